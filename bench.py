@@ -1,0 +1,163 @@
+#!/usr/bin/env python3
+"""Throughput of the E-field propagation path on one MI355X.
+
+Workload (default, id 2): 1024^2 pupil, 9 wavelengths (600-660 nm) x 3 point
+sources (0, 2, 6 lambda/D) through aperture/apodizer, phase aberration, a
+217-hex phase FPM (standard 8-ring layout), Lyot stop and a 128^2 image:
+27 fields per step, each carried pupil -> FPM -> Lyot -> image plus |E|^2 and
+the 3-10 lambda/D dark-hole mean.  Inputs are synthetic (Philox, seed
+20261203 + id) and resident in HBM before timing; the calibration
+(unocculted) propagation is computed once and amortised.
+
+One JSON line: value = propagations/s of the mixed-precision path; the FP64
+plain path is timed on the same workload beside it, with the contrast error of
+the mixed path against it, the roofline of the contraction kernel and a CPU
+baseline (the oracle restatement of csim, OpenMP, timed on this host).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+PEAKS = {  # MI355X dense peaks (TFLOP/s)
+    "fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0, "bf16x6": 2500.0,
+}
+PRODUCTS = {"fp64": 1, "fp32": 1, "bf16x3": 3, "bf16x6": 6}
+
+
+def build_chain(ctx, w, precision):
+    from csim_amd import workloads as W
+    from csim_amd.chain import Chain
+    ch = Chain(ctx, w.script)
+    W.apply(ch, w)
+    ch.set_precision(precision)
+    return ch
+
+
+def time_chain(ch, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        ch.contrast()
+    torch.cuda.synchronize()
+    ch.set_profiling(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        c, _ = ch.contrast()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    prof = ch.profile()
+    ch.set_profiling(False)
+    return dt, c, prof
+
+
+def gemm_roofline(prof, precision):
+    key = f"gemm_{precision}"
+    st = prof.get(key)
+    if not st or st["launches"] == 0:
+        return None
+    per_launch_flops = st["flops_total"] / st["launches"]
+    avg_s = st["ms_total"] / st["launches"] / 1e3
+    achieved = per_launch_flops / avg_s / 1e12
+    peak = PEAKS[precision]
+    return {"bound": "mfma", "kernel": key, "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+            "frac": round(achieved / peak, 4), "traffic": None,
+            "avg_launch_ms": round(st["ms_total"] / st["launches"], 4),
+            "flops_per_launch": per_launch_flops, "mfma_products": PRODUCTS[precision],
+            "issued_frac": round(achieved * PRODUCTS[precision] / peak, 4)}
+
+
+def cpu_baseline(w, budget_s=12.0):
+    """The oracle (FP64 CPU restatement of csim, Bluestein zoomFft, OpenMP) on a
+    bounded sample of the same workload's fields."""
+    import oracle
+    from oracle.chain import OracleChain
+    oc = OracleChain(w.script, w.arrays, sags=w.meta.get("sags"))
+    nl = len(oc.m.lambdas)
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        s, li = divmod(n % (oc.m.nsrc * nl), nl)
+        oc.field(s, li)
+        n += 1
+        if time.perf_counter() - t0 > budget_s or n >= oc.m.nsrc * nl:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": round(n / dt, 4), "unit": "propagations/s", "cores": oracle.num_threads(), "kind": "port",
+            "sample": f"{n} of the workload's fields (pupil->FPM->Lyot->image), Bluestein chirp-z zoomFft, "
+                      f"{dt:.1f} s"}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--precision", default="bf16x6", choices=["fp32", "bf16x3", "bf16x6"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    a = ap.parse_args(argv)
+    if a.gpus != 1:
+        print("bench.py measures one GPU: every field shards independently, so the whole-job rate at N GPUs is "
+              "N times this one-GPU figure; multi-GPU runs are not part of this build.", file=sys.stderr)
+        return 2
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py needs a HIP device", file=sys.stderr)
+        return 1
+    from csim_amd import workloads as W
+    from csim_amd.ops import Context
+    ctx = Context(0)
+    w = W.workload2()
+    fields = w.meta["fields"]
+
+    mix = build_chain(ctx, w, a.precision)
+    dt_mix, c_mix, prof_mix = time_chain(mix, a.steps, a.warmup)
+    img_mix = mix.normalized_image()
+    out = {
+        "metric": "propagations/s at 1024^2 pupil: mixed vs FP64 plain, contrast err <= 1e-11",
+        "value": round(fields * a.steps / dt_mix, 3),
+        "unit": "propagations/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(dt_mix / a.steps * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": a.precision, "data": "synthetic (Philox seed 20261203+2)",
+        "config": {"workload": "1024^2 pupil, 9 lambda 600-660 nm x 3 sources (0,2,6 l/D), 217-hex FPM, "
+                               "Lyot 0.9D, 128^2 image over +-25 l/D; 27 fields per step",
+                   "fields_per_step": fields, "precision": a.precision},
+    }
+    rf = gemm_roofline(prof_mix, a.precision)
+    if rf:
+        out["roofline"] = rf
+    out["kernels_mixed"] = prof_mix
+    if not a.no_fp64:
+        plain = build_chain(ctx, w, "fp64")
+        dt64, c64, prof64 = time_chain(plain, max(3, a.steps // 2), 1)
+        steps64 = max(3, a.steps // 2)
+        img64 = plain.normalized_image()
+        from oracle.chain import OracleChain
+        sel = OracleChain(w.script, w.arrays, sags=w.meta["sags"]).region_index(
+            (128, -(128 * (W.F_RATIO * 630e-9 / 2.56) - W.F_RATIO * 630e-9 / 2.56) / 2, W.F_RATIO * 630e-9 / 2.56))
+        out["fp64_plain"] = {"value": round(fields * steps64 / dt64, 3), "unit": "propagations/s",
+                             "ms_per_step": round(dt64 / steps64 * 1e3, 4),
+                             "roofline": gemm_roofline(prof64, "fp64")}
+        out["speedup_vs_fp64"] = round(out["value"] / out["fp64_plain"]["value"], 3)
+        out["contrast_error"] = {"dark_hole_mean_abs": float(abs(c_mix[0] - c64[0])),
+                                 "dark_hole_max_pixel_abs": float(np.max(np.abs(img_mix - img64)[sel])),
+                                 "image_max_pixel_abs": float(np.max(np.abs(img_mix - img64))),
+                                 "dark_hole_mean_fp64": float(c64[0]), "gate": 1e-11}
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(w)
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

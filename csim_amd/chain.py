@@ -1,0 +1,133 @@
+"""Element-chain API: csim input scripts run on the GPU (Python host mirror).
+
+Mirrors the reference's script-driven flow (``src/csim.cpp:54-108``): a script
+with ``telescope``, ``efield``, ``coronagraph`` (the element list,
+``src/coronagraph/coronagraph.cpp:67-155``) and ``regionContrast`` blocks.
+FITS ``filename:`` keys are honoured; ``set_array`` binds the same data from
+memory.  Arrays are numpy ``(rows, cols)`` in csim's index order.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import ChainInfo, PRECISIONS, Region, Report, check, lib
+from .ops import Context
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _colmajor(a, dtype):
+    return np.asfortranarray(np.asarray(a, dtype=dtype))
+
+
+class Chain:
+    def __init__(self, ctx: Context, script: str):
+        self.ctx = ctx
+        p = C.c_void_p()
+        check(lib().csx_chain_from_script(ctx.handle, script.encode(), C.byref(p)))
+        self._p = p
+
+    def close(self):
+        if self._p:
+            lib().csx_chain_destroy(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -------------------------------------------------------------- setup --
+    def set_array(self, element: str, key: str, a):
+        a = _colmajor(a, np.float64)
+        if a.ndim == 1:
+            a = a.reshape(-1, 1, order="F")
+        check(lib().csx_chain_set_array(self._p, element.encode(), key.encode(), _ptr(a), a.shape[0], a.shape[1]))
+
+    def set_complex_array(self, element: str, key: str, a, slice_index: int = 0):
+        a = _colmajor(a, np.complex128)
+        check(lib().csx_chain_set_complex_array(self._p, element.encode(), key.encode(), _ptr(a),
+                                                a.shape[0], a.shape[1], int(slice_index)))
+
+    def set_opt_data(self, element: str, name: str, v):
+        v = np.ascontiguousarray(v, dtype=np.float64).ravel()
+        check(lib().csx_chain_set_opt_data(self._p, element.encode(), name.encode(), _ptr(v), v.size))
+
+    def get_opt_data(self, element: str, name: str):
+        n = C.c_size_t(0)
+        check(lib().csx_chain_get_opt_data(self._p, element.encode(), name.encode(), None, C.byref(n)))
+        v = np.zeros(n.value)
+        check(lib().csx_chain_get_opt_data(self._p, element.encode(), name.encode(), _ptr(v), C.byref(n)))
+        return v
+
+    def set_candidates(self, element: str, sags):
+        """sags: (n_cand, n_hex) array, or None to return to the single sag set."""
+        if sags is None:
+            check(lib().csx_chain_set_candidates(self._p, element.encode(), None, 0, 0))
+            return
+        s = np.ascontiguousarray(sags, dtype=np.float64)
+        check(lib().csx_chain_set_candidates(self._p, element.encode(), _ptr(s), s.shape[1], s.shape[0]))
+
+    def set_precision(self, precision: str):
+        check(lib().csx_chain_set_precision(self._p, PRECISIONS[precision]))
+
+    def info(self) -> dict:
+        i = ChainInfo()
+        check(lib().csx_chain_get_info(self._p, C.byref(i)))
+        return {f: getattr(i, f) for f, _ in ChainInfo._fields_}
+
+    def lambdas(self):
+        n = check(lib().csx_chain_get_lambdas(self._p, None, 0))
+        out = np.zeros(n)
+        check(lib().csx_chain_get_lambdas(self._p, _ptr(out), n))
+        return out
+
+    # ------------------------------------------------------------ execute --
+    def propagate(self, calibrating: bool = False):
+        check(lib().csx_chain_propagate(self._p, int(bool(calibrating))))
+
+    def field(self, cand: int = 0, source: int = 0, lam: int = 0):
+        inf = self.info()
+        out = np.zeros((inf["out_cols"], inf["out_rows"]), dtype=np.complex128)
+        check(lib().csx_chain_get_field(self._p, cand, source, lam, _ptr(out)))
+        return out.T.copy()
+
+    def intensity(self, cand: int = 0):
+        inf = self.info()
+        out = np.zeros((inf["out_cols"], inf["out_rows"]))
+        check(lib().csx_chain_get_intensity(self._p, cand, _ptr(out)))
+        return out.T.copy()
+
+    def contrast(self, radii=None, angles=(-180.0, 180.0), xlim=(-1e16, 1e16), reference_lambda=-1.0):
+        """Region-mean contrast per candidate (regionContrast::compute_contrast)."""
+        n = self.info()["n_candidates"]
+        out = np.zeros(n)
+        rep = Report()
+        if radii is None:
+            check(lib().csx_chain_contrast(self._p, None, _ptr(out), C.byref(rep)))
+        else:
+            r = Region(float(radii[0]), float(radii[1]), float(angles[0]), float(angles[1]),
+                       float(xlim[0]), float(xlim[1]), float(reference_lambda))
+            check(lib().csx_chain_contrast(self._p, C.byref(r), _ptr(out), C.byref(rep)))
+        return out, {f: getattr(rep, f) for f, _ in Report._fields_}
+
+    def set_profiling(self, on: bool):
+        check(lib().csx_chain_set_profiling(self._p, int(bool(on))))
+
+    def profile(self) -> dict:
+        import json
+        n = check(lib().csx_chain_profile_json(self._p, None, 0))
+        buf = C.create_string_buffer(n + 1)
+        check(lib().csx_chain_profile_json(self._p, buf, n + 1))
+        return json.loads(buf.value.decode())
+
+    def normalized_image(self, cand: int = 0):
+        inf = self.info()
+        out = np.zeros((inf["out_cols"], inf["out_rows"]))
+        check(lib().csx_chain_get_normalized_image(self._p, cand, _ptr(out)))
+        return out.T.copy()

@@ -1,0 +1,180 @@
+// C ABI of the element / script API.
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "engine.hpp"
+
+using namespace csx;
+
+struct csx_chain {
+  Chain impl;
+  explicit csx_chain(csx_ctx* c) : impl(c) {}
+};
+
+#define CSX_GUARD_BEGIN try {
+#define CSX_GUARD_END                                                               \
+  }                                                                                 \
+  catch (const std::bad_alloc&) { return fail(CSX_ERR_NOMEM, "host allocation failed"); } \
+  catch (const std::exception& e) { return fail(CSX_ERR_STATE, e.what()); }          \
+  catch (...) { return fail(CSX_ERR_STATE, "unknown exception"); }
+
+static int chk(csx_chain* c) {
+  if (!c) return fail(CSX_ERR_ARG, "null chain");
+  CSX_HIP(hipSetDevice(c->impl.ctx()->device));
+  return CSX_OK;
+}
+
+extern "C" {
+
+int csx_chain_from_script(csx_ctx* ctx, const char* text, csx_chain** out) {
+  CSX_GUARD_BEGIN
+  if (!ctx || !text || !out) return fail(CSX_ERR_ARG, "csx_chain_from_script: null argument");
+  *out = nullptr;
+  CSX_HIP(hipSetDevice(ctx->device));
+  csx_chain* c = new csx_chain(ctx);
+  int rc = c->impl.load_script(text);
+  if (rc != CSX_OK) { delete c; return rc; }
+  *out = c;
+  return CSX_OK;
+  CSX_GUARD_END
+}
+
+void csx_chain_destroy(csx_chain* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->impl.ctx()->device);
+  (void)hipStreamSynchronize(c->impl.stream());
+  delete c;
+}
+
+int csx_chain_set_array(csx_chain* c, const char* e, const char* k, const double* d, int rows, int cols) {
+  CSX_GUARD_BEGIN
+  CSX_TRY(chk(c));
+  if (!e || !k || !d) return fail(CSX_ERR_ARG, "csx_chain_set_array: null argument");
+  return c->impl.set_array(e, k, d, rows, cols);
+  CSX_GUARD_END
+}
+
+int csx_chain_set_complex_array(csx_chain* c, const char* e, const char* k, const double* d, int rows, int cols,
+                                int slice) {
+  CSX_GUARD_BEGIN
+  CSX_TRY(chk(c));
+  if (!e || !k || !d) return fail(CSX_ERR_ARG, "csx_chain_set_complex_array: null argument");
+  return c->impl.set_complex_array(e, k, d, rows, cols, slice);
+  CSX_GUARD_END
+}
+
+int csx_chain_set_opt_data(csx_chain* c, const char* e, const char* n, const double* v, size_t len) {
+  CSX_GUARD_BEGIN
+  CSX_TRY(chk(c));
+  if (!e || !n || (!v && len)) return fail(CSX_ERR_ARG, "csx_chain_set_opt_data: null argument");
+  return c->impl.set_opt(e, n, std::vector<double>(v, v + len));
+  CSX_GUARD_END
+}
+
+int csx_chain_get_opt_data(csx_chain* c, const char* e, const char* n, double* v, size_t* len) {
+  CSX_GUARD_BEGIN
+  CSX_TRY(chk(c));
+  if (!e || !n || !len) return fail(CSX_ERR_ARG, "csx_chain_get_opt_data: null argument");
+  std::vector<double> out;
+  CSX_TRY(c->impl.get_opt(e, n, out));
+  if (v) {
+    if (*len < out.size()) { *len = out.size(); return fail(CSX_ERR_ARG, "output buffer too small"); }
+    std::memcpy(v, out.data(), sizeof(double) * out.size());
+  }
+  *len = out.size();
+  return CSX_OK;
+  CSX_GUARD_END
+}
+
+int csx_chain_set_candidates(csx_chain* c, const char* e, const double* sags, int n_hex, int n_cand) {
+  CSX_GUARD_BEGIN
+  CSX_TRY(chk(c));
+  if (!e) return fail(CSX_ERR_ARG, "csx_chain_set_candidates: null element name");
+  return c->impl.set_candidates(e, sags, n_hex, n_cand);
+  CSX_GUARD_END
+}
+
+int csx_chain_set_precision(csx_chain* c, csx_precision p) {
+  CSX_TRY(chk(c));
+  if (p < CSX_FP64 || p > CSX_BF16X6) return fail(CSX_ERR_ARG, "unknown precision");
+  return c->impl.set_precision(p);
+}
+
+int csx_chain_get_info(csx_chain* c, csx_chain_info* info) {
+  CSX_GUARD_BEGIN
+  CSX_TRY(chk(c));
+  if (!info) return fail(CSX_ERR_ARG, "null info");
+  return c->impl.info(info);
+  CSX_GUARD_END
+}
+
+int csx_chain_get_lambdas(csx_chain* c, double* lambdas, int n) {
+  CSX_TRY(chk(c));
+  const auto& l = c->impl.efield().lambdas;
+  if (!lambdas) return (int)l.size();
+  if (n < (int)l.size()) return fail(CSX_ERR_ARG, "output too small");
+  std::memcpy(lambdas, l.data(), sizeof(double) * l.size());
+  return (int)l.size();
+}
+
+int csx_chain_propagate(csx_chain* c, int calibrating) {
+  CSX_GUARD_BEGIN
+  CSX_TRY(chk(c));
+  return c->impl.propagate(calibrating != 0);
+  CSX_GUARD_END
+}
+
+int csx_chain_get_field(csx_chain* c, int cand, int source, int lambda_index, double* out) {
+  CSX_GUARD_BEGIN
+  CSX_TRY(chk(c));
+  if (!out) return fail(CSX_ERR_ARG, "null output");
+  return c->impl.get_field(cand, source, lambda_index, out);
+  CSX_GUARD_END
+}
+
+int csx_chain_get_intensity(csx_chain* c, int cand, double* out) {
+  CSX_GUARD_BEGIN
+  CSX_TRY(chk(c));
+  if (!out) return fail(CSX_ERR_ARG, "null output");
+  return c->impl.get_intensity(cand, out);
+  CSX_GUARD_END
+}
+
+int csx_chain_contrast(csx_chain* c, const csx_region* r, double* out, csx_report* rep) {
+  CSX_GUARD_BEGIN
+  CSX_TRY(chk(c));
+  if (!out) return fail(CSX_ERR_ARG, "null output");
+  return c->impl.contrast(r, out, rep);
+  CSX_GUARD_END
+}
+
+int csx_chain_get_normalized_image(csx_chain* c, int cand, double* out) {
+  CSX_GUARD_BEGIN
+  CSX_TRY(chk(c));
+  if (!out) return fail(CSX_ERR_ARG, "null output");
+  return c->impl.get_normalized_image(cand, out);
+  CSX_GUARD_END
+}
+
+int csx_chain_set_profiling(csx_chain* c, int enable) {
+  CSX_TRY(chk(c));
+  return c->impl.set_profiling(enable != 0);
+}
+
+int csx_chain_profile_json(csx_chain* c, char* buf, size_t cap) {
+  CSX_GUARD_BEGIN
+  CSX_TRY(chk(c));
+  std::string j = c->impl.profile_json();
+  if (buf && cap) {
+    size_t n = std::min(cap - 1, j.size());
+    std::memcpy(buf, j.data(), n);
+    buf[n] = 0;
+  }
+  return (int)j.size();
+  CSX_GUARD_END
+}
+
+}  // extern "C"

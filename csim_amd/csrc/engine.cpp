@@ -1,0 +1,1320 @@
+// Element chain engine: csim input scripts compiled into GPU stages.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+
+#include "gemm.hpp"
+#include "ops.hpp"
+
+namespace csx {
+
+// ================================================================ parsing ==
+// initCommandSet(file) (src/lib/csim_parser.cpp:60-82) and add_command (:93-118):
+// leading blanks stripped, '#' lines and empty lines skipped, "key:arg" split at
+// the first ':' (a line without ':' is a bare command).
+Block parse_script_text(const std::string& text) {
+  Block b;
+  std::istringstream in(text);
+  std::string line;
+  while (std::getline(in, line)) {
+    if (!line.empty() && line.back() == '\r') line.pop_back();
+    size_t p = line.find_first_not_of(" \t");
+    if (p == std::string::npos) continue;
+    line.erase(0, p);
+    if (line[0] == '#') continue;
+    size_t c = line.find(':');
+    if (c == std::string::npos) b.cmds.push_back({line, ""});
+    else b.cmds.push_back({line.substr(0, c), line.substr(c + 1)});
+  }
+  return b;
+}
+
+// find_command_blocks (src/lib/csim_parser.cpp:131-190): top-level start/end
+// blocks (nested ones kept inside); commands outside any block form one extra
+// block placed first.
+std::vector<Block> find_blocks(const Block& b) {
+  std::vector<Block> out;
+  Block loose;
+  bool has_loose = false;
+  int depth = 0;
+  for (const Cmd& c : b.cmds) {
+    if (c.key == "start") {
+      if (depth == 0) out.push_back(Block());
+      else out.back().cmds.push_back(c);
+      depth++;
+    } else if (c.key == "end") {
+      depth--;
+      if (depth > 0) out.back().cmds.push_back(c);
+    } else if (depth == 0) {
+      loose.cmds.push_back(c);
+      has_loose = true;
+    } else {
+      out.back().cmds.push_back(c);
+    }
+  }
+  if (has_loose) out.insert(out.begin(), loose);
+  return out;
+}
+
+static std::string trim(const std::string& s) {
+  size_t a = s.find_first_not_of(" \t"), b = s.find_last_not_of(" \t");
+  return a == std::string::npos ? std::string() : s.substr(a, b - a + 1);
+}
+static std::vector<double> parse_list(const std::string& s) {
+  std::vector<double> v;
+  std::string t = s;
+  for (char& ch : t) if (ch == ',') ch = ' ';
+  std::istringstream in(t);
+  double x;
+  while (in >> x) v.push_back(x);
+  return v;
+}
+// sscanf("%f, %f") as regionContrast::set does: values pass through float.
+static std::vector<double> parse_float_pair(const std::string& s) {
+  float a = 0, b = 0;
+  std::sscanf(s.c_str(), "%f, %f", &a, &b);
+  return {(double)a, (double)b};
+}
+
+// ---------------------------------------------------------------- FITS ---
+int read_fits(const std::string& path, std::vector<double>& data, std::vector<long>& dims) {
+  std::ifstream f(path, std::ios::binary);
+  if (!f) return fail(CSX_ERR_ARG, "cannot open FITS file " + path);
+  int bitpix = 0, naxis = 0;
+  long ax[3] = {1, 1, 1};
+  double bscale = 1, bzero = 0;
+  bool end = false;
+  char card[81];
+  card[80] = 0;
+  while (!end) {
+    for (int i = 0; i < 36; i++) {
+      if (!f.read(card, 80)) return fail(CSX_ERR_PARSE, "truncated FITS header in " + path);
+      std::string k = trim(std::string(card, 8));
+      std::string v = std::string(card + 10, 70);
+      if (k == "END") end = true;
+      else if (k == "BITPIX") bitpix = std::atoi(v.c_str());
+      else if (k == "NAXIS") naxis = std::atoi(v.c_str());
+      else if (k == "NAXIS1") ax[0] = std::atol(v.c_str());
+      else if (k == "NAXIS2") ax[1] = std::atol(v.c_str());
+      else if (k == "NAXIS3") ax[2] = std::atol(v.c_str());
+      else if (k == "BSCALE") bscale = std::atof(v.c_str());
+      else if (k == "BZERO") bzero = std::atof(v.c_str());
+    }
+  }
+  if (naxis < 1 || naxis > 3) return fail(CSX_ERR_UNSUPPORTED, "FITS NAXIS must be 1..3");
+  dims.assign(ax, ax + naxis);
+  size_t n = 1;
+  for (int i = 0; i < naxis; i++) n *= (size_t)ax[i];
+  int bytes = std::abs(bitpix) / 8;
+  std::vector<unsigned char> raw(n * bytes);
+  if (!f.read(reinterpret_cast<char*>(raw.data()), raw.size())) return fail(CSX_ERR_PARSE, "truncated FITS data");
+  data.resize(n);
+  for (size_t i = 0; i < n; i++) {
+    unsigned char* p = &raw[i * bytes];
+    uint64_t u = 0;
+    for (int b = 0; b < bytes; b++) u = (u << 8) | p[b];   // big endian
+    double v;
+    switch (bitpix) {
+      case -64: { double d; std::memcpy(&d, &u, 8); v = d; break; }
+      case -32: { uint32_t w = (uint32_t)u; float x; std::memcpy(&x, &w, 4); v = x; break; }
+      case 32: v = (double)(int32_t)(uint32_t)u; break;
+      case 16: v = (double)(int16_t)(uint16_t)u; break;
+      case 8: v = (double)(uint8_t)u; break;
+      default: return fail(CSX_ERR_UNSUPPORTED, "FITS BITPIX not supported");
+    }
+    data[i] = bzero + bscale * v;
+  }
+  return CSX_OK;
+}
+
+// FITS matrix -> column-major rows x cols (rows = NAXIS2, cols = NAXIS1), csim_fits.cpp:195-233.
+static int fits_matrix(const std::string& path, std::vector<double>& m, int& rows, int& cols, int& slices) {
+  std::vector<double> d;
+  std::vector<long> dims;
+  CSX_TRY(read_fits(path, d, dims));
+  cols = (int)dims[0];
+  rows = dims.size() > 1 ? (int)dims[1] : 1;
+  slices = dims.size() > 2 ? (int)dims[2] : 1;
+  if (dims.size() == 1) { rows = cols; cols = 1; }   // vectors: n x 1
+  m.resize(d.size());
+  for (int s = 0; s < slices; s++)
+    for (int r = 0; r < rows; r++)
+      for (int c = 0; c < cols; c++)
+        m[(size_t)s * rows * cols + (size_t)c * rows + r] =
+            dims.size() == 1 ? d[r] : d[(size_t)s * rows * cols + (size_t)r * cols + c];
+  return CSX_OK;
+}
+
+// ================================================================ geometry ==
+Geom geom_symmetric(int nx, int ny, double dx, double dy, double ox, double oy) {
+  Geom g;
+  g.nx = nx; g.ny = ny; g.dx = dx; g.dy = dy;
+  g.x0 = -(dx * nx - dx) / 2.0 - ox;
+  g.y0 = -(dy * ny - dy) / 2.0 - oy;
+  return g;
+}
+Geom geom_offset(int nx, int ny, double dx, double dy) {
+  Geom g;
+  g.nx = nx; g.ny = ny; g.dx = dx; g.dy = dy;
+  g.x0 = (0.0 - ((double)nx) / 2.) * dx;
+  g.y0 = (0.0 - ((double)ny) / 2.) * dy;
+  return g;
+}
+static csx_grid gx(const Geom& g) { return csx_grid{g.nx, g.x0, g.dx}; }
+static csx_grid gy(const Geom& g) { return csx_grid{g.ny, g.y0, g.dy}; }
+
+// ================================================================ elements ==
+bool Element::set_common(const std::string& key, const std::string& arg) {
+  std::string a = trim(arg);
+  if (key == "position") position = std::atof(a.c_str());
+  else if (key == "name") name = a;
+  else if (key == "disableForCalibration") disableForCalibration = (a == "on");
+  else if (key == "draw" || key == "save" || key == "break") { /* host-side debugging hooks: no GPU effect */ }
+  else return false;
+  return true;
+}
+int Element::set_opt(Chain&, const std::string& n, const std::vector<double>& v) {
+  if (n == "position" && !v.empty()) { position = v[0]; return CSX_OK; }   // celem.cpp:132-145
+  return fail(CSX_ERR_ARG, "element '" + name + "' has no optimisation data '" + n + "'");
+}
+int Element::get_opt(Chain&, const std::string& n, std::vector<double>& v) {
+  if (n == "position") { v.assign(1, position); return CSX_OK; }
+  return fail(CSX_ERR_ARG, "element '" + name + "' has no optimisation data '" + n + "'");
+}
+int Element::set_array(Chain&, const std::string& key, const double*, int, int) {
+  return fail(CSX_ERR_ARG, "element '" + name + "' (" + type + ") takes no array '" + key + "'");
+}
+int Element::set_complex_array(Chain&, const std::string& key, const double*, int, int, int) {
+  return fail(CSX_ERR_ARG, "element '" + name + "' (" + type + ") takes no complex array '" + key + "'");
+}
+
+static int unknown_key(const std::string& type, const std::string& key) {
+  std::fprintf(stderr, "!!! %s bad set field name: %s\n", type.c_str(), key.c_str());
+  return CSX_OK;   // csim prints and continues
+}
+
+// Real mask / mirror data shared helper: resampled to the current grid with
+// interp2 from the initial efield grid when sizes differ (mask.cpp:28-35,
+// mirror.cpp:26-31, csim_geom.cpp:331-339).
+struct RealMap {
+  std::vector<double> host;
+  int rows = 0, cols = 0;
+  std::map<long, DevBuf> dev;   // keyed by rows*65536+cols of the grid it is resampled to
+  int device_for(Chain& ch, const Geom& cur, const double** out) {
+    long key = (long)cur.ny * 65536 + cur.nx;
+    auto it = dev.find(key);
+    if (it == dev.end()) {
+      std::vector<double> m;
+      const Efield& ef = ch.efield();
+      if (rows == cur.ny && cols == cur.nx) {
+        m = host;
+      } else {
+        if (ef.rows == cur.ny && ef.cols == cur.nx) {
+          m = host;   // reference returns without resampling when the grids match in size
+          if (rows != cur.ny || cols != cur.nx) return fail(CSX_ERR_ARG, "mask size does not match the field");
+        } else {
+          // separable linear interpolation from the initial grid (pixelX/pixelY)
+          auto inX = grid_symmetric(ef.cols, ef.pixelScale), inY = grid_symmetric(ef.rows, ef.pixelScale);
+          std::vector<double> outX(cur.nx), outY(cur.ny);
+          for (int i = 0; i < cur.nx; i++) outX[i] = cur.x0 + i * cur.dx;
+          for (int i = 0; i < cur.ny; i++) outY[i] = cur.y0 + i * cur.dy;
+          if (rows != ef.rows || cols != ef.cols) return fail(CSX_ERR_ARG, "mask size does not match the efield");
+          auto tx = interp_table(inX, outX), ty = interp_table(inY, outY);
+          std::vector<double> s1((size_t)rows * cur.nx);
+          for (int r = 0; r < rows; r++)
+            for (int c = 0; c < cur.nx; c++) {
+              const InterpTable& t = tx[c];
+              s1[(size_t)c * rows + r] = t.lo < 0 ? 0.0
+                  : (1.0 - t.w) * host[(size_t)t.lo * rows + r] + t.w * host[(size_t)t.hi * rows + r];
+            }
+          m.assign((size_t)cur.ny * cur.nx, 0.0);
+          for (int c = 0; c < cur.nx; c++)
+            for (int r = 0; r < cur.ny; r++) {
+              const InterpTable& t = ty[r];
+              m[(size_t)c * cur.ny + r] = t.lo < 0 ? 0.0
+                  : (1.0 - t.w) * s1[(size_t)c * rows + t.lo] + t.w * s1[(size_t)c * rows + t.hi];
+            }
+        }
+      }
+      DevBuf b;
+      CSX_TRY(b.reserve(m.size() * sizeof(double)));
+      CSX_HIP(hipMemcpy(b.p, m.data(), m.size() * sizeof(double), hipMemcpyHostToDevice));
+      it = dev.emplace(key, std::move(b)).first;
+    }
+    *out = it->second.as<double>();
+    return CSX_OK;
+  }
+  int set(const double* d, int r, int c) {
+    if (!d || r <= 0 || c <= 0) return fail(CSX_ERR_ARG, "empty array");
+    host.assign(d, d + (size_t)r * c);
+    rows = r; cols = c;
+    dev.clear();
+    return CSX_OK;
+  }
+  int load(const std::string& path) {
+    int r, c, s;
+    std::vector<double> m;
+    CSX_TRY(fits_matrix(trim(path), m, r, c, s));
+    return set(m.data(), r, c);
+  }
+};
+
+// ---- mask (src/elements/mask.cpp): E *= (1+i) m
+struct MaskElem : Element {
+  RealMap map;
+  int set(Chain&, const std::string& k, const std::string& a) override {
+    if (set_common(k, a) || k == "mask") return CSX_OK;
+    if (k == "filename") return map.load(a);
+    return unknown_key(type, k);
+  }
+  int set_array(Chain& ch, const std::string& k, const double* d, int r, int c) override {
+    if (k != "filename" && k != "mask") return Element::set_array(ch, k, d, r, c);
+    ch.invalidate();
+    return map.set(d, r, c);
+  }
+  int compile(Chain& ch, Compile& c, int) override {
+    if (map.host.empty()) return fail(CSX_ERR_STATE, "mask '" + name + "' has no data");
+    const double* m = nullptr;
+    CSX_TRY(map.device_for(ch, c.geom, &m));
+    double2* E = ch.field_buf(c.buf);
+    long npix = c.geom.npix();
+    int nf = ch.nfield_for(c);
+    c.stages.push_back([=](hipStream_t s) { return k_apply_mask(E, npix, nf, m, s); });
+    c.stage_names.push_back("mask " + name);
+    return CSX_OK;
+  }
+};
+
+// ---- mirror (src/elements/mirror.cpp): E *= exp(-sign 4 pi i h / lambda)
+struct MirrorElem : Element {
+  RealMap map;
+  double sign = 1;
+  int set(Chain&, const std::string& k, const std::string& a) override {
+    if (set_common(k, a) || k == "mirror") return CSX_OK;
+    if (k == "filename") return map.load(a);
+    if (k == "sign") {
+      double v = std::atof(a.c_str());
+      if (v == -1 || v == 1) sign = v;
+      else std::fprintf(stderr, "!!!! mirror: illegal sign, ignoring\n");
+      return CSX_OK;
+    }
+    return unknown_key(type, k);
+  }
+  int set_array(Chain& ch, const std::string& k, const double* d, int r, int c) override {
+    if (k != "filename" && k != "sag") return Element::set_array(ch, k, d, r, c);
+    ch.invalidate();
+    return map.set(d, r, c);
+  }
+  int compile(Chain& ch, Compile& c, int) override {
+    if (map.host.empty()) return fail(CSX_ERR_STATE, "mirror '" + name + "' has no data");
+    const double* h = nullptr;
+    CSX_TRY(map.device_for(ch, c.geom, &h));
+    double2* E = ch.field_buf(c.buf);
+    long npix = c.geom.npix();
+    int nf = ch.nfield_for(c);
+    double sg = sign;
+    Chain* chp = &ch;
+    bool cal = c.calibrating;
+    c.stages.push_back([=](hipStream_t s) {
+      const double* l = cal ? chp->scratch("lam_cal").as<double>() : chp->lambdas_dev();
+      return k_apply_mirror(E, npix, nf, h, sg, l, s);
+    });
+    c.stage_names.push_back("mirror " + name);
+    return CSX_OK;
+  }
+};
+
+// ---- complexMask (src/elements/complexMask.cpp): centred window *= amp e^{-i ph}
+struct ComplexMaskElem : Element {
+  std::vector<double2> cube;   // rows x cols x slices (amp e^{-i ph})
+  int rows = 0, cols = 0, slices = 0, maskIndex = 0;
+  DevBuf dev;
+  int set(Chain&, const std::string& k, const std::string& a) override {
+    if (set_common(k, a) || k == "complexMask") return CSX_OK;
+    if (k == "maskIndex") { maskIndex = std::atoi(a.c_str()); return CSX_OK; }
+    if (k == "filename") {
+      std::string s = a;
+      size_t cpos = s.find(',');
+      if (cpos == std::string::npos) return fail(CSX_ERR_PARSE, "complexMask filename needs amp,ph");
+      std::vector<double> am, ph;
+      int r, c, sl, r2, c2, sl2;
+      CSX_TRY(fits_matrix(trim(s.substr(0, cpos)), am, r, c, sl));
+      CSX_TRY(fits_matrix(trim(s.substr(cpos + 1)), ph, r2, c2, sl2));
+      if (r != r2 || c != c2 || sl != sl2) return fail(CSX_ERR_ARG, "complexMask amp/ph sizes differ");
+      rows = r; cols = c; slices = sl;
+      cube.resize(am.size());
+      for (size_t i = 0; i < am.size(); i++) cube[i] = make_double2(am[i] * std::cos(-ph[i]), am[i] * std::sin(-ph[i]));
+      return CSX_OK;
+    }
+    return unknown_key(type, k);
+  }
+  int set_complex_array(Chain& ch, const std::string& k, const double* d, int r, int c, int slice) override {
+    if (k != "mask") return Element::set_complex_array(ch, k, d, r, c, slice);
+    if (slice < 0) return fail(CSX_ERR_ARG, "negative slice");
+    if (rows != r || cols != c) { rows = r; cols = c; slices = 0; cube.clear(); }
+    if (slice >= slices) { slices = slice + 1; cube.resize((size_t)rows * cols * slices); }
+    std::memcpy(&cube[(size_t)slice * rows * cols], d, sizeof(double2) * rows * cols);
+    ch.invalidate();
+    return CSX_OK;
+  }
+  int compile(Chain& ch, Compile& c, int) override {
+    if (skipped(c)) return CSX_OK;
+    if (cube.empty() || maskIndex >= slices) return fail(CSX_ERR_STATE, "complexMask '" + name + "' has no data");
+    CSX_TRY(dev.reserve(sizeof(double2) * rows * cols));
+    CSX_HIP(hipMemcpy(dev.p, &cube[(size_t)maskIndex * rows * cols], sizeof(double2) * rows * cols,
+                      hipMemcpyHostToDevice));
+    // expand to a full-size factor once (centred window, 1 elsewhere)
+    Geom g = c.geom;
+    std::vector<double2> full((size_t)g.npix(), make_double2(1, 0));
+    int r0 = g.ny / 2 - rows / 2, c0 = g.nx / 2 - cols / 2;
+    for (int cc = 0; cc < cols; cc++)
+      for (int rr = 0; rr < rows; rr++) {
+        int R = r0 + rr, C = c0 + cc;
+        if (R >= 0 && R < g.ny && C >= 0 && C < g.nx)
+          full[(size_t)C * g.ny + R] = cube[(size_t)maskIndex * rows * cols + (size_t)cc * rows + rr];
+      }
+    CSX_TRY(dev.reserve(sizeof(double2) * full.size()));
+    CSX_HIP(hipMemcpy(dev.p, full.data(), sizeof(double2) * full.size(), hipMemcpyHostToDevice));
+    double2* E = ch.field_buf(c.buf);
+    const double2* P = dev.as<double2>();
+    long npix = g.npix();
+    int nf = ch.nfield_for(c);
+    c.stages.push_back([=](hipStream_t s) { return k_apply_cfactor(E, npix, nf, P, nf, s); });
+    c.stage_names.push_back("complexMask " + name);
+    return CSX_OK;
+  }
+};
+
+// ---- fresnelPropagator (src/elements/fresnelPropagator.cpp + csim_fft.cpp:371-392)
+struct FresnelElem : Element {
+  double sign = 1;
+  int padFactor = 1;
+  int set(Chain&, const std::string& k, const std::string& a) override {
+    if (set_common(k, a) || k == "fresnelPropagator") return CSX_OK;
+    if (k == "propagationSign") {
+      double v = std::atof(a.c_str());
+      if (v == -1 || v == 1) sign = v;
+      return CSX_OK;
+    }
+    if (k == "padFactor") { padFactor = std::atoi(a.c_str()); return CSX_OK; }
+    return unknown_key(type, k);
+  }
+  int compile(Chain& ch, Compile& c, int index) override {
+    Element* prev = ch.elem_at(index - 1);
+    Element* next = ch.elem_at(index + 1);
+    if (!prev || !next) return fail(CSX_ERR_STATE, "fresnelPropagator needs a previous and a next element");
+    double z = sign * std::fabs(next->position - prev->position);
+    if (c.geom.nx != c.geom.ny) return fail(CSX_ERR_UNSUPPORTED, "fresnelPropagator: square arrays only");
+    int n = c.geom.nx;
+    double apRad = ch.efield().arrayRadius;
+    double2* E = ch.field_buf(c.buf);
+    int nf = ch.nfield_for(c);
+    Chain* chp = &ch;
+    bool cal = c.calibrating;
+    if (padFactor == 1) {
+      c.stages.push_back([=](hipStream_t s) {
+        const double* l = cal ? chp->scratch("lam_cal").as<double>() : chp->lambdas_dev();
+        return fresnel_batched(E, n, nf, l, apRad, z, s);
+      });
+    } else {
+      // embed in a 2n zero pad with apRad * padFactor (fresnelPropagator.cpp:45-54)
+      double2* P = ch.field_buf(1 - c.buf);
+      double ap = padFactor * apRad;
+      c.stages.push_back([=](hipStream_t s) {
+        const double* l = cal ? chp->scratch("lam_cal").as<double>() : chp->lambdas_dev();
+        CSX_TRY(embed_center(E, n, n, P, 2 * n, 2 * n, nf, 0, s));
+        CSX_TRY(fresnel_batched(P, 2 * n, nf, l, ap, z, s));
+        return embed_center(P, 2 * n, 2 * n, E, n, n, nf, 1, s);
+      });
+    }
+    c.stage_names.push_back("fresnelPropagator " + name);
+    return CSX_OK;
+  }
+};
+
+// ---- downsample (src/elements/downsample.cpp)
+struct DownsampleElem : Element {
+  int factor = 1;
+  DevBuf tables;
+  int set(Chain&, const std::string& k, const std::string& a) override {
+    if (set_common(k, a) || k == "downsample") return CSX_OK;
+    if (k == "sampleFactor") { factor = std::atoi(a.c_str()); return CSX_OK; }
+    return unknown_key(type, k);
+  }
+  int geometry(Chain&, const Geom& in, Geom& out) override {
+    if (factor < 1) return fail(CSX_ERR_ARG, "downsample: sampleFactor must be >= 1");
+    // newGeom.set_geometry(n/factor, ...): symmetric grid of pixel factor*d (downsample.cpp:35-40)
+    out = geom_symmetric(in.nx / factor, in.ny / factor, factor * in.dx, factor * in.dy);
+    return CSX_OK;
+  }
+  int compile(Chain& ch, Compile& c, int) override {
+    Geom out;
+    CSX_TRY(geometry(ch, c.geom, out));
+    if (c.geom.nx != c.geom.ny) return fail(CSX_ERR_UNSUPPORTED, "downsample: square arrays only");
+    std::vector<double> inX(c.geom.nx), inY(c.geom.ny), outX(out.nx), outY(out.ny);
+    for (int i = 0; i < c.geom.nx; i++) inX[i] = c.geom.x0 + i * c.geom.dx;
+    for (int i = 0; i < c.geom.ny; i++) inY[i] = c.geom.y0 + i * c.geom.dy;
+    for (int i = 0; i < out.nx; i++) outX[i] = out.x0 + i * out.dx;
+    for (int i = 0; i < out.ny; i++) outY[i] = out.y0 + i * out.dy;
+    auto tx = interp_table(inX, outX), ty = interp_table(inY, outY);
+    CSX_TRY(tables.reserve(sizeof(InterpTable) * (tx.size() + ty.size())));
+    CSX_HIP(hipMemcpy(tables.p, tx.data(), sizeof(InterpTable) * tx.size(), hipMemcpyHostToDevice));
+    CSX_HIP(hipMemcpy(tables.as<InterpTable>() + tx.size(), ty.data(), sizeof(InterpTable) * ty.size(),
+                      hipMemcpyHostToDevice));
+    const InterpTable* dtx = tables.as<InterpTable>();
+    const InterpTable* dty = dtx + tx.size();
+    double2* in = ch.field_buf(c.buf);
+    double2* o = ch.field_buf(1 - c.buf);
+    int nf = ch.nfield_for(c);
+    Geom g = c.geom;
+    c.stages.push_back([=](hipStream_t s) {
+      return resample_batched(in, g.ny, g.nx, nf, o, out.ny, out.nx, dty, dtx, s);
+    });
+    c.stage_names.push_back("downsample " + name);
+    c.buf = 1 - c.buf;
+    c.geom = out;
+    return CSX_OK;
+  }
+};
+
+// ---- MFT helpers shared by the Fraunhofer-type elements
+// Builds one MFT over all fields: from fields in buffer c.buf (geometry gin) to
+// the other buffer (geometry gout per lambda), lambda*f per field.
+static int compile_mft(Chain& ch, Compile& c, const std::vector<Geom>& gout_per_lam,
+                       const std::vector<double>& lf_per_lam, const std::string& label,
+                       const std::vector<MftField>* masks = nullptr) {
+  const int nf = ch.nfield_for(c), nl = ch.nlam();
+  const Geom gin = c.geom;
+  const bool c64 = ch.precision() != CSX_FP64;
+  double2* in = ch.field_buf(c.buf);
+  double2* out = ch.field_buf(1 - c.buf);
+  float2* in64 = static_cast<float2*>(ch.c64_buf());
+  std::vector<MftField> F(nf);
+  const Geom& g0 = gout_per_lam[0];
+  for (int f = 0; f < nf; f++) {
+    int l = f % nl;
+    const Geom& go = gout_per_lam[l];
+    if (go.nx != g0.nx || go.ny != g0.ny) return fail(CSX_ERR_UNSUPPORTED, label + ": output size varies with lambda");
+    MftField& m = F[f];
+    m.in = c64 ? (const void*)(in64 + (size_t)f * gin.npix()) : (const void*)(in + (size_t)f * gin.npix());
+    m.in_c64 = c64;
+    m.out = out + (size_t)f * go.npix();
+    m.out_c64 = 0;
+    m.in_x = gx(gin); m.in_y = gy(gin); m.out_x = gx(go); m.out_y = gy(go);
+    m.lambda_f = lf_per_lam[l];
+    if (masks) {
+      const MftField& mk = (*masks)[f];
+      m.mask_kind = mk.mask_kind; m.mask = mk.mask;
+      m.mrows = mk.mrows; m.mcols = mk.mcols; m.row0 = mk.row0; m.col0 = mk.col0;
+    }
+  }
+  auto plan = std::make_unique<MftPlan>();
+  CSX_TRY(plan->build(F, ch.precision(), ch.stream()));
+  const MftPlan* pp = plan.get();
+  c.plans.push_back(std::move(plan));
+  size_t nconv = (size_t)nf * gin.npix();
+  Chain* chp = &ch;
+  c.stages.push_back([=](hipStream_t s) {
+    KernelTimer* t = chp->timer();
+    if (c64) {
+      if (t) CSX_TRY(t->begin("convert_c128_to_c64", 0.0, 24.0 * nconv, s));
+      CSX_TRY(convert_c128_to_c64(in, in64, nconv, s));
+      if (t) CSX_TRY(t->end(s));
+    }
+    return pp->run(s, t);
+  });
+  c.stage_names.push_back(label);
+  c.buf = 1 - c.buf;
+  c.geom = g0;
+  return CSX_OK;
+}
+
+// ---- fraunhoferFocal (src/elements/fraunhoferFocal.cpp)
+struct FraunhoferFocalElem : Element {
+  double referenceLambda = 0, samplesPerflD = 0, FOVflD = 0;
+  int set(Chain&, const std::string& k, const std::string& a) override {
+    if (set_common(k, a) || k == "fraunhoferFocal") return CSX_OK;
+    if (k == "referenceLambda") referenceLambda = std::atof(a.c_str());
+    else if (k == "samplesPerflD") samplesPerflD = std::atof(a.c_str());
+    else if (k == "FOVflD") FOVflD = std::atof(a.c_str());
+    else return unknown_key(type, k);
+    return CSX_OK;
+  }
+  Geom out_geom(Chain& ch) const {   // set_geometry (:63-70)
+    double scale = ch.telescope().loD(referenceLambda);
+    int N = (int)std::ceil(FOVflD * samplesPerflD);
+    return geom_symmetric(N, N, scale / samplesPerflD, scale / samplesPerflD);
+  }
+  int geometry(Chain& ch, const Geom&, Geom& out) override { out = out_geom(ch); return CSX_OK; }
+  int compile(Chain& ch, Compile& c, int) override {
+    if (samplesPerflD <= 0 || FOVflD <= 0) return fail(CSX_ERR_ARG, "fraunhoferFocal: samplesPerflD/FOVflD unset");
+    const Efield& ef = ch.efield();
+    std::vector<Geom> go(ch.nlam(), out_geom(ch));
+    std::vector<double> lf(ch.nlam());
+    for (int l = 0; l < ch.nlam(); l++) lf[l] = ef.lambdas[l] * 2 * ef.beamRadius * ch.telescope().F;   // :38
+    return compile_mft(ch, c, go, lf, "fraunhoferFocal " + name);
+  }
+};
+
+// ---- zoomFftPropagator (src/elements/zoomFftPropagator.cpp:23-91)
+struct ZoomFftPropElem : Element {
+  int dir = 1;
+  double zoomFactor = 1.0, padFactor = 1.0;
+  int set(Chain&, const std::string& k, const std::string& a) override {
+    if (set_common(k, a) || k == "zoomFftPropagator") return CSX_OK;
+    if (k == "dir") dir = std::atoi(a.c_str());
+    else if (k == "zoomFactor") zoomFactor = std::atof(a.c_str());
+    else if (k == "padFactor") padFactor = std::atof(a.c_str());
+    else return unknown_key(type, k);
+    return CSX_OK;
+  }
+  int out_geom(Chain& ch, const Geom& in, Geom& out, double& lf) const {
+    const Efield& ef = ch.efield();
+    if (dir == 1) {
+      double dxp = 2 * (ef.beamRadius / (ef.pixelScale * in.ny * zoomFactor)) * ef.lambdas[0] * 80;
+      out = geom_offset((int)(padFactor * in.nx), (int)(padFactor * in.ny), dxp, dxp);
+      lf = ef.lambdas[0] * 3.52;
+    } else if (dir == -1) {
+      out = geom_offset((int)(in.nx / padFactor), (int)(in.ny / padFactor), ef.pixelScale, ef.pixelScale);
+      lf = -ef.lambdas[0] * 3.52;
+    } else {
+      return fail(CSX_ERR_ARG, "zoomFftPropagator: dir must be 1 or -1");
+    }
+    return CSX_OK;
+  }
+  int geometry(Chain& ch, const Geom& in, Geom& out) override {
+    double lf;
+    return out_geom(ch, in, out, lf);
+  }
+  int compile(Chain& ch, Compile& c, int) override {
+    if (skipped(c)) return CSX_OK;
+    Geom out;
+    double lf;
+    CSX_TRY(out_geom(ch, c.geom, out, lf));
+    if (out.nx != out.ny) return fail(CSX_ERR_UNSUPPORTED, "zoomFft output must be square");
+    std::vector<Geom> go(ch.nlam(), out);
+    std::vector<double> lfs(ch.nlam(), lf);   // the reference uses lambda[0] for every slice
+    return compile_mft(ch, c, go, lfs, "zoomFftPropagator " + name);
+  }
+};
+
+// ---- complexHexMaskFPM sub-element (src/elements/complexHexMaskFPM.cpp)
+struct HexFpm {
+  int maskNRings = -1, fpmArraySize = -1, nSubPix = -1, fpmMaskSize = -1, useOnlyThisHex = -1;
+  int layout = 0;   // additive key hexLayout: csim (0) | standard (1)
+  double hexStep = -1, fpmScaleFactor = -1, hexGap = -1, lambdaRef = -1, phaseSign = 1.0, fpmFRatio = -1;
+  double fpmRadius = -1, fpmMaskPhysicalSize = -1, pixelScale = -1, zoomFactor = 1.0;
+  std::vector<double> sags;
+  HexRaster raster;
+  bool rasterized = false;
+  DevBuf d_nsub, d_hexnum, d_sags, d_masks;
+  int set(const std::string& k, const std::string& a) {
+    if (k == "complexHexMaskFPM" || k == "name" || k == "position" || k == "draw" || k == "save") return CSX_OK;
+    if (k == "maskNRings") maskNRings = std::atoi(a.c_str());
+    else if (k == "fpmArraySize") fpmArraySize = std::atoi(a.c_str());
+    else if (k == "nSubPix") nSubPix = std::atoi(a.c_str());
+    else if (k == "hexStep") hexStep = std::atof(a.c_str());
+    else if (k == "fpmScaleFactor") fpmScaleFactor = std::atof(a.c_str());
+    else if (k == "hexGap") hexGap = std::atof(a.c_str());
+    else if (k == "lambdaRef") lambdaRef = std::atof(a.c_str());
+    else if (k == "phaseSign") phaseSign = std::atof(a.c_str());
+    else if (k == "fpmFRatio") fpmFRatio = std::atof(a.c_str());
+    else if (k == "fpmRadius") fpmRadius = std::atof(a.c_str());
+    else if (k == "fpmMaskSize") fpmMaskSize = std::atoi(a.c_str());
+    else if (k == "fpmMaskPhysicalSize") fpmMaskPhysicalSize = std::atof(a.c_str());
+    else if (k == "pixelScale") pixelScale = std::atof(a.c_str());
+    else if (k == "hexLayout") layout = trim(a) == "standard" ? 1 : 0;
+    else if (k == "sagFilename") {
+      int r, c, s;
+      std::vector<double> m;
+      CSX_TRY(fits_matrix(trim(a), m, r, c, s));
+      sags = m;
+    } else if (k == "maskFilenameRoot") { /* the rasterisation is recomputed, not cached on disk */ }
+    else if (k == "setSag") {
+      int i = 0; float v = 0;
+      std::sscanf(a.c_str(), "%d, %f", &i, &v);
+      if (i < 0) return fail(CSX_ERR_ARG, "setSag: negative index");
+      if ((size_t)i >= sags.size()) sags.resize(i + 1, 0.0);
+      sags[i] = v;
+    } else if (k == "useOnlyThisHex") useOnlyThisHex = std::atoi(a.c_str());
+    else return unknown_key("complexHexMaskFPM", k);
+    return CSX_OK;
+  }
+  // init_mask (:193-234): pixel scale, then the rasterisation.
+  int init(const Efield& ef) {
+    if (fpmMaskSize <= 0) return fail(CSX_ERR_ARG, "complexHexMaskFPM: fpmMaskSize unset");
+    if (pixelScale == -1) {
+      if (fpmMaskPhysicalSize > -1) pixelScale = fpmMaskPhysicalSize / fpmMaskSize;
+      else if (zoomFactor > -1 && lambdaRef > -1 && fpmFRatio > -1)
+        pixelScale = 2 * (ef.beamRadius / (ef.pixelScale * fpmMaskSize * zoomFactor)) * lambdaRef * fpmFRatio;
+    }
+    if (pixelScale <= 0) return fail(CSX_ERR_ARG, "complexHexMaskFPM: pixel scale undetermined");
+    CSX_TRY(hex_rasterize(layout, maskNRings, hexStep > 0 ? hexStep : 1.0, fpmArraySize, fpmScaleFactor, hexGap,
+                          fpmMaskSize, nSubPix, pixelScale, fpmRadius, raster));
+    if (sags.size() < (size_t)raster.nhex) sags.resize(raster.nhex, 0.0);
+    CSX_TRY(d_sags.reserve(sizeof(double) * sags.size()));
+    CSX_HIP(hipMemcpy(d_sags.p, sags.data(), sizeof(double) * sags.size(), hipMemcpyHostToDevice));
+    CSX_TRY(d_nsub.reserve(raster.nsub.size() * 4));
+    CSX_TRY(d_hexnum.reserve(raster.hexnum.size() * 4));
+    CSX_HIP(hipMemcpy(d_nsub.p, raster.nsub.data(), raster.nsub.size() * 4, hipMemcpyHostToDevice));
+    CSX_HIP(hipMemcpy(d_hexnum.p, raster.hexnum.data(), raster.hexnum.size() * 4, hipMemcpyHostToDevice));
+    rasterized = true;
+    return CSX_OK;
+  }
+};
+
+// ---- doubleFraunhoferFPM / doubleFraunhoferInterpFPM
+// (src/elements/doubleFraunhoferFPM.cpp:23-110, doubleFraunhoferInterpFPM.cpp:24-125):
+// MFT to the FPM plane, centred mask multiply (fused into the MFT epilogue),
+// MFT back to the pupil grid.  Additive keys fpmSamplesPerflD / fpmFOVflD /
+// referenceLambda place the FPM on a symmetric grid in lambda_ref/D units with
+// lambda*f = lambda 2R F# (as fraunhoferFocal), and occulterRadiusFld builds an
+// opaque disk mask analytically.
+struct DoubleFraunhoferElem : Element {
+  bool interp = false;   // doubleFraunhoferInterpFPM (hex mask)
+  double zoomFactor = 1.0, padFactor = 1.0;
+  int maskIndex = -1;
+  // explicit geometry (additive)
+  double fpmSamplesPerflD = 0, fpmFOVflD = 0, referenceLambda = 0, occulterRadiusFld = -1;
+  // complex mask cube (maskFilename amp,ph or set_complex_array "mask")
+  std::vector<double2> cube;
+  int mrows = 0, mcols = 0, mslices = 0;
+  DevBuf d_mask;
+  HexFpm hex;
+  bool has_hex = false;
+
+  int set(Chain&, const std::string& k, const std::string& a) override {
+    if (set_common(k, a) || k == "doubleFraunhoferFPM" || k == "doubleFraunhoferInterpFPM") return CSX_OK;
+    if (k == "zoomFactor") zoomFactor = std::atof(a.c_str());
+    else if (k == "padFactor") padFactor = std::atof(a.c_str());
+    else if (k == "maskIndex") maskIndex = std::atoi(a.c_str());
+    else if (k == "fpmSamplesPerflD") fpmSamplesPerflD = std::atof(a.c_str());
+    else if (k == "fpmFOVflD") fpmFOVflD = std::atof(a.c_str());
+    else if (k == "referenceLambda") referenceLambda = std::atof(a.c_str());
+    else if (k == "occulterRadiusFld") occulterRadiusFld = std::atof(a.c_str());
+    else if (k == "maskFilename") {
+      size_t cpos = a.find(',');
+      if (cpos == std::string::npos) return fail(CSX_ERR_PARSE, "maskFilename needs amp,ph");
+      std::vector<double> am, ph;
+      int r, c, s, r2, c2, s2;
+      CSX_TRY(fits_matrix(trim(a.substr(0, cpos)), am, r, c, s));
+      CSX_TRY(fits_matrix(trim(a.substr(cpos + 1)), ph, r2, c2, s2));
+      if (r != r2 || c != c2 || s != s2) return fail(CSX_ERR_ARG, "maskFilename amp/ph sizes differ");
+      mrows = r; mcols = c; mslices = s;
+      cube.resize(am.size());
+      for (size_t i = 0; i < am.size(); i++)   // initMask: amp % exp(-i ph)
+        cube[i] = make_double2(am[i] * std::cos(-ph[i]), am[i] * std::sin(-ph[i]));
+    } else return unknown_key(type, k);
+    return CSX_OK;
+  }
+  int set_complex_array(Chain& ch, const std::string& k, const double* d, int r, int c, int slice) override {
+    if (k != "mask") return Element::set_complex_array(ch, k, d, r, c, slice);
+    if (slice < 0) return fail(CSX_ERR_ARG, "negative slice");
+    if (mrows != r || mcols != c) { mrows = r; mcols = c; mslices = 0; cube.clear(); }
+    if (slice >= mslices) { mslices = slice + 1; cube.resize((size_t)mrows * mcols * mslices); }
+    std::memcpy(&cube[(size_t)slice * mrows * mcols], d, sizeof(double2) * mrows * mcols);
+    ch.invalidate();
+    return CSX_OK;
+  }
+  int set_opt(Chain& ch, const std::string& n, const std::vector<double>& v) override {
+    if (has_hex && n == "sags") {
+      // new sags are uploaded in place: the compiled stages read them from the device
+      hex.sags = v;
+      if (!disableForCalibration) ch.invalidate_calibration();
+      if (hex.rasterized && (int)v.size() == hex.raster.nhex && hex.d_sags.bytes >= sizeof(double) * v.size()) {
+        CSX_HIP(hipMemcpy(hex.d_sags.p, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice));
+        return CSX_OK;
+      }
+      ch.invalidate();
+      return CSX_OK;
+    }
+    if (has_hex && n == "useOnlyThisHex" && !v.empty()) { hex.useOnlyThisHex = (int)v[0]; ch.invalidate(); return CSX_OK; }
+    return Element::set_opt(ch, n, v);
+  }
+  int get_opt(Chain& ch, const std::string& n, std::vector<double>& v) override {
+    if (has_hex && n == "sags") { v = hex.sags; return CSX_OK; }
+    if (has_hex && n == "useOnlyThisHex") { v.assign(1, hex.useOnlyThisHex); return CSX_OK; }
+    return Element::get_opt(ch, n, v);
+  }
+  bool explicit_geom() const { return fpmSamplesPerflD > 0 && fpmFOVflD > 0; }
+  // FPM-plane geometry and lambda*f per wavelength
+  int fpm_geom(Chain& ch, const Geom& E, int l, Geom& g, double& lf) const {
+    const Efield& ef = ch.efield();
+    double lam = ef.lambdas[l];
+    if (explicit_geom()) {
+      double refl = referenceLambda > 0 ? referenceLambda : ef.lambdas[0];
+      double pix = ch.telescope().loD(refl) / fpmSamplesPerflD;
+      int N = (int)std::ceil(fpmFOVflD * fpmSamplesPerflD);
+      g = geom_symmetric(N, N, pix, pix);
+      lf = lam * 2 * ef.beamRadius * ch.telescope().F;
+      return CSX_OK;
+    }
+    int nr = (int)(padFactor * E.ny), nc = (int)(padFactor * E.nx);
+    double dxp;
+    if (interp) {
+      dxp = hex.pixelScale;   // init_geom: zoomGeom pixel = hex fpmScale
+    } else {
+      dxp = 2 * (ef.beamRadius / (ef.pixelScale * E.ny * zoomFactor)) * lam * 80;   // :51
+    }
+    g = geom_offset(nc, nr, dxp, dxp);
+    lf = lam * 3.52;   // :45
+    return CSX_OK;
+  }
+  int init(Chain& ch) override {
+    if (has_hex) { hex.zoomFactor = zoomFactor; CSX_TRY(hex.init(ch.efield())); }
+    return CSX_OK;
+  }
+  int compile(Chain& ch, Compile& c, int) override {
+    if (skipped(c)) return CSX_OK;
+    const int nl = ch.nlam(), nf = ch.nfield_for(c), ncand = ch.ncand_for(c);
+    const Geom E = c.geom;
+    std::vector<Geom> gf(nl);
+    std::vector<double> lf(nl), lfo(nl);
+    for (int l = 0; l < nl; l++) { CSX_TRY(fpm_geom(ch, E, l, gf[l], lf[l])); lfo[l] = -lf[l]; }
+    // FPM masks: per field pointer into d_mask
+    std::vector<MftField> mk(nf);
+    Chain* chp = &ch;
+    std::function<int(hipStream_t)> mask_stage;
+    if (has_hex) {
+      const int S = hex.raster.S;
+      size_t plane = (size_t)S * S;
+      CSX_TRY(hex.d_masks.reserve(sizeof(double2) * plane * ncand * nl));
+      const double* sags_dev = nullptr;
+      int nhex = hex.raster.nhex, ncd = 1;
+      if (!c.calibrating && chp->candidate_sags(name, &sags_dev, &nhex, &ncd) == CSX_OK && sags_dev) {
+        // candidate batch: sags already on the device
+      } else {
+        if (hex.d_sags.bytes < sizeof(double) * hex.sags.size()) {
+          CSX_TRY(hex.d_sags.reserve(sizeof(double) * hex.sags.size()));
+        }
+        CSX_HIP(hipMemcpy(hex.d_sags.p, hex.sags.data(), sizeof(double) * hex.sags.size(), hipMemcpyHostToDevice));
+        sags_dev = hex.d_sags.as<double>();
+        nhex = (int)hex.sags.size();
+        ncd = 1;
+      }
+      // lambdas of the wavelength axis (device)
+      DevBuf& lamb = ch.scratch("hex_lam_" + name);
+      CSX_TRY(lamb.reserve(sizeof(double) * nl));
+      CSX_HIP(hipMemcpy(lamb.p, ch.efield().lambdas.data(), sizeof(double) * nl, hipMemcpyHostToDevice));
+      const int32_t* dn = hex.d_nsub.as<int32_t>();
+      const int32_t* dh = hex.d_hexnum.as<int32_t>();
+      double2* dm = hex.d_masks.as<double2>();
+      const double* dl = lamb.as<double>();
+      int nsp = hex.raster.nsubpix, use = hex.useOnlyThisHex;
+      double ps = hex.phaseSign;
+      int ncm = std::min(ncd, ncand);
+      mask_stage = [=](hipStream_t s) {
+        return hex_masks(dn, dh, S, nsp, sags_dev, nhex, ncm, dl, nl, ps, use, dm, s);
+      };
+      for (int f = 0; f < nf; f++) {
+        int l = f % nl, cand = f / (nf / ncand);
+        if (cand >= ncm) cand = ncm - 1;
+        MftField& m = mk[f];
+        m.mask_kind = MASK_COMPLEX;
+        m.mask = dm + ((size_t)cand * nl + l) * plane;
+        // the hex mask is fliplr'd? no: applied as computed (doubleFraunhoferInterpFPM)
+        m.mrows = S; m.mcols = S;
+        m.row0 = gf[l].ny / 2 - S / 2; m.col0 = gf[l].nx / 2 - S / 2;
+      }
+    } else if (!cube.empty() || occulterRadiusFld >= 0) {
+      // one mask for every slice (maskIndex latches to the first slice, :86-87)
+      std::vector<double2> m;
+      int r = mrows, cc = mcols;
+      if (occulterRadiusFld >= 0) {
+        // analytic occulting disk on the FPM grid of lambda index 0
+        const Geom& g = gf[0];
+        double loD = ch.telescope().loD(referenceLambda > 0 ? referenceLambda : ch.efield().lambdas[0]);
+        r = g.ny; cc = g.nx;
+        m.assign((size_t)r * cc, make_double2(1, 0));
+        for (int j = 0; j < cc; j++)
+          for (int i = 0; i < r; i++) {
+            double x = g.x0 + j * g.dx, y = g.y0 + i * g.dy;
+            if (std::sqrt(x * x + y * y) / loD < occulterRadiusFld) m[(size_t)j * r + i] = make_double2(0, 0);
+          }
+      } else {
+        int mi = maskIndex < 0 ? 0 : maskIndex;
+        if (mi >= mslices) return fail(CSX_ERR_ARG, "doubleFraunhoferFPM: maskIndex out of range");
+        m.assign(cube.begin() + (size_t)mi * r * cc, cube.begin() + (size_t)(mi + 1) * r * cc);
+      }
+      CSX_TRY(d_mask.reserve(sizeof(double2) * m.size()));
+      CSX_HIP(hipMemcpy(d_mask.p, m.data(), sizeof(double2) * m.size(), hipMemcpyHostToDevice));
+      for (int f = 0; f < nf; f++) {
+        int l = f % nl;
+        MftField& mm = mk[f];
+        mm.mask_kind = MASK_COMPLEX;
+        mm.mask = d_mask.p;
+        mm.mrows = r; mm.mcols = cc;
+        mm.row0 = gf[l].ny / 2 - r / 2; mm.col0 = gf[l].nx / 2 - cc / 2;
+      }
+    }
+    if (mask_stage) { c.stages.push_back(mask_stage); c.stage_names.push_back("hex masks " + name); }
+    // pupil -> FPM (mask in the epilogue) -> pupil grid of the input
+    CSX_TRY(compile_mft(ch, c, gf, lf, type + " in " + name, &mk));
+    std::vector<Geom> back(nl, E);
+    return compile_mft(ch, c, back, lfo, type + " out " + name);
+  }
+};
+
+// ---- element factory (src/coronagraph/coronagraph.cpp:67-155)
+int Chain::make_element(const Block& b, std::unique_ptr<Element>& out) {
+  const std::string t = b.cmds.empty() ? "" : b.cmds[0].key;
+  std::unique_ptr<Element> e;
+  if (t == "mask") e.reset(new MaskElem());
+  else if (t == "mirror") e.reset(new MirrorElem());
+  else if (t == "complexMask") e.reset(new ComplexMaskElem());
+  else if (t == "fresnelPropagator") e.reset(new FresnelElem());
+  else if (t == "downsample") e.reset(new DownsampleElem());
+  else if (t == "fraunhoferFocal") e.reset(new FraunhoferFocalElem());
+  else if (t == "zoomFftPropagator") e.reset(new ZoomFftPropElem());
+  else if (t == "doubleFraunhoferFPM" || t == "doubleFraunhoferInterpFPM") {
+    auto* d = new DoubleFraunhoferElem();
+    d->interp = t == "doubleFraunhoferInterpFPM";
+    e.reset(d);
+  } else {
+    std::fprintf(stderr, "!!! coronagraph init unknown object name: %s\n", t.c_str());
+    return fail(CSX_ERR_UNSUPPORTED, "element type not supported: " + t);
+  }
+  e->type = t;
+  e->name = t;
+  if (auto* d = dynamic_cast<DoubleFraunhoferElem*>(e.get())) {
+    // own keys and a complexHexMaskFPM sub-block (doubleFraunhoferInterpFPM::init)
+    std::vector<Block> sub = find_blocks(b);
+    for (const Block& sb : sub) {
+      if (sb.cmds.empty()) continue;
+      if (sb.cmds[0].key == "complexHexMaskFPM") {
+        d->has_hex = true;
+        for (const Cmd& c : sb.cmds) CSX_TRY(d->hex.set(c.key, c.arg));
+      } else {
+        for (const Cmd& c : sb.cmds) CSX_TRY(e->set(*this, c.key, c.arg));
+      }
+    }
+  } else {
+    for (const Cmd& c : b.cmds) CSX_TRY(e->set(*this, c.key, c.arg));
+  }
+  out = std::move(e);
+  return CSX_OK;
+}
+
+// ================================================================== chain ==
+Element* Chain::elem_by_name(const std::string& n) {
+  for (auto& e : elems_)
+    if (e->name == n) return e.get();
+  return nullptr;
+}
+
+int Chain::parse_top(const std::vector<Block>& blocks) {
+  for (const Block& b : blocks) {
+    if (b.cmds.empty()) continue;
+    const std::string& t = b.cmds[0].key;
+    if (t == "telescope") {   // telescope.cpp:21-46
+      for (const Cmd& c : b.cmds) {
+        std::string a = trim(c.arg);
+        if (c.key == "magnification") tel_.mag = std::atof(a.c_str());
+        else if (c.key == "primaryDiameter") tel_.D = std::atof(a.c_str());
+        else if (c.key == "primaryFRatio" || c.key == "primaryfRatio") tel_.F = std::atof(a.c_str());
+      }
+      if (tel_.D == 0 || tel_.F == 0) return fail(CSX_ERR_PARSE, "telescope: primaryDiameter and primaryFRatio required");
+      tel_.fLength = tel_.F * tel_.D;
+    } else if (t == "efield") {   // efield.cpp:246-349
+      int nl_size = 0;
+      for (const Cmd& c : b.cmds) {
+        const std::string& k = c.key;
+        const std::string a = trim(c.arg);
+        if (k == "efield" || k == "name" || k == "outputDirectory") continue;
+        if (k == "setSize") {
+          int r = 0, cc = 0, n = 0;
+          std::sscanf(a.c_str(), "%d, %d, %d", &r, &cc, &n);
+          ef_.rows = r; ef_.cols = cc; nl_size = n;
+        } else if (k == "constantValue") { ef_.constant = true; ef_.constantValue = std::atof(a.c_str()); }
+        else if (k == "pointSource") {
+          double fl = 0, tx = 0, ty = 0;
+          std::sscanf(a.c_str(), "%lf, %lf, %lf", &fl, &tx, &ty);
+          ef_.sources.push_back({std::sqrt(fl), tx * tel_.mag, ty * tel_.mag});
+        } else if (k == "referenceWavelength") ef_.referenceWavelength = std::atof(a.c_str());
+        else if (k == "wavelength") ef_.lambdas.assign(1, std::atof(a.c_str()));
+        else if (k == "wavelengthRange") {
+          double s0 = 0, s1 = 0, d = 0;
+          std::sscanf(a.c_str(), "%lf, %lf, %lf", &s0, &s1, &d);
+          int n = (int)std::round((s1 - s0) / d) + 1;
+          ef_.lambdas.clear();
+          for (int i = 0; i < n; i++) ef_.lambdas.push_back(s0 + i * d);
+        } else if (k == "wavelengthBandwidth") {
+          double cw = 0, fr = 0; int n = 0;
+          std::sscanf(a.c_str(), "%lf, %lf, %d", &cw, &fr, &n);
+          double s0 = cw - cw * fr / 2, s1 = cw + cw * fr / 2, d = (s1 - s0) / ((double)n - 1);
+          ef_.lambdas.clear();
+          for (int i = 0; i < n; i++) ef_.lambdas.push_back(s0 + i * d);
+        } else if (k == "wavelengthList") ef_.lambdas = parse_list(a);
+        else if (k == "beamRadiusPhysical") ef_.beamRadius = std::atof(a.c_str());
+        else if (k == "pixelScale") ef_.pixelScale = std::atof(a.c_str());
+        else if (k == "arraySizePixels") ef_.arraySizePixels = std::atoi(a.c_str());
+        else if (k == "filename") {
+          size_t cpos = a.find(',');
+          if (cpos == std::string::npos) return fail(CSX_ERR_PARSE, "efield filename needs re,im");
+          std::vector<double> re, im;
+          int r, cc, s, r2, c2, s2;
+          CSX_TRY(fits_matrix(trim(a.substr(0, cpos)), re, r, cc, s));
+          CSX_TRY(fits_matrix(trim(a.substr(cpos + 1)), im, r2, c2, s2));
+          ef_.rows = r; ef_.cols = cc;
+          for (int sl = 0; sl < s; sl++) {
+            std::vector<double2> f((size_t)r * cc);
+            for (size_t i = 0; i < f.size(); i++)
+              f[i] = make_double2(re[(size_t)sl * r * cc + i], im[(size_t)sl * r * cc + i]);
+            ef_.fields[sl] = f;
+          }
+        } else {
+          std::fprintf(stderr, "!!!!! efield set: unknown fieldName: %s\n", k.c_str());
+        }
+      }
+      if (ef_.lambdas.empty()) return fail(CSX_ERR_PARSE, "efield: no wavelength");
+      if (nl_size && (int)ef_.lambdas.size() != nl_size && ef_.lambdas.size() == 1)
+        ef_.lambdas.resize(nl_size, ef_.lambdas[0]);
+      if (ef_.beamRadius == 0 || ef_.pixelScale == 0 || ef_.arraySizePixels == 0)
+        return fail(CSX_ERR_PARSE, "efield: beamRadiusPhysical, pixelScale and arraySizePixels required");
+      ef_.beamSizePixels = std::ceil(2 * ef_.beamRadius / ef_.pixelScale);          // :424
+      ef_.arrayRadius = ef_.beamRadius * ef_.arraySizePixels / ef_.beamSizePixels;  // :427
+      if (ef_.rows <= 0 || ef_.cols <= 0) return fail(CSX_ERR_PARSE, "efield: setSize required");
+    } else if (t == "coronagraph") {
+      std::vector<Block> sub = find_blocks(b);
+      for (const Block& sb : sub) {
+        if (sb.cmds.empty() || sb.cmds[0].key == "coronagraph") continue;
+        std::unique_ptr<Element> e;
+        CSX_TRY(make_element(sb, e));
+        elems_.push_back(std::move(e));
+      }
+    } else if (t == "regionContrast") {   // regionContrast.cpp:36-87
+      region_.present = true;
+      for (const Cmd& c : b.cmds) {
+        if (c.key == "referenceLambda") region_.refLambda = std::atof(c.arg.c_str());
+        else if (c.key == "xlimit") { auto v = parse_float_pair(c.arg); region_.x1 = v[0]; region_.x2 = v[1]; }
+        else if (c.key == "anglesInDegrees") { auto v = parse_float_pair(c.arg); region_.a1 = v[0]; region_.a2 = v[1]; }
+        else if (c.key == "radii") { auto v = parse_float_pair(c.arg); region_.r1 = v[0]; region_.r2 = v[1]; }
+      }
+    } else {
+      // other top-level blocks (optimisers, response builders, contrast curves) are
+      // driven from the host API, not from the propagation chain
+    }
+  }
+  return CSX_OK;
+}
+
+int Chain::load_script(const std::string& text) {
+  Block all = parse_script_text(text);
+  std::vector<Block> blocks = find_blocks(all);
+  // the telescope must be known before efield keys use the magnification
+  std::vector<Block> ordered;
+  for (const Block& b : blocks) if (!b.cmds.empty() && b.cmds[0].key == "telescope") ordered.push_back(b);
+  for (const Block& b : blocks) if (b.cmds.empty() || b.cmds[0].key != "telescope") ordered.push_back(b);
+  CSX_TRY(parse_top(ordered));
+  if (tel_.D == 0) return fail(CSX_ERR_PARSE, "script has no telescope block");
+  if (ef_.rows == 0) return fail(CSX_ERR_PARSE, "script has no efield block");
+  for (auto& e : elems_) CSX_TRY(e->init(*this));
+  return CSX_OK;
+}
+
+int Chain::set_array(const std::string& elem, const std::string& key, const double* d, int rows, int cols) {
+  Element* e = elem_by_name(elem);
+  if (!e) return fail(CSX_ERR_ARG, "no element named " + elem);
+  invalidate();
+  return e->set_array(*this, key, d, rows, cols);
+}
+
+int Chain::set_complex_array(const std::string& elem, const std::string& key, const double* d, int rows, int cols,
+                             int slice) {
+  if (elem == "efield" && key == "field") {
+    if (rows != ef_.rows || cols != ef_.cols) return fail(CSX_ERR_ARG, "field size does not match setSize");
+    if (slice < 0) return fail(CSX_ERR_ARG, "negative slice");
+    std::vector<double2> f((size_t)rows * cols);
+    std::memcpy(f.data(), d, sizeof(double2) * f.size());
+    ef_.fields[slice] = std::move(f);
+    invalidate();
+    return CSX_OK;
+  }
+  Element* e = elem_by_name(elem);
+  if (!e) return fail(CSX_ERR_ARG, "no element named " + elem);
+  invalidate();
+  return e->set_complex_array(*this, key, d, rows, cols, slice);
+}
+
+int Chain::set_opt(const std::string& elem, const std::string& n, const std::vector<double>& v) {
+  Element* e = elem_by_name(elem);
+  if (!e) return fail(CSX_ERR_ARG, "no element named " + elem);
+  return e->set_opt(*this, n, v);
+}
+int Chain::get_opt(const std::string& elem, const std::string& n, std::vector<double>& v) {
+  Element* e = elem_by_name(elem);
+  if (!e) return fail(CSX_ERR_ARG, "no element named " + elem);
+  return e->get_opt(*this, n, v);
+}
+
+int Chain::set_candidates(const std::string& elem, const double* sags, int nhex, int ncand) {
+  Element* e = elem_by_name(elem);
+  auto* d = dynamic_cast<DoubleFraunhoferElem*>(e);
+  if (!d || !d->has_hex) return fail(CSX_ERR_ARG, "no hex-FPM element named " + elem);
+  if (ncand < 0 || (ncand > 0 && (!sags || nhex <= 0))) return fail(CSX_ERR_ARG, "bad candidate batch");
+  if (ncand > 0 && nhex != d->hex.raster.nhex)
+    return fail(CSX_ERR_ARG, "candidate vectors must have one sag per hex (" + std::to_string(d->hex.raster.nhex) + ")");
+  bool shape_change = ncand != ncand_;
+  if (ncand > 0) {
+    CSX_TRY(cand_sags_.reserve(sizeof(double) * (size_t)nhex * ncand));
+    CSX_HIP(hipMemcpyAsync(cand_sags_.p, sags, sizeof(double) * (size_t)nhex * ncand, hipMemcpyHostToDevice, stream()));
+    CSX_HIP(hipStreamSynchronize(stream()));
+  }
+  ncand_ = ncand;
+  cand_elem_ = elem;
+  cand_nhex_ = nhex;
+  if (shape_change) invalidate();
+  return CSX_OK;
+}
+
+int Chain::candidate_sags(const std::string& elem, const double** dev, int* nhex, int* ncand) {
+  if (ncand_ <= 0 || elem != cand_elem_) { *dev = nullptr; return CSX_ERR_STATE; }
+  *dev = cand_sags_.as<double>();
+  *nhex = cand_nhex_;
+  *ncand = ncand_;
+  return CSX_OK;
+}
+
+int Chain::compile_one(bool calibrating, Compile& c) {
+  c.calibrating = calibrating;
+  c.stages.clear();
+  c.stage_names.clear();
+  c.plans.clear();
+  c.geom = geom_symmetric(ef_.cols, ef_.rows, ef_.pixelScale, ef_.pixelScale);
+  c.buf = 0;
+  // initial fields (efield::init_point_sources / init / filename slices)
+  const int nf = nfield_for(c), nl = nlam(), ns = nsrc();
+  const long npix = c.geom.npix();
+  double2* E0 = field_buf(0);
+  DevBuf& prm = calibrating ? src_prm_cal_ : src_prm_;
+  std::vector<double> p((size_t)3 * nf);
+  for (int f = 0; f < nf; f++) {
+    int s = (f / nl) % ns;
+    if (!ef_.sources.empty()) { p[3 * f] = ef_.sources[s].flux; p[3 * f + 1] = ef_.sources[s].tipX; p[3 * f + 2] = ef_.sources[s].tiltY; }
+    else { p[3 * f] = ef_.constant ? ef_.constantValue : 1.0; p[3 * f + 1] = 0; p[3 * f + 2] = 0; }
+  }
+  CSX_TRY(prm.reserve(sizeof(double) * p.size()));
+  CSX_HIP(hipMemcpy(prm.p, p.data(), sizeof(double) * p.size(), hipMemcpyHostToDevice));
+  if (!ef_.fields.empty()) {
+    // fields bound per lambda slice (one source), replicated over candidates
+    DevBuf& fb = scratch(calibrating ? "init_fields_cal" : "init_fields");
+    CSX_TRY(fb.reserve(sizeof(double2) * npix * nl));
+    for (int l = 0; l < nl; l++) {
+      auto it = ef_.fields.find(l);
+      if (it == ef_.fields.end()) it = ef_.fields.begin();
+      CSX_HIP(hipMemcpy(fb.as<double2>() + (size_t)l * npix, it->second.data(), sizeof(double2) * npix,
+                        hipMemcpyHostToDevice));
+    }
+    const double2* src = fb.as<double2>();
+    c.stages.push_back([=](hipStream_t s) {
+      for (int f0 = 0; f0 < nf; f0 += nl)
+        CSX_HIP(hipMemcpyAsync(E0 + (size_t)f0 * npix, src, sizeof(double2) * npix * nl, hipMemcpyDeviceToDevice, s));
+      return CSX_OK;
+    });
+  } else {
+    const double* dp = prm.as<double>();
+    int rows = ef_.rows, cols = ef_.cols;
+    double ps = ef_.pixelScale;
+    c.stages.push_back([=](hipStream_t s) { return k_point_sources(E0, rows, cols, nf, dp, ps, s); });
+  }
+  c.stage_names.push_back("efield init");
+  for (int i = 0; i < (int)elems_.size(); i++) {
+    Element* e = elems_[i].get();
+    if (e->skipped(c)) continue;
+    CSX_TRY(e->compile(*this, c, i));
+  }
+  return CSX_OK;
+}
+
+int Chain::compile_all() {
+  if (compiled_) return CSX_OK;
+  CSX_HIP(hipSetDevice(ctx_->device));
+  // pass 1: largest field over both chains
+  long maxe = 0;
+  for (int cal = 0; cal < 2; cal++) {
+    Compile dummy;
+    dummy.calibrating = cal;
+    Geom g = geom_symmetric(ef_.cols, ef_.rows, ef_.pixelScale, ef_.pixelScale);
+    long nf = nfield_for(dummy);
+    maxe = std::max(maxe, g.npix() * nf);
+    for (auto& e : elems_) {
+      if (e->skipped(dummy)) continue;
+      Geom o;
+      CSX_TRY(e->geometry(*this, g, o));
+      if (auto* d = dynamic_cast<DoubleFraunhoferElem*>(e.get())) {
+        for (int l = 0; l < nlam(); l++) {
+          Geom gf; double lf;
+          CSX_TRY(d->fpm_geom(*this, g, l, gf, lf));
+          maxe = std::max(maxe, gf.npix() * nf);
+        }
+      }
+      if (auto* fr = dynamic_cast<FresnelElem*>(e.get()))
+        if (fr->padFactor != 1) maxe = std::max(maxe, 4 * g.npix() * nf);
+      g = o;
+      maxe = std::max(maxe, g.npix() * nf);
+    }
+  }
+  max_field_elems_ = maxe;
+  CSX_TRY(bufs_[0].reserve(sizeof(double2) * maxe));
+  CSX_TRY(bufs_[1].reserve(sizeof(double2) * maxe));
+  if (prec_ != CSX_FP64) CSX_TRY(c64_.reserve(sizeof(float2) * maxe));
+  // per-field wavelengths (candidate-major, then source, then lambda)
+  for (int cal = 0; cal < 2; cal++) {
+    Compile dummy;
+    dummy.calibrating = cal;
+    int nf = nfield_for(dummy);
+    std::vector<double> lam(nf);
+    for (int f = 0; f < nf; f++) lam[f] = ef_.lambdas[f % nlam()];
+    DevBuf& b = cal ? scratch("lam_cal") : lam_dev_;
+    CSX_TRY(b.reserve(sizeof(double) * nf));
+    CSX_HIP(hipMemcpy(b.p, lam.data(), sizeof(double) * nf, hipMemcpyHostToDevice));
+  }
+  CSX_TRY(compile_one(false, plan_full_));
+  out_geom_ = plan_full_.geom; out_buf_ = plan_full_.buf;
+  CSX_TRY(compile_one(true, plan_cal_));
+  out_geom_cal_ = plan_cal_.geom; out_buf_cal_ = plan_cal_.buf;
+  compiled_ = true;
+  have_full_ = have_cal_ = false;
+  calib_valid_ = false;
+  return CSX_OK;
+}
+
+int Chain::propagate(bool calibrating) {
+  CSX_TRY(compile_all());
+  CSX_HIP(hipSetDevice(ctx_->device));
+  Compile& c = calibrating ? plan_cal_ : plan_full_;
+  for (size_t i = 0; i < c.stages.size(); i++) {
+    int rc = c.stages[i](stream());
+    if (rc != CSX_OK) return fail(rc, "stage '" + c.stage_names[i] + "': " + csx_last_error());
+  }
+  CSX_HIP(hipStreamSynchronize(stream()));
+  if (timer_.enabled()) CSX_TRY(timer_.collect());
+  if (calibrating) { have_cal_ = true; have_full_ = false; } else { have_full_ = true; have_cal_ = false; }
+  return CSX_OK;
+}
+
+int Chain::get_field(int cand, int src, int lam, double* out) {
+  bool cal = have_cal_;
+  if (!have_full_ && !have_cal_) return fail(CSX_ERR_STATE, "no propagation has run");
+  int ncd = cal ? 1 : ncand();
+  if (cand < 0 || cand >= ncd || src < 0 || src >= nsrc() || lam < 0 || lam >= nlam())
+    return fail(CSX_ERR_ARG, "field index out of range");
+  const Geom& g = cal ? out_geom_cal_ : out_geom_;
+  int b = cal ? out_buf_cal_ : out_buf_;
+  size_t f = ((size_t)cand * nsrc() + src) * nlam() + lam;
+  CSX_HIP(hipMemcpy(out, field_buf(b) + f * g.npix(), sizeof(double2) * g.npix(), hipMemcpyDeviceToHost));
+  return CSX_OK;
+}
+
+int Chain::get_intensity(int cand, double* out) {
+  bool cal = have_cal_;
+  if (!have_full_ && !have_cal_) return fail(CSX_ERR_STATE, "no propagation has run");
+  int ncd = cal ? 1 : ncand();
+  if (cand < 0 || cand >= ncd) return fail(CSX_ERR_ARG, "candidate out of range");
+  const Geom& g = cal ? out_geom_cal_ : out_geom_;
+  int b = cal ? out_buf_cal_ : out_buf_;
+  DevBuf& I = scratch("inten_tmp");
+  CSX_TRY(I.reserve(sizeof(double) * g.npix()));
+  int per = nsrc() * nlam();
+  CSX_TRY(intensity_groups(field_buf(b) + (size_t)cand * per * g.npix(), g.npix(), 1, per, I.as<double>(), stream()));
+  CSX_HIP(hipMemcpyAsync(out, I.p, sizeof(double) * g.npix(), hipMemcpyDeviceToHost, stream()));
+  CSX_HIP(hipStreamSynchronize(stream()));
+  return CSX_OK;
+}
+
+int Chain::contrast(const csx_region* r, double* out, csx_report* rep) {
+  Region R = region_;
+  if (r) {
+    R.r1 = r->radius1; R.r2 = r->radius2; R.a1 = r->angle1; R.a2 = r->angle2;
+    R.x1 = r->xlim1; R.x2 = r->xlim2; R.refLambda = r->reference_lambda > 0 ? r->reference_lambda : -1.0;
+  }
+  CSX_TRY(compile_all());
+  int per = nsrc() * nlam();
+  auto t0 = std::chrono::steady_clock::now();
+  if (!calib_valid_) {   // calibration (unocculted) run, cached until the chain changes
+    CSX_TRY(propagate(true));
+    const Geom gc = out_geom_cal_;
+    CSX_TRY(inten_cal_.reserve(sizeof(double) * gc.npix()));
+    CSX_TRY(peak_.reserve(sizeof(double)));
+    CSX_TRY(intensity_groups(field_buf(out_buf_cal_), gc.npix(), 1, per, inten_cal_.as<double>(), stream()));
+    CSX_TRY(max_groups(inten_cal_.as<double>(), gc.npix(), 1, peak_.as<double>(), stream()));
+    CSX_HIP(hipMemcpyAsync(&calib_peak_, peak_.p, sizeof(double), hipMemcpyDeviceToHost, stream()));
+    CSX_HIP(hipStreamSynchronize(stream()));
+    calib_valid_ = true;
+  }
+  auto t1 = std::chrono::steady_clock::now();
+  auto t2 = t1;
+  CSX_TRY(propagate(false));
+  auto t3 = std::chrono::steady_clock::now();
+  const Geom g = out_geom_;
+  int nc = ncand();
+  CSX_TRY(inten_.reserve(sizeof(double) * g.npix() * nc));
+  CSX_TRY(intensity_groups(field_buf(out_buf_), g.npix(), nc, per, inten_.as<double>(), stream()));
+  // region pixels (regionContrast::get_region_pixels, :89-101), column-major order
+  double refl = R.refLambda;
+  if (refl == -1.0) {
+    double s = 0;
+    for (double l : ef_.lambdas) s += l;
+    refl = s / nlam();
+  }
+  double loD = tel_.loD(refl);
+  std::vector<int32_t> idx;
+  for (int c = 0; c < g.nx; c++)
+    for (int rr = 0; rr < g.ny; rr++) {
+      double XX = g.x0 + c * g.dx, YY = g.y0 + rr * g.dy;
+      double RR = std::sqrt(std::pow(XX, 2) + std::pow(YY, 2)), TT = std::atan2(YY, XX);
+      if ((XX / loD >= R.x1) && (XX / loD <= R.x2) && (RR / loD >= R.r1) && (RR / loD <= R.r2) &&
+          (TT > R.a1 * M_PI / 180) && (TT < R.a2 * M_PI / 180))
+        idx.push_back(c * g.ny + rr);
+    }
+  n_region_ = (int)idx.size();
+  if (idx.empty()) return fail(CSX_ERR_ARG, "contrast region contains no pixel");
+  CSX_TRY(region_idx_.reserve(sizeof(int32_t) * idx.size()));
+  CSX_HIP(hipMemcpyAsync(region_idx_.p, idx.data(), sizeof(int32_t) * idx.size(), hipMemcpyHostToDevice, stream()));
+  CSX_TRY(contrast_.reserve(sizeof(double) * nc));
+  CSX_TRY(region_mean_groups(inten_.as<double>(), g.npix(), nc, region_idx_.as<int32_t>(), (int)idx.size(),
+                             peak_.as<double>(), 0.0, contrast_.as<double>(), stream()));
+  CSX_HIP(hipMemcpyAsync(out, contrast_.p, sizeof(double) * nc, hipMemcpyDeviceToHost, stream()));
+  CSX_HIP(hipStreamSynchronize(stream()));
+  if (rep) {
+    rep->calib_peak = calib_peak_;
+    rep->n_region_pixels = n_region_;
+    rep->seconds_calibration = std::chrono::duration<double>(t1 - t0).count();
+    rep->seconds_propagation = std::chrono::duration<double>(t3 - t2).count();
+  }
+  return CSX_OK;
+}
+
+int Chain::get_normalized_image(int cand, double* out) {
+  if (calib_peak_ == 0) return fail(CSX_ERR_STATE, "run csx_chain_contrast first");
+  if (cand < 0 || cand >= ncand()) return fail(CSX_ERR_ARG, "candidate out of range");
+  const Geom g = out_geom_;
+  CSX_HIP(hipMemcpy(out, inten_.as<double>() + (size_t)cand * g.npix(), sizeof(double) * g.npix(),
+                    hipMemcpyDeviceToHost));
+  for (long i = 0; i < g.npix(); i++) out[i] /= calib_peak_;
+  return CSX_OK;
+}
+
+int Chain::info(csx_chain_info* o) {
+  CSX_TRY(compile_all());
+  o->n_sources = nsrc();
+  o->n_lambdas = nlam();
+  o->n_candidates = ncand();
+  o->in_rows = ef_.rows;
+  o->in_cols = ef_.cols;
+  o->out_rows = out_geom_.ny;
+  o->out_cols = out_geom_.nx;
+  o->out_pixel_size = out_geom_.dx;
+  double refl = region_.refLambda;
+  if (refl == -1.0) { double s = 0; for (double l : ef_.lambdas) s += l; refl = s / nlam(); }
+  o->lambda_over_d = tel_.loD(refl);
+  return CSX_OK;
+}
+
+}  // namespace csx

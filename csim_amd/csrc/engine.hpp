@@ -1,0 +1,173 @@
+// Element chain: csim's input-script element API (src/elements/celem.hpp,
+// src/coronagraph/coronagraph.cpp) compiled into a list of GPU stages that run
+// over every field of a batch (candidate x source x wavelength).
+#pragma once
+#include <functional>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "hostgeom.hpp"
+#include "mft.hpp"
+
+namespace csx {
+
+// ------------------------------------------------------------- script text --
+// initCommandSet / find_command_blocks (src/lib/csim_parser.cpp:60-190).
+struct Cmd { std::string key, arg; };
+struct Block { std::vector<Cmd> cmds; };
+Block parse_script_text(const std::string& text);
+std::vector<Block> find_blocks(const Block& b);
+
+// Minimal FITS reader (BITPIX 8/16/32/-32/-64, NAXIS <= 3): values in FITS order
+// (NAXIS1 fastest).  csim reads matrices as rows = NAXIS2, cols = NAXIS1.
+int read_fits(const std::string& path, std::vector<double>& data, std::vector<long>& dims);
+
+// --------------------------------------------------------------- geometry --
+// arrayGeom with uniform grids: x along columns, y along rows.
+struct Geom {
+  int nx = 0, ny = 0;
+  double x0 = 0, y0 = 0, dx = 0, dy = 0;
+  int rows() const { return ny; }
+  int cols() const { return nx; }
+  long npix() const { return (long)nx * ny; }
+};
+Geom geom_symmetric(int nx, int ny, double dx, double dy, double ox = 0, double oy = 0);
+Geom geom_offset(int nx, int ny, double dx, double dy);
+
+struct Telescope {
+  double D = 0, F = 0, mag = 1, fLength = 0;
+  double loD(double lam) const { return mag * fLength * lam / D; }   // telescope.cpp:58-60
+};
+
+struct Source { double flux, tipX, tiltY; };
+
+struct Efield {
+  int rows = 0, cols = 0;
+  std::vector<double> lambdas;
+  double beamRadius = 0, pixelScale = 0, beamSizePixels = 0, arrayRadius = 0, referenceWavelength = 0;
+  int arraySizePixels = 0;
+  std::vector<Source> sources;
+  bool constant = false;
+  double constantValue = 0;
+  // complex fields bound from memory / FITS, one per (source, lambda) slice
+  std::map<int, std::vector<double2>> fields;
+  int nsources() const { return std::max<int>(1, std::max<int>((int)sources.size(), fields.empty() ? 0 : 1)); }
+};
+
+struct Region {
+  double r1 = 1.0, r2 = 2.0, a1 = 0.0, a2 = 30.0, x1 = -1e16, x2 = 1e16, refLambda = -1.0;
+  bool present = false;
+};
+
+class Chain;
+
+// Compile-time state threaded through the elements.
+struct Compile {
+  bool calibrating = false;
+  Geom geom;            // geometry of the current fields
+  int buf = 0;          // ping-pong buffer holding the fields (complex128)
+  std::vector<std::function<int(hipStream_t)>> stages;
+  std::vector<std::string> stage_names;
+  std::vector<std::unique_ptr<MftPlan>> plans;
+};
+
+// ---------------------------------------------------------------- elements --
+struct Element {
+  std::string type, name;
+  double position = 0;
+  bool disableForCalibration = false;
+  virtual ~Element() = default;
+  // celem::set (src/elements/celem.cpp:97-161) then the element's own keys.
+  bool set_common(const std::string& key, const std::string& arg);
+  virtual int set(Chain& ch, const std::string& key, const std::string& arg) = 0;
+  virtual int init(Chain& ch) { (void)ch; return CSX_OK; }
+  // Output geometry for an input geometry (no GPU work).
+  virtual int geometry(Chain& ch, const Geom& in, Geom& out) { (void)ch; out = in; return CSX_OK; }
+  // Appends the GPU stages of this element.
+  virtual int compile(Chain& ch, Compile& c, int index) = 0;
+  virtual int set_opt(Chain& ch, const std::string& name, const std::vector<double>& v);
+  virtual int get_opt(Chain& ch, const std::string& name, std::vector<double>& v);
+  virtual int set_array(Chain& ch, const std::string& key, const double* d, int rows, int cols);
+  virtual int set_complex_array(Chain& ch, const std::string& key, const double* d, int rows, int cols, int slice);
+  bool skipped(const Compile& c) const { return disableForCalibration && c.calibrating; }
+};
+
+class Chain {
+ public:
+  explicit Chain(csx_ctx* ctx) : ctx_(ctx) {}
+  int load_script(const std::string& text);
+  int set_array(const std::string& elem, const std::string& key, const double* d, int rows, int cols);
+  int set_complex_array(const std::string& elem, const std::string& key, const double* d, int rows, int cols,
+                        int slice);
+  int set_opt(const std::string& elem, const std::string& name, const std::vector<double>& v);
+  int get_opt(const std::string& elem, const std::string& name, std::vector<double>& v);
+  int set_candidates(const std::string& elem, const double* sags, int nhex, int ncand);
+  int set_precision(csx_precision p) { if (p != prec_) { prec_ = p; invalidate(); } return CSX_OK; }
+  int propagate(bool calibrating);
+  int get_field(int cand, int src, int lam, double* out);
+  int get_intensity(int cand, double* out);
+  int contrast(const csx_region* r, double* out, csx_report* rep);
+  int get_normalized_image(int cand, double* out);
+  int info(csx_chain_info* out);
+  int set_profiling(bool on) { timer_.enable(on); return CSX_OK; }
+  std::string profile_json() const { return timer_.json(); }
+  KernelTimer* timer() { return timer_.enabled() ? &timer_ : nullptr; }
+
+  // ---- shared with elements
+  csx_ctx* ctx() const { return ctx_; }
+  hipStream_t stream() const { return ctx_->stream; }
+  const Telescope& telescope() const { return tel_; }
+  const Efield& efield() const { return ef_; }
+  csx_precision precision() const { return prec_; }
+  int ncand() const { return std::max(1, ncand_); }
+  int ncand_for(const Compile& c) const { return c.calibrating ? 1 : ncand(); }
+  int nfield_for(const Compile& c) const { return ncand_for(c) * ef_.nsources() * (int)ef_.lambdas.size(); }
+  int nlam() const { return (int)ef_.lambdas.size(); }
+  int nsrc() const { return ef_.nsources(); }
+  double2* field_buf(int i) const { return bufs_[i].as<double2>(); }
+  void* c64_buf() const { return c64_.p; }
+  const double* lambdas_dev() const { return lam_dev_.as<double>(); }   // per field (candidate-major)
+  Element* elem_by_name(const std::string& n);
+  Element* elem_at(int i) { return (i >= 0 && i < (int)elems_.size()) ? elems_[i].get() : nullptr; }
+  int n_elems() const { return (int)elems_.size(); }
+  void invalidate() { compiled_ = false; calib_valid_ = false; }
+  void invalidate_calibration() { calib_valid_ = false; }
+  DevBuf& scratch(const std::string& key) { return scratch_[key]; }
+  int candidate_sags(const std::string& elem, const double** dev, int* nhex, int* ncand);
+
+ private:
+  int compile_all();
+  int compile_one(bool calibrating, Compile& c);
+  int parse_top(const std::vector<Block>& blocks);
+  int make_element(const Block& b, std::unique_ptr<Element>& out);
+
+  csx_ctx* ctx_;
+  Telescope tel_;
+  Efield ef_;
+  Region region_;
+  std::vector<std::unique_ptr<Element>> elems_;
+  csx_precision prec_ = CSX_FP64;
+  int ncand_ = 0;
+  std::string cand_elem_;
+  DevBuf cand_sags_;
+  int cand_nhex_ = 0;
+
+  bool compiled_ = false;
+  Compile plan_full_, plan_cal_;
+  DevBuf bufs_[2], c64_, lam_dev_, lam_cal_dev_, src_prm_, src_prm_cal_;
+  std::map<std::string, DevBuf> scratch_;
+  long max_field_elems_ = 0;
+  Geom out_geom_, out_geom_cal_;
+  int out_buf_ = 0, out_buf_cal_ = 0;
+  bool have_full_ = false, have_cal_ = false;
+  DevBuf inten_, inten_cal_, peak_, region_idx_, contrast_;
+  double calib_peak_ = 0;
+  bool calib_valid_ = false;
+  int n_region_ = 0;
+  KernelTimer timer_;
+};
+
+}  // namespace csx

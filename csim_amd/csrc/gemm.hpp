@@ -1,0 +1,58 @@
+// Grouped complex GEMM on CDNA4 MFMA: the contraction engine of every matrix
+// Fourier transform (csim zoomFft, src/lib/csim_fft.cpp:670-826, restated as
+// E_out = s * A_y * E_in * B_x^T).
+#pragma once
+#include "common.hpp"
+
+namespace csx {
+
+enum MaskKind : int { MASK_NONE = 0, MASK_COMPLEX = 1, MASK_REAL_1PI = 2, MASK_REAL = 3 };
+enum OutKind : int { OUT_C128 = 0, OUT_C64 = 1 };
+
+// One problem of a grouped launch:  C(m, n) = alpha * sum_k A(m, k) B(n, k),
+// then the epilogue mask.  A and B are K-contiguous (row-major M x K, N x K).
+// Element types are complex128 (FP64 kernel) or complex64 (FP32 / split-BF16).
+struct GemmProb {
+  const void* A;
+  const void* B;
+  void* C;
+  long lda, ldb, ldc;
+  int M, N, K;
+  int transC;          // 0: C[m*ldc + n]; 1: C[n*ldc + m]
+  int out_kind;        // OutKind
+  double alpha_re, alpha_im;
+  int mask_kind;       // MaskKind; applied at output (m, n)
+  const void* mask;    // element (m - mm0, n - mn0) at [(m - mm0) * ldm + (n - mn0)]
+  long ldm;
+  int mm0, mn0, mM, mN;
+  int tiles_m, tiles_n, tile_start;
+};
+
+// Tile of the FP64 / FP32 MFMA kernels (64 x 64 outputs, 4 waves of 32 x 32).
+constexpr int kTileM = 64, kTileN = 64;
+
+// Fills tiles_m/tiles_n/tile_start; returns the total tile count.
+int plan_tiles(GemmProb* probs, int n, int tile_m, int tile_n);
+
+// probs_dev: device copy of the problem array (n entries), total_tiles from plan_tiles.
+int gemm_f64(const GemmProb* probs_dev, int n, int total_tiles, hipStream_t s);
+int gemm_f32(const GemmProb* probs_dev, int n, int total_tiles, hipStream_t s);
+// Split-BF16 products: terms = 2 (hi, lo: 3 products) or 3 (hi, mid, lo: 6 products);
+// kblock_f64 > 0 flushes FP32 accumulators into FP64 every kblock_f64 of K.
+int gemm_bf16split(const GemmProb* probs_dev, int n, int total_tiles, int terms, hipStream_t s);
+
+// DFT kernel matrix M[o][i] = exp(-2 pi i (xo0 + o dxo)(xi0 + i dxi) / lf), one
+// per (problem) entry, FP64 phase argument reduced to [-1/2, 1/2] cycles.
+struct DftSpec {
+  void* out;          // complex128 or complex64 (nout x nin, row-major)
+  int nout, nin;
+  double xo0, dxo, xi0, dxi, lf;
+  int out_c64;
+};
+int dft_matrices(const DftSpec* specs_dev, int n, int max_elems, hipStream_t s);
+
+// complex128 <-> complex64 conversions.
+int convert_c128_to_c64(const void* in, void* out, size_t n, hipStream_t s);
+int convert_c64_to_c128(const void* in, void* out, size_t n, hipStream_t s);
+
+}  // namespace csx

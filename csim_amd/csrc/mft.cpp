@@ -1,0 +1,213 @@
+// Matrix Fourier transform plans (host side).
+#include "mft.hpp"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <tuple>
+
+namespace csx {
+
+int launch_gemm(const GemmProb* probs_dev, int n, int tiles, csx_precision p, hipStream_t s) {
+  switch (p) {
+    case CSX_FP64: return gemm_f64(probs_dev, n, tiles, s);
+    case CSX_FP32: return gemm_f32(probs_dev, n, tiles, s);
+    case CSX_BF16X3: return gemm_bf16split(probs_dev, n, tiles, 2, s);
+    case CSX_BF16X6: return gemm_bf16split(probs_dev, n, tiles, 3, s);
+  }
+  return fail(CSX_ERR_ARG, "unknown precision");
+}
+
+namespace {
+struct DftKey {
+  int nout, nin;
+  double xo0, dxo, xi0, dxi, lf;
+  bool operator<(const DftKey& o) const {
+    return std::tie(nout, nin, xo0, dxo, xi0, dxi, lf) < std::tie(o.nout, o.nin, o.xo0, o.dxo, o.xi0, o.dxi, o.lf);
+  }
+};
+}  // namespace
+
+int MftPlan::build(const std::vector<MftField>& fields, csx_precision p, hipStream_t s) {
+  prec_ = p;
+  nf_ = (int)fields.size();
+  flops_ = flops1_ = flops2_ = bytes1_ = bytes2_ = 0;
+  if (nf_ == 0) return CSX_OK;
+  const bool c64 = p != CSX_FP64;
+  const size_t esz = c64 ? sizeof(float2) : sizeof(double2);
+  for (const auto& f : fields) {
+    if (f.in_x.n <= 0 || f.in_y.n <= 0 || f.out_x.n <= 0 || f.out_y.n <= 0)
+      return fail(CSX_ERR_ARG, "mft: empty grid");
+    if (f.lambda_f == 0) return fail(CSX_ERR_ARG, "mft: lambda*f is zero");
+    if (!f.in || !f.out) return fail(CSX_ERR_ARG, "mft: null field pointer");
+    if (c64 != (f.in_c64 != 0)) return fail(CSX_ERR_ARG, "mft: input type does not match the precision");
+  }
+  // distinct DFT kernels
+  std::map<DftKey, size_t> index;     // key -> element offset
+  std::vector<DftKey> keys;
+  size_t elems = 0;
+  auto add = [&](const csx_grid& o, const csx_grid& i, double lf) {
+    DftKey k{o.n, i.n, o.x0, o.dx, i.x0, i.dx, lf};
+    auto it = index.find(k);
+    if (it != index.end()) return it->second;
+    size_t off = elems;
+    index[k] = off;
+    keys.push_back(k);
+    elems += (size_t)o.n * i.n;
+    return off;
+  };
+  std::vector<size_t> offAy(nf_), offBx(nf_);
+  for (int f = 0; f < nf_; f++) {
+    offAy[f] = add(fields[f].out_y, fields[f].in_y, fields[f].lambda_f);
+    offBx[f] = add(fields[f].out_x, fields[f].in_x, fields[f].lambda_f);
+  }
+  CSX_TRY(dft_.reserve(elems * esz));
+  std::vector<DftSpec> specs;
+  int maxe = 0;
+  for (const auto& k : keys) {
+    DftSpec d;
+    d.out = static_cast<char*>(dft_.p) + index[k] * esz;
+    d.nout = k.nout; d.nin = k.nin;
+    d.xo0 = k.xo0; d.dxo = k.dxo; d.xi0 = k.xi0; d.dxi = k.dxi; d.lf = k.lf;
+    d.out_c64 = c64 ? 1 : 0;
+    specs.push_back(d);
+    maxe = std::max(maxe, k.nout * k.nin);
+  }
+  CSX_TRY(specs_.reserve(specs.size() * sizeof(DftSpec)));
+  CSX_HIP(hipMemcpyAsync(specs_.p, specs.data(), specs.size() * sizeof(DftSpec), hipMemcpyHostToDevice, s));
+  CSX_TRY(dft_matrices(specs_.as<DftSpec>(), (int)specs.size(), maxe, s));
+
+  // stage-1 workspace T^T (out_y.n x in_x.n per field)
+  std::vector<size_t> offT(nf_);
+  size_t telems = 0;
+  for (int f = 0; f < nf_; f++) { offT[f] = telems; telems += (size_t)fields[f].out_y.n * fields[f].in_x.n; }
+  CSX_TRY(tbuf_.reserve(telems * esz));
+
+  std::vector<GemmProb> p1(nf_), p2(nf_);
+  for (int f = 0; f < nf_; f++) {
+    const MftField& F = fields[f];
+    GemmProb& a = p1[f];
+    std::memset(&a, 0, sizeof(a));
+    a.A = F.in; a.lda = F.in_y.n;
+    a.B = static_cast<char*>(dft_.p) + offAy[f] * esz; a.ldb = F.in_y.n;
+    a.C = static_cast<char*>(tbuf_.p) + offT[f] * esz; a.ldc = F.in_x.n;
+    a.M = F.in_x.n; a.N = F.out_y.n; a.K = F.in_y.n;
+    a.transC = 1;
+    a.out_kind = c64 ? OUT_C64 : OUT_C128;
+    a.alpha_re = 1.0; a.alpha_im = 0.0;
+    a.mask_kind = MASK_NONE;
+
+    GemmProb& b = p2[f];
+    std::memset(&b, 0, sizeof(b));
+    b.A = static_cast<char*>(dft_.p) + offBx[f] * esz; b.lda = F.in_x.n;
+    b.B = a.C; b.ldb = F.in_x.n;
+    b.C = F.out; b.ldc = F.out_y.n;
+    b.M = F.out_x.n; b.N = F.out_y.n; b.K = F.in_x.n;
+    b.transC = 0;
+    b.out_kind = F.out_c64 ? OUT_C64 : OUT_C128;
+    // s = dx dy / (i lf) = -i dx dy / lf
+    b.alpha_re = 0.0; b.alpha_im = -(F.in_x.dx * F.in_y.dx) / F.lambda_f;
+    b.mask_kind = F.mask_kind;
+    if (F.mask_kind != MASK_NONE) {
+      b.mask = F.mask; b.ldm = F.mrows;
+      b.mm0 = F.col0; b.mn0 = F.row0; b.mM = F.mcols; b.mN = F.mrows;
+    }
+    double f1 = 8.0 * (double)F.in_x.n * F.in_y.n * F.out_y.n;
+    double f2 = 8.0 * (double)F.out_x.n * F.in_x.n * F.out_y.n;
+    flops1_ += f1; flops2_ += f2; flops_ += f1 + f2;
+    double ein = F.in_c64 ? 8.0 : 16.0, eout = F.out_c64 ? 8.0 : 16.0;
+    bytes1_ += ein * F.in_x.n * F.in_y.n + (double)esz * F.out_y.n * F.in_x.n;
+    bytes2_ += (double)esz * F.out_y.n * F.in_x.n + eout * F.out_x.n * F.out_y.n;
+  }
+  // DFT kernels are read once per launch as well
+  for (const auto& k : keys) { (void)k; }
+  tiles1_ = plan_tiles(p1.data(), nf_, kTileM, kTileN);
+  tiles2_ = plan_tiles(p2.data(), nf_, kTileM, kTileN);
+  CSX_TRY(probs1_.reserve(nf_ * sizeof(GemmProb)));
+  CSX_TRY(probs2_.reserve(nf_ * sizeof(GemmProb)));
+  CSX_HIP(hipMemcpyAsync(probs1_.p, p1.data(), nf_ * sizeof(GemmProb), hipMemcpyHostToDevice, s));
+  CSX_HIP(hipMemcpyAsync(probs2_.p, p2.data(), nf_ * sizeof(GemmProb), hipMemcpyHostToDevice, s));
+  CSX_HIP(hipStreamSynchronize(s));   // host vectors go out of scope
+  return CSX_OK;
+}
+
+static const char* prec_name(csx_precision p) {
+  switch (p) {
+    case CSX_FP64: return "fp64";
+    case CSX_FP32: return "fp32";
+    case CSX_BF16X3: return "bf16x3";
+    case CSX_BF16X6: return "bf16x6";
+  }
+  return "?";
+}
+
+int MftPlan::run(hipStream_t s, KernelTimer* t) const {
+  if (nf_ == 0) return CSX_OK;
+  const bool timed = t && t->enabled();
+  std::string base = std::string("gemm_") + prec_name(prec_);
+  if (timed) CSX_TRY(t->begin(base, flops1_, bytes1_, s));
+  CSX_TRY(launch_gemm(probs1_.as<GemmProb>(), nf_, tiles1_, prec_, s));
+  if (timed) CSX_TRY(t->end(s));
+  if (timed) CSX_TRY(t->begin(base, flops2_, bytes2_, s));
+  CSX_TRY(launch_gemm(probs2_.as<GemmProb>(), nf_, tiles2_, prec_, s));
+  if (timed) CSX_TRY(t->end(s));
+  return CSX_OK;
+}
+
+// ------------------------------------------------------------ KernelTimer --
+KernelTimer::~KernelTimer() {
+  for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
+}
+void KernelTimer::enable(bool on) {
+  on_ = on;
+  stats_.clear();
+  open_.clear();
+  used_ = 0;
+}
+int KernelTimer::begin(const std::string& name, double flops, double bytes, hipStream_t s) {
+  if (used_ + 2 > (int)pool_.size()) {
+    for (int i = 0; i < 64; i++) {
+      hipEvent_t e;
+      CSX_HIP(hipEventCreate(&e));
+      pool_.push_back(e);
+    }
+  }
+  open_.push_back(Rec{name, flops, bytes, used_});
+  CSX_HIP(hipEventRecord(pool_[used_], s));
+  used_ += 2;
+  return CSX_OK;
+}
+int KernelTimer::end(hipStream_t s) {
+  if (open_.empty()) return fail(CSX_ERR_STATE, "KernelTimer::end without begin");
+  CSX_HIP(hipEventRecord(pool_[open_.back().ev + 1], s));
+  return CSX_OK;
+}
+int KernelTimer::collect() {
+  for (const Rec& r : open_) {
+    float ms = 0;
+    CSX_HIP(hipEventElapsedTime(&ms, pool_[r.ev], pool_[r.ev + 1]));
+    Stat& st = stats_[r.name];
+    st.count++;
+    st.ms += ms;
+    st.flops += r.flops;
+    st.bytes += r.bytes;
+  }
+  open_.clear();
+  used_ = 0;
+  return CSX_OK;
+}
+std::string KernelTimer::json() const {
+  std::string o = "{";
+  bool first = true;
+  for (const auto& kv : stats_) {
+    char buf[512];
+    std::snprintf(buf, sizeof buf, "%s\"%s\": {\"launches\": %ld, \"ms_total\": %.9g, \"flops_total\": %.9g, \"bytes_total\": %.9g}",
+                  first ? "" : ", ", kv.first.c_str(), kv.second.count, kv.second.ms, kv.second.flops, kv.second.bytes);
+    o += buf;
+    first = false;
+  }
+  return o + "}";
+}
+
+}  // namespace csx

@@ -1,0 +1,75 @@
+// Matrix Fourier transform plans: csim zoomFft (src/lib/csim_fft.cpp:670-826) as
+// two grouped complex GEMMs over a whole batch of fields.
+#pragma once
+#include <map>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "gemm.hpp"
+
+namespace csx {
+
+// Per-launch timing with HIP events recorded on the launch stream; enabled for
+// measurement runs only.  Event pairs are pooled and read after a synchronise.
+class KernelTimer {
+ public:
+  ~KernelTimer();
+  void enable(bool on);
+  bool enabled() const { return on_; }
+  int begin(const std::string& name, double flops, double bytes, hipStream_t s);
+  int end(hipStream_t s);
+  int collect();                        // after the stream is synchronised
+  std::string json() const;
+  struct Stat { long count = 0; double ms = 0, flops = 0, bytes = 0; };
+  const std::map<std::string, Stat>& stats() const { return stats_; }
+
+ private:
+  struct Rec { std::string name; double flops, bytes; int ev; };
+  bool on_ = false;
+  std::vector<hipEvent_t> pool_;
+  int used_ = 0;
+  std::vector<Rec> open_;
+  std::map<std::string, Stat> stats_;
+};
+
+// One field of a grouped MFT.  Fields are column-major (rows = y).
+//   E_out = dx_in dy_in / (i lf) * A_y * E_in * B_x^T
+// computed y-first:  T^T = (X A_y^T)^T  then  X_out = B_x T  with X = E^T
+// (the column-major field read as a row-major matrix).
+struct MftField {
+  const void* in = nullptr;    // complex128, or complex64 when in_c64
+  void* out = nullptr;         // complex128, or complex64 when out_c64
+  int in_c64 = 0, out_c64 = 0;
+  csx_grid in_x{}, in_y{}, out_x{}, out_y{};
+  double lambda_f = 0;
+  // optional mask applied to the output (E_out(r, c) *= mask(r - row0, c - col0))
+  int mask_kind = MASK_NONE;
+  const void* mask = nullptr;  // complex128 / double, column-major mrows x mcols
+  int mrows = 0, mcols = 0, row0 = 0, col0 = 0;
+};
+
+class MftPlan {
+ public:
+  // Builds DFT kernel matrices (FP64 phases, stored at the working precision),
+  // the stage-1 workspace and both problem arrays.  Field pointers are baked in.
+  int build(const std::vector<MftField>& fields, csx_precision p, hipStream_t s);
+  // Enqueues both GEMM stages (no allocation, no host sync: graph-capturable).
+  int run(hipStream_t s, KernelTimer* timer = nullptr) const;
+  size_t workspace_bytes() const { return dft_.bytes + tbuf_.bytes; }
+  int n_fields() const { return nf_; }
+  double flops() const { return flops_; }   // algorithmic, 8 per complex MAC
+  double flops_stage(int i) const { return i == 0 ? flops1_ : flops2_; }
+  // algorithmic HBM bytes of each stage: operands read once, output written once
+  double bytes_stage(int i) const { return i == 0 ? bytes1_ : bytes2_; }
+
+ private:
+  csx_precision prec_ = CSX_FP64;
+  DevBuf dft_, tbuf_, probs1_, probs2_, specs_;
+  int nf_ = 0, tiles1_ = 0, tiles2_ = 0;
+  double flops_ = 0, flops1_ = 0, flops2_ = 0, bytes1_ = 0, bytes2_ = 0;
+};
+
+int launch_gemm(const GemmProb* probs_dev, int n, int tiles, csx_precision p, hipStream_t s);
+
+}  // namespace csx

@@ -1,0 +1,499 @@
+// Pointwise, FFT, resampling, hex-mask and reduction kernels (gfx950).
+// Fields are complex128, column-major (rows = y), batches field-major.
+#include "ops.hpp"
+
+#include <algorithm>
+
+namespace csx {
+
+static inline int grid_for(long n, int block = 256, int cap = 8192) {
+  long g = (n + block - 1) / block;
+  return (int)std::max<long>(1, std::min<long>(g, cap));
+}
+
+// ----------------------------------------------------------------- pointwise --
+__global__ void k_mask(double2* __restrict__ E, long npix, long total, const double* __restrict__ m) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    double w = m[i % npix];
+    double2 e = E[i];
+    // (er + i ei)(w + i w)
+    E[i] = make_double2(e.x * w - e.y * w, e.x * w + e.y * w);
+  }
+}
+int k_apply_mask(double2* E, long npix, int nfield, const double* m, hipStream_t s) {
+  long total = npix * nfield;
+  hipLaunchKernelGGL(k_mask, dim3(grid_for(total)), dim3(256), 0, s, E, npix, total, m);
+  CSX_HIP(hipGetLastError());
+  return CSX_OK;
+}
+
+__global__ void k_mirror(double2* __restrict__ E, long npix, long total, const double* __restrict__ h,
+                         double sign, const double* __restrict__ lambdas) {
+#pragma clang fp contract(off)
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    long f = i / npix;
+    // csim: exp((-mirrorSign*2*2*M_PI/lambda) * i * h), evaluated left to right
+    double a = -sign * 2 * 2 * M_PI / lambdas[f];
+    double ph = a * h[i - f * npix];
+    double sn, cs;
+    sincos(ph, &sn, &cs);
+    double2 e = E[i];
+    E[i] = make_double2(e.x * cs - e.y * sn, e.x * sn + e.y * cs);
+  }
+}
+int k_apply_mirror(double2* E, long npix, int nfield, const double* h, double sign,
+                   const double* lambdas_dev, hipStream_t s) {
+  long total = npix * nfield;
+  hipLaunchKernelGGL(k_mirror, dim3(grid_for(total)), dim3(256), 0, s, E, npix, total, h, sign, lambdas_dev);
+  CSX_HIP(hipGetLastError());
+  return CSX_OK;
+}
+
+__global__ void k_cfactor(double2* __restrict__ E, long npix, long total, const double2* __restrict__ P, int per) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    long f = i / npix, p = i - f * npix;
+    double2 w = P[(f / per) * npix + p];
+    double2 e = E[i];
+    E[i] = make_double2(e.x * w.x - e.y * w.y, e.x * w.y + e.y * w.x);
+  }
+}
+int k_apply_cfactor(double2* E, long npix, int nfield, const double2* P, int per, hipStream_t s) {
+  long total = npix * nfield;
+  hipLaunchKernelGGL(k_cfactor, dim3(grid_for(total)), dim3(256), 0, s, E, npix, total, P, per);
+  CSX_HIP(hipGetLastError());
+  return CSX_OK;
+}
+
+// params per field: flux, tipX, tiltY (tip/tilt already times magnification)
+__global__ void k_psrc(double2* __restrict__ E, int rows, int cols, int nfield,
+                       const double* __restrict__ prm, double ps) {
+#pragma clang fp contract(off)
+  long npix = (long)rows * cols, total = npix * nfield;
+  double Lx = ps * cols, Ly = ps * rows;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    long f = i / npix, p = i - f * npix;
+    int c = (int)(p / rows), r = (int)(p - (long)c * rows);
+    double x = -(Lx - ps) / 2.0 + c * ps;
+    double y = -(Ly - ps) / 2.0 + r * ps;
+    double flux = prm[3 * f], tipX = prm[3 * f + 1], tiltY = prm[3 * f + 2];
+    double v = 2 * x * tipX / Lx + 2 * y * tiltY / Ly;
+    double ph = 2.0 * M_PI * v;
+    double sn, cs;
+    sincos(ph, &sn, &cs);
+    E[i] = make_double2(flux * cs, flux * sn);
+  }
+}
+int k_point_sources(double2* E, int rows, int cols, int nfield, const double* params_dev,
+                    double pixel_scale, hipStream_t s) {
+  long total = (long)rows * cols * nfield;
+  hipLaunchKernelGGL(k_psrc, dim3(grid_for(total)), dim3(256), 0, s, E, rows, cols, nfield, params_dev, pixel_scale);
+  CSX_HIP(hipGetLastError());
+  return CSX_OK;
+}
+
+// ---------------------------------------------------------------------- FFT --
+// Radix-2 Stockham autosort transforms staged in LDS.  Stage m = 2^s:
+//   y[k + 2jm] = x[k + jm] + w x[k + jm + n/2],  y[k + 2jm + m] = x[k + jm] - w x[k + jm + n/2],
+//   w = exp(sign i pi k / m) = tw[k n/(2m)],  tw[q] = exp(i 2 pi q / n) (conjugated for sign -1).
+// Each stage reads its operands into registers, synchronises, writes in place.
+constexpr int kFftThreads = 256;
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// sin and cos of a large FP64 argument (|x| up to ~1e9 rad) with the argument
+// reduced by 2 pi in extended precision: k = rint(x / 2pi); the product k * C1
+// and its exact rounding error (fma) are removed, then k * C2, C1 + C2 = 2 pi
+// to ~1e-32.  The reduced |r| <= pi keeps the full accuracy of sincos.
+__device__ __forceinline__ void sincos_reduced(double x, double* s, double* c) {
+  const double C1 = 6.283185307179586232;           // 2 pi rounded to FP64
+  const double C2 = 2.4492935982947064e-16;         // 2 pi - C1
+  double k = rint(x * 0.15915494309189535);
+  double p = k * C1;
+  double e = __fma_rn(k, C1, -p);                    // exact error of k * C1
+  double r = ((x - p) - e) - k * C2;
+  sincos(r, s, c);
+}
+
+// count independent transforms of length n stored at buf + t*n; Q >= butterflies per thread.
+template <int Q>
+__device__ void lds_fft(double2* buf, int n, int count, int sign, const double2* __restrict__ tw) {
+  const int half = n >> 1;
+  const int nb = half * count;
+  for (int m = 1; m < n; m <<= 1) {
+    const int l = n / (2 * m);
+    double2 o0[Q], o1[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+      int b = threadIdx.x + q * kFftThreads;
+      if (b < nb) {
+        int t = b / half, bb = b - t * half;
+        int j = bb / m, k = bb - j * m;
+        const double2* x = buf + t * n;
+        double2 c0 = x[k + j * m], c1 = x[k + j * m + half];
+        double2 w = tw[k * l];
+        if (sign < 0) w.y = -w.y;
+        double2 wc1 = cmul(w, c1);
+        o0[q] = make_double2(c0.x + wc1.x, c0.y + wc1.y);
+        o1[q] = make_double2(c0.x - wc1.x, c0.y - wc1.y);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+      int b = threadIdx.x + q * kFftThreads;
+      if (b < nb) {
+        int t = b / half, bb = b - t * half;
+        int j = bb / m, k = bb - j * m;
+        double2* x = buf + t * n;
+        x[k + 2 * j * m] = o0[q];
+        x[k + 2 * j * m + m] = o1[q];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_twiddles(double2* tw, int n) {
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n / 2; q += gridDim.x * blockDim.x) {
+    double sn, cs;
+    sincospi(2.0 * (double)q / (double)n, &sn, &cs);
+    tw[q] = make_double2(cs, sn);
+  }
+}
+
+// Along the contiguous (row) index: one column of one field per block.
+template <int Q>
+__global__ __launch_bounds__(kFftThreads) void k_fft_cols(double2* __restrict__ E, int n, int sign, double scale,
+                                                          const double2* __restrict__ tw) {
+  extern __shared__ double2 sbuf[];
+  long base = (long)blockIdx.x * n;         // field*n*n + col*n
+  for (int i = threadIdx.x; i < n; i += blockDim.x) sbuf[i] = E[base + i];
+  __syncthreads();
+  lds_fft<Q>(sbuf, n, 1, sign, tw);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    double2 v = sbuf[i];
+    E[base + i] = make_double2(v.x * scale, v.y * scale);
+  }
+}
+
+// Along the strided (column) index: R consecutive rows of one field per block.
+// mode 0: plain transform.  mode 1: forward, multiply by the angular-spectrum
+// transfer function, inverse (the Fresnel step), all in LDS.
+template <int Q>
+__global__ __launch_bounds__(kFftThreads) void k_fft_rows(double2* __restrict__ E, int n, int R, int sign,
+                                                          int mode, const double* __restrict__ lambdas,
+                                                          double ap_rad, double z, const double2* __restrict__ tw) {
+  extern __shared__ double2 sbuf[];          // [R][n]
+  int blocks_per_field = n / R;
+  int field = blockIdx.x / blocks_per_field;
+  int r0 = (blockIdx.x - field * blocks_per_field) * R;
+  double2* F = E + (long)field * n * n;
+  for (int e = threadIdx.x; e < R * n; e += blockDim.x) {
+    int c = e / R, rr = e - c * R;           // consecutive threads: consecutive rows of one column
+    sbuf[rr * n + c] = F[(long)c * n + r0 + rr];
+  }
+  __syncthreads();
+  if (mode == 0) {
+    lds_fft<Q>(sbuf, n, R, sign, tw);
+  } else {
+    lds_fft<Q>(sbuf, n, R, -1, tw);
+    {
+#pragma clang fp contract(off)
+      // csim: exp(i*k*z * sqrt(1. - 1./4.*pow(lambda/apRad, 2.)*H)), Goodman 4-20
+      double lam = lambdas[field];
+      double k = 2. * M_PI / lam;
+      double q = lam / ap_rad;
+      double c4 = 1. / 4. * (q * q);
+      for (int e = threadIdx.x; e < R * n; e += blockDim.x) {
+        int rr = e / n, c = e - rr * n;
+        int r = r0 + rr;
+        int pr = r < n / 2 ? r : r - n;        // fft_shift of regspace(-n/2, n/2-1)
+        int pc = c < n / 2 ? c : c - n;
+        double H = (double)pc * (double)pc + (double)pr * (double)pr;
+        double ph = k * z * sqrt(1. - c4 * H);
+        double sn, cs;
+        sincos_reduced(ph, &sn, &cs);
+        sbuf[e] = cmul(sbuf[e], make_double2(cs, sn));
+      }
+    }
+    __syncthreads();
+    lds_fft<Q>(sbuf, n, R, +1, tw);
+  }
+  for (int e = threadIdx.x; e < R * n; e += blockDim.x) {
+    int c = e / R, rr = e - c * R;
+    F[(long)c * n + r0 + rr] = sbuf[rr * n + c];
+  }
+}
+
+static int rows_per_block(int n) {
+  int R = 4096 / n;                          // 64 KiB of LDS
+  if (R > 8) R = 8;
+  if (R < 1) R = 1;
+  while (R > 1 && n % R) R >>= 1;
+  while (R > 1 && R * (n / 2) > 16 * kFftThreads) R >>= 1;
+  return R;
+}
+
+static int check_fft_n(int n) {
+  if (n < 2 || n > 4096 || (n & (n - 1))) return fail(CSX_ERR_ARG, "fft: n must be a power of two in [2, 4096]");
+  return CSX_OK;
+}
+
+// Twiddle tables, one per transform length, kept for the process lifetime.
+static const double2* twiddles(int n, hipStream_t s) {
+  static double2* tables[13] = {nullptr};
+  int lg = 0;
+  while ((1 << lg) < n) lg++;
+  if (!tables[lg]) {
+    double2* p = nullptr;
+    if (hipMalloc(&p, sizeof(double2) * std::max(1, n / 2)) != hipSuccess) return nullptr;
+    hipLaunchKernelGGL(k_twiddles, dim3(std::max(1, (n / 2 + 255) / 256)), dim3(256), 0, s, p, n);
+    tables[lg] = p;
+  }
+  return tables[lg];
+}
+
+static int qfor(long nb) {
+  long q = (nb + kFftThreads - 1) / kFftThreads;
+  int Q = 1;
+  while (Q < q) Q <<= 1;
+  return Q;
+}
+
+#define CSX_FFT_DISPATCH(Q, KERNEL, ...)                                              \
+  switch (Q) {                                                                        \
+    case 1: hipLaunchKernelGGL(KERNEL<1>, __VA_ARGS__); break;                       \
+    case 2: hipLaunchKernelGGL(KERNEL<2>, __VA_ARGS__); break;                       \
+    case 4: hipLaunchKernelGGL(KERNEL<4>, __VA_ARGS__); break;                       \
+    case 8: hipLaunchKernelGGL(KERNEL<8>, __VA_ARGS__); break;                       \
+    case 16: hipLaunchKernelGGL(KERNEL<16>, __VA_ARGS__); break;                     \
+    default: return fail(CSX_ERR_ARG, "fft: unsupported size");                      \
+  }
+
+static int launch_cols(double2* E, int n, int nfield, int sign, double scale, const double2* tw, hipStream_t s) {
+  int Q = qfor(n / 2);
+  size_t lds = sizeof(double2) * n;
+  CSX_FFT_DISPATCH(Q, k_fft_cols, dim3(n * nfield), dim3(kFftThreads), lds, s, E, n, sign, scale, tw);
+  CSX_HIP(hipGetLastError());
+  return CSX_OK;
+}
+static int launch_rows(double2* E, int n, int nfield, int sign, int mode, const double* lambdas, double ap_rad,
+                       double z, const double2* tw, hipStream_t s) {
+  int R = rows_per_block(n);
+  int Q = qfor((long)R * (n / 2));
+  size_t lds = sizeof(double2) * n * R;
+  CSX_FFT_DISPATCH(Q, k_fft_rows, dim3((n / R) * nfield), dim3(kFftThreads), lds, s, E, n, R, sign, mode, lambdas,
+                   ap_rad, z, tw);
+  CSX_HIP(hipGetLastError());
+  return CSX_OK;
+}
+
+int fft2_batched(double2* E, int n, int nfield, int dir, hipStream_t s) {
+  CSX_TRY(check_fft_n(n));
+  const double2* tw = twiddles(n, s);
+  if (!tw) return fail(CSX_ERR_NOMEM, "fft: twiddle table allocation failed");
+  if (dir < 0) {
+    CSX_TRY(launch_cols(E, n, nfield, -1, 1.0, tw, s));
+    CSX_TRY(launch_rows(E, n, nfield, -1, 0, nullptr, 0, 0, tw, s));
+  } else {
+    CSX_TRY(launch_rows(E, n, nfield, +1, 0, nullptr, 0, 0, tw, s));
+    CSX_TRY(launch_cols(E, n, nfield, +1, 1.0 / ((double)n * (double)n), tw, s));
+  }
+  return CSX_OK;
+}
+
+int fresnel_batched(double2* E, int n, int nfield, const double* lambdas_dev, double ap_rad,
+                    double z, hipStream_t s) {
+  CSX_TRY(check_fft_n(n));
+  const double2* tw = twiddles(n, s);
+  if (!tw) return fail(CSX_ERR_NOMEM, "fft: twiddle table allocation failed");
+  CSX_TRY(launch_cols(E, n, nfield, -1, 1.0, tw, s));
+  CSX_TRY(launch_rows(E, n, nfield, -1, 1, lambdas_dev, ap_rad, z, tw, s));
+  CSX_TRY(launch_cols(E, n, nfield, +1, 1.0 / ((double)n * (double)n), tw, s));
+  return CSX_OK;
+}
+
+// --------------------------------------------------------------- resampling --
+// out(r', c') = (1-wy)[(1-wx) in(ylo,xlo) + wx in(ylo,xhi)] + wy[(1-wx) in(yhi,xlo) + wx in(yhi,xhi)]
+// (interp along x first, then y, as src/lib/csim_geom.cpp:244-266).
+__global__ void k_resample(const double2* __restrict__ in, int rows, int cols, int nfield,
+                           double2* __restrict__ out, int orows, int ocols,
+                           const InterpTable* __restrict__ ty, const InterpTable* __restrict__ tx) {
+#pragma clang fp contract(off)
+  long onpix = (long)orows * ocols, total = onpix * nfield;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    long f = i / onpix, p = i - f * onpix;
+    int c = (int)(p / orows), r = (int)(p - (long)c * orows);
+    InterpTable X = tx[c], Y = ty[r];
+    double2 v = make_double2(0, 0);
+    if (X.lo >= 0 && Y.lo >= 0) {
+      const double2* F = in + f * (long)rows * cols;
+      double2 a = F[(long)X.lo * rows + Y.lo], b = F[(long)X.hi * rows + Y.lo];
+      double2 c2 = F[(long)X.lo * rows + Y.hi], d = F[(long)X.hi * rows + Y.hi];
+      double s1r = (1.0 - X.w) * a.x + X.w * b.x, s1i = (1.0 - X.w) * a.y + X.w * b.y;
+      double s2r = (1.0 - X.w) * c2.x + X.w * d.x, s2i = (1.0 - X.w) * c2.y + X.w * d.y;
+      v = make_double2((1.0 - Y.w) * s1r + Y.w * s2r, (1.0 - Y.w) * s1i + Y.w * s2i);
+    }
+    out[i] = v;
+  }
+}
+int resample_batched(const double2* in, int rows, int cols, int nfield, double2* out, int orows,
+                     int ocols, const InterpTable* ty_dev, const InterpTable* tx_dev, hipStream_t s) {
+  long total = (long)orows * ocols * nfield;
+  hipLaunchKernelGGL(k_resample, dim3(grid_for(total)), dim3(256), 0, s, in, rows, cols, nfield, out, orows,
+                     ocols, ty_dev, tx_dev);
+  CSX_HIP(hipGetLastError());
+  return CSX_OK;
+}
+
+// ------------------------------------------------------------------ hex mask --
+__global__ void k_hex(const int32_t* __restrict__ nsub, const int32_t* __restrict__ hexnum, int S,
+                      int nsubpix, const double* __restrict__ sags, int nhex, int ncand,
+                      const double* __restrict__ lambdas, int nlam, double phase_sign, int use_only,
+                      double2* __restrict__ out) {
+#pragma clang fp contract(off)
+  long plane = (long)S * S, total = plane * ncand * nlam;
+  double norm = (double)nsubpix * nsubpix;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    long cl = i / plane, p = i - cl * plane;
+    int cand = (int)(cl / nlam), li = (int)(cl - (long)cand * nlam);
+    double lam = lambdas[li];
+    const double* sg = sags + (long)cand * nhex;
+    double ar = 0, ai = 0;
+#pragma unroll
+    for (int s = 0; s < 3; s++) {
+      int h = hexnum[p + s * plane];
+      double ns = (double)nsub[p + s * plane];
+      if (use_only > -1) {
+        if (h == use_only) ;
+        else if (use_only == nhex && h == -1) ;
+        else ns = 0;
+      }
+      double sag = h > -1 ? sg[h] : 0.0;
+      // csim: exp(phaseSign * 2 * M_PI * i * 2.0 * sag / lambda)
+      double ph = phase_sign * 2 * M_PI * 2.0 * sag / lam;
+      double sn, cs;
+      sincos(ph, &sn, &cs);
+      ar += ns * cs;
+      ai += ns * sn;
+    }
+    out[i] = make_double2(ar / norm, ai / norm);
+  }
+}
+int hex_masks(const int32_t* nsub, const int32_t* hexnum, int S, int nsubpix, const double* sags,
+              int nhex, int ncand, const double* lambdas_dev, int nlam, double phase_sign,
+              int use_only, double2* out, hipStream_t s) {
+  long total = (long)S * S * ncand * nlam;
+  hipLaunchKernelGGL(k_hex, dim3(grid_for(total)), dim3(256), 0, s, nsub, hexnum, S, nsubpix, sags, nhex,
+                     ncand, lambdas_dev, nlam, phase_sign, use_only, out);
+  CSX_HIP(hipGetLastError());
+  return CSX_OK;
+}
+
+// --------------------------------------------------------------- reductions --
+__global__ void k_intensity(const double2* __restrict__ E, long npix, int ngroups, int per, double* __restrict__ I) {
+  long total = npix * ngroups;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    long g = i / npix, p = i - g * npix;
+    double acc = 0;
+    for (int f = 0; f < per; f++) {
+      double2 e = E[(g * per + f) * npix + p];
+      acc += e.x * e.x + e.y * e.y;
+    }
+    I[i] = acc;
+  }
+}
+int intensity_groups(const double2* E, long npix, int ngroups, int per, double* I, hipStream_t s) {
+  long total = npix * ngroups;
+  hipLaunchKernelGGL(k_intensity, dim3(grid_for(total)), dim3(256), 0, s, E, npix, ngroups, per, I);
+  CSX_HIP(hipGetLastError());
+  return CSX_OK;
+}
+
+__device__ double block_reduce(double v, int op_max) {
+  __shared__ double red[32];
+  for (int o = 32; o > 0; o >>= 1) {
+    double w = __shfl_down(v, o, 64);
+    v = op_max ? fmax(v, w) : v + w;
+  }
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int nw = (blockDim.x + 63) >> 6;
+    double r = red[0];
+    for (int i = 1; i < nw; i++) r = op_max ? fmax(r, red[i]) : r + red[i];
+    red[0] = r;
+  }
+  __syncthreads();
+  return red[0];
+}
+
+__global__ void k_max(const double* __restrict__ I, long npix, double* __restrict__ out) {
+  const double* x = I + blockIdx.x * npix;
+  double v = -INFINITY;
+  for (long p = threadIdx.x; p < npix; p += blockDim.x) v = fmax(v, x[p]);
+  v = block_reduce(v, 1);
+  if (threadIdx.x == 0) out[blockIdx.x] = v;
+}
+int max_groups(const double* I, long npix, int ngroups, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_max, dim3(ngroups), dim3(256), 0, s, I, npix, out);
+  CSX_HIP(hipGetLastError());
+  return CSX_OK;
+}
+
+// Fixed-order (deterministic) region sum: each thread sums a strided subset.
+__global__ void k_region_mean(const double* __restrict__ I, long npix, const int32_t* __restrict__ idx,
+                              int nidx, const double* __restrict__ peak_dev, double peak,
+                              double* __restrict__ out) {
+  const double* x = I + blockIdx.x * npix;
+  double v = 0;
+  for (int q = threadIdx.x; q < nidx; q += blockDim.x) v += x[idx[q]];
+  v = block_reduce(v, 0);
+  if (threadIdx.x == 0) {
+    double pk = peak_dev ? peak_dev[0] : peak;
+    out[blockIdx.x] = (v / nidx) / pk;
+  }
+}
+int region_mean_groups(const double* I, long npix, int ngroups, const int32_t* idx, int nidx,
+                       const double* peak_dev, double peak, double* out, hipStream_t s) {
+  if (nidx <= 0) return fail(CSX_ERR_ARG, "region_mean: empty region");
+  hipLaunchKernelGGL(k_region_mean, dim3(ngroups), dim3(256), 0, s, I, npix, idx, nidx, peak_dev, peak, out);
+  CSX_HIP(hipGetLastError());
+  return CSX_OK;
+}
+
+// ------------------------------------------------------------------- embed --
+__global__ void k_embed(const double2* __restrict__ src, int srows, int scols, double2* __restrict__ dst,
+                        int drows, int dcols, int nfield, int crop) {
+  // crop == 0: dst larger, src placed at (drows/2 - srows/2, dcols/2 - scols/2), zeros around.
+  // crop == 1: dst smaller, dst = src window at (srows/2 - drows/2, scols/2 - dcols/2).
+  long dn = (long)drows * dcols, total = dn * nfield;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    long f = i / dn, p = i - f * dn;
+    int c = (int)(p / drows), r = (int)(p - (long)c * drows);
+    double2 v = make_double2(0, 0);
+    if (!crop) {
+      int r0 = drows / 2 - srows / 2, c0 = dcols / 2 - scols / 2;
+      int sr = r - r0, sc = c - c0;
+      if (sr >= 0 && sr < srows && sc >= 0 && sc < scols) v = src[f * (long)srows * scols + (long)sc * srows + sr];
+    } else {
+      int r0 = srows / 2 - drows / 2, c0 = scols / 2 - dcols / 2;
+      v = src[f * (long)srows * scols + (long)(c + c0) * srows + (r + r0)];
+    }
+    dst[i] = v;
+  }
+}
+int embed_center(const double2* src, int srows, int scols, double2* dst, int drows, int dcols,
+                 int nfield, int crop, hipStream_t s) {
+  long total = (long)drows * dcols * nfield;
+  hipLaunchKernelGGL(k_embed, dim3(grid_for(total)), dim3(256), 0, s, src, srows, scols, dst, drows, dcols,
+                     nfield, crop);
+  CSX_HIP(hipGetLastError());
+  return CSX_OK;
+}
+
+}  // namespace csx

@@ -1,0 +1,47 @@
+// Pointwise, FFT, resampling, hex-mask and reduction kernels (gfx950).
+#pragma once
+#include "common.hpp"
+
+namespace csx {
+
+// E[f][p] *= (1+i) m[p]            (csim mask, src/elements/mask.cpp:32-46)
+int k_apply_mask(double2* E, long npix, int nfield, const double* m, hipStream_t s);
+// E[f][p] *= exp(-sign 4 pi i h[p] / lambda[f])   (mirror.cpp:66-72); lambdas on device
+int k_apply_mirror(double2* E, long npix, int nfield, const double* h, double sign,
+                   const double* lambdas_dev, hipStream_t s);
+// E[f][p] *= P[f / per][p]   complex factor shared by `per` consecutive fields
+int k_apply_cfactor(double2* E, long npix, int nfield, const double2* P, int per, hipStream_t s);
+// Point sources (efield.cpp:179-206): E[f] = flux_f exp(i 2 pi (2 x tipX_f/Lx + 2 y tiltY_f/Ly))
+int k_point_sources(double2* E, int rows, int cols, int nfield, const double* params_dev,
+                    double pixel_scale, hipStream_t s);
+
+// 2-D FFT, csim semantics: dir -1 forward unnormalised, +1 inverse / n^2.
+int fft2_batched(double2* E, int n, int nfield, int dir, hipStream_t s);
+// Angular-spectrum Fresnel step (src/lib/csim_fft.cpp:371-392); lambdas on device.
+int fresnel_batched(double2* E, int n, int nfield, const double* lambdas_dev, double ap_rad,
+                    double z, hipStream_t s);
+
+// Separable linear resampling with precomputed (index, weight) tables, extrap 0.
+struct InterpTable { int lo, hi; double w; };
+int resample_batched(const double2* in, int rows, int cols, int nfield, double2* out, int orows,
+                     int ocols, const InterpTable* ty_dev, const InterpTable* tx_dev, hipStream_t s);
+
+// Hex FPM masks for (candidate, lambda) pairs (complexHexMaskFPM.cpp:44-99).
+int hex_masks(const int32_t* nsub, const int32_t* hexnum, int S, int nsubpix, const double* sags,
+              int nhex, int ncand, const double* lambdas_dev, int nlam, double phase_sign,
+              int use_only, double2* out, hipStream_t s);
+
+// I[g][p] = sum_{f < per} |E[g*per + f][p]|^2
+int intensity_groups(const double2* E, long npix, int ngroups, int per, double* I, hipStream_t s);
+// out[g] = max_p I[g][p]
+int max_groups(const double* I, long npix, int ngroups, double* out, hipStream_t s);
+// out[g] = mean_{idx} I[g][idx] / peak_dev[0] (peak on device) or / peak
+int region_mean_groups(const double* I, long npix, int ngroups, const int32_t* idx, int nidx,
+                       const double* peak_dev, double peak, double* out, hipStream_t s);
+
+// Centred copy: dst (drows x dcols) <- src (srows x scols) placed at the centre
+// offsets csim uses (rows/2 - srows/2); zero elsewhere (pad) or crop.
+int embed_center(const double2* src, int srows, int scols, double2* dst, int drows, int dcols,
+                 int nfield, int crop, hipStream_t s);
+
+}  // namespace csx

@@ -1,0 +1,114 @@
+"""GPU element chain (csim scripts) against the oracle chain on the same inputs.
+
+Gates: plain FP64 GPU path <= 1e-12 relative field RMS against the CPU
+restatement of csim (Bluestein chirp-z zoomFft); mixed precision <= 1e-11
+absolute contrast (dark-hole mean, 3-10 lambda/D) against the FP64 GPU path,
+per-pixel maximum reported.
+"""
+import numpy as np
+import pytest
+
+from csim_amd import workloads as W
+from csim_amd.chain import Chain
+from oracle.chain import OracleChain
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_rms(a, b):
+    return float(np.sqrt(np.mean(np.abs(a - b) ** 2)) / np.sqrt(np.mean(np.abs(b) ** 2)))
+
+
+def build(ctx, w, precision="fp64"):
+    ch = Chain(ctx, w.script)
+    W.apply(ch, w)
+    ch.set_precision(precision)
+    return ch
+
+
+def test_workload0_fields_and_contrast(ctx):
+    w = W.workload0()
+    ch = build(ctx, w)
+    oc = OracleChain(w.script, w.arrays)
+    ch.propagate(calibrating=True)
+    e_cal = rel_rms(ch.field(), oc.field(0, 0, calibrating=True))
+    ch.propagate(calibrating=False)
+    e_full = rel_rms(ch.field(), oc.field(0, 0, calibrating=False))
+    print(f"workload0 calibration field {e_cal:.3e}, occulted field {e_full:.3e}")
+    assert e_cal < 1e-12 and e_full < 1e-12
+    c, rep = ch.contrast()
+    c_ref, img_ref, peak_ref = oc.contrast()
+    print(f"contrast gpu {c[0]:.12e} oracle {c_ref:.12e} peak {rep['calib_peak']:.6e}")
+    assert abs(rep["calib_peak"] - peak_ref) <= 1e-12 * peak_ref
+    assert abs(c[0] - c_ref) <= 1e-12 * c_ref
+    img = ch.normalized_image()
+    assert np.max(np.abs(img - img_ref)) <= 1e-12 * img_ref.max()
+
+
+def test_workload2_reduced_fp64(ctx):
+    w = W.workload2(n_lambda=3, offsets=(0.0, 2.0, 6.0))
+    ch = build(ctx, w)
+    oc = OracleChain(w.script, w.arrays, sags=w.meta["sags"])
+    ch.propagate(False)
+    worst = 0.0
+    for s in range(3):
+        for li in range(3):
+            worst = max(worst, rel_rms(ch.field(0, s, li), oc.field(s, li)))
+    print(f"workload2 (3 lambda x 3 sources) worst field rel rms {worst:.3e}")
+    assert worst < 1e-12
+    c, _ = ch.contrast()
+    c_ref, _, _ = oc.contrast()
+    assert abs(c[0] - c_ref) <= 1e-12 * c_ref
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "bf16x6"])
+def test_workload2_mixed_contrast_error(ctx, precision):
+    w = W.workload2()
+    ref = build(ctx, w, "fp64")
+    c64, _ = ref.contrast()
+    img64 = ref.normalized_image()
+    mix = build(ctx, w, precision)
+    cm, _ = mix.contrast()
+    imgm = mix.normalized_image()
+    sel = OracleChain(w.script, w.arrays, sags=w.meta["sags"]).region_index(
+        (128, -(128 * (W.F_RATIO * 630e-9 / 2.56) - W.F_RATIO * 630e-9 / 2.56) / 2, W.F_RATIO * 630e-9 / 2.56))
+    dmean = abs(cm[0] - c64[0])
+    dmax = float(np.max(np.abs(imgm - img64)[sel]))
+    print(f"{precision}: dark-hole mean contrast {c64[0]:.4e}, |delta mean| {dmean:.3e}, max |delta C| in hole {dmax:.3e}")
+    if precision in ("fp32", "bf16x6"):
+        assert dmean <= 1e-11
+
+
+def test_candidate_batch_matches_single(ctx):
+    w = W.workload2(n_lambda=2, offsets=(0.0,))
+    ch = build(ctx, w)
+    n_hex = w.meta["n_hex"]
+    t = W.hex_thickness(w.meta["seed"], 7, n_hex, n_cand=3)
+    sags = W.thickness_to_sag(t)
+    ch.set_candidates("fpm", sags)
+    cb, _ = ch.contrast()
+    for k in range(3):
+        single = build(ctx, w)
+        single.set_opt_data("fpm", "sags", sags[k])
+        cs, _ = single.contrast()
+        assert abs(cb[k] - cs[0]) <= 1e-13 * cs[0]
+
+
+def test_opt_data_roundtrip(ctx):
+    w = W.workload2(n_lambda=1, offsets=(0.0,))
+    ch = build(ctx, w)
+    v = np.linspace(0, 1e-7, w.meta["n_hex"])
+    ch.set_opt_data("fpm", "sags", v)
+    assert np.array_equal(ch.get_opt_data("fpm", "sags"), v)
+    ch.set_opt_data("lyot", "position", [2.5])
+    assert ch.get_opt_data("lyot", "position")[0] == 2.5
+
+
+def test_workload1_fresnel_chain(ctx):
+    w = W.workload1()
+    ch = build(ctx, w)
+    oc = OracleChain(w.script, w.arrays)
+    ch.propagate(False)
+    e = rel_rms(ch.field(), oc.field(0, 0))
+    print(f"workload1 (2048^2 PIAA + Fresnel + downsample + MFT) field rel rms {e:.3e}")
+    assert e < 1e-12

@@ -1,0 +1,168 @@
+"""GPU parity of the operator API against the FP64 oracle (csim restatement).
+
+Tolerances: FP64 paths <= 1e-12 relative field RMS (the plain-path gate);
+mixed paths report their error level (the contrast gate is in the chain tests).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from csim_amd import ops
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_rms(a, b):
+    return float(np.sqrt(np.mean(np.abs(a - b) ** 2)) / np.sqrt(np.mean(np.abs(b) ** 2)))
+
+
+def rand_field(rng, rows, cols):
+    return rng.standard_normal((rows, cols)) + 1j * rng.standard_normal((rows, cols))
+
+
+def sym(n, d):
+    return (n, -(n * d - d) / 2.0, d)
+
+
+@pytest.mark.parametrize("nin,nout", [(16, 16), (64, 32), (96, 40), (256, 64)])
+def test_mft_fp64_small_vs_czt(ctx, nin, nout):
+    rng = np.random.default_rng(nin + nout)
+    E = rand_field(rng, nin, nin)
+    dx, lf = 1e-5, 650e-9 * 0.7
+    dxo = lf / (nin * dx) / 3.0
+    gi, go = sym(nin, dx), sym(nout, dxo)
+    ref = oracle.zoomfft(E, (gi[1], gi[1], dx, dx), (nout, go[1], go[1], dxo, dxo), lf)
+    out = ops.to_host(ops.mft(ctx, ops.to_device(E), gi, gi, go, go, lf))[0]
+    assert rel_rms(out, ref) < 1e-12
+
+
+def test_mft_fp64_rectangular_vs_direct(ctx):
+    rng = np.random.default_rng(7)
+    rows, cols, orows, ocols = 80, 48, 36, 20
+    E = rand_field(rng, rows, cols)
+    lf = -1.3e-6
+    gix, giy = (cols, -2.2e-4, 1e-5), (rows, -4e-4, 1e-5)
+    gox, goy = (ocols, -1.1e-3, 1.1e-4), (orows, 0.5e-3, -0.7e-4)
+    ref = oracle.mft_direct(E, (gix[1], giy[1], gix[2], giy[2]), orows, ocols, (gox[1], goy[1], gox[2], goy[2]), lf)
+    out = ops.to_host(ops.mft(ctx, ops.to_device(E), gix, giy, gox, goy, lf))[0]
+    assert rel_rms(out, ref) < 1e-12
+
+
+def test_mft_fp64_1024_to_256_batched(ctx):
+    rng = np.random.default_rng(11)
+    N, M, B = 1024, 256, 3
+    dx = 1e-5
+    lams = np.array([600e-9, 630e-9, 660e-9])
+    lf = lams * 2 * 5e-3 * 10.0
+    Es = [rand_field(rng, N, N) for _ in range(B)]
+    gi = sym(N, dx)
+    outs = []
+    go_list = []
+    dxo = 10.0 * 630e-9 / 8
+    go = sym(M, dxo)
+    out = ops.to_host(ops.mft(ctx, ops.to_device(np.stack(Es)), gi, gi, go, go, lf))
+    for b in range(B):
+        ref = oracle.zoomfft(Es[b], (gi[1], gi[1], dx, dx), (M, go[1], go[1], dxo, dxo), lf[b])
+        e = rel_rms(out[b], ref)
+        print(f"field {b}: rel rms vs Bluestein CZT {e:.3e}")
+        assert e < 1e-12
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32", 2e-6), ("bf16x3", 5e-5), ("bf16x6", 5e-6)])
+def test_mft_mixed_error_level(ctx, prec, tol):
+    rng = np.random.default_rng(3)
+    N, M = 256, 64
+    E = rand_field(rng, N, N)
+    dx, lf = 1e-5, 650e-9
+    dxo = lf / (N * dx) / 2
+    gi, go = sym(N, dx), sym(M, dxo)
+    ref = ops.to_host(ops.mft(ctx, ops.to_device(E), gi, gi, go, go, lf, "fp64"))[0]
+    out = ops.to_host(ops.mft(ctx, ops.to_device(E), gi, gi, go, go, lf, prec))[0]
+    e = rel_rms(out, ref)
+    print(f"{prec}: rel rms vs fp64 {e:.3e}")
+    assert e < tol
+
+
+@pytest.mark.parametrize("n", [8, 64, 256, 2048])
+def test_fft2_vs_numpy(ctx, n):
+    rng = np.random.default_rng(n)
+    E = rand_field(rng, n, n)
+    d = ops.to_device(E)
+    out = ops.to_host(ops.fft2(ctx, d, -1))[0]
+    assert rel_rms(out, np.fft.fft2(E)) < 1e-14
+    back = ops.to_host(ops.fft2(ctx, d, +1))[0]
+    assert rel_rms(back, E) < 1e-14
+
+
+@pytest.mark.parametrize("n", [128, 2048])
+def test_fresnel_vs_oracle(ctx, n):
+    rng = np.random.default_rng(5)
+    E = rand_field(rng, n, n)
+    lam, z = 650e-9, 1.0
+    ap = n * 10e-6 / 2
+    ref = oracle.fresnel_as(E, lam, ap, z)
+    out = ops.to_host(ops.fresnel_as(ctx, ops.to_device(E), [lam], ap, z))[0]
+    e = rel_rms(out, ref)
+    print(f"fresnel n={n}: {e:.3e}")
+    assert e < 1e-12
+
+
+def test_downsample_vs_oracle(ctx):
+    rng = np.random.default_rng(9)
+    n, f, dx = 256, 2, 1e-5
+    E = rand_field(rng, n, n)
+    ref = oracle.downsample(E, f, dx)
+    out = ops.to_host(ops.downsample(ctx, ops.to_device(E), f, dx))[0]
+    assert rel_rms(out, ref) < 1e-14
+
+
+def test_mask_mirror_vs_oracle(ctx):
+    rng = np.random.default_rng(13)
+    n = 128
+    E = rand_field(rng, n, n)
+    m = rng.random((n, n))
+    h = rng.standard_normal((n, n)) * 1e-7
+    lam = 633e-9
+    ref = oracle.mirror_apply(oracle.mask_apply(E, m), h, lam, sign=-1.0)
+    d = ops.to_device(E)
+    ops.apply_mask(ctx, d, ops.real_to_device(m))
+    ops.apply_mirror(ctx, d, ops.real_to_device(h), [lam], sign=-1.0)
+    assert rel_rms(ops.to_host(d)[0], ref) < 1e-14
+
+
+def test_hex_mask_vs_oracle(ctx):
+    import torch
+    S, nsp = 48, 8
+    nsub, hexnum, nhex = ops.hex_rasterize(1, 4, 1.0, 512, 0.9, 0.02, S, nsp, 1.0 / 40, 0.5)
+    rng = np.random.default_rng(17)
+    ncand = 2
+    sags = rng.random((ncand, nhex)) * 0.5e-6
+    lams = np.array([600e-9, 650e-9])
+    d_nsub = torch.from_numpy(np.ascontiguousarray(np.transpose(nsub, (2, 1, 0)))).cuda()
+    d_hex = torch.from_numpy(np.ascontiguousarray(np.transpose(hexnum, (2, 1, 0)))).cuda()
+    d_sags = torch.from_numpy(sags).cuda()
+    out = ops.hex_masks(ctx, d_nsub, d_hex, S, nsp, d_sags, nhex, ncand, lams)
+    got = ops.to_host(out)
+    for c in range(ncand):
+        for li, lam in enumerate(lams):
+            ref = oracle.hex_mask(nsub, hexnum, nsp, sags[c], lam)
+            assert rel_rms(got[c * len(lams) + li], ref) < 1e-14
+
+
+def test_intensity_region_mean(ctx):
+    import torch
+    rng = np.random.default_rng(21)
+    n, per, groups = 32, 3, 2
+    Es = np.stack([rand_field(rng, n, n) for _ in range(per * groups)])
+    I = ops.intensity(ctx, ops.to_device(Es), per)
+    got = np.transpose(I.cpu().numpy(), (0, 2, 1))
+    for g in range(groups):
+        ref = np.zeros((n, n), order="F")
+        for f in range(per):
+            oracle.intensity_accumulate(Es[g * per + f], ref)
+        assert np.allclose(got[g], ref, rtol=1e-15, atol=0)
+    idx = oracle.region_pixels(n, 1.0, 2.0, radii=(1.0, 5.0), angles=(-180, 180))
+    means = ops.region_mean(ctx, I, torch.from_numpy(idx).cuda(), 7.0).cpu().numpy()
+    for g in range(groups):
+        ref = np.asfortranarray(got[g]).ravel(order="F")[idx].mean() / 7.0
+        assert abs(means[g] - ref) <= 1e-14 * abs(ref)
