@@ -74,6 +74,15 @@ def gemm_roofline(prof, precision):
             "issued_frac": round(achieved * PRODUCTS[precision] / peak, 4)}
 
 
+def region_mask(ch):
+    """3-10 lambda/D annulus of the chain's output grid (regionContrast block)."""
+    inf = ch.info()
+    n, d, loD = inf["out_rows"], inf["out_pixel_size"], inf["lambda_over_d"]
+    x = -(n * d - d) / 2 + np.arange(n) * d
+    r = np.sqrt(x[None, :] ** 2 + x[:, None] ** 2) / loD
+    return (r >= 3) & (r <= 10)
+
+
 def cpu_baseline(w, budget_s=12.0):
     """The oracle (FP64 CPU restatement of csim, Bluestein zoomFft, OpenMP) on a
     bounded sample of the same workload's fields."""
@@ -142,17 +151,28 @@ def main(argv=None):
         dt64, c64, prof64 = time_chain(plain, max(3, a.steps // 2), 1)
         steps64 = max(3, a.steps // 2)
         img64 = plain.normalized_image()
-        from oracle.chain import OracleChain
-        sel = OracleChain(w.script, w.arrays, sags=w.meta["sags"]).region_index(
-            (128, -(128 * (W.F_RATIO * 630e-9 / 2.56) - W.F_RATIO * 630e-9 / 2.56) / 2, W.F_RATIO * 630e-9 / 2.56))
+        sel = region_mask(plain)
         out["fp64_plain"] = {"value": round(fields * steps64 / dt64, 3), "unit": "propagations/s",
                              "ms_per_step": round(dt64 / steps64 * 1e3, 4),
                              "roofline": gemm_roofline(prof64, "fp64")}
         out["speedup_vs_fp64"] = round(out["value"] / out["fp64_plain"]["value"], 3)
-        out["contrast_error"] = {"dark_hole_mean_abs": float(abs(c_mix[0] - c64[0])),
-                                 "dark_hole_max_pixel_abs": float(np.max(np.abs(img_mix - img64)[sel])),
-                                 "image_max_pixel_abs": float(np.max(np.abs(img_mix - img64))),
-                                 "dark_hole_mean_fp64": float(c64[0]), "gate": 1e-11}
+        # this workload's random hex FPM leaves the 3-10 l/D region bright (C ~ 1e-2): relative error
+        err = {"workload_region_mean_fp64": float(c64[0]),
+               "workload_region_mean_rel": float(abs(c_mix[0] - c64[0]) / c64[0]),
+               "workload_max_pixel_abs": float(np.max(np.abs(img_mix - img64)[sel]))}
+        # the 1e-11 absolute gate on a real dark hole: same pupil, 9 wavelengths, on-axis, occulter
+        wd = W.workload2_dark()
+        dk64, dkm = build_chain(ctx, wd, "fp64"), build_chain(ctx, wd, a.precision)
+        cd64, _ = dk64.contrast()
+        cdm, _ = dkm.contrast()
+        seld = region_mask(dk64)
+        err.update({"dark_hole_mean_fp64": float(cd64[0]), "dark_hole_mean_abs": float(abs(cdm[0] - cd64[0])),
+                    "dark_hole_max_pixel_abs": float(np.max(np.abs(dkm.normalized_image() - dk64.normalized_image())[seld])),
+                    "gate": 1e-11, "gate_on": "dark-hole mean, 9-lambda on-axis occulter chain"})
+        err["pass"] = err["dark_hole_mean_abs"] <= 1e-11
+        out["contrast_error"] = err
+        for ch in (dk64, dkm, plain):
+            ch.close()
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w)
     print(json.dumps(out))

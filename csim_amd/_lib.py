@@ -63,6 +63,7 @@ SIGNATURES = {
     "csx_chain_get_intensity": (_i, [_p, _i, _p]),
     "csx_chain_contrast": (_i, [_p, C.POINTER(Region), _p, C.POINTER(Report)]),
     "csx_chain_get_normalized_image": (_i, [_p, _i, _p]),
+    "csx_chain_field_profiles": (_i, [_p, _p, _i, _p]),
     "csx_chain_set_profiling": (_i, [_p, _i]),
     "csx_chain_profile_json": (_i, [_p, C.c_char_p, _sz]),
     "csx_mft_workspace_size": (_sz, [Grid, Grid, Grid, Grid, _i, _i]),

@@ -30,6 +30,7 @@ class Chain:
         p = C.c_void_p()
         check(lib().csx_chain_from_script(ctx.handle, script.encode(), C.byref(p)))
         self._p = p
+        ctx.adopt(self)
 
     def close(self):
         if self._p:
@@ -115,6 +116,16 @@ class Chain:
                        float(xlim[0]), float(xlim[1]), float(reference_lambda))
             check(lib().csx_chain_contrast(self._p, C.byref(r), _ptr(out), C.byref(rep)))
         return out, {f: getattr(rep, f) for f, _ in Report._fields_}
+
+    def field_profiles(self, bins, nbins):
+        """Sum of |E|^2 per bin for every field of the last propagation -> (n_fields, nbins).
+        bins: (out_rows, out_cols) int array, -1 outside every bin."""
+        b = np.asfortranarray(np.asarray(bins, dtype=np.int32))
+        inf = self.info()
+        nf = inf["n_candidates"] * inf["n_sources"] * inf["n_lambdas"]
+        out = np.zeros((nf, nbins))
+        check(lib().csx_chain_field_profiles(self._p, _ptr(b), int(nbins), _ptr(out)))
+        return out
 
     def set_profiling(self, on: bool):
         check(lib().csx_chain_set_profiling(self._p, int(bool(on))))

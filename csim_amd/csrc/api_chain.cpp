@@ -159,6 +159,14 @@ int csx_chain_get_normalized_image(csx_chain* c, int cand, double* out) {
   CSX_GUARD_END
 }
 
+int csx_chain_field_profiles(csx_chain* c, const int32_t* bin_of_pixel, int nbins, double* out) {
+  CSX_GUARD_BEGIN
+  CSX_TRY(chk(c));
+  if (!bin_of_pixel || !out) return fail(CSX_ERR_ARG, "null argument");
+  return c->impl.field_profiles(bin_of_pixel, nbins, out);
+  CSX_GUARD_END
+}
+
 int csx_chain_set_profiling(csx_chain* c, int enable) {
   CSX_TRY(chk(c));
   return c->impl.set_profiling(enable != 0);

@@ -82,17 +82,16 @@ int csx_ctx_synchronize(csx_ctx* ctx) {
 
 // ------------------------------------------------------------------- MFT ---
 static int mft_fields(csx_grid in_x, csx_grid in_y, csx_grid out_x, csx_grid out_y, const double* lambda_f,
-                      int batch, const void* d_in, void* d_out, bool c64, std::vector<MftField>& F) {
+                      int batch, const void* d_in, void* d_out, int in_kind, size_t ein, std::vector<MftField>& F) {
   if (batch <= 0 || !lambda_f || !d_in || !d_out) return fail(CSX_ERR_ARG, "csx_mft: bad arguments");
   size_t nin = (size_t)in_x.n * in_y.n, nout = (size_t)out_x.n * out_y.n;
-  size_t ein = c64 ? sizeof(float2) : sizeof(double2);
   F.resize(batch);
   for (int b = 0; b < batch; b++) {
     MftField& f = F[b];
     f.in = static_cast<const char*>(d_in) + b * nin * ein;
-    f.in_c64 = c64;
+    f.in_kind = in_kind;
     f.out = static_cast<char*>(d_out) + b * nout * sizeof(double2);
-    f.out_c64 = 0;
+    f.out_kind = FK_C128;
     f.in_x = in_x; f.in_y = in_y; f.out_x = out_x; f.out_y = out_y;
     f.lambda_f = lambda_f[b];
   }
@@ -101,8 +100,8 @@ static int mft_fields(csx_grid in_x, csx_grid in_y, csx_grid out_x, csx_grid out
 
 size_t csx_mft_workspace_size(csx_grid in_x, csx_grid in_y, csx_grid out_x, csx_grid out_y, int batch,
                               csx_precision p) {
-  size_t e = p == CSX_FP64 ? sizeof(double2) : sizeof(float2);
-  size_t conv = p == CSX_FP64 ? 0 : (size_t)batch * in_x.n * in_y.n * sizeof(float2);
+  size_t e = p == CSX_FP64 ? sizeof(double2) : (p == CSX_FP32 ? sizeof(float2) : (size_t)4 * terms_of(p));
+  size_t conv = p == CSX_FP64 ? 0 : (size_t)batch * in_x.n * in_y.n * e;
   return conv + (size_t)batch * out_y.n * in_x.n * e +
          (size_t)batch * ((size_t)out_y.n * in_y.n + (size_t)out_x.n * in_x.n) * e;
 }
@@ -112,16 +111,22 @@ int csx_mft(csx_ctx* ctx, csx_grid in_x, csx_grid in_y, csx_grid out_x, csx_grid
   CSX_GUARD_BEGIN
   CSX_TRY(use_device(ctx));
   (void)d_work;
-  const bool c64 = p != CSX_FP64;
+  const int T = terms_of(p);
   const void* src = d_in;
-  if (c64) {
-    size_t n = (size_t)batch * in_x.n * in_y.n;
+  int kind = FK_C128;
+  size_t ein = sizeof(double2);
+  size_t n = (size_t)batch * in_x.n * in_y.n;
+  if (p == CSX_FP32) {
     CSX_TRY(ctx->work.reserve(n * sizeof(float2)));
     CSX_TRY(convert_c128_to_c64(d_in, ctx->work.p, n, ctx->stream));
-    src = ctx->work.p;
+    src = ctx->work.p; kind = FK_C64; ein = sizeof(float2);
+  } else if (T) {
+    CSX_TRY(ctx->work.reserve(n * 4 * T));
+    CSX_TRY(split_c128_to_planes(d_in, ctx->work.p, (long)in_x.n * in_y.n, batch, T, ctx->stream));
+    src = ctx->work.p; kind = FK_PLANES; ein = 4 * T;
   }
   std::vector<MftField> F;
-  CSX_TRY(mft_fields(in_x, in_y, out_x, out_y, lambda_f, batch, src, d_out, c64, F));
+  CSX_TRY(mft_fields(in_x, in_y, out_x, out_y, lambda_f, batch, src, d_out, kind, ein, F));
   MftPlan plan;
   CSX_TRY(plan.build(F, p, ctx->stream));
   CSX_TRY(plan.run(ctx->stream));

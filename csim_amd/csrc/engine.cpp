@@ -491,10 +491,12 @@ static int compile_mft(Chain& ch, Compile& c, const std::vector<Geom>& gout_per_
                        const std::vector<MftField>* masks = nullptr) {
   const int nf = ch.nfield_for(c), nl = ch.nlam();
   const Geom gin = c.geom;
-  const bool c64 = ch.precision() != CSX_FP64;
+  const int T = terms_of(ch.precision());
+  const bool c64 = ch.precision() == CSX_FP32;
   double2* in = ch.field_buf(c.buf);
   double2* out = ch.field_buf(1 - c.buf);
-  float2* in64 = static_cast<float2*>(ch.c64_buf());
+  char* conv = static_cast<char*>(ch.c64_buf());
+  const size_t ein = T ? (size_t)4 * T : sizeof(float2);
   std::vector<MftField> F(nf);
   const Geom& g0 = gout_per_lam[0];
   for (int f = 0; f < nf; f++) {
@@ -502,10 +504,10 @@ static int compile_mft(Chain& ch, Compile& c, const std::vector<Geom>& gout_per_
     const Geom& go = gout_per_lam[l];
     if (go.nx != g0.nx || go.ny != g0.ny) return fail(CSX_ERR_UNSUPPORTED, label + ": output size varies with lambda");
     MftField& m = F[f];
-    m.in = c64 ? (const void*)(in64 + (size_t)f * gin.npix()) : (const void*)(in + (size_t)f * gin.npix());
-    m.in_c64 = c64;
+    m.in = (c64 || T) ? (const void*)(conv + (size_t)f * gin.npix() * ein) : (const void*)(in + (size_t)f * gin.npix());
+    m.in_kind = T ? FK_PLANES : (c64 ? FK_C64 : FK_C128);
     m.out = out + (size_t)f * go.npix();
-    m.out_c64 = 0;
+    m.out_kind = FK_C128;
     m.in_x = gx(gin); m.in_y = gy(gin); m.out_x = gx(go); m.out_y = gy(go);
     m.lambda_f = lf_per_lam[l];
     if (masks) {
@@ -519,12 +521,17 @@ static int compile_mft(Chain& ch, Compile& c, const std::vector<Geom>& gout_per_
   const MftPlan* pp = plan.get();
   c.plans.push_back(std::move(plan));
   size_t nconv = (size_t)nf * gin.npix();
+  long npix_in = gin.npix();
   Chain* chp = &ch;
   c.stages.push_back([=](hipStream_t s) {
     KernelTimer* t = chp->timer();
     if (c64) {
       if (t) CSX_TRY(t->begin("convert_c128_to_c64", 0.0, 24.0 * nconv, s));
-      CSX_TRY(convert_c128_to_c64(in, in64, nconv, s));
+      CSX_TRY(convert_c128_to_c64(in, conv, nconv, s));
+      if (t) CSX_TRY(t->end(s));
+    } else if (T) {
+      if (t) CSX_TRY(t->begin("split_c128_to_planes", 0.0, (16.0 + 4.0 * T) * nconv, s));
+      CSX_TRY(split_c128_to_planes(in, conv, npix_in, nf, T, s));
       if (t) CSX_TRY(t->end(s));
     }
     return pp->run(s, t);
@@ -1163,7 +1170,8 @@ int Chain::compile_all() {
   max_field_elems_ = maxe;
   CSX_TRY(bufs_[0].reserve(sizeof(double2) * maxe));
   CSX_TRY(bufs_[1].reserve(sizeof(double2) * maxe));
-  if (prec_ != CSX_FP64) CSX_TRY(c64_.reserve(sizeof(float2) * maxe));
+  if (prec_ == CSX_FP32) CSX_TRY(c64_.reserve(sizeof(float2) * maxe));
+  else if (prec_ != CSX_FP64) CSX_TRY(c64_.reserve((size_t)4 * terms_of(prec_) * maxe));
   // per-field wavelengths (candidate-major, then source, then lambda)
   for (int cal = 0; cal < 2; cal++) {
     Compile dummy;
@@ -1298,6 +1306,23 @@ int Chain::get_normalized_image(int cand, double* out) {
   CSX_HIP(hipMemcpy(out, inten_.as<double>() + (size_t)cand * g.npix(), sizeof(double) * g.npix(),
                     hipMemcpyDeviceToHost));
   for (long i = 0; i < g.npix(); i++) out[i] /= calib_peak_;
+  return CSX_OK;
+}
+
+int Chain::field_profiles(const int32_t* bins, int nbins, double* out) {
+  if (!have_full_ && !have_cal_) return fail(CSX_ERR_STATE, "no propagation has run");
+  bool cal = have_cal_;
+  const Geom& g = cal ? out_geom_cal_ : out_geom_;
+  int b = cal ? out_buf_cal_ : out_buf_;
+  int nf = (cal ? 1 : ncand()) * nsrc() * nlam();
+  DevBuf& db = scratch("profile_bins");
+  DevBuf& dout = scratch("profile_out");
+  CSX_TRY(db.reserve(sizeof(int32_t) * g.npix()));
+  CSX_TRY(dout.reserve(sizeof(double) * (size_t)nf * nbins));
+  CSX_HIP(hipMemcpyAsync(db.p, bins, sizeof(int32_t) * g.npix(), hipMemcpyHostToDevice, stream()));
+  CSX_TRY(bin_sums(field_buf(b), g.npix(), nf, db.as<int32_t>(), nbins, dout.as<double>(), stream()));
+  CSX_HIP(hipMemcpyAsync(out, dout.p, sizeof(double) * (size_t)nf * nbins, hipMemcpyDeviceToHost, stream()));
+  CSX_HIP(hipStreamSynchronize(stream()));
   return CSX_OK;
 }
 

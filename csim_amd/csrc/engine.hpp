@@ -111,6 +111,7 @@ class Chain {
   int get_intensity(int cand, double* out);
   int contrast(const csx_region* r, double* out, csx_report* rep);
   int get_normalized_image(int cand, double* out);
+  int field_profiles(const int32_t* bins, int nbins, double* out);
   int info(csx_chain_info* out);
   int set_profiling(bool on) { timer_.enable(on); return CSX_OK; }
   std::string profile_json() const { return timer_.json(); }

@@ -43,6 +43,27 @@ __device__ __forceinline__ int find_prob(const GemmProb* __restrict__ probs, int
   return lo;
 }
 
+__device__ __forceinline__ unsigned short bf16_rne(float x) {
+  unsigned int u = __float_as_uint(x);
+  unsigned int r = u + 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(r >> 16);
+}
+__device__ __forceinline__ float bf16_val(unsigned short h) { return __uint_as_float(((unsigned int)h) << 16); }
+
+// (re, im) split into `terms` BF16 parts each: x = hi + mid + lo, every part the
+// round-to-nearest-even of what the previous ones leave (from the FP32 value).
+__device__ __forceinline__ void store_planes(unsigned short* base, long off, long stride, int terms,
+                                             double vr, double vi) {
+  float xr = (float)vr, xi = (float)vi;
+  for (int t = 0; t < terms; t++) {
+    unsigned short hr = bf16_rne(xr), hi = bf16_rne(xi);
+    base[(2 * t) * stride + off] = hr;
+    base[(2 * t + 1) * stride + off] = hi;
+    xr -= bf16_val(hr);
+    xi -= bf16_val(hi);
+  }
+}
+
 // Epilogue: alpha, mask, store (shared by every kernel).
 __device__ __forceinline__ void epilogue_store(const GemmProb& P, int m, int n, double re, double im) {
   if (m >= P.M || n >= P.N) return;
@@ -71,7 +92,8 @@ __device__ __forceinline__ void epilogue_store(const GemmProb& P, int m, int n, 
   }
   long off = P.transC ? (long)n * P.ldc + m : (long)m * P.ldc + n;
   if (P.out_kind == OUT_C128) static_cast<double2*>(P.C)[off] = make_double2(vr, vi);
-  else static_cast<float2*>(P.C)[off] = make_float2((float)vr, (float)vi);
+  else if (P.out_kind == OUT_C64) static_cast<float2*>(P.C)[off] = make_float2((float)vr, (float)vi);
+  else store_planes(static_cast<unsigned short*>(P.C), off, P.sC, P.terms, vr, vi);
 }
 
 // ------------------------------------------------------ FP64 / FP32 MFMA GEMM --
@@ -194,14 +216,8 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ unsigned short bf16_rne_bits(float x) {
-  unsigned int u = __float_as_uint(x);
-  unsigned int r = u + 0x7FFFu + ((u >> 16) & 1u);
-  return (unsigned short)(r >> 16);
-}
-__device__ __forceinline__ float bf16_bits_to_float(unsigned short h) {
-  return __uint_as_float(((unsigned int)h) << 16);
-}
+__device__ __forceinline__ unsigned short bf16_rne_bits(float x) { return bf16_rne(x); }
+__device__ __forceinline__ float bf16_bits_to_float(unsigned short h) { return bf16_val(h); }
 
 template <int TERMS>
 __global__ __launch_bounds__(256) void k_gemm_bf16split(const GemmProb* __restrict__ probs, int nprob) {
@@ -333,6 +349,156 @@ int gemm_bf16split(const GemmProb* probs_dev, int n, int total_tiles, int terms,
   return CSX_OK;
 }
 
+// ------------------------------------------------- BF16-plane MFMA GEMM --
+// Both operands arrive as BF16 planes (split once: DFT kernels at plan time,
+// fields by the producing epilogue), so the main loop is 16-byte copies into
+// LDS and v_mfma_f32_16x16x32_bf16; products kept: (a, b) terms with a + b < TERMS.
+// FP32 accumulators are flushed into FP64 after every K slab of 32 when TERMS == 3.
+template <int TERMS>
+__global__ __launch_bounds__(256) void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
+  constexpr int BK = 32, LDK = BK + 8, NP = 2 * TERMS;
+  __shared__ __align__(16) unsigned short As[NP][kTileM][LDK];
+  __shared__ __align__(16) unsigned short Bs[NP][kTileN][LDK];
+
+  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  const GemmProb& P = probs[find_prob(probs, nprob, tile)];
+  const int local = tile - P.tile_start;
+  const int tm = local % P.tiles_m, tn = local / P.tiles_m;
+  const int m0 = tm * kTileM, n0 = tn * kTileN;
+  const unsigned short* __restrict__ A = static_cast<const unsigned short*>(P.A);
+  const unsigned short* __restrict__ B = static_cast<const unsigned short*>(P.B);
+
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int lrow = t >> 2, lk = (t & 3) * 8;     // loader: row, 8-element chunk
+
+  f32x4 accR[2][2], accI[2][2];
+  double dR[2][2][4], dI[2][2][4];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      accR[i][j] = f32x4{0, 0, 0, 0}; accI[i][j] = f32x4{0, 0, 0, 0};
+#pragma unroll
+      for (int v = 0; v < 4; v++) { dR[i][j][v] = 0; dI[i][j][v] = 0; }
+    }
+
+  u32x4 ra[NP], rb[NP];
+  auto load_chunk = [&](const unsigned short* X, long ld, long sp, int rows, int row, int k0, int q) -> u32x4 {
+    u32x4 v = u32x4{0, 0, 0, 0};
+    int k = k0 + lk;
+    if (row < rows) {
+      const unsigned short* p = X + q * sp + (long)row * ld + k;
+      if (k + 8 <= P.K) {
+        v = *reinterpret_cast<const u32x4*>(p);
+      } else {
+        unsigned short h[8];
+#pragma unroll
+        for (int e = 0; e < 8; e++) h[e] = (k + e < P.K) ? p[e] : (unsigned short)0;
+        v = u32x4{h[0] | ((unsigned)h[1] << 16), h[2] | ((unsigned)h[3] << 16), h[4] | ((unsigned)h[5] << 16),
+                  h[6] | ((unsigned)h[7] << 16)};
+      }
+    }
+    return v;
+  };
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < NP; q++) {
+      ra[q] = load_chunk(A, P.lda, P.sA, P.M, m0 + lrow, k0, q);
+      rb[q] = load_chunk(B, P.ldb, P.sB, P.N, n0 + lrow, k0, q);
+    }
+  };
+
+  gload(0);
+  for (int k0 = 0; k0 < P.K; k0 += BK) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NP; q++) {
+      *reinterpret_cast<u32x4*>(&As[q][lrow][lk]) = ra[q];
+      *reinterpret_cast<u32x4*>(&Bs[q][lrow][lk]) = rb[q];
+    }
+    __syncthreads();
+    if (k0 + BK < P.K) gload(k0 + BK);
+    const int fr = lane & 15, fk = 8 * (lane >> 4);
+    u32x4 a[NP][2], b[NP][2];
+#pragma unroll
+    for (int q = 0; q < NP; q++)
+#pragma unroll
+      for (int s2 = 0; s2 < 2; s2++) {
+        a[q][s2] = *reinterpret_cast<const u32x4*>(&As[q][wm * 32 + s2 * 16 + fr][fk]);
+        b[q][s2] = *reinterpret_cast<const u32x4*>(&Bs[q][wn * 32 + s2 * 16 + fr][fk]);
+      }
+#pragma unroll
+    for (int ta = 0; ta < TERMS; ta++)
+#pragma unroll
+      for (int tb = 0; tb < TERMS - ta; tb++)
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+          for (int j = 0; j < 2; j++) {
+            bf16x8 ar = __builtin_bit_cast(bf16x8, a[2 * ta][i]);
+            bf16x8 ai = __builtin_bit_cast(bf16x8, a[2 * ta + 1][i]);
+            bf16x8 ain = __builtin_bit_cast(bf16x8, a[2 * ta + 1][i] ^ u32x4{0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u});
+            bf16x8 br = __builtin_bit_cast(bf16x8, b[2 * tb][j]);
+            bf16x8 bi = __builtin_bit_cast(bf16x8, b[2 * tb + 1][j]);
+            accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ar, br, accR[i][j], 0, 0, 0);
+            accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ar, bi, accI[i][j], 0, 0, 0);
+            accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ain, bi, accR[i][j], 0, 0, 0);
+            accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ai, br, accI[i][j], 0, 0, 0);
+          }
+    if (TERMS >= 3) {
+#pragma unroll
+      for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+#pragma unroll
+          for (int v = 0; v < 4; v++) { dR[i][j][v] += accR[i][j][v]; dI[i][j][v] += accI[i][j][v]; }
+          accR[i][j] = f32x4{0, 0, 0, 0}; accI[i][j] = f32x4{0, 0, 0, 0};
+        }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        int m = m0 + wm * 32 + i * 16 + 4 * (lane >> 4) + v;
+        int n = n0 + wn * 32 + j * 16 + (lane & 15);
+        epilogue_store(P, m, n, dR[i][j][v] + (double)accR[i][j][v], dI[i][j][v] + (double)accI[i][j][v]);
+      }
+}
+
+int gemm_bf16planes(const GemmProb* probs_dev, int n, int total_tiles, int terms, hipStream_t s) {
+  if (total_tiles <= 0) return CSX_OK;
+  if (terms == 2)
+    hipLaunchKernelGGL(k_gemm_planes<2>, dim3(total_tiles), dim3(256), 0, s, probs_dev, n);
+  else if (terms == 3)
+    hipLaunchKernelGGL(k_gemm_planes<3>, dim3(total_tiles), dim3(256), 0, s, probs_dev, n);
+  else
+    return fail(CSX_ERR_ARG, "gemm_bf16planes: terms must be 2 or 3");
+  CSX_HIP(hipGetLastError());
+  return CSX_OK;
+}
+
+__global__ void k_split_planes(const double2* __restrict__ in, unsigned short* __restrict__ out, long npix,
+                               long total, int terms) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    long f = i / npix, p = i - f * npix;
+    double2 v = in[i];
+    store_planes(out + f * 2 * terms * npix, p, npix, terms, v.x, v.y);
+  }
+}
+int split_c128_to_planes(const void* in, void* out, long npix, int nfield, int terms, hipStream_t s) {
+  long total = npix * nfield;
+  if (total <= 0) return CSX_OK;
+  int blocks = (int)std::min<long>(8192, (total + 255) / 256);
+  hipLaunchKernelGGL(k_split_planes, dim3(blocks), dim3(256), 0, s, (const double2*)in, (unsigned short*)out, npix,
+                     total, terms);
+  CSX_HIP(hipGetLastError());
+  return CSX_OK;
+}
+
 // ---------------------------------------------------------------- DFT kernels --
 // M[o][i] = exp(-2 pi i t), t = (xo0 + o dxo)(xi0 + i dxi) / lf in cycles, FP64,
 // reduced to [-1/2, 1/2] before sincospi so the argument keeps full accuracy.
@@ -346,7 +512,8 @@ __global__ void k_dft_matrices(const DftSpec* __restrict__ specs, int nspec) {
     double f = t - rint(t);
     double sn, cs;
     sincospi(-2.0 * f, &sn, &cs);
-    if (S.out_c64) static_cast<float2*>(S.out)[e] = make_float2((float)cs, (float)sn);
+    if (S.terms > 0) store_planes(static_cast<unsigned short*>(S.out), e, total, S.terms, cs, sn);
+    else if (S.out_c64) static_cast<float2*>(S.out)[e] = make_float2((float)cs, (float)sn);
     else static_cast<double2*>(S.out)[e] = make_double2(cs, sn);
   }
 }

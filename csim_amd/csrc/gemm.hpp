@@ -7,7 +7,7 @@
 namespace csx {
 
 enum MaskKind : int { MASK_NONE = 0, MASK_COMPLEX = 1, MASK_REAL_1PI = 2, MASK_REAL = 3 };
-enum OutKind : int { OUT_C128 = 0, OUT_C64 = 1 };
+enum OutKind : int { OUT_C128 = 0, OUT_C64 = 1, OUT_PLANES = 2 };
 
 // One problem of a grouped launch:  C(m, n) = alpha * sum_k A(m, k) B(n, k),
 // then the epilogue mask.  A and B are K-contiguous (row-major M x K, N x K).
@@ -26,6 +26,10 @@ struct GemmProb {
   long ldm;
   int mm0, mn0, mM, mN;
   int tiles_m, tiles_n, tile_start;
+  // split-BF16 planes: plane q = 2 t + c (term t, c = re/im) of an operand sits
+  // q * sX bf16 elements after its base pointer
+  long sA, sB, sC;
+  int terms;
 };
 
 // Tile of the FP64 / FP32 MFMA kernels (64 x 64 outputs, 4 waves of 32 x 32).
@@ -40,6 +44,12 @@ int gemm_f32(const GemmProb* probs_dev, int n, int total_tiles, hipStream_t s);
 // Split-BF16 products: terms = 2 (hi, lo: 3 products) or 3 (hi, mid, lo: 6 products);
 // kblock_f64 > 0 flushes FP32 accumulators into FP64 every kblock_f64 of K.
 int gemm_bf16split(const GemmProb* probs_dev, int n, int total_tiles, int terms, hipStream_t s);
+// Operands already split into BF16 planes (A, B K-contiguous per plane, lda/ldb and
+// plane strides multiples of 8); output complex128/complex64 or planes (OUT_PLANES).
+int gemm_bf16planes(const GemmProb* probs_dev, int n, int total_tiles, int terms, hipStream_t s);
+// complex128 -> `terms` BF16 planes per complex element (hi, mid, lo of re and im),
+// field f's planes at out + f * 2 * terms * npix, plane q at + q * npix.
+int split_c128_to_planes(const void* in, void* out, long npix, int nfield, int terms, hipStream_t s);
 
 // DFT kernel matrix M[o][i] = exp(-2 pi i (xo0 + o dxo)(xi0 + i dxi) / lf), one
 // per (problem) entry, FP64 phase argument reduced to [-1/2, 1/2] cycles.
@@ -48,6 +58,7 @@ struct DftSpec {
   int nout, nin;
   double xo0, dxo, xi0, dxi, lf;
   int out_c64;
+  int terms;          // > 0: write `terms` BF16 planes (plane stride nout * nin) instead
 };
 int dft_matrices(const DftSpec* specs_dev, int n, int max_elems, hipStream_t s);
 

@@ -9,12 +9,14 @@
 
 namespace csx {
 
+int terms_of(csx_precision p) { return p == CSX_BF16X3 ? 2 : (p == CSX_BF16X6 ? 3 : 0); }
+
 int launch_gemm(const GemmProb* probs_dev, int n, int tiles, csx_precision p, hipStream_t s) {
   switch (p) {
     case CSX_FP64: return gemm_f64(probs_dev, n, tiles, s);
     case CSX_FP32: return gemm_f32(probs_dev, n, tiles, s);
-    case CSX_BF16X3: return gemm_bf16split(probs_dev, n, tiles, 2, s);
-    case CSX_BF16X6: return gemm_bf16split(probs_dev, n, tiles, 3, s);
+    case CSX_BF16X3: return gemm_bf16planes(probs_dev, n, tiles, 2, s);
+    case CSX_BF16X6: return gemm_bf16planes(probs_dev, n, tiles, 3, s);
   }
   return fail(CSX_ERR_ARG, "unknown precision");
 }
@@ -34,14 +36,19 @@ int MftPlan::build(const std::vector<MftField>& fields, csx_precision p, hipStre
   nf_ = (int)fields.size();
   flops_ = flops1_ = flops2_ = bytes1_ = bytes2_ = 0;
   if (nf_ == 0) return CSX_OK;
-  const bool c64 = p != CSX_FP64;
-  const size_t esz = c64 ? sizeof(float2) : sizeof(double2);
+  const int T = terms_of(p);
+  const bool c64 = p == CSX_FP32;
+  // bytes per complex element of the working operands (DFT kernels, stage-1 output)
+  const size_t esz = T ? (size_t)4 * T : (c64 ? sizeof(float2) : sizeof(double2));
+  const int want_in = T ? FK_PLANES : (c64 ? FK_C64 : FK_C128);
   for (const auto& f : fields) {
     if (f.in_x.n <= 0 || f.in_y.n <= 0 || f.out_x.n <= 0 || f.out_y.n <= 0)
       return fail(CSX_ERR_ARG, "mft: empty grid");
     if (f.lambda_f == 0) return fail(CSX_ERR_ARG, "mft: lambda*f is zero");
     if (!f.in || !f.out) return fail(CSX_ERR_ARG, "mft: null field pointer");
-    if (c64 != (f.in_c64 != 0)) return fail(CSX_ERR_ARG, "mft: input type does not match the precision");
+    if (f.in_kind != want_in) return fail(CSX_ERR_ARG, "mft: input storage does not match the precision");
+    if (T && (f.in_x.n % 8 || f.in_y.n % 8 || f.out_y.n % 8))
+      return fail(CSX_ERR_UNSUPPORTED, "mft: split-BF16 precisions need input sizes and output rows divisible by 8");
   }
   // distinct DFT kernels
   std::map<DftKey, size_t> index;     // key -> element offset
@@ -71,6 +78,7 @@ int MftPlan::build(const std::vector<MftField>& fields, csx_precision p, hipStre
     d.nout = k.nout; d.nin = k.nin;
     d.xo0 = k.xo0; d.dxo = k.dxo; d.xi0 = k.xi0; d.dxi = k.dxi; d.lf = k.lf;
     d.out_c64 = c64 ? 1 : 0;
+    d.terms = T;
     specs.push_back(d);
     maxe = std::max(maxe, k.nout * k.nin);
   }
@@ -87,25 +95,28 @@ int MftPlan::build(const std::vector<MftField>& fields, csx_precision p, hipStre
   std::vector<GemmProb> p1(nf_), p2(nf_);
   for (int f = 0; f < nf_; f++) {
     const MftField& F = fields[f];
+    const long npix_in = (long)F.in_x.n * F.in_y.n, nt = (long)F.out_y.n * F.in_x.n;
     GemmProb& a = p1[f];
     std::memset(&a, 0, sizeof(a));
-    a.A = F.in; a.lda = F.in_y.n;
-    a.B = static_cast<char*>(dft_.p) + offAy[f] * esz; a.ldb = F.in_y.n;
-    a.C = static_cast<char*>(tbuf_.p) + offT[f] * esz; a.ldc = F.in_x.n;
+    a.A = F.in; a.lda = F.in_y.n; a.sA = npix_in;
+    a.B = static_cast<char*>(dft_.p) + offAy[f] * esz; a.ldb = F.in_y.n; a.sB = (long)F.out_y.n * F.in_y.n;
+    a.C = static_cast<char*>(tbuf_.p) + offT[f] * esz; a.ldc = F.in_x.n; a.sC = nt;
     a.M = F.in_x.n; a.N = F.out_y.n; a.K = F.in_y.n;
     a.transC = 1;
-    a.out_kind = c64 ? OUT_C64 : OUT_C128;
+    a.out_kind = T ? OUT_PLANES : (c64 ? OUT_C64 : OUT_C128);
+    a.terms = T;
     a.alpha_re = 1.0; a.alpha_im = 0.0;
     a.mask_kind = MASK_NONE;
 
     GemmProb& b = p2[f];
     std::memset(&b, 0, sizeof(b));
-    b.A = static_cast<char*>(dft_.p) + offBx[f] * esz; b.lda = F.in_x.n;
-    b.B = a.C; b.ldb = F.in_x.n;
-    b.C = F.out; b.ldc = F.out_y.n;
+    b.A = static_cast<char*>(dft_.p) + offBx[f] * esz; b.lda = F.in_x.n; b.sA = (long)F.out_x.n * F.in_x.n;
+    b.B = a.C; b.ldb = F.in_x.n; b.sB = nt;
+    b.C = F.out; b.ldc = F.out_y.n; b.sC = (long)F.out_x.n * F.out_y.n;
     b.M = F.out_x.n; b.N = F.out_y.n; b.K = F.in_x.n;
     b.transC = 0;
-    b.out_kind = F.out_c64 ? OUT_C64 : OUT_C128;
+    b.out_kind = F.out_kind == FK_PLANES ? OUT_PLANES : (F.out_kind == FK_C64 ? OUT_C64 : OUT_C128);
+    b.terms = T;
     // s = dx dy / (i lf) = -i dx dy / lf
     b.alpha_re = 0.0; b.alpha_im = -(F.in_x.dx * F.in_y.dx) / F.lambda_f;
     b.mask_kind = F.mask_kind;
@@ -116,7 +127,8 @@ int MftPlan::build(const std::vector<MftField>& fields, csx_precision p, hipStre
     double f1 = 8.0 * (double)F.in_x.n * F.in_y.n * F.out_y.n;
     double f2 = 8.0 * (double)F.out_x.n * F.in_x.n * F.out_y.n;
     flops1_ += f1; flops2_ += f2; flops_ += f1 + f2;
-    double ein = F.in_c64 ? 8.0 : 16.0, eout = F.out_c64 ? 8.0 : 16.0;
+    double ein = F.in_kind == FK_PLANES ? 4.0 * T : (F.in_kind == FK_C64 ? 8.0 : 16.0);
+    double eout = F.out_kind == FK_PLANES ? 4.0 * T : (F.out_kind == FK_C64 ? 8.0 : 16.0);
     bytes1_ += ein * F.in_x.n * F.in_y.n + (double)esz * F.out_y.n * F.in_x.n;
     bytes2_ += (double)esz * F.out_y.n * F.in_x.n + eout * F.out_x.n * F.out_y.n;
   }

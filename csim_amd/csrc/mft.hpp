@@ -37,10 +37,14 @@ class KernelTimer {
 //   E_out = dx_in dy_in / (i lf) * A_y * E_in * B_x^T
 // computed y-first:  T^T = (X A_y^T)^T  then  X_out = B_x T  with X = E^T
 // (the column-major field read as a row-major matrix).
+// Element storage of a field operand.
+enum FieldKind : int { FK_C128 = 0, FK_C64 = 1, FK_PLANES = 2 };
+int terms_of(csx_precision p);   // BF16 terms per value (0 for FP64 / FP32)
+
 struct MftField {
-  const void* in = nullptr;    // complex128, or complex64 when in_c64
-  void* out = nullptr;         // complex128, or complex64 when out_c64
-  int in_c64 = 0, out_c64 = 0;
+  const void* in = nullptr;    // FieldKind in_kind: complex128, complex64 or BF16 planes
+  void* out = nullptr;         // FieldKind out_kind
+  int in_kind = FK_C128, out_kind = FK_C128;
   csx_grid in_x{}, in_y{}, out_x{}, out_y{};
   double lambda_f = 0;
   // optional mask applied to the output (E_out(r, c) *= mask(r - row0, c - col0))

@@ -102,17 +102,17 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// sin and cos of a large FP64 argument (|x| up to ~1e9 rad) with the argument
-// reduced by 2 pi in extended precision: k = rint(x / 2pi); the product k * C1
-// and its exact rounding error (fma) are removed, then k * C2, C1 + C2 = 2 pi
-// to ~1e-32.  The reduced |r| <= pi keeps the full accuracy of sincos.
+// sin and cos of a large FP64 argument (|x| up to ~1e9 rad): x is reduced by
+// 2 pi = C1 + C2 (C1 the FP64 nearest 2 pi, C2 the next 53 bits) with fused
+// multiply-adds, r = (x - k C1) - k C2, each step rounded once; |r| <= pi then
+// keeps the full accuracy of sincos.
 __device__ __forceinline__ void sincos_reduced(double x, double* s, double* c) {
+#pragma clang fp contract(off)
   const double C1 = 6.283185307179586232;           // 2 pi rounded to FP64
   const double C2 = 2.4492935982947064e-16;         // 2 pi - C1
   double k = rint(x * 0.15915494309189535);
-  double p = k * C1;
-  double e = __fma_rn(k, C1, -p);                    // exact error of k * C1
-  double r = ((x - p) - e) - k * C2;
+  double r = __fma_rn(-k, C1, x);                    // x - k C1, exact product, one rounding
+  r = __fma_rn(-k, C2, r);
   sincos(r, s, c);
 }
 
@@ -462,6 +462,39 @@ int region_mean_groups(const double* I, long npix, int ngroups, const int32_t* i
                        const double* peak_dev, double peak, double* out, hipStream_t s) {
   if (nidx <= 0) return fail(CSX_ERR_ARG, "region_mean: empty region");
   hipLaunchKernelGGL(k_region_mean, dim3(ngroups), dim3(256), 0, s, I, npix, idx, nidx, peak_dev, peak, out);
+  CSX_HIP(hipGetLastError());
+  return CSX_OK;
+}
+
+// ------------------------------------------------------------ bin profiles --
+// out[f][b] = sum over pixels p with bin[p] == b of |E_f[p]|^2, fixed summation
+// order (per-thread partial sums, then a sequential sum over threads).
+constexpr int kBinThreads = 128;
+__global__ __launch_bounds__(kBinThreads) void k_bin_sums(const double2* __restrict__ E, long npix,
+                                                          const int32_t* __restrict__ bin, int nbins,
+                                                          double* __restrict__ out) {
+  extern __shared__ double part[];            // [kBinThreads][nbins]
+  const double2* F = E + blockIdx.x * npix;
+  double* mine = part + threadIdx.x * nbins;
+  for (int b = 0; b < nbins; b++) mine[b] = 0.0;
+  for (long p = threadIdx.x; p < npix; p += blockDim.x) {
+    int b = bin[p];
+    if (b >= 0 && b < nbins) {
+      double2 e = F[p];
+      mine[b] += e.x * e.x + e.y * e.y;
+    }
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < nbins; b += blockDim.x) {
+    double acc = 0;
+    for (int t = 0; t < (int)blockDim.x; t++) acc += part[t * nbins + b];
+    out[(long)blockIdx.x * nbins + b] = acc;
+  }
+}
+int bin_sums(const double2* E, long npix, int nfield, const int32_t* bin, int nbins, double* out, hipStream_t s) {
+  if (nbins <= 0 || nbins > 64) return fail(CSX_ERR_ARG, "bin_sums: nbins must be in [1, 64]");
+  size_t lds = sizeof(double) * kBinThreads * nbins;
+  hipLaunchKernelGGL(k_bin_sums, dim3(nfield), dim3(kBinThreads), lds, s, E, npix, bin, nbins, out);
   CSX_HIP(hipGetLastError());
   return CSX_OK;
 }

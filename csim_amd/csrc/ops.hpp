@@ -39,6 +39,9 @@ int max_groups(const double* I, long npix, int ngroups, double* out, hipStream_t
 int region_mean_groups(const double* I, long npix, int ngroups, const int32_t* idx, int nidx,
                        const double* peak_dev, double peak, double* out, hipStream_t s);
 
+// out[f][b] = sum_{p : bin[p] == b} |E_f[p]|^2 over nfield fields (bins < 0 skipped)
+int bin_sums(const double2* E, long npix, int nfield, const int32_t* bin, int nbins, double* out, hipStream_t s);
+
 // Centred copy: dst (drows x dcols) <- src (srows x scols) placed at the centre
 // offsets csim uses (rows/2 - srows/2); zero elsewhere (pad) or crop.
 int embed_center(const double2* src, int srows, int scols, double2* dst, int drows, int dcols,

@@ -7,7 +7,9 @@ from / to numpy arrays of shape ``(rows, cols)`` (or ``(batch, rows, cols)``).
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -19,23 +21,36 @@ def _torch():
     return torch
 
 
+_live_contexts = weakref.WeakSet()
+
+
 class Context:
-    """One HIP device and stream (csx_ctx)."""
+    """One HIP device and stream (csx_ctx).  Objects created on a context
+    (chains, surrogates) register with it and are closed before it."""
 
     def __init__(self, device: int = 0):
         p = C.c_void_p()
         check(lib().csx_ctx_create(device, 0, C.byref(p)))
         self._p = p
         self.device = device
+        self._children = weakref.WeakSet()
+        _live_contexts.add(self)
 
     @property
     def handle(self):
+        if not self._p:
+            raise RuntimeError("context is closed")
         return self._p
+
+    def adopt(self, obj):
+        self._children.add(obj)
 
     def synchronize(self):
         check(lib().csx_ctx_synchronize(self._p))
 
     def close(self):
+        for c in list(self._children):
+            c.close()
         if self._p:
             lib().csx_ctx_destroy(self._p)
             self._p = None
@@ -43,6 +58,16 @@ class Context:
     def __del__(self):
         try:
             self.close()
+        except Exception:
+            pass
+
+
+@atexit.register
+def _close_all():
+    # release device objects while the HIP runtime is still up
+    for c in list(_live_contexts):
+        try:
+            c.close()
         except Exception:
             pass
 

@@ -248,6 +248,27 @@ def workload2(n=1024, n_sub_pix=16, n_lambda=9, offsets=(0.0, 2.0, 6.0), layout=
                           "n_hex": n_hex, "hex": hopt})
 
 
+def workload2_dark(n=1024, n_lambda=9):
+    """Workload 2's pupil, wavelengths and image with the on-axis source only and the
+    3 lambda/D occulting disk in place of the random hex mask: a real dark hole
+    (mean contrast ~1e-6), on which the 1e-11 absolute-contrast gate is evaluated."""
+    seed = BASE_SEED + 2
+    pix = PIX * 1024 / n
+    r_ap = 500 * n // 1024
+    beam_r = r_ap * pix
+    lams = poly_lambdas(n_lambda)
+    lam_ref = 630e-9
+    script = (_telescope(2 * beam_r) + _efield(n, pix, beam_r, lams, [(1.0, 0.0, 0.0)]) +
+              "start\n\tcoronagraph\n\tstart\n\t\tmask\n\t\tname:pupil\n\tend\n"
+              "\tstart\n\t\tmirror\n\t\tname:aberration\n\tend\n" +
+              _fpm_block("doubleFraunhoferFPM", lam_ref, occulter=3.0) + "\n" + _tail(lam_ref))
+    arrays = {("pupil", "filename"): pupil_amplitude(n, r_ap, 400 * n / 1024),
+              ("aberration", "filename"): aberration_sag(BASE_SEED + 0, n),
+              ("lyot", "filename"): circle(n, 450 * n / 1024)}
+    return Workload(2, f"{n_lambda} lambda on-axis, occulter 3 l/D, {n}^2 pupil (dark hole)", script, arrays,
+                    meta={"seed": seed, "n": n, "fields": n_lambda, "lambdas": lams})
+
+
 def workload3(n=1024, n_sub_pix=16, n_lambda=9):
     """One pupil size of the DE batch (on-axis, hex FPM); 512^2 halves every radius."""
     w = workload2(n=n, n_sub_pix=n_sub_pix, n_lambda=n_lambda, offsets=(0.0,))

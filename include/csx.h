@@ -136,6 +136,12 @@ int csx_chain_contrast(csx_chain* chain, const csx_region* region, double* contr
 /* Normalised image I / max(I_cal) of one candidate (after csx_chain_contrast). */
 int csx_chain_get_normalized_image(csx_chain* chain, int cand, double* out);
 
+/* Per-field binned intensity of the last propagation: out[f][b] = sum of |E_f|^2
+ * over the output pixels p with bin_of_pixel[p] == b (column-major pixel order,
+ * bins < 0 ignored, nbins <= 64), for every field f = (cand * n_sources + s) *
+ * n_lambdas + l.  Annulus profiles of this kind are the surrogate's targets. */
+int csx_chain_field_profiles(csx_chain* chain, const int32_t* bin_of_pixel, int nbins, double* out);
+
 /* Measurement: with profiling on, every contraction launch (and its input
  * conversion) is bracketed by HIP events on the chain's stream; the JSON
  * report gives per kernel {launches, ms_total, flops_total, bytes_total}

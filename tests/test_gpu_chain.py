@@ -61,22 +61,38 @@ def test_workload2_reduced_fp64(ctx):
     assert abs(c[0] - c_ref) <= 1e-12 * c_ref
 
 
-@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "bf16x6"])
-def test_workload2_mixed_contrast_error(ctx, precision):
-    w = W.workload2()
+def _dark_hole_errors(ctx, w, precision):
     ref = build(ctx, w, "fp64")
     c64, _ = ref.contrast()
     img64 = ref.normalized_image()
     mix = build(ctx, w, precision)
     cm, _ = mix.contrast()
     imgm = mix.normalized_image()
-    sel = OracleChain(w.script, w.arrays, sags=w.meta["sags"]).region_index(
-        (128, -(128 * (W.F_RATIO * 630e-9 / 2.56) - W.F_RATIO * 630e-9 / 2.56) / 2, W.F_RATIO * 630e-9 / 2.56))
-    dmean = abs(cm[0] - c64[0])
-    dmax = float(np.max(np.abs(imgm - img64)[sel]))
-    print(f"{precision}: dark-hole mean contrast {c64[0]:.4e}, |delta mean| {dmean:.3e}, max |delta C| in hole {dmax:.3e}")
+    inf = ref.info()
+    n, d = inf["out_rows"], inf["out_pixel_size"]
+    sel = OracleChain(w.script, w.arrays, sags=w.meta.get("sags")).region_index((n, -(n * d - d) / 2, d))
+    return c64[0], abs(cm[0] - c64[0]), float(np.max(np.abs(imgm - img64)[sel]))
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "bf16x6"])
+def test_mixed_contrast_gate_on_dark_hole(ctx, precision):
+    """1e-11 absolute contrast gate (3-10 lambda/D dark-hole mean) on the 9-wavelength
+    occulter chain; the per-pixel maximum is reported."""
+    w = W.workload2_dark()
+    c, dmean, dmax = _dark_hole_errors(ctx, w, precision)
+    print(f"{precision}: dark hole mean C {c:.4e}; |delta mean| {dmean:.3e}; max |delta C| per pixel {dmax:.3e}")
     if precision in ("fp32", "bf16x6"):
         assert dmean <= 1e-11
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x6"])
+def test_workload2_mixed_relative_error(ctx, precision):
+    """The random 217-hex FPM leaves the region bright (C ~ 1e-2): the mixed path's
+    relative error of the region mean is what is bounded there."""
+    w = W.workload2()
+    c, dmean, dmax = _dark_hole_errors(ctx, w, precision)
+    print(f"{precision}: region mean C {c:.4e}; relative error {dmean / c:.3e}; max |delta C| {dmax:.3e}")
+    assert dmean / c < 1e-6
 
 
 def test_candidate_batch_matches_single(ctx):

@@ -166,3 +166,28 @@ def test_intensity_region_mean(ctx):
     for g in range(groups):
         ref = np.asfortranarray(got[g]).ravel(order="F")[idx].mean() / 7.0
         assert abs(means[g] - ref) <= 1e-14 * abs(ref)
+
+
+def test_hex_resp_objective_known_answer_and_batch(ctx):
+    """The reference's GPU path (its zgemm, optObjective.cpp:46) on MFMA: the shipped
+    response matrix and DE optimum give the logged 1.35012e-10."""
+    import ctypes as C
+    import os
+    import torch
+    from csim_amd._lib import check, lib
+    fx = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "mono_kat.npz"))
+    M = fx["M"]                                           # 636 x 500
+    rng = np.random.default_rng(23)
+    ncand = 37
+    X = np.concatenate([fx["best"][None, :], rng.uniform(-5e-8, 5e-8, (ncand - 1, 499))])   # candidate-major
+    d_M = torch.from_numpy(np.ascontiguousarray(M.T)).cuda()    # column-major 636 x 500
+    d_X = torch.from_numpy(np.ascontiguousarray(X)).cuda()
+    d_o = torch.zeros(ncand, dtype=torch.float64, device="cuda")
+    lam = np.array([float(fx["wavelength"])])
+    check(lib().csx_hex_resp_objective(ctx.handle, C.c_void_p(d_M.data_ptr()), 636, 500, 1,
+                                       lam.ctypes.data_as(C.c_void_p), 1.0, float(fx["calib"]),
+                                       C.c_void_p(d_X.data_ptr()), 499, ncand, 0, C.c_void_p(d_o.data_ptr()), 0))
+    got = d_o.cpu().numpy()
+    ref = oracle.hex_resp_objective(M[:, :, None], lam, 1.0, float(fx["calib"]), X.T)
+    assert abs(got[0] - 1.3501189541e-10) < 1e-19
+    assert np.max(np.abs(got - ref) / ref) < 1e-12
