@@ -169,6 +169,35 @@ Geom geom_offset(int nx, int ny, double dx, double dy) {
 static csx_grid gx(const Geom& g) { return csx_grid{g.nx, g.x0, g.dx}; }
 static csx_grid gy(const Geom& g) { return csx_grid{g.ny, g.y0, g.dy}; }
 
+// ======================================================= fused pointwise ==
+// Consecutive pointwise elements (and the point-source initialisation) are
+// applied in one pass over the fields; any other element flushes them first.
+static int flush_pointwise(Chain& ch, Compile& c) {
+  if (c.pending.empty()) return CSX_OK;
+  if (c.in_conv) return fail(CSX_ERR_STATE, "pointwise factors pending on converted fields");
+  PwChain pc{};
+  pc.nops = (int)c.pending.size();
+  for (int i = 0; i < pc.nops; i++) pc.op[i] = c.pending[i];
+  pc.rows = c.geom.ny;
+  pc.cols = c.geom.nx;
+  pc.pixel_scale = ch.efield().pixelScale;
+  pc.src_params = ch.source_params(c.calibrating);
+  pc.lambdas = c.calibrating ? ch.scratch("lam_cal").as<double>() : ch.lambdas_dev();
+  double2* E = ch.field_buf(c.buf);
+  int nf = ch.nfield_for(c);
+  std::string name = "pointwise x" + std::to_string(pc.nops);
+  c.stages.push_back([=](hipStream_t s) { return pointwise_chain(E, nf, pc, s); });
+  c.stage_names.push_back(name);
+  c.pending.clear();
+  return CSX_OK;
+}
+
+static int push_pointwise(Chain& ch, Compile& c, PwOp op) {
+  if ((int)c.pending.size() >= kMaxPwOps) CSX_TRY(flush_pointwise(ch, c));
+  c.pending.push_back(op);
+  return CSX_OK;
+}
+
 // ================================================================ elements ==
 bool Element::set_common(const std::string& key, const std::string& arg) {
   std::string a = trim(arg);
@@ -282,12 +311,7 @@ struct MaskElem : Element {
     if (map.host.empty()) return fail(CSX_ERR_STATE, "mask '" + name + "' has no data");
     const double* m = nullptr;
     CSX_TRY(map.device_for(ch, c.geom, &m));
-    double2* E = ch.field_buf(c.buf);
-    long npix = c.geom.npix();
-    int nf = ch.nfield_for(c);
-    c.stages.push_back([=](hipStream_t s) { return k_apply_mask(E, npix, nf, m, s); });
-    c.stage_names.push_back("mask " + name);
-    return CSX_OK;
+    return push_pointwise(ch, c, PwOp{PW_MASK_1PI, m, 0.0, 1});
   }
 };
 
@@ -315,18 +339,7 @@ struct MirrorElem : Element {
     if (map.host.empty()) return fail(CSX_ERR_STATE, "mirror '" + name + "' has no data");
     const double* h = nullptr;
     CSX_TRY(map.device_for(ch, c.geom, &h));
-    double2* E = ch.field_buf(c.buf);
-    long npix = c.geom.npix();
-    int nf = ch.nfield_for(c);
-    double sg = sign;
-    Chain* chp = &ch;
-    bool cal = c.calibrating;
-    c.stages.push_back([=](hipStream_t s) {
-      const double* l = cal ? chp->scratch("lam_cal").as<double>() : chp->lambdas_dev();
-      return k_apply_mirror(E, npix, nf, h, sg, l, s);
-    });
-    c.stage_names.push_back("mirror " + name);
-    return CSX_OK;
+    return push_pointwise(ch, c, PwOp{PW_MIRROR, h, sign, 1});
   }
 };
 
@@ -381,13 +394,8 @@ struct ComplexMaskElem : Element {
       }
     CSX_TRY(dev.reserve(sizeof(double2) * full.size()));
     CSX_HIP(hipMemcpy(dev.p, full.data(), sizeof(double2) * full.size(), hipMemcpyHostToDevice));
-    double2* E = ch.field_buf(c.buf);
-    const double2* P = dev.as<double2>();
-    long npix = g.npix();
     int nf = ch.nfield_for(c);
-    c.stages.push_back([=](hipStream_t s) { return k_apply_cfactor(E, npix, nf, P, nf, s); });
-    c.stage_names.push_back("complexMask " + name);
-    return CSX_OK;
+    return push_pointwise(ch, c, PwOp{PW_CFACTOR, dev.p, 0.0, nf});
   }
 };
 
@@ -406,6 +414,7 @@ struct FresnelElem : Element {
     return unknown_key(type, k);
   }
   int compile(Chain& ch, Compile& c, int index) override {
+    CSX_TRY(flush_pointwise(ch, c));
     Element* prev = ch.elem_at(index - 1);
     Element* next = ch.elem_at(index + 1);
     if (!prev || !next) return fail(CSX_ERR_STATE, "fresnelPropagator needs a previous and a next element");
@@ -454,6 +463,7 @@ struct DownsampleElem : Element {
     return CSX_OK;
   }
   int compile(Chain& ch, Compile& c, int) override {
+    CSX_TRY(flush_pointwise(ch, c));
     Geom out;
     CSX_TRY(geometry(ch, c.geom, out));
     if (c.geom.nx != c.geom.ny) return fail(CSX_ERR_UNSUPPORTED, "downsample: square arrays only");
@@ -484,19 +494,28 @@ struct DownsampleElem : Element {
 };
 
 // ---- MFT helpers shared by the Fraunhofer-type elements
-// Builds one MFT over all fields: from fields in buffer c.buf (geometry gin) to
-// the other buffer (geometry gout per lambda), lambda*f per field.
+// One grouped MFT over all fields, from the current fields (geometry c.geom)
+// to geometry gout[lambda] with lambda*f lf[lambda].  Pending pointwise factors
+// are applied first.  Optional epilogue: per-field masks (FPM plane) or one real
+// csim mask (1+i) m over the whole output grid.  out_conv writes the result at
+// the working precision into the conversion buffer for a following MFT.
 static int compile_mft(Chain& ch, Compile& c, const std::vector<Geom>& gout_per_lam,
                        const std::vector<double>& lf_per_lam, const std::string& label,
-                       const std::vector<MftField>* masks = nullptr) {
+                       const std::vector<MftField>* masks = nullptr, bool out_conv = false,
+                       const double* epi_mask_1pi = nullptr) {
+  CSX_TRY(flush_pointwise(ch, c));
   const int nf = ch.nfield_for(c), nl = ch.nlam();
   const Geom gin = c.geom;
   const int T = terms_of(ch.precision());
   const bool c64 = ch.precision() == CSX_FP32;
+  const bool mixed = c64 || T;
+  if (out_conv && !mixed) out_conv = false;
   double2* in = ch.field_buf(c.buf);
   double2* out = ch.field_buf(1 - c.buf);
   char* conv = static_cast<char*>(ch.c64_buf());
   const size_t ein = T ? (size_t)4 * T : sizeof(float2);
+  const int wkind = T ? FK_PLANES : (c64 ? FK_C64 : FK_C128);
+  const bool from_conv = c.in_conv;
   std::vector<MftField> F(nf);
   const Geom& g0 = gout_per_lam[0];
   for (int f = 0; f < nf; f++) {
@@ -504,16 +523,24 @@ static int compile_mft(Chain& ch, Compile& c, const std::vector<Geom>& gout_per_
     const Geom& go = gout_per_lam[l];
     if (go.nx != g0.nx || go.ny != g0.ny) return fail(CSX_ERR_UNSUPPORTED, label + ": output size varies with lambda");
     MftField& m = F[f];
-    m.in = (c64 || T) ? (const void*)(conv + (size_t)f * gin.npix() * ein) : (const void*)(in + (size_t)f * gin.npix());
-    m.in_kind = T ? FK_PLANES : (c64 ? FK_C64 : FK_C128);
-    m.out = out + (size_t)f * go.npix();
-    m.out_kind = FK_C128;
+    m.in = mixed ? (const void*)(conv + (size_t)f * gin.npix() * ein) : (const void*)(in + (size_t)f * gin.npix());
+    m.in_kind = wkind;
+    if (out_conv) {
+      m.out = conv + (size_t)f * go.npix() * ein;
+      m.out_kind = wkind;
+    } else {
+      m.out = out + (size_t)f * go.npix();
+      m.out_kind = FK_C128;
+    }
     m.in_x = gx(gin); m.in_y = gy(gin); m.out_x = gx(go); m.out_y = gy(go);
     m.lambda_f = lf_per_lam[l];
     if (masks) {
       const MftField& mk = (*masks)[f];
       m.mask_kind = mk.mask_kind; m.mask = mk.mask;
       m.mrows = mk.mrows; m.mcols = mk.mcols; m.row0 = mk.row0; m.col0 = mk.col0;
+    } else if (epi_mask_1pi) {
+      m.mask_kind = MASK_REAL_1PI; m.mask = epi_mask_1pi;
+      m.mrows = go.ny; m.mcols = go.nx; m.row0 = 0; m.col0 = 0;
     }
   }
   auto plan = std::make_unique<MftPlan>();
@@ -525,11 +552,11 @@ static int compile_mft(Chain& ch, Compile& c, const std::vector<Geom>& gout_per_
   Chain* chp = &ch;
   c.stages.push_back([=](hipStream_t s) {
     KernelTimer* t = chp->timer();
-    if (c64) {
+    if (!from_conv && c64) {
       if (t) CSX_TRY(t->begin("convert_c128_to_c64", 0.0, 24.0 * nconv, s));
       CSX_TRY(convert_c128_to_c64(in, conv, nconv, s));
       if (t) CSX_TRY(t->end(s));
-    } else if (T) {
+    } else if (!from_conv && T) {
       if (t) CSX_TRY(t->begin("split_c128_to_planes", 0.0, (16.0 + 4.0 * T) * nconv, s));
       CSX_TRY(split_c128_to_planes(in, conv, npix_in, nf, T, s));
       if (t) CSX_TRY(t->end(s));
@@ -537,7 +564,12 @@ static int compile_mft(Chain& ch, Compile& c, const std::vector<Geom>& gout_per_
     return pp->run(s, t);
   });
   c.stage_names.push_back(label);
-  c.buf = 1 - c.buf;
+  if (out_conv) {
+    c.in_conv = true;
+  } else {
+    c.in_conv = false;
+    c.buf = 1 - c.buf;
+  }
   c.geom = g0;
   return CSX_OK;
 }
@@ -610,6 +642,8 @@ struct ZoomFftPropElem : Element {
     return compile_mft(ch, c, go, lfs, "zoomFftPropagator " + name);
   }
 };
+
+static bool is_mft_element(Element* e, const Compile& c);
 
 // ---- complexHexMaskFPM sub-element (src/elements/complexHexMaskFPM.cpp)
 struct HexFpm {
@@ -777,7 +811,7 @@ struct DoubleFraunhoferElem : Element {
     if (has_hex) { hex.zoomFactor = zoomFactor; CSX_TRY(hex.init(ch.efield())); }
     return CSX_OK;
   }
-  int compile(Chain& ch, Compile& c, int) override {
+  int compile(Chain& ch, Compile& c, int index) override {
     if (skipped(c)) return CSX_OK;
     const int nl = ch.nlam(), nf = ch.nfield_for(c), ncand = ch.ncand_for(c);
     const Geom E = c.geom;
@@ -862,11 +896,33 @@ struct DoubleFraunhoferElem : Element {
     }
     if (mask_stage) { c.stages.push_back(mask_stage); c.stage_names.push_back("hex masks " + name); }
     // pupil -> FPM (mask in the epilogue) -> pupil grid of the input
-    CSX_TRY(compile_mft(ch, c, gf, lf, type + " in " + name, &mk));
+    CSX_TRY(flush_pointwise(ch, c));
+    CSX_TRY(compile_mft(ch, c, gf, lf, type + " in " + name, &mk, /*out_conv=*/true));
+    // a mask element right after folds into the epilogue of the return MFT
+    const double* lyot = nullptr;
+    int claim = 0;
+    if (Element* nx = ch.elem_at(index + 1)) {
+      if (auto* me = dynamic_cast<MaskElem*>(nx)) {
+        if (!me->skipped(c) && !me->map.host.empty()) {
+          CSX_TRY(me->map.device_for(ch, E, &lyot));
+          claim = 1;
+        }
+      }
+    }
+    bool next_mft = is_mft_element(ch.elem_at(index + 1 + claim), c);
     std::vector<Geom> back(nl, E);
-    return compile_mft(ch, c, back, lfo, type + " out " + name);
+    CSX_TRY(compile_mft(ch, c, back, lfo, type + " out " + name, nullptr, next_mft, lyot));
+    c.claimed = claim;
+    return CSX_OK;
   }
 };
+
+// An element whose first GPU work is an MFT over the current fields.
+static bool is_mft_element(Element* e, const Compile& c) {
+  if (!e || e->skipped(c)) return false;
+  return dynamic_cast<FraunhoferFocalElem*>(e) || dynamic_cast<ZoomFftPropElem*>(e) ||
+         dynamic_cast<DoubleFraunhoferElem*>(e);
+}
 
 // ---- element factory (src/coronagraph/coronagraph.cpp:67-155)
 int Chain::make_element(const Block& b, std::unique_ptr<Element>& out) {
@@ -1099,6 +1155,7 @@ int Chain::compile_one(bool calibrating, Compile& c) {
   const int nf = nfield_for(c), nl = nlam(), ns = nsrc();
   const long npix = c.geom.npix();
   double2* E0 = field_buf(0);
+  c.pending.clear();
   DevBuf& prm = calibrating ? src_prm_cal_ : src_prm_;
   std::vector<double> p((size_t)3 * nf);
   for (int f = 0; f < nf; f++) {
@@ -1124,18 +1181,20 @@ int Chain::compile_one(bool calibrating, Compile& c) {
         CSX_HIP(hipMemcpyAsync(E0 + (size_t)f0 * npix, src, sizeof(double2) * npix * nl, hipMemcpyDeviceToDevice, s));
       return CSX_OK;
     });
+    c.stage_names.push_back("efield init");
   } else {
-    const double* dp = prm.as<double>();
-    int rows = ef_.rows, cols = ef_.cols;
-    double ps = ef_.pixelScale;
-    c.stages.push_back([=](hipStream_t s) { return k_point_sources(E0, rows, cols, nf, dp, ps, s); });
+    c.pending.push_back(PwOp{PW_SOURCE, nullptr, 0.0, 1});   // fused with the following factors
   }
-  c.stage_names.push_back("efield init");
+  c.in_conv = false;
+  c.claimed = 0;
   for (int i = 0; i < (int)elems_.size(); i++) {
     Element* e = elems_[i].get();
+    if (c.claimed > 0) { c.claimed--; continue; }
     if (e->skipped(c)) continue;
     CSX_TRY(e->compile(*this, c, i));
   }
+  CSX_TRY(flush_pointwise(*this, c));
+  if (c.in_conv) return fail(CSX_ERR_STATE, "chain ends on converted fields");
   return CSX_OK;
 }
 

@@ -11,6 +11,7 @@
 #include "common.hpp"
 #include "hostgeom.hpp"
 #include "mft.hpp"
+#include "ops.hpp"
 
 namespace csx {
 
@@ -69,6 +70,9 @@ struct Compile {
   bool calibrating = false;
   Geom geom;            // geometry of the current fields
   int buf = 0;          // ping-pong buffer holding the fields (complex128)
+  bool in_conv = false; // fields live in the conversion buffer at the working precision instead
+  int claimed = 0;      // following elements already folded into the last one compiled
+  std::vector<PwOp> pending;   // pointwise factors not yet applied (one fused pass)
   std::vector<std::function<int(hipStream_t)>> stages;
   std::vector<std::string> stage_names;
   std::vector<std::unique_ptr<MftPlan>> plans;
@@ -131,6 +135,7 @@ class Chain {
   double2* field_buf(int i) const { return bufs_[i].as<double2>(); }
   void* c64_buf() const { return c64_.p; }
   const double* lambdas_dev() const { return lam_dev_.as<double>(); }   // per field (candidate-major)
+  const double* source_params(bool cal) const { return (cal ? src_prm_cal_ : src_prm_).as<double>(); }
   Element* elem_by_name(const std::string& n);
   Element* elem_at(int i) { return (i >= 0 && i < (int)elems_.size()) ? elems_[i].get() : nullptr; }
   int n_elems() const { return (int)elems_.size(); }

@@ -91,6 +91,54 @@ int k_point_sources(double2* E, int rows, int cols, int nfield, const double* pa
   return CSX_OK;
 }
 
+__global__ void k_pointwise(double2* __restrict__ E, int nfield, PwChain pc) {
+#pragma clang fp contract(off)
+  const long npix = (long)pc.rows * pc.cols, total = npix * nfield;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    long f = i / npix, p = i - f * npix;
+    double2 e;
+    int first = 0;
+    if (pc.nops > 0 && pc.op[0].kind == PW_SOURCE) {
+      // efield::init_point_sources (src/data/efield.cpp:179-206)
+      int c = (int)(p / pc.rows), r = (int)(p - (long)c * pc.rows);
+      double ps = pc.pixel_scale, Lx = ps * pc.cols, Ly = ps * pc.rows;
+      double x = -(Lx - ps) / 2.0 + c * ps, y = -(Ly - ps) / 2.0 + r * ps;
+      const double* prm = pc.src_params + 3 * f;
+      double v = 2 * x * prm[1] / Lx + 2 * y * prm[2] / Ly;
+      double sn, cs;
+      sincos(2.0 * M_PI * v, &sn, &cs);
+      e = make_double2(prm[0] * cs, prm[0] * sn);
+      first = 1;
+    } else {
+      e = E[i];
+    }
+    for (int k = first; k < pc.nops; k++) {
+      const PwOp& o = pc.op[k];
+      if (o.kind == PW_MASK_1PI) {           // mask.cpp: E *= (1+i) m
+        double w = static_cast<const double*>(o.p)[p];
+        e = make_double2(e.x * w - e.y * w, e.x * w + e.y * w);
+      } else if (o.kind == PW_MIRROR) {      // mirror.cpp:66-72
+        double a = -o.sign * 2 * 2 * M_PI / pc.lambdas[f];
+        double ph = a * static_cast<const double*>(o.p)[p];
+        double sn, cs;
+        sincos(ph, &sn, &cs);
+        e = make_double2(e.x * cs - e.y * sn, e.x * sn + e.y * cs);
+      } else if (o.kind == PW_CFACTOR) {
+        double2 w = static_cast<const double2*>(o.p)[(f / o.per) * npix + p];
+        e = make_double2(e.x * w.x - e.y * w.y, e.x * w.y + e.y * w.x);
+      }
+    }
+    E[i] = e;
+  }
+}
+int pointwise_chain(double2* E, int nfield, const PwChain& pc, hipStream_t s) {
+  if (pc.nops <= 0) return CSX_OK;
+  long total = (long)pc.rows * pc.cols * nfield;
+  hipLaunchKernelGGL(k_pointwise, dim3(grid_for(total)), dim3(256), 0, s, E, nfield, pc);
+  CSX_HIP(hipGetLastError());
+  return CSX_OK;
+}
+
 // ---------------------------------------------------------------------- FFT --
 // Radix-2 Stockham autosort transforms staged in LDS.  Stage m = 2^s:
 //   y[k + 2jm] = x[k + jm] + w x[k + jm + n/2],  y[k + 2jm + m] = x[k + jm] - w x[k + jm + n/2],

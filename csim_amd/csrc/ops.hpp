@@ -15,6 +15,21 @@ int k_apply_cfactor(double2* E, long npix, int nfield, const double2* P, int per
 int k_point_sources(double2* E, int rows, int cols, int nfield, const double* params_dev,
                     double pixel_scale, hipStream_t s);
 
+// Fused pointwise pass over a batch: optional point-source generation followed by
+// up to kMaxPwOps element factors, one read and one write of every field value.
+enum PwKind : int { PW_SOURCE = 0, PW_MASK_1PI = 1, PW_MIRROR = 2, PW_CFACTOR = 3 };
+constexpr int kMaxPwOps = 6;
+struct PwOp { int kind; const void* p; double sign; int per; };   // per: fields sharing a PW_CFACTOR plane
+struct PwChain {
+  PwOp op[kMaxPwOps];
+  int nops;
+  int rows, cols;
+  double pixel_scale;               // PW_SOURCE grid
+  const double* src_params;         // PW_SOURCE: flux, tipX, tiltY per field
+  const double* lambdas;            // PW_MIRROR: per field
+};
+int pointwise_chain(double2* E, int nfield, const PwChain& pc, hipStream_t s);
+
 // 2-D FFT, csim semantics: dir -1 forward unnormalised, +1 inverse / n^2.
 int fft2_batched(double2* E, int n, int nfield, int dir, hipStream_t s);
 // Angular-spectrum Fresnel step (src/lib/csim_fft.cpp:371-392); lambdas on device.
