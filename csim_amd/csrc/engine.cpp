@@ -1,6 +1,8 @@
 // Element chain engine: csim input scripts compiled into GPU stages.
 #include "engine.hpp"
 
+#include <complex>
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -917,6 +919,272 @@ struct DoubleFraunhoferElem : Element {
   }
 };
 
+// ---- fpmPupToLyot (src/elements/fpmPupToLyot.cpp:24-167): the FPM applied as a
+// convolution in the pupil on a 2n x 2n zero-padded grid,
+//   E' = s * ifft2( fft2(fft_shift(z * zoomFft(K))) .* fft2(pad E) ) (+ pad E)  -> centre,
+// K = mask - 1 with the Babinet masks (fpmCMCForPupToLyot, fpmIntHexCMCForPupToLyot,
+// the hex mask mirrored left-right) or the opaque mask itself (fpmBinaryForPupToLyot),
+// s = dx^2 / (-i fRatioSign lambda f).  Mask spectra are recomputed on every
+// propagation (they follow the hex sags and the candidate batch).
+struct FpmPupToLyotElem : Element {
+  enum MaskType { MT_NONE = 0, MT_CMC = 1, MT_HEX = 2, MT_BINARY = 3 };
+  int mtype = MT_NONE;
+  double focalRatio = -1, fRatioSign = 1, zoomFftSign = 1;
+  bool babinet = false;
+  // fpmCMCForPupToLyot
+  std::vector<double2> cube;
+  int mrows = 0, mcols = 0, mslices = 0, maskIndex = -1;
+  double fpmMatScale = 0;
+  // fpmIntHexCMCForPupToLyot
+  HexFpm hex;
+  double zoomFactor = -1;
+  // fpmBinaryForPupToLyot
+  std::vector<double> fpmMat, calibMat;
+  int brows = 0, bcols = 0, crows = 0, ccols = 0;
+  double referenceLambda = 0, innerRadiusFld = 0, outerRadiusFld = 0;
+  // device buffers, [0] full run, [1] calibration run
+  DevBuf d_src[2], d_K[2], d_H[2], d_P[2], d_scale[2], d_lam;
+
+  int set(Chain&, const std::string& k, const std::string& a) override {
+    if (set_common(k, a) || k == "fpmPupToLyot") return CSX_OK;
+    if (k == "focalRatio") focalRatio = std::atof(a.c_str());
+    else if (k == "fRatioSign") fRatioSign = std::atof(a.c_str());
+    else if (k == "zoomFftSign") zoomFftSign = std::atof(a.c_str());
+    else return unknown_key(type, k);
+    return CSX_OK;
+  }
+  static int load_amp_ph(const std::string& a, bool reim, std::vector<double2>& out, int& r, int& c, int& s) {
+    size_t cpos = a.find(',');
+    if (cpos == std::string::npos) return fail(CSX_ERR_PARSE, "mask file names need the form first,second");
+    std::vector<double> x, y;
+    int r2, c2, s2;
+    CSX_TRY(fits_matrix(trim(a.substr(0, cpos)), x, r, c, s));
+    CSX_TRY(fits_matrix(trim(a.substr(cpos + 1)), y, r2, c2, s2));
+    if (r != r2 || c != c2 || s != s2) return fail(CSX_ERR_ARG, "mask file sizes differ");
+    out.resize(x.size());
+    for (size_t i = 0; i < x.size(); i++)
+      out[i] = reim ? make_double2(x[i], y[i])                                        // initMaskReIm
+                    : make_double2(x[i] * std::cos(-y[i]), x[i] * std::sin(-y[i]));   // initMask: amp exp(-i ph)
+    return CSX_OK;
+  }
+  // keys of the mask sub-block (fpmPupToLyot.cpp:250-330, 377-392, 463-496)
+  int set_mask(const std::string& k, const std::string& a) {
+    if (k == "fpmCMCForPupToLyot" || k == "fpmIntHexCMCForPupToLyot" || k == "fpmBinaryForPupToLyot") return CSX_OK;
+    if (mtype == MT_CMC) {
+      if (k == "maskIndex") maskIndex = std::atoi(a.c_str());
+      else if (k == "fpmMatScale") fpmMatScale = std::atof(a.c_str());
+      else if (k == "maskFilename") return load_amp_ph(a, false, cube, mrows, mcols, mslices);
+      else if (k == "maskFilenameReIm") return load_amp_ph(a, true, cube, mrows, mcols, mslices);
+      else return unknown_key("fpmCMCForPupToLyot", k);
+    } else if (mtype == MT_HEX) {
+      if (k == "zoomFactor") zoomFactor = std::atof(a.c_str());
+      else return unknown_key("fpmIntHexCMCForPupToLyot", k);
+    } else if (mtype == MT_BINARY) {
+      int s = 0;
+      if (k == "referenceLambda") referenceLambda = std::atof(a.c_str());
+      else if (k == "innerRadiusFld") innerRadiusFld = std::atof(a.c_str());
+      else if (k == "outerRadiusFld") outerRadiusFld = std::atof(a.c_str());
+      else if (k == "maskFilename") return fits_matrix(trim(a), fpmMat, brows, bcols, s);
+      else if (k == "calibFilename") return fits_matrix(trim(a), calibMat, crows, ccols, s);
+      else return unknown_key("fpmBinaryForPupToLyot", k);
+    }
+    return CSX_OK;
+  }
+  int set_array(Chain& ch, const std::string& k, const double* d, int r, int c) override {
+    if (mtype == MT_BINARY && (k == "maskFilename" || k == "calibFilename")) {
+      if (!d || r <= 0 || c <= 0) return fail(CSX_ERR_ARG, "empty array");
+      auto& dst = k == "maskFilename" ? fpmMat : calibMat;
+      dst.assign(d, d + (size_t)r * c);
+      if (k == "maskFilename") { brows = r; bcols = c; } else { crows = r; ccols = c; }
+      ch.invalidate();
+      return CSX_OK;
+    }
+    return Element::set_array(ch, k, d, r, c);
+  }
+  int set_complex_array(Chain& ch, const std::string& k, const double* d, int r, int c, int slice) override {
+    if (mtype != MT_CMC || k != "mask") return Element::set_complex_array(ch, k, d, r, c, slice);
+    if (slice < 0) return fail(CSX_ERR_ARG, "negative slice");
+    if (mrows != r || mcols != c) { mrows = r; mcols = c; mslices = 0; cube.clear(); }
+    if (slice >= mslices) { mslices = slice + 1; cube.resize((size_t)mrows * mcols * mslices); }
+    std::memcpy(&cube[(size_t)slice * mrows * mcols], d, sizeof(double2) * mrows * mcols);
+    ch.invalidate();
+    return CSX_OK;
+  }
+  int set_opt(Chain& ch, const std::string& n, const std::vector<double>& v) override {
+    if (n == "reinit") return CSX_OK;   // spectra are rebuilt on every propagation
+    if (mtype == MT_HEX && n == "setBabinet" && !v.empty()) { babinet = v[0] != 0; ch.invalidate(); return CSX_OK; }
+    if (mtype == MT_HEX && n == "sags") {
+      hex.sags = v;
+      ch.invalidate_calibration();
+      if (hex.rasterized && (int)v.size() == hex.raster.nhex && hex.d_sags.bytes >= sizeof(double) * v.size()) {
+        CSX_HIP(hipMemcpy(hex.d_sags.p, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice));
+        return CSX_OK;
+      }
+      ch.invalidate();
+      return CSX_OK;
+    }
+    if (mtype == MT_HEX && n == "useOnlyThisHex" && !v.empty()) { hex.useOnlyThisHex = (int)v[0]; ch.invalidate(); return CSX_OK; }
+    return Element::set_opt(ch, n, v);
+  }
+  int get_opt(Chain& ch, const std::string& n, std::vector<double>& v) override {
+    if (mtype == MT_HEX && n == "sags") { v = hex.sags; return CSX_OK; }
+    if (mtype == MT_HEX && n == "useOnlyThisHex") { v.assign(1, hex.useOnlyThisHex); return CSX_OK; }
+    return Element::get_opt(ch, n, v);
+  }
+  int init(Chain& ch) override {
+    if (mtype == MT_HEX) {
+      if (zoomFactor <= 0) return fail(CSX_ERR_ARG, "fpmIntHexCMCForPupToLyot: zoomFactor must be set before the hex mask");
+      hex.zoomFactor = zoomFactor;
+      CSX_TRY(hex.init(ch.efield()));
+    }
+    return CSX_OK;
+  }
+  // mask grid (maskGeom.set_xy) and lambda * f of slice l (set_geometry)
+  int mask_geom(Chain& ch, Geom& g, std::vector<double>& lf) const {
+    const Efield& ef = ch.efield();
+    const int nl = ch.nlam();
+    lf.resize(nl);
+    if (mtype == MT_CMC || mtype == MT_HEX) {
+      if (focalRatio <= 0) return fail(CSX_ERR_ARG, "fpmPupToLyot: focalRatio unset");
+      if (mtype == MT_CMC) {
+        if (cube.empty() || fpmMatScale <= 0) return fail(CSX_ERR_ARG, "fpmCMCForPupToLyot: mask or fpmMatScale unset");
+        g = geom_symmetric(mcols, mrows, fpmMatScale, fpmMatScale);
+      } else {
+        g = geom_symmetric(hex.raster.S, hex.raster.S, hex.pixelScale, hex.pixelScale);
+      }
+      double cFRatio = focalRatio * 2 * ef.beamRadius;
+      for (int l = 0; l < nl; l++) lf[l] = fRatioSign * ef.lambdas[l] * cFRatio;
+    } else if (mtype == MT_BINARY) {
+      if (fpmMat.empty() || outerRadiusFld <= 0) return fail(CSX_ERR_ARG, "fpmBinaryForPupToLyot: mask or outerRadiusFld unset");
+      double flD = ch.telescope().loD(referenceLambda);
+      double pix = 2 * outerRadiusFld * flD / brows;
+      g = geom_symmetric(bcols, brows, pix, pix);
+      for (int l = 0; l < nl; l++) lf[l] = fRatioSign * ef.lambdas[l] * ch.telescope().fLength;
+    } else {
+      return fail(CSX_ERR_ARG, "fpmPupToLyot: no mask sub-block");
+    }
+    return CSX_OK;
+  }
+  int compile(Chain& ch, Compile& c, int) override {
+    const bool binary = mtype == MT_BINARY;
+    if (skipped(c) && !binary) return CSX_OK;   // Babinet masks: K = exp(0) - 1 = 0, E unchanged
+    const bool use_calib = binary && skipped(c);
+    if (use_calib && calibMat.empty()) return fail(CSX_ERR_ARG, "fpmBinaryForPupToLyot: calibFilename unset");
+    if (use_calib && (crows != brows || ccols != bcols)) return fail(CSX_ERR_ARG, "fpmBinaryForPupToLyot: calib size");
+    const Geom E = c.geom;
+    if (E.nx != E.ny || (E.nx & 1)) return fail(CSX_ERR_UNSUPPORTED, "fpmPupToLyot: fields must be square and even");
+    const int n = E.nx, n2 = 2 * n;
+    if (n2 > 4096 || (n2 & (n2 - 1))) return fail(CSX_ERR_UNSUPPORTED, "fpmPupToLyot: 2n must be a power of two <= 4096");
+    CSX_TRY(flush_pointwise(ch, c));
+    if (c.in_conv) return fail(CSX_ERR_STATE, "fpmPupToLyot: fields at working precision");
+    Geom mg;
+    std::vector<double> lf;
+    CSX_TRY(mask_geom(ch, mg, lf));
+    const int nl = ch.nlam(), nf = ch.nfield_for(c), ncand = ch.ncand_for(c), nsrc = ch.nsrc();
+    const int which = c.calibrating ? 1 : 0;
+    const long S2 = (long)mg.nx * mg.ny, P2 = (long)n2 * n2;
+    // source masks and the number of mask candidates
+    int ncm = 1;
+    const double2* src = nullptr;
+    long src_stride = 0;
+    int fl = 0, minus1 = 0;
+    std::function<int(hipStream_t)> hex_stage;
+    if (mtype == MT_HEX) {
+      const int S = hex.raster.S;
+      const double* sags_dev = nullptr;
+      int nhex = hex.raster.nhex, ncd = 1;
+      if (!(!c.calibrating && ch.candidate_sags(name, &sags_dev, &nhex, &ncd) == CSX_OK && sags_dev)) {
+        if (hex.d_sags.bytes < sizeof(double) * hex.sags.size()) CSX_TRY(hex.d_sags.reserve(sizeof(double) * hex.sags.size()));
+        CSX_HIP(hipMemcpy(hex.d_sags.p, hex.sags.data(), sizeof(double) * hex.sags.size(), hipMemcpyHostToDevice));
+        sags_dev = hex.d_sags.as<double>();
+        nhex = (int)hex.sags.size();
+        ncd = 1;
+      }
+      ncm = std::min(ncd, ncand);
+      CSX_TRY(d_src[which].reserve(sizeof(double2) * S2 * ncm * nl));
+      CSX_TRY(d_lam.reserve(sizeof(double) * nl));
+      CSX_HIP(hipMemcpy(d_lam.p, ch.efield().lambdas.data(), sizeof(double) * nl, hipMemcpyHostToDevice));
+      const int32_t* dn = hex.d_nsub.as<int32_t>();
+      const int32_t* dh = hex.d_hexnum.as<int32_t>();
+      double2* dm = d_src[which].as<double2>();
+      const double* dl = d_lam.as<double>();
+      int nsp = hex.raster.nsubpix, use = hex.useOnlyThisHex, ncmc = ncm;
+      double ps = hex.phaseSign;
+      hex_stage = [=](hipStream_t s) { return hex_masks(dn, dh, S, nsp, sags_dev, nhex, ncmc, dl, nl, ps, use, dm, s); };
+      src = dm; src_stride = S2; fl = 1; minus1 = 1;
+    } else {
+      std::vector<double2> m((size_t)S2);
+      if (mtype == MT_CMC) {
+        int mi = maskIndex < 0 ? 0 : maskIndex;   // maskIndex latches to the first slice executed
+        if (mi >= mslices) return fail(CSX_ERR_ARG, "fpmCMCForPupToLyot: maskIndex out of range");
+        std::copy(cube.begin() + (size_t)mi * S2, cube.begin() + (size_t)(mi + 1) * S2, m.begin());
+        minus1 = 1;
+      } else {
+        const std::vector<double>& b = use_calib ? calibMat : fpmMat;
+        for (long i = 0; i < S2; i++) m[i] = make_double2(b[i], 0.0);
+      }
+      CSX_TRY(d_src[which].reserve(sizeof(double2) * S2));
+      CSX_HIP(hipMemcpy(d_src[which].p, m.data(), sizeof(double2) * S2, hipMemcpyHostToDevice));
+      src = d_src[which].as<double2>(); src_stride = 0;   // the same mask for every slice
+    }
+    babinet = mtype != MT_BINARY;
+    // kernel inputs K (one per (mask candidate, lambda)), their transforms H
+    const int nk = ncm * nl;
+    CSX_TRY(d_K[which].reserve(sizeof(double2) * S2 * nk));
+    CSX_TRY(d_H[which].reserve(sizeof(double2) * P2 * nk));
+    CSX_TRY(d_P[which].reserve(sizeof(double2) * P2 * nf));
+    // s = dx^2 / (-i fRatioSign lambda f) per slice (fftMHatTimesfftPaddedEScale)
+    std::vector<double2> sc(nl);
+    const std::complex<double> i1(0, 1);
+    for (int l = 0; l < nl; l++) {
+      std::complex<double> v = std::pow(E.dx, 2) / (-i1 * fRatioSign * lf[l]);
+      sc[l] = make_double2(v.real(), v.imag());
+    }
+    CSX_TRY(d_scale[which].reserve(sizeof(double2) * nl));
+    CSX_HIP(hipMemcpy(d_scale[which].p, sc.data(), sizeof(double2) * nl, hipMemcpyHostToDevice));
+    // zoomFft of K from the mask grid to the padded pupil grid, FP64
+    const Geom pg = geom_symmetric(n2, n2, E.dx, E.dy);
+    std::vector<MftField> F(nk);
+    double2* dK = d_K[which].as<double2>();
+    double2* dH = d_H[which].as<double2>();
+    for (int k = 0; k < nk; k++) {
+      MftField& m = F[k];
+      m.in = dK + (size_t)k * S2; m.out = dH + (size_t)k * P2;
+      m.in_kind = FK_C128; m.out_kind = FK_C128;
+      m.in_x = gx(mg); m.in_y = gy(mg); m.out_x = gx(pg); m.out_y = gy(pg);
+      m.lambda_f = lf[k % nl];
+    }
+    auto plan = std::make_unique<MftPlan>();
+    CSX_TRY(plan->build(F, CSX_FP64, ch.stream()));
+    const MftPlan* pp = plan.get();
+    c.plans.push_back(std::move(plan));
+    const int rows = mg.ny, cols = mg.nx, fli = fl, mi1 = minus1;
+    const double zs = zoomFftSign;
+    Chain* chp = &ch;
+    c.stages.push_back([=](hipStream_t s) {
+      if (hex_stage) CSX_TRY(hex_stage(s));
+      // src_stride 0: the same mask for every slice
+      CSX_TRY(fpm_prep(src, src_stride, rows, cols, nk, fli, mi1, zs, dK, s));
+      CSX_TRY(pp->run(s, chp->timer()));
+      return fft2_batched(dH, n2, nk, -1, s);
+    });
+    c.stage_names.push_back("fpmPupToLyot mask spectra " + name);
+    double2* Ef = ch.field_buf(c.buf);
+    double2* P = d_P[which].as<double2>();
+    const double2* dsc = d_scale[which].as<double2>();
+    const int bab = babinet ? 1 : 0, per_cand = nsrc * nl, ncmc = ncm;
+    c.stages.push_back([=](hipStream_t s) {
+      CSX_TRY(embed_center(Ef, n, n, P, n2, n2, nf, 0, s));
+      CSX_TRY(fft2_batched(P, n2, nf, -1, s));
+      CSX_TRY(spectrum_mul(P, n2, nf, dH, per_cand, nl, ncmc, s));
+      CSX_TRY(fft2_batched(P, n2, nf, +1, s));
+      return crop_add(Ef, n, P, n2, nf, nl, dsc, bab, s);
+    });
+    c.stage_names.push_back("fpmPupToLyot convolution " + name);
+    return CSX_OK;
+  }
+};
+
 // An element whose first GPU work is an MFT over the current fields.
 static bool is_mft_element(Element* e, const Compile& c) {
   if (!e || e->skipped(c)) return false;
@@ -939,6 +1207,8 @@ int Chain::make_element(const Block& b, std::unique_ptr<Element>& out) {
     auto* d = new DoubleFraunhoferElem();
     d->interp = t == "doubleFraunhoferInterpFPM";
     e.reset(d);
+  } else if (t == "fpmPupToLyot") {
+    e.reset(new FpmPupToLyotElem());
   } else {
     std::fprintf(stderr, "!!! coronagraph init unknown object name: %s\n", t.c_str());
     return fail(CSX_ERR_UNSUPPORTED, "element type not supported: " + t);
@@ -953,6 +1223,27 @@ int Chain::make_element(const Block& b, std::unique_ptr<Element>& out) {
       if (sb.cmds[0].key == "complexHexMaskFPM") {
         d->has_hex = true;
         for (const Cmd& c : sb.cmds) CSX_TRY(d->hex.set(c.key, c.arg));
+      } else {
+        for (const Cmd& c : sb.cmds) CSX_TRY(e->set(*this, c.key, c.arg));
+      }
+    }
+  } else if (auto* p = dynamic_cast<FpmPupToLyotElem*>(e.get())) {
+    // own keys, then one mask sub-block (fpmPupToLyot::init, :172-195), which may
+    // hold a complexHexMaskFPM block after its own keys
+    for (const Block& sb : find_blocks(b)) {
+      if (sb.cmds.empty()) continue;
+      const std::string& h = sb.cmds[0].key;
+      if (h == "fpmCMCForPupToLyot" || h == "fpmIntHexCMCForPupToLyot" || h == "fpmBinaryForPupToLyot") {
+        p->mtype = h == "fpmCMCForPupToLyot" ? FpmPupToLyotElem::MT_CMC
+                 : (h == "fpmIntHexCMCForPupToLyot" ? FpmPupToLyotElem::MT_HEX : FpmPupToLyotElem::MT_BINARY);
+        for (const Block& mb : find_blocks(sb)) {
+          if (mb.cmds.empty()) continue;
+          if (mb.cmds[0].key == "complexHexMaskFPM") {
+            for (const Cmd& c : mb.cmds) CSX_TRY(p->hex.set(c.key, c.arg));
+          } else {
+            for (const Cmd& c : mb.cmds) CSX_TRY(p->set_mask(c.key, c.arg));
+          }
+        }
       } else {
         for (const Cmd& c : sb.cmds) CSX_TRY(e->set(*this, c.key, c.arg));
       }

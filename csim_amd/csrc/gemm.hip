@@ -50,10 +50,21 @@ __device__ __forceinline__ unsigned short bf16_rne(float x) {
 }
 __device__ __forceinline__ float bf16_val(unsigned short h) { return __uint_as_float(((unsigned int)h) << 16); }
 
+// Pointers read from the problem array are generic; casting them to the global
+// address space makes the loads global_load (vmcnt only) instead of flat_load,
+// which also counts against lgkmcnt and so stalls every LDS wait behind HBM.
+#define GLOBAL __attribute__((address_space(1)))
+template <typename T> __device__ __forceinline__ const GLOBAL T* gin(const void* p) { return (const GLOBAL T*)p; }
+template <typename T> __device__ __forceinline__ GLOBAL T* gout(void* p) { return (GLOBAL T*)p; }
+// builtin vector types for complex values behind global pointers (the HIP vector
+// classes have no address-space-qualified copy)
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
 // (re, im) split into `terms` BF16 parts each: x = hi + mid + lo, every part the
 // round-to-nearest-even of what the previous ones leave (from the FP32 value).
-__device__ __forceinline__ void store_planes(unsigned short* base, long off, long stride, int terms,
-                                             double vr, double vi) {
+template <typename Ptr>
+__device__ __forceinline__ void store_planes(Ptr base, long off, long stride, int terms, double vr, double vi) {
   float xr = (float)vr, xi = (float)vi;
   for (int t = 0; t < terms; t++) {
     unsigned short hr = bf16_rne(xr), hi = bf16_rne(xi);
@@ -74,12 +85,12 @@ __device__ __forceinline__ void epilogue_store(const GemmProb& P, int m, int n, 
     if (mr >= 0 && mr < P.mM && nc >= 0 && nc < P.mN) {
       long off = (long)mr * P.ldm + nc;
       if (P.mask_kind == MASK_COMPLEX) {
-        double2 w = static_cast<const double2*>(P.mask)[off];
+        f64x2 w = gin<f64x2>(P.mask)[off];
         double tr = vr * w.x - vi * w.y;
         vi = vr * w.y + vi * w.x;
         vr = tr;
       } else {
-        double w = static_cast<const double*>(P.mask)[off];
+        double w = gin<double>(P.mask)[off];
         if (P.mask_kind == MASK_REAL_1PI) {   // csim mask: (1+i) w
           double tr = (vr - vi) * w;
           vi = (vr + vi) * w;
@@ -91,9 +102,9 @@ __device__ __forceinline__ void epilogue_store(const GemmProb& P, int m, int n, 
     }
   }
   long off = P.transC ? (long)n * P.ldc + m : (long)m * P.ldc + n;
-  if (P.out_kind == OUT_C128) static_cast<double2*>(P.C)[off] = make_double2(vr, vi);
-  else if (P.out_kind == OUT_C64) static_cast<float2*>(P.C)[off] = make_float2((float)vr, (float)vi);
-  else store_planes(static_cast<unsigned short*>(P.C), off, P.sC, P.terms, vr, vi);
+  if (P.out_kind == OUT_C128) gout<f64x2>(P.C)[off] = f64x2{vr, vi};
+  else if (P.out_kind == OUT_C64) gout<f32x2>(P.C)[off] = f32x2{(float)vr, (float)vi};
+  else store_planes(gout<unsigned short>(P.C), off, P.sC, P.terms, vr, vi);
 }
 
 // ------------------------------------------------------ FP64 / FP32 MFMA GEMM --
@@ -102,7 +113,7 @@ __device__ __forceinline__ void epilogue_store(const GemmProb& P, int m, int n, 
 template <typename T> struct Mma;
 template <> struct Mma<double> {
   typedef double v4 __attribute__((ext_vector_type(4)));
-  typedef double2 c2;
+  typedef f64x2 c2;
   static __device__ __forceinline__ v4 mma(double a, double b, v4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
@@ -110,7 +121,7 @@ template <> struct Mma<double> {
 };
 template <> struct Mma<float> {
   typedef float v4 __attribute__((ext_vector_type(4)));
-  typedef float2 c2;
+  typedef f32x2 c2;
   static __device__ __forceinline__ v4 mma(float a, float b, v4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
@@ -133,8 +144,10 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const GemmProb* __restrict__ pr
   const int local = tile - P.tile_start;
   const int tm = local % P.tiles_m, tn = local / P.tiles_m;
   const int m0 = tm * kTileM, n0 = tn * kTileN;
-  const C2* __restrict__ A = static_cast<const C2*>(P.A);
-  const C2* __restrict__ B = static_cast<const C2*>(P.B);
+  const GLOBAL C2* A = gin<C2>(P.A);
+  const GLOBAL C2* B = gin<C2>(P.B);
+  const int M = P.M, N = P.N, K = P.K;
+  const long lda = P.lda, ldb = P.ldb;
 
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int wm = wave >> 1, wn = wave & 1;
@@ -147,22 +160,30 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const GemmProb* __restrict__ pr
     for (int j = 0; j < 2; j++) { accR[i][j] = V4{0, 0, 0, 0}; accI[i][j] = V4{0, 0, 0, 0}; }
 
   C2 ra[4], rb[4];
+  // unconditional loads from clamped (in-bounds) addresses, zeroed by select
+  const int am = m0 + lrow, bn = n0 + lrow;
+  const long arow = (long)min(am, M - 1) * lda, brow = (long)min(bn, N - 1) * ldb;
+  int kl = 0;   // k of the loaded slab; the select is applied at the LDS store
   auto gload = [&](int k0) {
-    int am = m0 + lrow, bn = n0 + lrow;
+    kl = k0;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      int k = k0 + lk + q;
-      ra[q] = (am < P.M && k < P.K) ? A[(long)am * P.lda + k] : C2{0, 0};
-      rb[q] = (bn < P.N && k < P.K) ? B[(long)bn * P.ldb + k] : C2{0, 0};
+      int kc = min(k0 + lk + q, K - 1);
+      ra[q] = A[arow + kc];
+      rb[q] = B[brow + kc];
     }
   };
   gload(0);
-  for (int k0 = 0; k0 < P.K; k0 += BK) {
+  for (int k0 = 0; k0 < K; k0 += BK) {
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < 4; q++) { As[lrow][lk + q] = ra[q]; Bs[lrow][lk + q] = rb[q]; }
+    for (int q = 0; q < 4; q++) {
+      const bool kin = kl + lk + q < K;
+      As[lrow][lk + q] = (am < M && kin) ? ra[q] : C2{0, 0};
+      Bs[lrow][lk + q] = (bn < N && kin) ? rb[q] : C2{0, 0};
+    }
     __syncthreads();
-    if (k0 + BK < P.K) gload(k0 + BK);
+    if (k0 + BK < K) gload(k0 + BK);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       C2 a[2], b[2];
@@ -206,148 +227,10 @@ int gemm_f32(const GemmProb* probs_dev, int n, int total_tiles, hipStream_t s) {
   return CSX_OK;
 }
 
-// ------------------------------------------------------- split-BF16 MFMA GEMM --
-// FP32 complex operands are split on the fly into TERMS bf16 planes
-// (x = hi + mid + lo, each round-to-nearest-even of the remainder) and
-// contracted with v_mfma_f32_16x16x32_bf16.  Products kept: every (a, b) term
-// pair with a + b < TERMS (3 for two terms, 6 for three).  FP32 accumulators
-// are flushed into FP64 after every K slab of 32 when TERMS == 3.
+// ------------------------------------------------------- split-BF16 types --
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ unsigned short bf16_rne_bits(float x) { return bf16_rne(x); }
-__device__ __forceinline__ float bf16_bits_to_float(unsigned short h) { return bf16_val(h); }
-
-template <int TERMS>
-__global__ __launch_bounds__(256) void k_gemm_bf16split(const GemmProb* __restrict__ probs, int nprob) {
-  constexpr int BK = 32;          // one MFMA K step
-  constexpr int LDK = BK + 8;     // bf16 elements per LDS row (16 B pad)
-  // planes: [term][re/im][row][k]
-  __shared__ unsigned short As[TERMS][2][kTileM][LDK];
-  __shared__ unsigned short Bs[TERMS][2][kTileN][LDK];
-
-  const int tile = xcd_remap(blockIdx.x, gridDim.x);
-  const GemmProb& P = probs[find_prob(probs, nprob, tile)];
-  const int local = tile - P.tile_start;
-  const int tm = local % P.tiles_m, tn = local / P.tiles_m;
-  const int m0 = tm * kTileM, n0 = tn * kTileN;
-  const float2* __restrict__ A = static_cast<const float2*>(P.A);
-  const float2* __restrict__ B = static_cast<const float2*>(P.B);
-
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
-  const int lrow = t >> 2, lk = (t & 3) * 8;   // loader: 64 rows x 4 chunks of 8
-
-  f32x4 accR[2][2], accI[2][2];
-  double dR[2][2][4], dI[2][2][4];
-#pragma unroll
-  for (int i = 0; i < 2; i++)
-#pragma unroll
-    for (int j = 0; j < 2; j++) {
-      accR[i][j] = f32x4{0, 0, 0, 0}; accI[i][j] = f32x4{0, 0, 0, 0};
-#pragma unroll
-      for (int v = 0; v < 4; v++) { dR[i][j][v] = 0; dI[i][j][v] = 0; }
-    }
-
-  float2 ra[8], rb[8];
-  auto gload = [&](int k0) {
-    int am = m0 + lrow, bn = n0 + lrow;
-#pragma unroll
-    for (int q = 0; q < 8; q++) {
-      int k = k0 + lk + q;
-      ra[q] = (am < P.M && k < P.K) ? A[(long)am * P.lda + k] : make_float2(0.f, 0.f);
-      rb[q] = (bn < P.N && k < P.K) ? B[(long)bn * P.ldb + k] : make_float2(0.f, 0.f);
-    }
-  };
-  auto split_store = [&](unsigned short (*S)[2][kTileM][LDK], const float2* r) {
-#pragma unroll
-    for (int q = 0; q < 8; q++) {
-      float xr = r[q].x, xi = r[q].y;
-#pragma unroll
-      for (int tt = 0; tt < TERMS; tt++) {
-        unsigned short hr = bf16_rne_bits(xr), hi = bf16_rne_bits(xi);
-        S[tt][0][lrow][lk + q] = hr;
-        S[tt][1][lrow][lk + q] = hi;
-        xr -= bf16_bits_to_float(hr);
-        xi -= bf16_bits_to_float(hi);
-      }
-    }
-  };
-
-  gload(0);
-  for (int k0 = 0; k0 < P.K; k0 += BK) {
-    __syncthreads();
-    split_store(As, ra);
-    split_store(Bs, rb);
-    __syncthreads();
-    if (k0 + BK < P.K) gload(k0 + BK);
-    // fragments: lane l holds row l%16, k = 8*(l/16) .. +7
-    const int fr = lane & 15, fk = 8 * (lane >> 4);
-    u32x4 a[TERMS][2][2], b[TERMS][2][2];   // [term][re/im][subtile]
-#pragma unroll
-    for (int tt = 0; tt < TERMS; tt++)
-#pragma unroll
-      for (int c = 0; c < 2; c++)
-#pragma unroll
-        for (int s = 0; s < 2; s++) {
-          a[tt][c][s] = *reinterpret_cast<const u32x4*>(&As[tt][c][wm * 32 + s * 16 + fr][fk]);
-          b[tt][c][s] = *reinterpret_cast<const u32x4*>(&Bs[tt][c][wn * 32 + s * 16 + fr][fk]);
-        }
-#pragma unroll
-    for (int ta = 0; ta < TERMS; ta++)
-#pragma unroll
-      for (int tb = 0; tb < TERMS - ta; tb++)
-#pragma unroll
-        for (int i = 0; i < 2; i++)
-#pragma unroll
-          for (int j = 0; j < 2; j++) {
-            bf16x8 ar = __builtin_bit_cast(bf16x8, a[ta][0][i]);
-            bf16x8 ai = __builtin_bit_cast(bf16x8, a[ta][1][i]);
-            bf16x8 ain = __builtin_bit_cast(bf16x8, a[ta][1][i] ^ u32x4{0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u});
-            bf16x8 br = __builtin_bit_cast(bf16x8, b[tb][0][j]);
-            bf16x8 bi = __builtin_bit_cast(bf16x8, b[tb][1][j]);
-            accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ar, br, accR[i][j], 0, 0, 0);
-            accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ar, bi, accI[i][j], 0, 0, 0);
-            accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ain, bi, accR[i][j], 0, 0, 0);
-            accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ai, br, accI[i][j], 0, 0, 0);
-          }
-    if (TERMS >= 3) {
-#pragma unroll
-      for (int i = 0; i < 2; i++)
-#pragma unroll
-        for (int j = 0; j < 2; j++) {
-#pragma unroll
-          for (int v = 0; v < 4; v++) { dR[i][j][v] += accR[i][j][v]; dI[i][j][v] += accI[i][j][v]; }
-          accR[i][j] = f32x4{0, 0, 0, 0}; accI[i][j] = f32x4{0, 0, 0, 0};
-        }
-    }
-  }
-#pragma unroll
-  for (int i = 0; i < 2; i++)
-#pragma unroll
-    for (int j = 0; j < 2; j++)
-#pragma unroll
-      for (int v = 0; v < 4; v++) {
-        int m = m0 + wm * 32 + i * 16 + 4 * (lane >> 4) + v;
-        int n = n0 + wn * 32 + j * 16 + (lane & 15);
-        double re = dR[i][j][v] + (double)accR[i][j][v];
-        double im = dI[i][j][v] + (double)accI[i][j][v];
-        epilogue_store(P, m, n, re, im);
-      }
-}
-
-int gemm_bf16split(const GemmProb* probs_dev, int n, int total_tiles, int terms, hipStream_t s) {
-  if (total_tiles <= 0) return CSX_OK;
-  if (terms == 2)
-    hipLaunchKernelGGL(k_gemm_bf16split<2>, dim3(total_tiles), dim3(256), 0, s, probs_dev, n);
-  else if (terms == 3)
-    hipLaunchKernelGGL(k_gemm_bf16split<3>, dim3(total_tiles), dim3(256), 0, s, probs_dev, n);
-  else
-    return fail(CSX_ERR_ARG, "gemm_bf16split: terms must be 2 or 3");
-  CSX_HIP(hipGetLastError());
-  return CSX_OK;
-}
 
 // ------------------------------------------------- BF16-plane MFMA GEMM --
 // Both operands arrive as BF16 planes (split once: DFT kernels at plan time,
@@ -365,8 +248,10 @@ __global__ __launch_bounds__(256) void k_gemm_planes(const GemmProb* __restrict_
   const int local = tile - P.tile_start;
   const int tm = local % P.tiles_m, tn = local / P.tiles_m;
   const int m0 = tm * kTileM, n0 = tn * kTileN;
-  const unsigned short* __restrict__ A = static_cast<const unsigned short*>(P.A);
-  const unsigned short* __restrict__ B = static_cast<const unsigned short*>(P.B);
+  const GLOBAL unsigned short* A = gin<unsigned short>(P.A);
+  const GLOBAL unsigned short* B = gin<unsigned short>(P.B);
+  const int M = P.M, N = P.N, K = P.K;
+  const long sA = P.sA, sB = P.sB;
 
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int wm = wave >> 1, wn = wave & 1;
@@ -384,41 +269,33 @@ __global__ __launch_bounds__(256) void k_gemm_planes(const GemmProb* __restrict_
     }
 
   u32x4 ra[NP], rb[NP];
-  auto load_chunk = [&](const unsigned short* X, long ld, long sp, int rows, int row, int k0, int q) -> u32x4 {
-    u32x4 v = u32x4{0, 0, 0, 0};
-    int k = k0 + lk;
-    if (row < rows) {
-      const unsigned short* p = X + q * sp + (long)row * ld + k;
-      if (k + 8 <= P.K) {
-        v = *reinterpret_cast<const u32x4*>(p);
-      } else {
-        unsigned short h[8];
-#pragma unroll
-        for (int e = 0; e < 8; e++) h[e] = (k + e < P.K) ? p[e] : (unsigned short)0;
-        v = u32x4{h[0] | ((unsigned)h[1] << 16), h[2] | ((unsigned)h[3] << 16), h[4] | ((unsigned)h[5] << 16),
-                  h[6] | ((unsigned)h[7] << 16)};
-      }
-    }
-    return v;
-  };
+  // K is a multiple of 8: a chunk is wholly inside or outside.  Unconditional
+  // loads from clamped addresses, zeroed by select.
+  const int am = m0 + lrow, bn = n0 + lrow;
+  const GLOBAL unsigned short* pa = A + (long)min(am, M - 1) * P.lda;
+  const GLOBAL unsigned short* pb = B + (long)min(bn, N - 1) * P.ldb;
+  bool ia = false, ib = false;   // select applied at the LDS store: loads stay in flight
   auto gload = [&](int k0) {
+    const int k = k0 + lk, kc = min(k, K - 8);
+    ia = am < M && k < K; ib = bn < N && k < K;
 #pragma unroll
     for (int q = 0; q < NP; q++) {
-      ra[q] = load_chunk(A, P.lda, P.sA, P.M, m0 + lrow, k0, q);
-      rb[q] = load_chunk(B, P.ldb, P.sB, P.N, n0 + lrow, k0, q);
+      ra[q] = *reinterpret_cast<const GLOBAL u32x4*>(pa + q * sA + kc);
+      rb[q] = *reinterpret_cast<const GLOBAL u32x4*>(pb + q * sB + kc);
     }
   };
 
   gload(0);
-  for (int k0 = 0; k0 < P.K; k0 += BK) {
+  for (int k0 = 0; k0 < K; k0 += BK) {
     __syncthreads();
+    const u32x4 z = u32x4{0, 0, 0, 0};
 #pragma unroll
     for (int q = 0; q < NP; q++) {
-      *reinterpret_cast<u32x4*>(&As[q][lrow][lk]) = ra[q];
-      *reinterpret_cast<u32x4*>(&Bs[q][lrow][lk]) = rb[q];
+      *reinterpret_cast<u32x4*>(&As[q][lrow][lk]) = ia ? ra[q] : z;
+      *reinterpret_cast<u32x4*>(&Bs[q][lrow][lk]) = ib ? rb[q] : z;
     }
     __syncthreads();
-    if (k0 + BK < P.K) gload(k0 + BK);
+    if (k0 + BK < K) gload(k0 + BK);
     const int fr = lane & 15, fk = 8 * (lane >> 4);
     u32x4 a[NP][2], b[NP][2];
 #pragma unroll
@@ -469,6 +346,136 @@ __global__ __launch_bounds__(256) void k_gemm_planes(const GemmProb* __restrict_
       }
 }
 
+// 128x128 tile of the two-term (3-product) split: 4 waves of 64x64 (4x4 MFMA
+// 16x16x32 sub-tiles, 32 FP32 accumulators each), K slabs of 32 double-buffered
+// in LDS (2 x 64 KiB, one barrier per slab).  LDS rows are 64 B (4 chunks of
+// 16 B) with chunk c of row r stored at c ^ ((r >> 1) & 3): the 8 rows a group
+// of 8 lanes reads (or writes) then fall on 8 distinct 16-byte bank groups.
+static_assert(kBigTileM == kBigTileN, "square big tile");
+constexpr int kBigTile = kBigTileM;
+__device__ __forceinline__ int swz(int row, int chunk) { return row * 32 + ((chunk ^ ((row >> 1) & 3)) << 3); }
+
+__global__ __launch_bounds__(256, 1) void k_gemm_planes_big(const GemmProb* __restrict__ probs, int nprob) {
+  constexpr int BK = 32, NP = 4, ROWS = kBigTile;
+  constexpr int PLANE = ROWS * BK;                  // bf16 elements of one plane of one operand tile
+  constexpr int STAGE = 2 * NP * PLANE;             // A and B, all planes
+  __shared__ __align__(16) unsigned short lds[2 * STAGE];
+
+  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  const GemmProb& P = probs[find_prob(probs, nprob, tile)];
+  const int local = tile - P.tile_start;
+  const int tm = local % P.tiles_m, tn = local / P.tiles_m;
+  const int m0 = tm * kBigTile, n0 = tn * kBigTile;
+  const GLOBAL unsigned short* A = gin<unsigned short>(P.A);
+  const GLOBAL unsigned short* B = gin<unsigned short>(P.B);
+  const int M = P.M, N = P.N, K = P.K;
+  const long lda = P.lda, ldb = P.ldb, sA = P.sA, sB = P.sB;
+
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+
+  f32x4 accR[4][4], accI[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) { accR[i][j] = f32x4{0, 0, 0, 0}; accI[i][j] = f32x4{0, 0, 0, 0}; }
+
+  // loader: per operand and plane 128 rows x 4 chunks = 512 chunks, 2 per thread
+  u32x4 ra[NP][2], rb[NP][2];
+  const GLOBAL unsigned short* pa[2];
+  const GLOBAL unsigned short* pb[2];
+  bool ina[2], inb[2];
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    const int row = (h * 256 + t) >> 2;
+    ina[h] = m0 + row < M; inb[h] = n0 + row < N;
+    pa[h] = A + (long)min(m0 + row, M - 1) * lda;
+    pb[h] = B + (long)min(n0 + row, N - 1) * ldb;
+  }
+  // K is a multiple of 8: unconditional loads from clamped addresses; the data are
+  // zeroed by select only when stored to LDS, so the loads stay in flight meanwhile
+  bool ka[2], kb[2];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int k = k0 + ((h * 256 + t) & 3) * 8, kc = min(k, K - 8);
+      ka[h] = ina[h] && k < K; kb[h] = inb[h] && k < K;
+#pragma unroll
+      for (int q = 0; q < NP; q++) {
+        ra[q][h] = *reinterpret_cast<const GLOBAL u32x4*>(pa[h] + q * sA + kc);
+        rb[q][h] = *reinterpret_cast<const GLOBAL u32x4*>(pb[h] + q * sB + kc);
+      }
+    }
+  };
+  auto sstore = [&](int buf) {
+    unsigned short* base = lds + buf * STAGE;
+    const u32x4 z = u32x4{0, 0, 0, 0};
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int id = h * 256 + t, row = id >> 2, c = id & 3, o = swz(row, c);
+#pragma unroll
+      for (int q = 0; q < NP; q++) {
+        *reinterpret_cast<u32x4*>(base + q * PLANE + o) = ka[h] ? ra[q][h] : z;
+        *reinterpret_cast<u32x4*>(base + (NP + q) * PLANE + o) = kb[h] ? rb[q][h] : z;
+      }
+    }
+  };
+
+  const int nslab = (K + BK - 1) / BK;
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  const int fr = lane & 15, fc = lane >> 4;
+  const u32x4 sign = u32x4{0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
+  for (int s = 0; s < nslab; s++) {
+    const int cur = s & 1;
+    if (s + 1 < nslab) gload((s + 1) * BK);
+    const unsigned short* base = lds + cur * STAGE;
+    u32x4 b[NP][4];
+#pragma unroll
+    for (int q = 0; q < NP; q++)
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        b[q][j] = *reinterpret_cast<const u32x4*>(base + (NP + q) * PLANE + swz(wn * 64 + j * 16 + fr, fc));
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      u32x4 a[NP];
+#pragma unroll
+      for (int q = 0; q < NP; q++)
+        a[q] = *reinterpret_cast<const u32x4*>(base + q * PLANE + swz(wm * 64 + i * 16 + fr, fc));
+#pragma unroll
+      for (int ta = 0; ta < 2; ta++) {
+        const bf16x8 ar = __builtin_bit_cast(bf16x8, a[2 * ta]);
+        const bf16x8 ai = __builtin_bit_cast(bf16x8, a[2 * ta + 1]);
+        const bf16x8 ain = __builtin_bit_cast(bf16x8, a[2 * ta + 1] ^ sign);
+#pragma unroll
+        for (int tb = 0; tb < 2 - ta; tb++)
+#pragma unroll
+          for (int j = 0; j < 4; j++) {
+            const bf16x8 br = __builtin_bit_cast(bf16x8, b[2 * tb][j]);
+            const bf16x8 bi = __builtin_bit_cast(bf16x8, b[2 * tb + 1][j]);
+            accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ar, br, accR[i][j], 0, 0, 0);
+            accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ar, bi, accI[i][j], 0, 0, 0);
+            accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ain, bi, accR[i][j], 0, 0, 0);
+            accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ai, br, accI[i][j], 0, 0, 0);
+          }
+      }
+    }
+    if (s + 1 < nslab) sstore(cur ^ 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        int m = m0 + wm * 64 + i * 16 + 4 * (lane >> 4) + v;
+        int n = n0 + wn * 64 + j * 16 + (lane & 15);
+        epilogue_store(P, m, n, (double)accR[i][j][v], (double)accI[i][j][v]);
+      }
+}
+
 int gemm_bf16planes(const GemmProb* probs_dev, int n, int total_tiles, int terms, hipStream_t s) {
   if (total_tiles <= 0) return CSX_OK;
   if (terms == 2)
@@ -477,6 +484,13 @@ int gemm_bf16planes(const GemmProb* probs_dev, int n, int total_tiles, int terms
     hipLaunchKernelGGL(k_gemm_planes<3>, dim3(total_tiles), dim3(256), 0, s, probs_dev, n);
   else
     return fail(CSX_ERR_ARG, "gemm_bf16planes: terms must be 2 or 3");
+  CSX_HIP(hipGetLastError());
+  return CSX_OK;
+}
+
+int gemm_bf16planes_big(const GemmProb* probs_dev, int n, int total_tiles, hipStream_t s) {
+  if (total_tiles <= 0) return CSX_OK;
+  hipLaunchKernelGGL(k_gemm_planes_big, dim3(total_tiles), dim3(256), 0, s, probs_dev, n);
   CSX_HIP(hipGetLastError());
   return CSX_OK;
 }

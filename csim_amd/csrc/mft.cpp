@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <tuple>
@@ -11,14 +12,100 @@ namespace csx {
 
 int terms_of(csx_precision p) { return p == CSX_BF16X3 ? 2 : (p == CSX_BF16X6 ? 3 : 0); }
 
-int launch_gemm(const GemmProb* probs_dev, int n, int tiles, csx_precision p, hipStream_t s) {
+int launch_gemm(const GemmProb* probs_dev, int n, int tiles, csx_precision p, hipStream_t s, int big) {
   switch (p) {
     case CSX_FP64: return gemm_f64(probs_dev, n, tiles, s);
     case CSX_FP32: return gemm_f32(probs_dev, n, tiles, s);
-    case CSX_BF16X3: return gemm_bf16planes(probs_dev, n, tiles, 2, s);
+    case CSX_BF16X3: return big ? gemm_bf16planes_big(probs_dev, n, tiles, s) : gemm_bf16planes(probs_dev, n, tiles, 2, s);
     case CSX_BF16X6: return gemm_bf16planes(probs_dev, n, tiles, 3, s);
   }
   return fail(CSX_ERR_ARG, "unknown precision");
+}
+
+// Tile variants a precision can launch: index 0 = 64 x 64, 1 = 128 x 128.
+static int n_variants(csx_precision p) { return p == CSX_BF16X3 ? 2 : 1; }
+
+// Forced variant from CSX_GEMM_TILE (64 or 128), else -1 (choose by timing).
+static int forced_variant() {
+  const char* e = std::getenv("CSX_GEMM_TILE");
+  if (!e || !*e) return -1;
+  int v = std::atoi(e);
+  return v == 128 ? 1 : (v == 64 ? 0 : -1);
+}
+
+// Autotuner cache: problem-list signature -> chosen variant (process lifetime).
+static std::map<std::string, int>& tune_cache() {
+  static std::map<std::string, int> c;
+  return c;
+}
+
+static std::string signature(const std::vector<GemmProb>& v, csx_precision p) {
+  std::string s = std::to_string((int)p);
+  for (const auto& g : v) {
+    s += ':';
+    s += std::to_string(g.M) + 'x' + std::to_string(g.N) + 'x' + std::to_string(g.K);
+  }
+  return s;
+}
+
+// Uploads the problem list planned for `variant` into `dev`; returns the tile count.
+static int upload(std::vector<GemmProb>& v, int variant, DevBuf& dev, hipStream_t s, int* tiles) {
+  int tm = variant ? kBigTileM : kTileM, tn = variant ? kBigTileN : kTileN;
+  *tiles = plan_tiles(v.data(), (int)v.size(), tm, tn);
+  CSX_TRY(dev.reserve(v.size() * sizeof(GemmProb)));
+  CSX_HIP(hipMemcpyAsync(dev.p, v.data(), v.size() * sizeof(GemmProb), hipMemcpyHostToDevice, s));
+  CSX_HIP(hipStreamSynchronize(s));
+  return CSX_OK;
+}
+
+// Picks the fastest tile variant of one grouped launch by timing each (median of
+// 3 launches after one warm-up, HIP events on the launch stream).  The launches
+// write the stage's own output buffers only, which the real run overwrites.
+static int choose_variant(std::vector<GemmProb>& v, csx_precision p, DevBuf& dev, hipStream_t s, int* variant,
+                          int* tiles) {
+  const int nv = n_variants(p);
+  int forced = forced_variant();
+  if (nv == 1 || forced >= 0) {
+    *variant = nv == 1 ? 0 : forced;
+    return upload(v, *variant, dev, s, tiles);
+  }
+  const std::string key = signature(v, p);
+  auto it = tune_cache().find(key);
+  if (it != tune_cache().end()) {
+    *variant = it->second;
+    return upload(v, *variant, dev, s, tiles);
+  }
+  hipEvent_t e0, e1;
+  CSX_HIP(hipEventCreate(&e0));
+  CSX_HIP(hipEventCreate(&e1));
+  float best = 1e30f;
+  int best_v = 0;
+  int rc = CSX_OK;
+  for (int var = 0; var < nv && rc == CSX_OK; var++) {
+    int nt = 0;
+    if ((rc = upload(v, var, dev, s, &nt)) != CSX_OK) break;
+    std::vector<float> ms;
+    for (int r = 0; r < 4 && rc == CSX_OK; r++) {
+      if (hipEventRecord(e0, s) != hipSuccess) { rc = fail(CSX_ERR_HIP, "autotune: event record"); break; }
+      if ((rc = launch_gemm(dev.as<GemmProb>(), (int)v.size(), nt, p, s, var)) != CSX_OK) break;
+      if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+        rc = fail(CSX_ERR_HIP, "autotune: event wait");
+        break;
+      }
+      float t = 0;
+      (void)hipEventElapsedTime(&t, e0, e1);
+      if (r > 0) ms.push_back(t);
+    }
+    if (rc != CSX_OK) break;
+    std::sort(ms.begin(), ms.end());
+    if (ms[1] < best) { best = ms[1]; best_v = var; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != CSX_OK) return rc;
+  tune_cache()[key] = best_v;
+  *variant = best_v;
+  return upload(v, best_v, dev, s, tiles);
 }
 
 namespace {
@@ -98,11 +185,11 @@ int MftPlan::build(const std::vector<MftField>& fields, csx_precision p, hipStre
     const long npix_in = (long)F.in_x.n * F.in_y.n, nt = (long)F.out_y.n * F.in_x.n;
     GemmProb& a = p1[f];
     std::memset(&a, 0, sizeof(a));
-    a.A = F.in; a.lda = F.in_y.n; a.sA = npix_in;
-    a.B = static_cast<char*>(dft_.p) + offAy[f] * esz; a.ldb = F.in_y.n; a.sB = (long)F.out_y.n * F.in_y.n;
+    a.A = static_cast<char*>(dft_.p) + offAy[f] * esz; a.lda = F.in_y.n; a.sA = (long)F.out_y.n * F.in_y.n;
+    a.B = F.in; a.ldb = F.in_y.n; a.sB = npix_in;
     a.C = static_cast<char*>(tbuf_.p) + offT[f] * esz; a.ldc = F.in_x.n; a.sC = nt;
-    a.M = F.in_x.n; a.N = F.out_y.n; a.K = F.in_y.n;
-    a.transC = 1;
+    a.M = F.out_y.n; a.N = F.in_x.n; a.K = F.in_y.n;
+    a.transC = 0;
     a.out_kind = T ? OUT_PLANES : (c64 ? OUT_C64 : OUT_C128);
     a.terms = T;
     a.alpha_re = 1.0; a.alpha_im = 0.0;
@@ -134,13 +221,9 @@ int MftPlan::build(const std::vector<MftField>& fields, csx_precision p, hipStre
   }
   // DFT kernels are read once per launch as well
   for (const auto& k : keys) { (void)k; }
-  tiles1_ = plan_tiles(p1.data(), nf_, kTileM, kTileN);
-  tiles2_ = plan_tiles(p2.data(), nf_, kTileM, kTileN);
-  CSX_TRY(probs1_.reserve(nf_ * sizeof(GemmProb)));
-  CSX_TRY(probs2_.reserve(nf_ * sizeof(GemmProb)));
-  CSX_HIP(hipMemcpyAsync(probs1_.p, p1.data(), nf_ * sizeof(GemmProb), hipMemcpyHostToDevice, s));
-  CSX_HIP(hipMemcpyAsync(probs2_.p, p2.data(), nf_ * sizeof(GemmProb), hipMemcpyHostToDevice, s));
-  CSX_HIP(hipStreamSynchronize(s));   // host vectors go out of scope
+  // stage 2 reads stage 1's output, so stage 1 is tuned (and left written) first
+  CSX_TRY(choose_variant(p1, p, probs1_, s, &var1_, &tiles1_));
+  CSX_TRY(choose_variant(p2, p, probs2_, s, &var2_, &tiles2_));
   return CSX_OK;
 }
 
@@ -159,10 +242,10 @@ int MftPlan::run(hipStream_t s, KernelTimer* t) const {
   const bool timed = t && t->enabled();
   std::string base = std::string("gemm_") + prec_name(prec_);
   if (timed) CSX_TRY(t->begin(base, flops1_, bytes1_, s));
-  CSX_TRY(launch_gemm(probs1_.as<GemmProb>(), nf_, tiles1_, prec_, s));
+  CSX_TRY(launch_gemm(probs1_.as<GemmProb>(), nf_, tiles1_, prec_, s, var1_));
   if (timed) CSX_TRY(t->end(s));
   if (timed) CSX_TRY(t->begin(base, flops2_, bytes2_, s));
-  CSX_TRY(launch_gemm(probs2_.as<GemmProb>(), nf_, tiles2_, prec_, s));
+  CSX_TRY(launch_gemm(probs2_.as<GemmProb>(), nf_, tiles2_, prec_, s, var2_));
   if (timed) CSX_TRY(t->end(s));
   return CSX_OK;
 }

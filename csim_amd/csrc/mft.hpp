@@ -35,8 +35,9 @@ class KernelTimer {
 
 // One field of a grouped MFT.  Fields are column-major (rows = y).
 //   E_out = dx_in dy_in / (i lf) * A_y * E_in * B_x^T
-// computed y-first:  T^T = (X A_y^T)^T  then  X_out = B_x T  with X = E^T
-// (the column-major field read as a row-major matrix).
+// computed y-first:  T^T = A_y E  (stage 1, row-major out_y x in_x)  then
+// X_out = B_x (T^T)^T = E_out^T  (stage 2), with X = E^T the column-major field
+// read as a row-major matrix.  Both stages are NT GEMMs (K-contiguous operands).
 // Element storage of a field operand.
 enum FieldKind : int { FK_C128 = 0, FK_C64 = 1, FK_PLANES = 2 };
 int terms_of(csx_precision p);   // BF16 terms per value (0 for FP64 / FP32)
@@ -61,6 +62,7 @@ class MftPlan {
   // Enqueues both GEMM stages (no allocation, no host sync: graph-capturable).
   int run(hipStream_t s, KernelTimer* timer = nullptr) const;
   size_t workspace_bytes() const { return dft_.bytes + tbuf_.bytes; }
+  int variant(int stage) const { return stage == 0 ? var1_ : var2_; }
   int n_fields() const { return nf_; }
   double flops() const { return flops_; }   // algorithmic, 8 per complex MAC
   double flops_stage(int i) const { return i == 0 ? flops1_ : flops2_; }
@@ -71,9 +73,10 @@ class MftPlan {
   csx_precision prec_ = CSX_FP64;
   DevBuf dft_, tbuf_, probs1_, probs2_, specs_;
   int nf_ = 0, tiles1_ = 0, tiles2_ = 0;
+  int var1_ = 0, var2_ = 0;   // tile variant of each stage (0: 64 x 64, 1: 128 x 128)
   double flops_ = 0, flops1_ = 0, flops2_ = 0, bytes1_ = 0, bytes2_ = 0;
 };
 
-int launch_gemm(const GemmProb* probs_dev, int n, int tiles, csx_precision p, hipStream_t s);
+int launch_gemm(const GemmProb* probs_dev, int n, int tiles, csx_precision p, hipStream_t s, int big = 0);
 
 }  // namespace csx

@@ -577,4 +577,81 @@ int embed_center(const double2* src, int srows, int scols, double2* dst, int dro
   return CSX_OK;
 }
 
+// ------------------------------------------------ convolution-form FPM --
+// fpmPupToLyot (src/elements/fpmPupToLyot.cpp:24-167).  Kernel input of the mask
+// transform: out[k] = factor * (src[k] - minus1), columns mirrored for fliplr.
+__global__ void k_fpm_prep(const double2* __restrict__ src, long src_stride, int rows, int cols, int n,
+                           int fliplr, int minus1, double factor, double2* __restrict__ out) {
+  long np = (long)rows * cols, total = np * n;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    long k = i / np, p = i - k * np;
+    int c = (int)(p / rows), r = (int)(p - (long)c * rows);
+    int sc = fliplr ? cols - 1 - c : c;
+    double2 v = src[k * src_stride + (long)sc * rows + r];
+    if (minus1) v.x -= 1.0;
+    out[i] = make_double2(factor * v.x, factor * v.y);
+  }
+}
+int fpm_prep(const double2* src, long src_stride, int rows, int cols, int n, int fliplr, int minus1, double factor,
+             double2* out, hipStream_t s) {
+  long total = (long)rows * cols * n;
+  if (total <= 0) return CSX_OK;
+  hipLaunchKernelGGL(k_fpm_prep, dim3(grid_for(total)), dim3(256), 0, s, src, src_stride, rows, cols, n, fliplr,
+                     minus1, factor, out);
+  CSX_HIP(hipGetLastError());
+  return CSX_OK;
+}
+
+// P[f](r, c) *= H[h(f)](r, c) (-1)^(r + c): the transform of fft_shift(maskHat)
+// for an even size is (-1)^(r+c) times the transform of maskHat.
+__global__ void k_spectrum_mul(double2* __restrict__ P, int n2, const double2* __restrict__ H, int per_cand, int nl,
+                               int nmask_cand) {
+  const int f = blockIdx.y;
+  int cand = f / per_cand, l = f % nl;
+  if (cand >= nmask_cand) cand = nmask_cand - 1;
+  const long np = (long)n2 * n2;
+  double2* p = P + (long)f * np;
+  const double2* h = H + ((long)cand * nl + l) * np;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < np; i += (long)gridDim.x * blockDim.x) {
+    int c = (int)(i / n2), r = (int)(i - (long)c * n2);
+    double2 a = h[i], b = p[i];
+    double2 v = make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    if ((r + c) & 1) { v.x = -v.x; v.y = -v.y; }
+    p[i] = v;
+  }
+}
+int spectrum_mul(double2* P, int n2, int nfield, const double2* H, int per_cand, int nl, int nmask_cand,
+                 hipStream_t s) {
+  if (nfield <= 0) return CSX_OK;
+  long np = (long)n2 * n2;
+  int bx = (int)std::min<long>(2048, (np + 255) / 256);
+  hipLaunchKernelGGL(k_spectrum_mul, dim3(bx, nfield), dim3(256), 0, s, P, n2, H, per_cand, nl, nmask_cand);
+  CSX_HIP(hipGetLastError());
+  return CSX_OK;
+}
+
+// E[f] = scale[l] * P[f](centre) + (babinet ? E[f] : 0), centre rows/cols n/2 .. 3n/2 - 1.
+__global__ void k_crop_add(double2* __restrict__ E, int n, const double2* __restrict__ P, int n2, int nfield, int nl,
+                           const double2* __restrict__ scale, int babinet) {
+  long np = (long)n * n, total = np * nfield;
+  int o = n / 2;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    long f = i / np, p = i - f * np;
+    int c = (int)(p / n), r = (int)(p - (long)c * n);
+    double2 a = P[f * (long)n2 * n2 + (long)(c + o) * n2 + (r + o)];
+    double2 sc = scale[f % nl];
+    double2 v = make_double2(a.x * sc.x - a.y * sc.y, a.x * sc.y + a.y * sc.x);
+    if (babinet) { double2 e = E[i]; v.x += e.x; v.y += e.y; }
+    E[i] = v;
+  }
+}
+int crop_add(double2* E, int n, const double2* P, int n2, int nfield, int nl, const double2* scale_dev, int babinet,
+             hipStream_t s) {
+  long total = (long)n * n * nfield;
+  if (total <= 0) return CSX_OK;
+  hipLaunchKernelGGL(k_crop_add, dim3(grid_for(total)), dim3(256), 0, s, E, n, P, n2, nfield, nl, scale_dev, babinet);
+  CSX_HIP(hipGetLastError());
+  return CSX_OK;
+}
+
 }  // namespace csx

@@ -62,4 +62,15 @@ int bin_sums(const double2* E, long npix, int nfield, const int32_t* bin, int nb
 int embed_center(const double2* src, int srows, int scols, double2* dst, int drows, int dcols,
                  int nfield, int crop, hipStream_t s);
 
+// Convolution-form FPM (fpmPupToLyot.cpp:24-167), square even fields, 2n padding.
+// out[k] = factor (src[k] - minus1) over n masks of rows x cols (columns mirrored if fliplr)
+int fpm_prep(const double2* src, long src_stride, int rows, int cols, int n, int fliplr, int minus1, double factor,
+             double2* out, hipStream_t s);
+// P[f] *= H[cand(f) * nl + l(f)] (-1)^(r+c), cand = f / per_cand (clamped), l = f % nl
+int spectrum_mul(double2* P, int n2, int nfield, const double2* H, int per_cand, int nl, int nmask_cand,
+                 hipStream_t s);
+// E[f] = scale[f % nl] * centre(P[f]) + (babinet ? E[f] : 0)
+int crop_add(double2* E, int n, const double2* P, int n2, int nfield, int nl, const double2* scale_dev, int babinet,
+             hipStream_t s);
+
 }  // namespace csx

@@ -92,6 +92,8 @@ extern "C" int csx_hex_resp_objective(csx_ctx* ctx, const void* d_M, int npix, i
   if (!ctx || !d_M || !lambdas || !d_X || !d_obj) return fail(CSX_ERR_ARG, "csx_hex_resp_objective: null argument");
   if (npix <= 0 || ncol != nopt + 1 || nslices <= 0 || ncand <= 0 || calib == 0)
     return fail(CSX_ERR_ARG, "csx_hex_resp_objective: bad sizes (ncol must be nopt + 1)");
+  if (p != CSX_FP64 && p != CSX_FP32)
+    return fail(CSX_ERR_UNSUPPORTED, "csx_hex_resp_objective: precision must be CSX_FP64 or CSX_FP32");
   CSX_HIP(hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
   const bool c64 = p != CSX_FP64;

@@ -12,6 +12,9 @@ Workloads (sizes and physics of the headline configurations):
   1  2048^2 field, two PIAA sag mirrors, 1 m angular-spectrum step, downsample x2, then 0
   2  1024^2 pupil, 9 wavelengths 600-660 nm x 3 sources (0, 2, 6 lambda/D), 217-hex FPM
   3  DE batch: 9 wavelengths, on-axis, hex FPM; pupil 1024^2 or 512^2 (radii halved)
+  p2l  the FPM as a pupil-plane convolution (csim's fpmPupToLyot) with each of its
+       three masks: hex (fpmIntHexCMCForPupToLyot), complex cube (fpmCMCForPupToLyot),
+       opaque disk (fpmBinaryForPupToLyot)
 """
 from __future__ import annotations
 
@@ -284,3 +287,54 @@ def apply(chain, w: Workload):
         chain.set_complex_array(elem, key, a, sl)
     if w.meta.get("sags") is not None:
         chain.set_opt_data("fpm", "sags", w.meta["sags"])
+
+
+def workload_p2l(mask="hex", n=256, n_lambda=2, offsets=(0.0, 2.0), rings=3, mask_size=64, n_sub_pix=4):
+    """pupil -> fpmPupToLyot (2n x 2n convolution) -> Lyot stop -> 128^2-class image."""
+    seed = BASE_SEED + 10
+    pix = PIX * 1024 / n
+    r_ap = 500 * n // 1024
+    beam_r = r_ap * pix
+    Lx_over_D = n / (2.0 * r_ap)
+    sources = [(1.0, t * Lx_over_D / 2.0, 0.0) for t in offsets]
+    lams = poly_lambdas(n_lambda)
+    lam_ref = 630e-9
+    loD = F_RATIO * lam_ref
+    lines = ["	start", "		fpmPupToLyot", "		name:fpm", "		disableForCalibration:on",
+             f"		focalRatio:{F_RATIO!r}", "		fRatioSign:-1", "		zoomFftSign:-1"]
+    arrays = {("pupil", "filename"): pupil_amplitude(n, r_ap, 400 * n / 1024),
+              ("lyot", "filename"): circle(n, 450 * n / 1024)}
+    complex_arrays = {}
+    meta = {"seed": seed, "n": n, "fields": n_lambda * len(offsets), "lambdas": lams}
+    if mask == "hex":
+        hopt = hex_options(lam_ref, n_sub_pix, "standard", rings, mask_size=mask_size)
+        lines += ["		start", "			fpmIntHexCMCForPupToLyot", "			zoomFactor:1", "			start",
+                  "				complexHexMaskFPM", "				name:hexmask"]
+        lines += [f"				{k}:{v}" for k, v in hopt.items()]
+        lines += ["			end", "		end"]
+        n_hex = 1 + 3 * rings * (rings + 1)
+        t = hex_thickness(seed, 1, n_hex)[0]
+        meta.update(n_hex=n_hex, thickness=t, sags=thickness_to_sag(t), hex=hopt)
+    elif mask == "cmc":
+        S = 48
+        amp = 0.5 + 0.5 * philox(seed, 2, S * S).reshape(S, S)
+        ph = 2 * math.pi * philox(seed, 3, S * S).reshape(S, S)
+        complex_arrays[("fpm", "mask", 0)] = amp * np.exp(-1j * ph)
+        lines += ["		start", "			fpmCMCForPupToLyot", f"			fpmMatScale:{loD / 6!r}", "		end"]
+    elif mask == "binary":
+        S = 64
+        outer = 4.0
+        x = (np.arange(S) - (S - 1) / 2.0) * (2 * outer / S)
+        rr = np.sqrt(x[None, :] ** 2 + x[:, None] ** 2)
+        arrays[("fpm", "maskFilename")] = (rr < 2.5).astype(np.float64)
+        arrays[("fpm", "calibFilename")] = np.ones((S, S))
+        lines += ["		start", "			fpmBinaryForPupToLyot", f"			referenceLambda:{lam_ref!r}",
+                  "			innerRadiusFld:0", f"			outerRadiusFld:{outer!r}", "		end"]
+    else:
+        raise ValueError(mask)
+    lines.append("	end")
+    script = (_telescope(2 * beam_r) + _efield(n, pix, beam_r, lams, sources) +
+              "start\n\tcoronagraph\n\tstart\n\t\tmask\n\t\tname:pupil\n\tend\n" +
+              "\n".join(lines) + "\n" + _tail(lam_ref))
+    return Workload(10, f"fpmPupToLyot ({mask}) {n}^2 pupil, {n_lambda} lambda x {len(offsets)} sources",
+                    script, arrays, complex_arrays, meta)

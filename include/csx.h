@@ -216,7 +216,8 @@ int csx_region_mean(csx_ctx* ctx, const double* d_intensity, int npix, int n_gro
 /* optHexRespObjective::execute (src/optimization/optObjective.cpp:24-69): the
  * response-matrix objective as a batched complex GEMM on MFMA.  d_M: nslices x
  * (ncol x npix) complex128 with M[s] column-major npix x ncol, ncol = nopt+1;
- * d_X: ncand x nopt; d_obj: ncand.  objective 0 = mean, 1 = max. */
+ * d_X: ncand x nopt; d_obj: ncand.  objective 0 = mean, 1 = max.
+ * Precision CSX_FP64 or CSX_FP32 (others: CSX_ERR_UNSUPPORTED). */
 int csx_hex_resp_objective(csx_ctx* ctx, const void* d_M, int npix, int ncol, int nslices,
                            const double* lambdas, double phase_sign, double calib,
                            const double* d_X, int nopt, int ncand, int objective,

@@ -15,6 +15,7 @@ product code:
   downsample      src/elements/downsample.cpp:23-96
   doubleFraunhofer src/elements/doubleFraunhoferFPM.cpp:23-110, doubleFraunhoferInterpFPM.cpp
   hex FPM         src/elements/complexHexMaskFPM.cpp
+  fpmPupToLyot    src/elements/fpmPupToLyot.cpp:24-167, 240-496 (three mask kinds)
   fraunhoferFocal src/elements/fraunhoferFocal.cpp:23-72
   region contrast src/tools/regionContrast.cpp:89-205
 The additive keys (fpmSamplesPerflD, fpmFOVflD, occulterRadiusFld, hexLayout)
@@ -27,8 +28,8 @@ import math
 
 import numpy as np
 
-from . import (center_mask_multiply, downsample, fresnel_as, grid_symmetric, hex_centers, hex_design_array,
-               hex_interp_data, hex_mask, interp2, mask_apply, mirror_apply, point_source, zoomfft)
+from . import (center_mask_multiply, downsample, fpm_pup_to_lyot, fresnel_as, grid_symmetric, hex_centers,
+               hex_design_array, hex_interp_data, hex_mask, interp2, mask_apply, mirror_apply, point_source, zoomfft)
 
 
 # ------------------------------------------------------------- parsing ---
@@ -70,6 +71,9 @@ def blocks(cmds):
 
 def _f(a):
     return float(a.strip())
+
+
+P2L_MASKS = ("fpmCMCForPupToLyot", "fpmIntHexCMCForPupToLyot", "fpmBinaryForPupToLyot")
 
 
 class Model:
@@ -127,10 +131,20 @@ class Model:
 
     @staticmethod
     def _element(sb):
-        e = {"type": sb[0][0], "name": sb[0][0], "position": 0.0, "disable": False, "hex": None}
+        e = {"type": sb[0][0], "name": sb[0][0], "position": 0.0, "disable": False, "hex": None,
+             "mask_type": None, "mask": {}}
         for sub in blocks(sb):
             if sub and sub[0][0] == "complexHexMaskFPM":
                 e["hex"] = {k: a.strip() for k, a in sub[1:]}
+            elif sub and sub[0][0] in P2L_MASKS:
+                # fpmPupToLyot mask sub-block (fpmPupToLyot.cpp:172-195), possibly
+                # holding a complexHexMaskFPM block
+                e["mask_type"] = sub[0][0]
+                for msub in blocks(sub):
+                    if msub and msub[0][0] == "complexHexMaskFPM":
+                        e["hex"] = {k: a.strip() for k, a in msub[1:]}
+                    else:
+                        e["mask"].update({k: a.strip() for k, a in msub if k != sub[0][0]})
             else:
                 for k, a in sub:
                     a = a.strip()
@@ -172,7 +186,10 @@ class HexGeometry:
         self.gap = float(h["hexGap"])
         self.nsp = int(h["nSubPix"])
         self.S = int(h["fpmMaskSize"])
-        self.pixel = float(h["pixelScale"])
+        if "pixelScale" in h:
+            self.pixel = float(h["pixelScale"])
+        else:   # init_mask (complexHexMaskFPM.cpp:193-234) with the physical mask size
+            self.pixel = float(h["fpmMaskPhysicalSize"]) / self.S
         self.radius = float(h["fpmRadius"])
         self.phase_sign = float(h.get("phaseSign", 1))
         hx, hy, _ = hex_centers(self.rings, self.layout, self.step)
@@ -221,7 +238,7 @@ class OracleChain:
         els = m.elements
         for i, e in enumerate(els):
             t = e["type"]
-            if calibrating and e["disable"]:
+            if calibrating and e["disable"] and t != "fpmPupToLyot":
                 continue
             if t == "mask":
                 E = mask_apply(E, resample_to(m.arrays[(e["name"], "filename")], m.pix, m.cols, g))
@@ -248,6 +265,8 @@ class OracleChain:
                     rr = np.sqrt(x[None, :] ** 2 + x[:, None] ** 2) / m.loD(ref)
                     Ef = Ef * np.where(rr < float(e["occulterRadiusFld"]), 0.0, 1.0)
                 E = zoomfft(Ef, (gf[1], gf[1], gf[2], gf[2]), (g[0], g[1], g[1], g[2], g[2]), -lf)
+            elif t == "fpmPupToLyot":
+                E = self._pup_to_lyot(e, E, g, lam, calibrating, cand_sags)
             elif t == "fraunhoferFocal":
                 ref = float(e["referenceLambda"])
                 spl, fov = float(e["samplesPerflD"]), float(e["FOVflD"])
@@ -258,6 +277,37 @@ class OracleChain:
                 raise NotImplementedError(t)
         self.out_grid = g
         return np.asarray(E)
+
+    def _pup_to_lyot(self, e, E, g, lam, calibrating, cand_sags):
+        """fpmPupToLyot::execute (src/elements/fpmPupToLyot.cpp:24-167) with the mask
+        of its sub-block: K = fliplr(hex mask) - 1, K = cube - 1 (Babinet, identity
+        when disabled for calibration) or the opaque mask itself."""
+        m = self.m
+        mt, mk = e["mask_type"], e["mask"]
+        frs = float(e.get("fRatioSign", 1))
+        zs = float(e.get("zoomFftSign", 1))
+        if mt != "fpmBinaryForPupToLyot":
+            if calibrating and e["disable"]:
+                return E                                  # K = exp(0) - 1 = 0
+            lf = frs * lam * float(e["focalRatio"]) * 2 * m.R
+            if mt == "fpmIntHexCMCForPupToLyot":
+                hg = self.hex[e["name"]]
+                sags = cand_sags if cand_sags is not None else self.sags
+                K = hg.mask(sags, lam)[:, ::-1] - 1.0
+                mdx = hg.pixel
+            else:
+                idx = int(mk.get("maskIndex", -1))
+                K = np.asarray(m.arrays[(e["name"], "mask")], dtype=np.complex128)
+                if K.ndim == 3:
+                    K = K[..., max(idx, 0)]
+                K = K - 1.0
+                mdx = float(mk["fpmMatScale"])
+            return fpm_pup_to_lyot(E, g[2], K, mdx, lf, zs, frs, True)
+        key = "calibFilename" if (calibrating and e["disable"]) else "maskFilename"
+        K = np.asarray(m.arrays[(e["name"], key)], dtype=np.complex128)
+        mdx = 2 * float(mk["outerRadiusFld"]) * m.loD(float(mk["referenceLambda"])) / K.shape[0]
+        lf = frs * lam * m.F * m.D
+        return fpm_pup_to_lyot(E, g[2], K, mdx, lf, zs, frs, False)
 
     def intensity(self, calibrating=False, cand_sags=None):
         I = None

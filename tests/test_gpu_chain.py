@@ -128,3 +128,31 @@ def test_workload1_fresnel_chain(ctx):
     e = rel_rms(ch.field(), oc.field(0, 0))
     print(f"workload1 (2048^2 PIAA + Fresnel + downsample + MFT) field rel rms {e:.3e}")
     assert e < 1e-12
+
+
+@pytest.mark.parametrize("mask", ["hex", "cmc", "binary"])
+def test_fpm_pup_to_lyot_vs_oracle(ctx, mask):
+    """fpmPupToLyot (2n x 2n FFT convolution, FP64) with each mask kind against the
+    oracle's restatement of csim: calibration and occulted fields, contrast."""
+    w = W.workload_p2l(mask)
+    ch = build(ctx, w)
+    arrays = dict(w.arrays)
+    for (elem, key, sl), a in w.complex_arrays.items():
+        arrays[(elem, key)] = a
+    oc = OracleChain(w.script, arrays, sags=w.meta.get("sags"))
+    if w.meta.get("sags") is not None:
+        ch.set_opt_data("fpm", "sags", w.meta["sags"])
+    nl, ns = len(w.meta["lambdas"]), 2
+    for cal in (True, False):
+        ch.propagate(calibrating=cal)
+        worst = 0.0
+        for s in range(ns):
+            for li in range(nl):
+                worst = max(worst, rel_rms(ch.field(0, s, li), oc.field(s, li, calibrating=cal)))
+        print(f"fpmPupToLyot {mask} calibrating={cal}: worst field rel rms {worst:.3e}")
+        assert worst < 1e-12
+    c, rep = ch.contrast()
+    c_ref, _, peak_ref = oc.contrast()
+    print(f"contrast gpu {c[0]:.12e} oracle {c_ref:.12e}")
+    assert abs(rep["calib_peak"] - peak_ref) <= 1e-12 * peak_ref
+    assert abs(c[0] - c_ref) <= 1e-11 * abs(c_ref)

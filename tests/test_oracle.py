@@ -285,6 +285,34 @@ def test_fpm_pup_to_lyot_babinet_identity_and_numpy():
     assert rel_rms(out, res[n // 2:n // 2 + n, n // 2:n // 2 + n]) < 1e-12
 
 
+def test_fpm_pup_to_lyot_equals_double_mft():
+    """The 2n-padded circular convolution never wraps on the centre n x n, so it equals
+    a linear convolution with maskHat sampled half a pixel off the pupil grid:
+      E' = E + s z c  B (K . (A E A^T)) B^T,  A = exp(+2 pi i u x / lf),
+      B = exp(-2 pi i u (x + dx/2) / lf),  c = du^2 / (i lf),  s = dx^2 / (-i fs lf).
+    A wrong pad offset, a missing fft_shift or a flipped sign fails it (the
+    correction term, not the whole field, is compared)."""
+    rng = np.random.default_rng(12)
+    n, S = 32, 12
+    dx, du, lf = 1.0, 0.5, 5.0          # a few cycles across both grids, O(1) correction
+    E = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    K = rng.standard_normal((S, S)) * 0.2 + 1j * rng.standard_normal((S, S)) * 0.2
+    x = sym(n, dx) + np.arange(n) * dx
+    u = sym(S, du) + np.arange(S) * du
+    A = np.exp(2j * np.pi * u[:, None] * x[None, :] / lf)
+    B = np.exp(-2j * np.pi * (x[:, None] + dx / 2) * u[None, :] / lf)
+    for zs, fs, bab in ((-1.0, -1.0, True), (1.0, 1.0, False)):
+        out = oracle.fpm_pup_to_lyot(E, dx, K, du, lf, zs, fs, bab)
+        corr = dx * dx / (-1j * fs * lf) * zs * (du * du / (1j * lf)) * (B @ (K * (A @ E @ A.T)) @ B.T)
+        base = E if bab else 0.0
+        assert np.sqrt(np.mean(np.abs(corr) ** 2)) > 0.1 * np.sqrt(np.mean(np.abs(E) ** 2))
+        assert rel_rms(out - base, corr) < 1e-12
+    # planted mistake: the kernel sampled on the pupil grid itself (no half-pixel shift)
+    B0 = np.exp(-2j * np.pi * x[:, None] * u[None, :] / lf)
+    bad = dx * dx / (1j * lf) * -1.0 * (du * du / (1j * lf)) * (B0 @ (K * (A @ E @ A.T)) @ B0.T)
+    assert rel_rms(oracle.fpm_pup_to_lyot(E, dx, K, du, lf, -1.0, -1.0, True) - E, bad) > 1e-3
+
+
 # -------------------------------------------------------------- region ---
 def test_region_pixels_annulus():
     n, d, loD = 64, 0.5, 2.0
