@@ -928,6 +928,7 @@ struct DoubleFraunhoferElem : Element {
 // propagation (they follow the hex sags and the candidate batch).
 struct FpmPupToLyotElem : Element {
   enum MaskType { MT_NONE = 0, MT_CMC = 1, MT_HEX = 2, MT_BINARY = 3 };
+  bool calibration_aware() const override { return true; }   // the binary mask has a calibration mask
   int mtype = MT_NONE;
   double focalRatio = -1, fRatioSign = 1, zoomFftSign = 1;
   bool babinet = false;
@@ -1409,11 +1410,13 @@ int Chain::get_opt(const std::string& elem, const std::string& n, std::vector<do
 
 int Chain::set_candidates(const std::string& elem, const double* sags, int nhex, int ncand) {
   Element* e = elem_by_name(elem);
-  auto* d = dynamic_cast<DoubleFraunhoferElem*>(e);
-  if (!d || !d->has_hex) return fail(CSX_ERR_ARG, "no hex-FPM element named " + elem);
+  const HexFpm* hx = nullptr;
+  if (auto* d = dynamic_cast<DoubleFraunhoferElem*>(e)) hx = d->has_hex ? &d->hex : nullptr;
+  else if (auto* p = dynamic_cast<FpmPupToLyotElem*>(e)) hx = p->mtype == FpmPupToLyotElem::MT_HEX ? &p->hex : nullptr;
+  if (!hx) return fail(CSX_ERR_ARG, "no hex-FPM element named " + elem);
   if (ncand < 0 || (ncand > 0 && (!sags || nhex <= 0))) return fail(CSX_ERR_ARG, "bad candidate batch");
-  if (ncand > 0 && nhex != d->hex.raster.nhex)
-    return fail(CSX_ERR_ARG, "candidate vectors must have one sag per hex (" + std::to_string(d->hex.raster.nhex) + ")");
+  if (ncand > 0 && nhex != hx->raster.nhex)
+    return fail(CSX_ERR_ARG, "candidate vectors must have one sag per hex (" + std::to_string(hx->raster.nhex) + ")");
   bool shape_change = ncand != ncand_;
   if (ncand > 0) {
     CSX_TRY(cand_sags_.reserve(sizeof(double) * (size_t)nhex * ncand));
@@ -1481,7 +1484,7 @@ int Chain::compile_one(bool calibrating, Compile& c) {
   for (int i = 0; i < (int)elems_.size(); i++) {
     Element* e = elems_[i].get();
     if (c.claimed > 0) { c.claimed--; continue; }
-    if (e->skipped(c)) continue;
+    if (e->skipped(c) && !e->calibration_aware()) continue;
     CSX_TRY(e->compile(*this, c, i));
   }
   CSX_TRY(flush_pointwise(*this, c));

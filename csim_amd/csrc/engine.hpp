@@ -97,6 +97,9 @@ struct Element {
   virtual int set_array(Chain& ch, const std::string& key, const double* d, int rows, int cols);
   virtual int set_complex_array(Chain& ch, const std::string& key, const double* d, int rows, int cols, int slice);
   bool skipped(const Compile& c) const { return disableForCalibration && c.calibrating; }
+  // true: compile() is called for the calibration run even when skipped(c) (the
+  // element then applies its own calibration behaviour)
+  virtual bool calibration_aware() const { return false; }
 };
 
 class Chain {

@@ -305,24 +305,35 @@ __global__ __launch_bounds__(256) void k_gemm_planes(const GemmProb* __restrict_
         a[q][s2] = *reinterpret_cast<const u32x4*>(&As[q][wm * 32 + s2 * 16 + fr][fk]);
         b[q][s2] = *reinterpret_cast<const u32x4*>(&Bs[q][wn * 32 + s2 * 16 + fr][fk]);
       }
+    // two sweeps per product over the 2x2 sub-tiles: consecutive updates of one
+    // accumulator are 8 MFMAs apart (no back-to-back dependency stalls)
+    const u32x4 sgn = u32x4{0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
 #pragma unroll
     for (int ta = 0; ta < TERMS; ta++)
 #pragma unroll
-      for (int tb = 0; tb < TERMS - ta; tb++)
+      for (int tb = 0; tb < TERMS - ta; tb++) {
 #pragma unroll
         for (int i = 0; i < 2; i++)
 #pragma unroll
           for (int j = 0; j < 2; j++) {
             bf16x8 ar = __builtin_bit_cast(bf16x8, a[2 * ta][i]);
-            bf16x8 ai = __builtin_bit_cast(bf16x8, a[2 * ta + 1][i]);
-            bf16x8 ain = __builtin_bit_cast(bf16x8, a[2 * ta + 1][i] ^ u32x4{0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u});
             bf16x8 br = __builtin_bit_cast(bf16x8, b[2 * tb][j]);
             bf16x8 bi = __builtin_bit_cast(bf16x8, b[2 * tb + 1][j]);
             accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ar, br, accR[i][j], 0, 0, 0);
             accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ar, bi, accI[i][j], 0, 0, 0);
+          }
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+          for (int j = 0; j < 2; j++) {
+            bf16x8 ai = __builtin_bit_cast(bf16x8, a[2 * ta + 1][i]);
+            bf16x8 ain = __builtin_bit_cast(bf16x8, a[2 * ta + 1][i] ^ sgn);
+            bf16x8 br = __builtin_bit_cast(bf16x8, b[2 * tb][j]);
+            bf16x8 bi = __builtin_bit_cast(bf16x8, b[2 * tb + 1][j]);
             accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ain, bi, accR[i][j], 0, 0, 0);
             accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ai, br, accI[i][j], 0, 0, 0);
           }
+      }
     if (TERMS >= 3) {
 #pragma unroll
       for (int i = 0; i < 2; i++)
@@ -444,21 +455,29 @@ __global__ __launch_bounds__(256, 1) void k_gemm_planes_big(const GemmProb* __re
       for (int q = 0; q < NP; q++)
         a[q] = *reinterpret_cast<const u32x4*>(base + q * PLANE + swz(wm * 64 + i * 16 + fr, fc));
 #pragma unroll
+      // two sweeps per product over j: consecutive updates of one accumulator are
+      // 8 MFMAs apart (no back-to-back dependency stalls with one wave per SIMD)
       for (int ta = 0; ta < 2; ta++) {
         const bf16x8 ar = __builtin_bit_cast(bf16x8, a[2 * ta]);
         const bf16x8 ai = __builtin_bit_cast(bf16x8, a[2 * ta + 1]);
         const bf16x8 ain = __builtin_bit_cast(bf16x8, a[2 * ta + 1] ^ sign);
 #pragma unroll
-        for (int tb = 0; tb < 2 - ta; tb++)
+        for (int tb = 0; tb < 2 - ta; tb++) {
 #pragma unroll
           for (int j = 0; j < 4; j++) {
             const bf16x8 br = __builtin_bit_cast(bf16x8, b[2 * tb][j]);
             const bf16x8 bi = __builtin_bit_cast(bf16x8, b[2 * tb + 1][j]);
             accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ar, br, accR[i][j], 0, 0, 0);
             accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ar, bi, accI[i][j], 0, 0, 0);
+          }
+#pragma unroll
+          for (int j = 0; j < 4; j++) {
+            const bf16x8 br = __builtin_bit_cast(bf16x8, b[2 * tb][j]);
+            const bf16x8 bi = __builtin_bit_cast(bf16x8, b[2 * tb + 1][j]);
             accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ain, bi, accR[i][j], 0, 0, 0);
             accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ai, br, accI[i][j], 0, 0, 0);
           }
+        }
       }
     }
     if (s + 1 < nslab) sstore(cur ^ 1);

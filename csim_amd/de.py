@@ -93,7 +93,8 @@ class DEResult:
 class DifferentialEvolution:
     def __init__(self, objective, dim, cfg: DEConfig, screen=None):
         """objective(pop (D, P)) -> values (P).  screen(pop (D, K)) -> indices of the
-        candidates worth evaluating exactly (surrogate screening); None evaluates all."""
+        candidates worth evaluating exactly (surrogate screening), or (indices, exact
+        values) when the screen evaluates its survivors itself; None evaluates all."""
         self.f = objective
         self.D = dim
         self.c = cfg
@@ -115,7 +116,9 @@ class DifferentialEvolution:
         return origin[:, None] + spread * c.init_spread * (self.rng.random((self.D, P)) - 0.5) * \
             (self.ihi - self.ilo)[:, None]
 
-    def run(self, time_budget=None, target=None) -> DEResult:
+    def run(self, time_budget=None, target=None, eval_budget=None) -> DEResult:
+        """Iterates until the csim stopping rules, a wall-time budget (s), a target
+        objective or a budget of exact evaluations is reached."""
         c = self.c
         t0 = time.perf_counter()
         origin = c.init_mean if c.init_mean is not None else 0.5 * (self.ilo + self.ihi)
@@ -152,6 +155,8 @@ class DifferentialEvolution:
                 break
             if target is not None and gbest_val <= target:
                 break
+            if eval_budget is not None and self.evals >= eval_budget:
+                break
             if small >= c.max_small_change or do_reinit or spread_mean < c.reinit_spread:
                 best_val = np.inf
                 strategy = STRATEGIES[c.reinit_strategy]
@@ -176,10 +181,16 @@ class DifferentialEvolution:
             cand = np.where(above, up[:, None] + r * (original - up[:, None]), cand)
             # selection (optionally on a surrogate-screened subset)
             if self.screen is not None:
-                idx = np.asarray(self.screen(cand), dtype=int)
+                res = self.screen(cand)
                 cv = np.full(cand.shape[1], np.inf)
-                if idx.size:
-                    cv[idx] = self._eval(cand[:, idx])
+                if isinstance(res, tuple):          # the screen evaluated its survivors itself
+                    idx = np.asarray(res[0], dtype=int)
+                    self.evals += idx.size
+                    cv[idx] = res[1]
+                else:
+                    idx = np.asarray(res, dtype=int)
+                    if idx.size:
+                        cv[idx] = self._eval(cand[:, idx])
             else:
                 cv = self._eval(cand)
             better = cv < val

@@ -1,0 +1,229 @@
+"""Surrogate-screened differential evolution against plain DE (no reference counterpart).
+
+The hex-FPM optimisation of csim (differential evolution over the sag vector,
+src/optimization/differentialEvolution/, objective = dark-hole mean contrast,
+src/tools/regionContrast.cpp:145-205) run two ways on the batched GPU chain:
+
+  plain      every mutant of a generation is propagated (9 wavelengths each) and
+             selected on its exact dark-hole contrast;
+  screened   a 512-mutant generation is ranked by the MLP surrogate (hex
+             thicknesses + wavelength -> 32 annulus log10-contrasts) and only the
+             top 32 are propagated; the exact values of those survivors decide the
+             selection, are the surrogate's measured error, and are added to its
+             training set (a few fine-tuning steps per generation).
+
+The surrogate is trained first on propagations of random thickness vectors
+(t ~ U(0, 1 um)) made by the same chain.  Usage:
+
+    python -m csim_amd.study [--n 512] [--train 22528] [--seconds 20]
+
+prints one JSON object (data-generation and training times, held-out rank
+correlation, and both DE traces with the time each reached the plain run's
+final contrast).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import time
+
+import numpy as np
+
+from . import workloads as W
+from .chain import Chain
+from .de import DEConfig, DifferentialEvolution
+from .surrogate import MLP, N_BINS, annulus_bins, features, objective_from_profiles
+
+
+class ExactObjective:
+    """Dark-hole mean contrast of every candidate of a population, in fixed-size
+    candidate batches (one compiled chain per batch size).  Optionally returns the
+    per-(candidate, wavelength) annulus means used as surrogate labels."""
+
+    def __init__(self, ctx, workload, batch, precision="bf16x3"):
+        self.w = workload
+        self.batch = batch
+        self.ch = Chain(ctx, workload.script)
+        W.apply(self.ch, workload)
+        self.ch.set_precision(precision)
+        self.nhex = workload.meta["n_hex"]
+        self.ch.set_candidates("fpm", np.zeros((batch, self.nhex)))
+        self.ch.contrast()                          # compile and calibrate once
+        self.bins, self.counts = annulus_bins(self.ch)
+        self.peak = self.ch.contrast()[1]["calib_peak"]
+        self.nl = self.ch.info()["n_lambdas"]
+        self.calls = 0
+
+    def evaluate(self, thickness, with_profiles=False):
+        """thickness: (P, n_hex) metres -> contrast (P,) [, labels (P, n_lambda, N_BINS)]."""
+        P = thickness.shape[0]
+        vals = np.empty(P)
+        prof = np.empty((P, self.nl, N_BINS)) if with_profiles else None
+        for i in range(0, P, self.batch):
+            chunk = thickness[i:i + self.batch]
+            k = chunk.shape[0]
+            if k < self.batch:                      # pad with copies: the batch size stays fixed
+                chunk = np.concatenate([chunk, np.repeat(chunk[-1:], self.batch - k, axis=0)])
+            self.ch.set_candidates("fpm", W.thickness_to_sag(chunk))
+            c, _ = self.ch.contrast()
+            vals[i:i + k] = c[:k]
+            if with_profiles:
+                p = self.ch.field_profiles(self.bins, N_BINS).reshape(self.batch, self.nl, N_BINS)
+                prof[i:i + k] = p[:k] / self.counts[None, None, :] / self.peak
+            self.calls += k
+        return (vals, prof) if with_profiles else vals
+
+    def __call__(self, pop):            # DE interface: pop (D, P)
+        return self.evaluate(np.ascontiguousarray(pop.T))
+
+
+def log_labels(prof):
+    return np.log10(np.maximum(prof, 1e-14)).astype(np.float32)
+
+
+def spearman(a, b):
+    ra = np.argsort(np.argsort(a))
+    rb = np.argsort(np.argsort(b))
+    return float(np.corrcoef(ra, rb)[0, 1])
+
+
+class ScreenedSurrogate:
+    """Ranks mutants by predicted objective, keeps the top `keep`, and learns from the
+    exact values of the survivors: their labels go into a device ring buffer and a few
+    Adam steps on random batches of it follow every generation."""
+
+    def __init__(self, mlp, lambdas, counts, exact, keep=32, tune_steps=4, lr=3e-4, capacity=32768, seed=5):
+        import torch
+        self.mlp, self.lambdas, self.counts, self.exact = mlp, list(lambdas), counts, exact
+        self.keep, self.tune_steps, self.lr = keep, tune_steps, lr
+        self.Xd = torch.empty((capacity, mlp.n_in), dtype=torch.float32, device="cuda")
+        self.Yd = torch.empty((capacity, mlp.n_out), dtype=torch.float32, device="cuda")
+        self.cap, self.pos, self.filled = capacity, 0, 0
+        self.gen = torch.Generator(device="cuda")
+        self.gen.manual_seed(seed)
+        self.errors = []                # per generation: Spearman(pred, exact) of the survivors
+
+    def predict(self, thickness):
+        y = self.mlp.predict(features(thickness, self.lambdas)).astype(np.float64)
+        C = 10.0 ** y.reshape(thickness.shape[0], len(self.lambdas), -1)
+        return objective_from_profiles(C, self.counts)
+
+    def _push(self, X, Y):
+        import torch
+        n = X.shape[0]
+        idx = (self.pos + np.arange(n)) % self.cap
+        it = torch.from_numpy(idx).cuda()
+        self.Xd.index_copy_(0, it, torch.from_numpy(X).cuda())
+        self.Yd.index_copy_(0, it, torch.from_numpy(Y).cuda())
+        self.pos = (self.pos + n) % self.cap
+        self.filled = min(self.cap, self.filled + n)
+
+    def screen_and_eval(self, pop):
+        """DE hook: pop (D, P) -> (indices evaluated, their exact values)."""
+        import torch
+        t = np.ascontiguousarray(pop.T)
+        pred = self.predict(t)
+        idx = np.argsort(pred)[:self.keep]
+        vals, prof = self.exact.evaluate(t[idx], with_profiles=True)
+        if idx.size > 2:
+            self.errors.append(spearman(pred[idx], vals))
+        self._push(features(t[idx], self.lambdas), log_labels(prof).reshape(-1, N_BINS))
+        nb = min(1024, self.filled)
+        for _ in range(self.tune_steps):
+            sel = torch.randint(0, self.filled, (nb,), device="cuda", generator=self.gen)
+            self.mlp.train_step(self.Xd.index_select(0, sel), self.Yd.index_select(0, sel), self.lr)
+        return idx, vals
+
+
+def run(n=512, n_train=22528, seconds=20.0, n_lambda=9, plain_pop=64, screen_pop=512, keep=32, epochs=30,
+        precision="bf16x3", seed=20261207, log=print):
+    import torch
+    from .ops import Context
+    ctx = Context(0)
+    w = W.workload3(n=n, n_lambda=n_lambda)
+    nhex = w.meta["n_hex"]
+    out = {"pupil": n, "n_lambda": n_lambda, "n_hex": nhex, "precision": precision}
+
+    # ---- training data: random thickness vectors, exact propagations
+    gen = ExactObjective(ctx, w, 256, precision)
+    lambdas = gen.ch.lambdas()
+    rng = np.random.Generator(np.random.Philox(seed))
+    T = rng.random((n_train, nhex)) * 1e-6
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    vals, prof = gen.evaluate(T, with_profiles=True)
+    torch.cuda.synchronize()
+    out["data_seconds"] = time.perf_counter() - t0
+    out["data_propagations"] = int(n_train * n_lambda)
+    log(f"data: {n_train} candidates x {n_lambda} lambda in {out['data_seconds']:.2f} s")
+    # profiles reproduce the exact objective (consistency of the labels)
+    rebuilt = objective_from_profiles(prof, gen.counts)
+    out["label_consistency_max_rel"] = float(np.max(np.abs(rebuilt - vals) / vals))
+
+    # ---- train (hold out the last 10 %)
+    nh = n_train // 10
+    X = features(T, lambdas)
+    Y = log_labels(prof).reshape(-1, N_BINS)
+    ntr = (n_train - nh) * n_lambda
+    mlp = MLP(ctx, X.shape[1], 512, 512, N_BINS, seed=seed & 0xffff)
+    t0 = time.perf_counter()
+    hist = mlp.fit(X[:ntr], Y[:ntr], epochs=epochs, batch=1024, lr=1e-3, seed=seed)
+    torch.cuda.synchronize()
+    out["train_seconds"] = time.perf_counter() - t0
+    out["train_loss"] = [float(hist[0]), float(hist[-1])]
+    sur = ScreenedSurrogate(mlp, lambdas, gen.counts, None, keep=keep)
+    pred = sur.predict(T[n_train - nh:])
+    out["holdout_spearman"] = spearman(pred, vals[n_train - nh:])
+    out["holdout_rel_err_median"] = float(np.median(np.abs(pred - vals[n_train - nh:]) / vals[n_train - nh:]))
+    log(f"train {out['train_seconds']:.2f} s, loss {hist[0]:.4f} -> {hist[-1]:.4f}, "
+        f"held-out Spearman {out['holdout_spearman']:.3f}")
+    gen.ch.close()
+
+    # ---- plain DE (every mutant propagated)
+    cfg = DEConfig(population_size=plain_pop, global_bounds=(0.0, 1e-6), init_bounds=(0.0, 1e-6),
+                   max_iterations=10 ** 9, seed=seed + 1)
+    exact_p = ExactObjective(ctx, w, plain_pop, precision)
+    de = DifferentialEvolution(exact_p, nhex, cfg)
+    torch.cuda.synchronize()
+    rp = de.run(time_budget=seconds)
+    out["plain"] = {"population": plain_pop, "best": rp.best_value, "seconds": rp.seconds,
+                    "evaluations": rp.evaluations, "generations": len(rp.history) - 1}
+    log(f"plain DE: best {rp.best_value:.4e} after {rp.seconds:.1f} s, {rp.evaluations} candidates")
+    exact_p.ch.close()
+
+    # ---- screened DE (512 mutants ranked, top `keep` propagated)
+    exact_s = ExactObjective(ctx, w, keep, precision)
+    sur.exact = exact_s
+    cfg_s = DEConfig(population_size=screen_pop, global_bounds=(0.0, 1e-6), init_bounds=(0.0, 1e-6),
+                     max_iterations=10 ** 9, seed=seed + 2)
+    de_s = DifferentialEvolution(exact_s, nhex, cfg_s, screen=sur.screen_and_eval)
+    rs = de_s.run(time_budget=seconds, target=rp.best_value)
+    reach = next((h[1] for h in rs.history if h[2] <= rp.best_value), None)
+    out["screened"] = {"population": screen_pop, "keep": keep, "best": rs.best_value, "seconds": rs.seconds,
+                       "evaluations": rs.evaluations, "generations": len(rs.history) - 1,
+                       "seconds_to_plain_best": reach,
+                       "survivor_spearman_median": float(np.median(sur.errors)) if sur.errors else None}
+    out["wall_ratio_vs_plain"] = (reach / rp.seconds) if reach else None
+    out["wall_ratio_incl_training"] = ((reach + out["data_seconds"] + out["train_seconds"]) / rp.seconds
+                                       if reach else None)
+    log(f"screened DE: best {rs.best_value:.4e} after {rs.seconds:.1f} s; reached plain best at {reach}")
+    exact_s.ch.close()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--train", type=int, default=22528)
+    ap.add_argument("--seconds", type=float, default=20.0)
+    ap.add_argument("--lambdas", type=int, default=9)
+    ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--precision", default="bf16x3")
+    a = ap.parse_args()
+    res = run(n=a.n, n_train=a.train, seconds=a.seconds, n_lambda=a.lambdas, epochs=a.epochs, precision=a.precision,
+              log=lambda s: print(s, flush=True))
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

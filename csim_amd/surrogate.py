@@ -1,7 +1,7 @@
 """Learned surrogate for DE screening (no reference counterpart).
 
 An MLP  (n_hex thicknesses + wavelength) -> 32 annulus log10-contrasts over
-3-10 lambda/D, trained on propagations of the batched GPU chain with the
+3-10 lambda/D (thicknesses embedded as hex transmission phasors), trained on propagations of the batched GPU chain with the
 library's own kernels (csx_mlp_*: BF16 operands, FP32 accumulation, Adam).
 The DE objective (dark-hole mean of sum_lambda I / peak) is rebuilt from the
 predicted per-wavelength annulus contrasts; the surrogate ranks a large
@@ -43,15 +43,26 @@ def labels_from_chain(chain, bins, counts, peak):
     return prof / counts[None, None, :] / peak
 
 
+N_INDEX = 1.46          # hex material index: phase 2 pi (n - 1) t / lambda
+
+
 def features(thickness, lambdas):
-    """(n_cand, n_hex) thicknesses x n_lambda -> (n_cand * n_lambda, n_hex + 1) float32."""
-    t = np.asarray(thickness) / T_SCALE
+    """(n_cand, n_hex) thicknesses x n_lambda -> (n_cand * n_lambda, 2 n_hex + 1) float32.
+
+    The network's input is the thickness vector and the wavelength; its first,
+    parameter-free layer is the physical embedding of each hex as its transmission
+    phasor cos / sin(2 pi (n-1) t / lambda), in which the focal-plane field is linear
+    (and the annulus intensities quadratic), followed by the normalised wavelength."""
+    t = np.asarray(thickness, dtype=np.float64)
+    lam = np.asarray(lambdas, dtype=np.float64)
     nc, nh = t.shape
-    nl = len(lambdas)
-    X = np.empty((nc, nl, nh + 1), dtype=np.float32)
-    X[:, :, :nh] = t[:, None, :]
-    X[:, :, nh] = (np.asarray(lambdas) - LAM_CENTER) / LAM_SCALE
-    return X.reshape(nc * nl, nh + 1)
+    nl = lam.size
+    ph = 2 * np.pi * (N_INDEX - 1.0) * t[:, None, :] / lam[None, :, None]     # (nc, nl, nh)
+    X = np.empty((nc, nl, 2 * nh + 1), dtype=np.float32)
+    X[:, :, :nh] = np.cos(ph)
+    X[:, :, nh:2 * nh] = np.sin(ph)
+    X[:, :, 2 * nh] = (lam - LAM_CENTER) / LAM_SCALE
+    return X.reshape(nc * nl, 2 * nh + 1)
 
 
 def objective_from_profiles(C_lam_bins, counts):
