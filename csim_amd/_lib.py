@@ -81,6 +81,7 @@ SIGNATURES = {
     "csx_hex_resp_objective": (_i, [_p, _p, _i, _i, _i, _p, _d, _d, _p, _i, _i, _i, _p, _i]),
     "csx_philox_fill": (_i, [_u64, _u64, _p, _sz, _i]),
     "csx_philox4x32_10": (None, [_p, _p, _p]),
+    "csx_hex_phasor_features": (_i, [_p, _p, _i, _i, _p, _i, _d, _d, _d, _p]),
     "csx_mlp_create": (_i, [_p, _i, _i, _i, _i, _u64, C.POINTER(_p)]),
     "csx_mlp_destroy": (None, [_p]),
     "csx_mlp_get_params": (_i, [_p, _p]),

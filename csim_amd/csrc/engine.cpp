@@ -174,10 +174,22 @@ static csx_grid gy(const Geom& g) { return csx_grid{g.ny, g.y0, g.dy}; }
 // ======================================================= fused pointwise ==
 // Consecutive pointwise elements (and the point-source initialisation) are
 // applied in one pass over the fields; any other element flushes them first.
-static int flush_pointwise(Chain& ch, Compile& c) {
+// to_conv: a following MFT reads the result at the working precision, which the
+// pass then writes straight into the conversion buffer (no separate split pass).
+static int flush_pointwise(Chain& ch, Compile& c, bool to_conv = false) {
   if (c.pending.empty()) return CSX_OK;
   if (c.in_conv) return fail(CSX_ERR_STATE, "pointwise factors pending on converted fields");
   PwChain pc{};
+  const int T = terms_of(ch.precision());
+  const bool c64 = ch.precision() == CSX_FP32;
+  if (to_conv && (T || c64)) {
+    pc.out_kind = T ? PW_OUT_PLANES : PW_OUT_C64;
+    pc.out = ch.c64_buf();
+    pc.terms = T;
+    c.in_conv = true;
+  } else {
+    pc.out_kind = PW_OUT_C128;
+  }
   pc.nops = (int)c.pending.size();
   for (int i = 0; i < pc.nops; i++) pc.op[i] = c.pending[i];
   pc.rows = c.geom.ny;
@@ -187,7 +199,7 @@ static int flush_pointwise(Chain& ch, Compile& c) {
   pc.lambdas = c.calibrating ? ch.scratch("lam_cal").as<double>() : ch.lambdas_dev();
   double2* E = ch.field_buf(c.buf);
   int nf = ch.nfield_for(c);
-  std::string name = "pointwise x" + std::to_string(pc.nops);
+  std::string name = "pointwise x" + std::to_string(pc.nops) + (pc.out_kind != PW_OUT_C128 ? " -> working precision" : "");
   c.stages.push_back([=](hipStream_t s) { return pointwise_chain(E, nf, pc, s); });
   c.stage_names.push_back(name);
   c.pending.clear();
@@ -505,7 +517,7 @@ static int compile_mft(Chain& ch, Compile& c, const std::vector<Geom>& gout_per_
                        const std::vector<double>& lf_per_lam, const std::string& label,
                        const std::vector<MftField>* masks = nullptr, bool out_conv = false,
                        const double* epi_mask_1pi = nullptr) {
-  CSX_TRY(flush_pointwise(ch, c));
+  CSX_TRY(flush_pointwise(ch, c, /*to_conv=*/true));
   const int nf = ch.nfield_for(c), nl = ch.nlam();
   const Geom gin = c.geom;
   const int T = terms_of(ch.precision());
@@ -898,7 +910,6 @@ struct DoubleFraunhoferElem : Element {
     }
     if (mask_stage) { c.stages.push_back(mask_stage); c.stage_names.push_back("hex masks " + name); }
     // pupil -> FPM (mask in the epilogue) -> pupil grid of the input
-    CSX_TRY(flush_pointwise(ch, c));
     CSX_TRY(compile_mft(ch, c, gf, lf, type + " in " + name, &mk, /*out_conv=*/true));
     // a mask element right after folds into the epilogue of the return MFT
     const double* lyot = nullptr;

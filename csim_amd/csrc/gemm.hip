@@ -6,6 +6,7 @@
 // (wavelength x source x candidate, uneven sizes allowed): a grouped GEMM whose
 // workgroups find their problem by binary search over per-problem tile offsets.
 #include "gemm.hpp"
+#include "planes.hpp"
 
 #include <hip/hip_bf16.h>
 #include <algorithm>
@@ -43,41 +44,9 @@ __device__ __forceinline__ int find_prob(const GemmProb* __restrict__ probs, int
   return lo;
 }
 
-__device__ __forceinline__ unsigned short bf16_rne(float x) {
-  unsigned int u = __float_as_uint(x);
-  unsigned int r = u + 0x7FFFu + ((u >> 16) & 1u);
-  return (unsigned short)(r >> 16);
-}
-__device__ __forceinline__ float bf16_val(unsigned short h) { return __uint_as_float(((unsigned int)h) << 16); }
-
-// Pointers read from the problem array are generic; casting them to the global
-// address space makes the loads global_load (vmcnt only) instead of flat_load,
-// which also counts against lgkmcnt and so stalls every LDS wait behind HBM.
-#define GLOBAL __attribute__((address_space(1)))
-template <typename T> __device__ __forceinline__ const GLOBAL T* gin(const void* p) { return (const GLOBAL T*)p; }
-template <typename T> __device__ __forceinline__ GLOBAL T* gout(void* p) { return (GLOBAL T*)p; }
-// builtin vector types for complex values behind global pointers (the HIP vector
-// classes have no address-space-qualified copy)
-typedef double f64x2 __attribute__((ext_vector_type(2)));
-typedef float f32x2 __attribute__((ext_vector_type(2)));
-
-// (re, im) split into `terms` BF16 parts each: x = hi + mid + lo, every part the
-// round-to-nearest-even of what the previous ones leave (from the FP32 value).
-template <typename Ptr>
-__device__ __forceinline__ void store_planes(Ptr base, long off, long stride, int terms, double vr, double vi) {
-  float xr = (float)vr, xi = (float)vi;
-  for (int t = 0; t < terms; t++) {
-    unsigned short hr = bf16_rne(xr), hi = bf16_rne(xi);
-    base[(2 * t) * stride + off] = hr;
-    base[(2 * t + 1) * stride + off] = hi;
-    xr -= bf16_val(hr);
-    xi -= bf16_val(hi);
-  }
-}
-
-// Epilogue: alpha, mask, store (shared by every kernel).
-__device__ __forceinline__ void epilogue_store(const GemmProb& P, int m, int n, double re, double im) {
-  if (m >= P.M || n >= P.N) return;
+// Epilogue: alpha, mask (epilogue_value), store (shared by every kernel).
+__device__ __forceinline__ void epilogue_value(const GemmProb& P, int m, int n, double re, double im, double& out_r,
+                                               double& out_i) {
   double vr = P.alpha_re * re - P.alpha_im * im;
   double vi = P.alpha_re * im + P.alpha_im * re;
   if (P.mask_kind != MASK_NONE) {
@@ -101,6 +70,13 @@ __device__ __forceinline__ void epilogue_store(const GemmProb& P, int m, int n, 
       }
     }
   }
+  out_r = vr;
+  out_i = vi;
+}
+__device__ __forceinline__ void epilogue_store(const GemmProb& P, int m, int n, double re, double im) {
+  if (m >= P.M || n >= P.N) return;
+  double vr, vi;
+  epilogue_value(P, m, n, re, im, vr, vi);
   long off = P.transC ? (long)n * P.ldc + m : (long)m * P.ldc + n;
   if (P.out_kind == OUT_C128) gout<f64x2>(P.C)[off] = f64x2{vr, vi};
   else if (P.out_kind == OUT_C64) gout<f32x2>(P.C)[off] = f32x2{(float)vr, (float)vi};
@@ -237,241 +213,149 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 // fields by the producing epilogue), so the main loop is 16-byte copies into
 // LDS and v_mfma_f32_16x16x32_bf16; products kept: (a, b) terms with a + b < TERMS.
 // FP32 accumulators are flushed into FP64 after every K slab of 32 when TERMS == 3.
-template <int TERMS>
-__global__ __launch_bounds__(256) void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
-  constexpr int BK = 32, LDK = BK + 8, NP = 2 * TERMS;
-  __shared__ __align__(16) unsigned short As[NP][kTileM][LDK];
-  __shared__ __align__(16) unsigned short Bs[NP][kTileN][LDK];
-
-  const int tile = xcd_remap(blockIdx.x, gridDim.x);
-  const GemmProb& P = probs[find_prob(probs, nprob, tile)];
-  const int local = tile - P.tile_start;
-  const int tm = local % P.tiles_m, tn = local / P.tiles_m;
-  const int m0 = tm * kTileM, n0 = tn * kTileN;
-  const GLOBAL unsigned short* A = gin<unsigned short>(P.A);
-  const GLOBAL unsigned short* B = gin<unsigned short>(P.B);
-  const int M = P.M, N = P.N, K = P.K;
-  const long sA = P.sA, sB = P.sB;
-
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
-  const int lrow = t >> 2, lk = (t & 3) * 8;     // loader: row, 8-element chunk
-
-  f32x4 accR[2][2], accI[2][2];
-  double dR[2][2][4], dI[2][2][4];
-#pragma unroll
-  for (int i = 0; i < 2; i++)
-#pragma unroll
-    for (int j = 0; j < 2; j++) {
-      accR[i][j] = f32x4{0, 0, 0, 0}; accI[i][j] = f32x4{0, 0, 0, 0};
-#pragma unroll
-      for (int v = 0; v < 4; v++) { dR[i][j][v] = 0; dI[i][j][v] = 0; }
-    }
-
-  u32x4 ra[NP], rb[NP];
-  // K is a multiple of 8: a chunk is wholly inside or outside.  Unconditional
-  // loads from clamped addresses, zeroed by select.
-  const int am = m0 + lrow, bn = n0 + lrow;
-  const GLOBAL unsigned short* pa = A + (long)min(am, M - 1) * P.lda;
-  const GLOBAL unsigned short* pb = B + (long)min(bn, N - 1) * P.ldb;
-  bool ia = false, ib = false;   // select applied at the LDS store: loads stay in flight
-  auto gload = [&](int k0) {
-    const int k = k0 + lk, kc = min(k, K - 8);
-    ia = am < M && k < K; ib = bn < N && k < K;
-#pragma unroll
-    for (int q = 0; q < NP; q++) {
-      ra[q] = *reinterpret_cast<const GLOBAL u32x4*>(pa + q * sA + kc);
-      rb[q] = *reinterpret_cast<const GLOBAL u32x4*>(pb + q * sB + kc);
-    }
-  };
-
-  gload(0);
-  for (int k0 = 0; k0 < K; k0 += BK) {
-    __syncthreads();
-    const u32x4 z = u32x4{0, 0, 0, 0};
-#pragma unroll
-    for (int q = 0; q < NP; q++) {
-      *reinterpret_cast<u32x4*>(&As[q][lrow][lk]) = ia ? ra[q] : z;
-      *reinterpret_cast<u32x4*>(&Bs[q][lrow][lk]) = ib ? rb[q] : z;
-    }
-    __syncthreads();
-    if (k0 + BK < K) gload(k0 + BK);
-    const int fr = lane & 15, fk = 8 * (lane >> 4);
-    u32x4 a[NP][2], b[NP][2];
-#pragma unroll
-    for (int q = 0; q < NP; q++)
-#pragma unroll
-      for (int s2 = 0; s2 < 2; s2++) {
-        a[q][s2] = *reinterpret_cast<const u32x4*>(&As[q][wm * 32 + s2 * 16 + fr][fk]);
-        b[q][s2] = *reinterpret_cast<const u32x4*>(&Bs[q][wn * 32 + s2 * 16 + fr][fk]);
-      }
-    // two sweeps per product over the 2x2 sub-tiles: consecutive updates of one
-    // accumulator are 8 MFMAs apart (no back-to-back dependency stalls)
-    const u32x4 sgn = u32x4{0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
-#pragma unroll
-    for (int ta = 0; ta < TERMS; ta++)
-#pragma unroll
-      for (int tb = 0; tb < TERMS - ta; tb++) {
-#pragma unroll
-        for (int i = 0; i < 2; i++)
-#pragma unroll
-          for (int j = 0; j < 2; j++) {
-            bf16x8 ar = __builtin_bit_cast(bf16x8, a[2 * ta][i]);
-            bf16x8 br = __builtin_bit_cast(bf16x8, b[2 * tb][j]);
-            bf16x8 bi = __builtin_bit_cast(bf16x8, b[2 * tb + 1][j]);
-            accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ar, br, accR[i][j], 0, 0, 0);
-            accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ar, bi, accI[i][j], 0, 0, 0);
-          }
-#pragma unroll
-        for (int i = 0; i < 2; i++)
-#pragma unroll
-          for (int j = 0; j < 2; j++) {
-            bf16x8 ai = __builtin_bit_cast(bf16x8, a[2 * ta + 1][i]);
-            bf16x8 ain = __builtin_bit_cast(bf16x8, a[2 * ta + 1][i] ^ sgn);
-            bf16x8 br = __builtin_bit_cast(bf16x8, b[2 * tb][j]);
-            bf16x8 bi = __builtin_bit_cast(bf16x8, b[2 * tb + 1][j]);
-            accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ain, bi, accR[i][j], 0, 0, 0);
-            accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ai, br, accI[i][j], 0, 0, 0);
-          }
-      }
-    if (TERMS >= 3) {
-#pragma unroll
-      for (int i = 0; i < 2; i++)
-#pragma unroll
-        for (int j = 0; j < 2; j++) {
-#pragma unroll
-          for (int v = 0; v < 4; v++) { dR[i][j][v] += accR[i][j][v]; dI[i][j][v] += accI[i][j][v]; }
-          accR[i][j] = f32x4{0, 0, 0, 0}; accI[i][j] = f32x4{0, 0, 0, 0};
-        }
-    }
-  }
-#pragma unroll
-  for (int i = 0; i < 2; i++)
-#pragma unroll
-    for (int j = 0; j < 2; j++)
-#pragma unroll
-      for (int v = 0; v < 4; v++) {
-        int m = m0 + wm * 32 + i * 16 + 4 * (lane >> 4) + v;
-        int n = n0 + wn * 32 + j * 16 + (lane & 15);
-        epilogue_store(P, m, n, dR[i][j][v] + (double)accR[i][j][v], dI[i][j][v] + (double)accI[i][j][v]);
-      }
+//
+// One template covers the tile variants the tuner chooses from: block tile
+// BM x BN, WM x WN waves (each (BM/WM) x (BN/WN) = IM x JN MFMA sub-tiles),
+// single (one barrier pair per slab, more blocks per CU) or double-buffered LDS
+// (one barrier per slab).  LDS rows are 64 B (4 chunks of 16 B) with chunk c of
+// row r stored at c ^ ((r >> 1) & 3): the 8 rows a group of 8 lanes reads or
+// writes fall on 8 distinct 16-byte bank groups.
+// With BK = 64 the rows are 128 B (8 chunks) and chunk c of row r sits at c ^ (r & 7).
+template <int BK>
+__device__ __forceinline__ int swz(int row, int chunk) {
+  return BK == 32 ? row * 32 + ((chunk ^ ((row >> 1) & 3)) << 3) : row * 64 + ((chunk ^ (row & 7)) << 3);
 }
 
-// 128x128 tile of the two-term (3-product) split: 4 waves of 64x64 (4x4 MFMA
-// 16x16x32 sub-tiles, 32 FP32 accumulators each), K slabs of 32 double-buffered
-// in LDS (2 x 64 KiB, one barrier per slab).  LDS rows are 64 B (4 chunks of
-// 16 B) with chunk c of row r stored at c ^ ((r >> 1) & 3): the 8 rows a group
-// of 8 lanes reads (or writes) then fall on 8 distinct 16-byte bank groups.
-static_assert(kBigTileM == kBigTileN, "square big tile");
-constexpr int kBigTile = kBigTileM;
-__device__ __forceinline__ int swz(int row, int chunk) { return row * 32 + ((chunk ^ ((row >> 1) & 3)) << 3); }
-
-__global__ __launch_bounds__(256, 1) void k_gemm_planes_big(const GemmProb* __restrict__ probs, int nprob) {
-  constexpr int BK = 32, NP = 4, ROWS = kBigTile;
-  constexpr int PLANE = ROWS * BK;                  // bf16 elements of one plane of one operand tile
-  constexpr int STAGE = 2 * NP * PLANE;             // A and B, all planes
-  __shared__ __align__(16) unsigned short lds[2 * STAGE];
+template <int TERMS, int BM, int BN, int WM, int WN, bool DB, int WPE, int BK>
+__global__ __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(WPE)))
+void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
+  constexpr int NT = 64 * WM * WN, NP = 2 * TERMS, CH = BK / 8;   // 16-byte chunks per LDS row
+  constexpr int IM = BM / WM / 16, JN = BN / WN / 16;
+  constexpr int PA = BM * BK, PB = BN * BK;        // bf16 elements of one plane of each operand tile
+  constexpr int STAGE = NP * (PA + PB);
+  constexpr int ITA = NP * BM * CH / NT, ITB = NP * BN * CH / NT;
+  static_assert(BK == 32 || BK == 64, "slab depth");
+  static_assert((NP * BM * CH) % NT == 0 && (NP * BN * CH) % NT == 0 && NT % CH == 0, "loader shape");
+  static_assert(TERMS == 2 || IM * JN <= 4, "FP64 flush registers");
+  __shared__ __align__(16) unsigned short lds[(DB ? 2 : 1) * STAGE];
 
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   const GemmProb& P = probs[find_prob(probs, nprob, tile)];
   const int local = tile - P.tile_start;
   const int tm = local % P.tiles_m, tn = local / P.tiles_m;
-  const int m0 = tm * kBigTile, n0 = tn * kBigTile;
+  const int m0 = tm * BM, n0 = tn * BN;
   const GLOBAL unsigned short* A = gin<unsigned short>(P.A);
   const GLOBAL unsigned short* B = gin<unsigned short>(P.B);
   const int M = P.M, N = P.N, K = P.K;
-  const long lda = P.lda, ldb = P.ldb, sA = P.sA, sB = P.sB;
 
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
+  const int wm = wave / WN, wn = wave % WN;
 
-  f32x4 accR[4][4], accI[4][4];
+  // loader: chunk e = it * NT + t of the (plane, row, chunk) list of each operand
+  // (the chunk index c = t % CH is the same for every it since NT % CH == 0);
+  // element offsets within the field (< 2^31) are fixed per it
+  const int ch = (t % CH) * 8;
+  unsigned offA[ITA], offB[ITB];
+  unsigned okA = 0, okB = 0;                      // row-in-range bits
 #pragma unroll
-  for (int i = 0; i < 4; i++)
-#pragma unroll
-    for (int j = 0; j < 4; j++) { accR[i][j] = f32x4{0, 0, 0, 0}; accI[i][j] = f32x4{0, 0, 0, 0}; }
-
-  // loader: per operand and plane 128 rows x 4 chunks = 512 chunks, 2 per thread
-  u32x4 ra[NP][2], rb[NP][2];
-  const GLOBAL unsigned short* pa[2];
-  const GLOBAL unsigned short* pb[2];
-  bool ina[2], inb[2];
-#pragma unroll
-  for (int h = 0; h < 2; h++) {
-    const int row = (h * 256 + t) >> 2;
-    ina[h] = m0 + row < M; inb[h] = n0 + row < N;
-    pa[h] = A + (long)min(m0 + row, M - 1) * lda;
-    pb[h] = B + (long)min(n0 + row, N - 1) * ldb;
+  for (int it = 0; it < ITA; it++) {
+    const int e = it * NT + t, q = e / (BM * CH), r = (e % (BM * CH)) / CH;
+    if (m0 + r < M) okA |= 1u << it;
+    offA[it] = (unsigned)(q * P.sA + (long)min(m0 + r, M - 1) * P.lda + ch);
   }
-  // K is a multiple of 8: unconditional loads from clamped addresses; the data are
-  // zeroed by select only when stored to LDS, so the loads stay in flight meanwhile
-  bool ka[2], kb[2];
+#pragma unroll
+  for (int it = 0; it < ITB; it++) {
+    const int e = it * NT + t, q = e / (BN * CH), r = (e % (BN * CH)) / CH;
+    if (n0 + r < N) okB |= 1u << it;
+    offB[it] = (unsigned)(q * P.sB + (long)min(n0 + r, N - 1) * P.ldb + ch);
+  }
+  u32x4 ra[ITA], rb[ITB];
+  unsigned vA = 0, vB = 0;                        // valid bits of the slab in registers
+  // K is a multiple of 8: a chunk is wholly inside or outside.  Loads are
+  // unconditional (clamped addresses); the data are zeroed only when stored to
+  // LDS, so the loads stay in flight meanwhile.
   auto gload = [&](int k0) {
+    if (k0 + BK <= K) {
 #pragma unroll
-    for (int h = 0; h < 2; h++) {
-      const int k = k0 + ((h * 256 + t) & 3) * 8, kc = min(k, K - 8);
-      ka[h] = ina[h] && k < K; kb[h] = inb[h] && k < K;
+      for (int it = 0; it < ITA; it++) ra[it] = *reinterpret_cast<const GLOBAL u32x4*>(A + offA[it] + k0);
 #pragma unroll
-      for (int q = 0; q < NP; q++) {
-        ra[q][h] = *reinterpret_cast<const GLOBAL u32x4*>(pa[h] + q * sA + kc);
-        rb[q][h] = *reinterpret_cast<const GLOBAL u32x4*>(pb[h] + q * sB + kc);
-      }
+      for (int it = 0; it < ITB; it++) rb[it] = *reinterpret_cast<const GLOBAL u32x4*>(B + offB[it] + k0);
+      vA = okA; vB = okB;
+    } else {
+      const int k = k0 + ch, kc = min(k, K - 8) - ch;
+      const bool kin = k < K;
+#pragma unroll
+      for (int it = 0; it < ITA; it++) ra[it] = *reinterpret_cast<const GLOBAL u32x4*>(A + offA[it] + kc);
+#pragma unroll
+      for (int it = 0; it < ITB; it++) rb[it] = *reinterpret_cast<const GLOBAL u32x4*>(B + offB[it] + kc);
+      vA = kin ? okA : 0u; vB = kin ? okB : 0u;
     }
   };
   auto sstore = [&](int buf) {
     unsigned short* base = lds + buf * STAGE;
     const u32x4 z = u32x4{0, 0, 0, 0};
+    const int c = t % CH;
 #pragma unroll
-    for (int h = 0; h < 2; h++) {
-      const int id = h * 256 + t, row = id >> 2, c = id & 3, o = swz(row, c);
+    for (int it = 0; it < ITA; it++) {
+      const int e = it * NT + t, q = e / (BM * CH), r = (e % (BM * CH)) / CH;
+      *reinterpret_cast<u32x4*>(base + q * PA + swz<BK>(r, c)) = (vA >> it) & 1 ? ra[it] : z;
+    }
 #pragma unroll
-      for (int q = 0; q < NP; q++) {
-        *reinterpret_cast<u32x4*>(base + q * PLANE + o) = ka[h] ? ra[q][h] : z;
-        *reinterpret_cast<u32x4*>(base + (NP + q) * PLANE + o) = kb[h] ? rb[q][h] : z;
-      }
+    for (int it = 0; it < ITB; it++) {
+      const int e = it * NT + t, q = e / (BN * CH), r = (e % (BN * CH)) / CH;
+      *reinterpret_cast<u32x4*>(base + NP * PA + q * PB + swz<BK>(r, c)) = (vB >> it) & 1 ? rb[it] : z;
     }
   };
 
-  const int nslab = (K + BK - 1) / BK;
-  gload(0);
-  sstore(0);
-  __syncthreads();
+  f32x4 accR[IM][JN], accI[IM][JN];
+  double dR[TERMS == 3 ? IM : 1][TERMS == 3 ? JN : 1][4], dI[TERMS == 3 ? IM : 1][TERMS == 3 ? JN : 1][4];
+#pragma unroll
+  for (int i = 0; i < IM; i++)
+#pragma unroll
+    for (int j = 0; j < JN; j++) { accR[i][j] = f32x4{0, 0, 0, 0}; accI[i][j] = f32x4{0, 0, 0, 0}; }
+  if (TERMS == 3) {
+#pragma unroll
+    for (int i = 0; i < (TERMS == 3 ? IM : 1); i++)
+#pragma unroll
+      for (int j = 0; j < (TERMS == 3 ? JN : 1); j++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) { dR[i][j][v] = 0; dI[i][j][v] = 0; }
+  }
+
   const int fr = lane & 15, fc = lane >> 4;
-  const u32x4 sign = u32x4{0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
-  for (int s = 0; s < nslab; s++) {
-    const int cur = s & 1;
-    if (s + 1 < nslab) gload((s + 1) * BK);
-    const unsigned short* base = lds + cur * STAGE;
-    u32x4 b[NP][4];
+  const u32x4 sgn = u32x4{0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
+  auto compute = [&](const unsigned short* base) {
+#pragma unroll
+   for (int kk = 0; kk < BK / 32; kk++) {
+    const int fk = kk * 4 + fc;                  // 16-byte chunk of this lane's 8 k values
+    u32x4 b[NP][JN];
 #pragma unroll
     for (int q = 0; q < NP; q++)
 #pragma unroll
-      for (int j = 0; j < 4; j++)
-        b[q][j] = *reinterpret_cast<const u32x4*>(base + (NP + q) * PLANE + swz(wn * 64 + j * 16 + fr, fc));
+      for (int j = 0; j < JN; j++)
+        b[q][j] = *reinterpret_cast<const u32x4*>(base + NP * PA + q * PB + swz<BK>(wn * (BN / WN) + j * 16 + fr, fk));
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
+    for (int i = 0; i < IM; i++) {
       u32x4 a[NP];
 #pragma unroll
-      for (int q = 0; q < NP; q++)
-        a[q] = *reinterpret_cast<const u32x4*>(base + q * PLANE + swz(wm * 64 + i * 16 + fr, fc));
+      for (int q = 0; q < NP; q++) a[q] = *reinterpret_cast<const u32x4*>(base + q * PA + swz<BK>(wm * (BM / WM) + i * 16 + fr, fk));
 #pragma unroll
-      // two sweeps per product over j: consecutive updates of one accumulator are
-      // 8 MFMAs apart (no back-to-back dependency stalls with one wave per SIMD)
-      for (int ta = 0; ta < 2; ta++) {
+      for (int ta = 0; ta < TERMS; ta++) {
         const bf16x8 ar = __builtin_bit_cast(bf16x8, a[2 * ta]);
         const bf16x8 ai = __builtin_bit_cast(bf16x8, a[2 * ta + 1]);
-        const bf16x8 ain = __builtin_bit_cast(bf16x8, a[2 * ta + 1] ^ sign);
+        const bf16x8 ain = __builtin_bit_cast(bf16x8, a[2 * ta + 1] ^ sgn);
 #pragma unroll
-        for (int tb = 0; tb < 2 - ta; tb++) {
+        for (int tb = 0; tb < TERMS - ta; tb++) {
+          // two sweeps over j: updates of one accumulator are 2 JN MFMAs apart
 #pragma unroll
-          for (int j = 0; j < 4; j++) {
+          for (int j = 0; j < JN; j++) {
             const bf16x8 br = __builtin_bit_cast(bf16x8, b[2 * tb][j]);
             const bf16x8 bi = __builtin_bit_cast(bf16x8, b[2 * tb + 1][j]);
             accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ar, br, accR[i][j], 0, 0, 0);
             accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ar, bi, accI[i][j], 0, 0, 0);
           }
 #pragma unroll
-          for (int j = 0; j < 4; j++) {
+          for (int j = 0; j < JN; j++) {
             const bf16x8 br = __builtin_bit_cast(bf16x8, b[2 * tb][j]);
             const bf16x8 bi = __builtin_bit_cast(bf16x8, b[2 * tb + 1][j]);
             accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ain, bi, accR[i][j], 0, 0, 0);
@@ -480,36 +364,133 @@ __global__ __launch_bounds__(256, 1) void k_gemm_planes_big(const GemmProb* __re
         }
       }
     }
-    if (s + 1 < nslab) sstore(cur ^ 1);
+    if (TERMS == 3) {
+#pragma unroll
+      for (int i = 0; i < (TERMS == 3 ? IM : 1); i++)
+#pragma unroll
+        for (int j = 0; j < (TERMS == 3 ? JN : 1); j++) {
+#pragma unroll
+          for (int v = 0; v < 4; v++) { dR[i][j][v] += accR[i][j][v]; dI[i][j][v] += accI[i][j][v]; }
+          accR[i][j] = f32x4{0, 0, 0, 0}; accI[i][j] = f32x4{0, 0, 0, 0};
+        }
+    }
+   }
+  };
+
+  const int nslab = (K + BK - 1) / BK;
+  if (DB) {
+    gload(0);
+    sstore(0);
     __syncthreads();
+    for (int s = 0; s < nslab; s++) {
+      const int cur = s & 1;
+      if (s + 1 < nslab) gload((s + 1) * BK);
+      compute(lds + cur * STAGE);
+      if (s + 1 < nslab) sstore(cur ^ 1);
+      __syncthreads();
+    }
+  } else {
+    gload(0);
+    for (int s = 0; s < nslab; s++) {
+      __syncthreads();
+      sstore(0);
+      __syncthreads();
+      if (s + 1 < nslab) gload((s + 1) * BK);
+      compute(lds);
+    }
+  }
+  // BF16-plane outputs are staged through LDS so that every lane stores 16
+  // contiguous bytes of one plane (the MFMA layout gives a lane 4 rows of one
+  // column).  Chunk cc (8 columns) of tile row ml sits at cc ^ (2 ((ml >> 2) & 3)):
+  // the 4 rows one store instruction covers land on distinct banks.
+  constexpr bool kStaged = 2 * TERMS * BM * BN <= (DB ? 2 : 1) * STAGE;
+  if (kStaged && P.out_kind == OUT_PLANES && !P.transC && P.terms == TERMS) {
+    __syncthreads();                               // every wave is done reading operand tiles
+#pragma unroll
+    for (int i = 0; i < IM; i++)
+#pragma unroll
+      for (int j = 0; j < JN; j++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int ml = wm * (BM / WM) + i * 16 + 4 * (lane >> 4) + v;
+          const int nl = wn * (BN / WN) + j * 16 + (lane & 15);
+          double re = accR[i][j][v], im = accI[i][j][v];
+          if (TERMS == 3) { re += dR[TERMS == 3 ? i : 0][TERMS == 3 ? j : 0][v]; im += dI[TERMS == 3 ? i : 0][TERMS == 3 ? j : 0][v]; }
+          double vr, vi;
+          epilogue_value(P, m0 + ml, n0 + nl, re, im, vr, vi);
+          const int pos = ml * BN + ((((nl >> 3) ^ (((ml >> 2) & 3) << 1))) << 3) + (nl & 7);
+          float xr = (float)vr, xi = (float)vi;
+#pragma unroll
+          for (int tt = 0; tt < TERMS; tt++) {
+            const unsigned short hr = bf16_rne(xr), hi = bf16_rne(xi);
+            lds[(2 * tt) * BM * BN + pos] = hr;
+            lds[(2 * tt + 1) * BM * BN + pos] = hi;
+            xr -= bf16_val(hr);
+            xi -= bf16_val(hi);
+          }
+        }
+    __syncthreads();
+    constexpr int CPR = BN / 8;
+    GLOBAL unsigned short* C = gout<unsigned short>(P.C);
+    for (int e = t; e < 2 * TERMS * BM * CPR; e += NT) {
+      const int q = e / (BM * CPR), rem = e % (BM * CPR), ml = rem / CPR, cc = rem % CPR;
+      const int m = m0 + ml, n = n0 + cc * 8;      // N is a multiple of 8: chunks are whole
+      if (m < M && n < N)
+        *reinterpret_cast<GLOBAL u32x4*>(C + q * P.sC + (long)m * P.ldc + n) =
+            *reinterpret_cast<const u32x4*>(lds + q * BM * BN + ml * BN + ((cc ^ (((ml >> 2) & 3) << 1)) << 3));
+    }
+    return;
   }
 #pragma unroll
-  for (int i = 0; i < 4; i++)
+  for (int i = 0; i < IM; i++)
 #pragma unroll
-    for (int j = 0; j < 4; j++)
+    for (int j = 0; j < JN; j++)
 #pragma unroll
       for (int v = 0; v < 4; v++) {
-        int m = m0 + wm * 64 + i * 16 + 4 * (lane >> 4) + v;
-        int n = n0 + wn * 64 + j * 16 + (lane & 15);
-        epilogue_store(P, m, n, (double)accR[i][j][v], (double)accI[i][j][v]);
+        const int m = m0 + wm * (BM / WM) + i * 16 + 4 * (lane >> 4) + v;
+        const int n = n0 + wn * (BN / WN) + j * 16 + (lane & 15);
+        double re = accR[i][j][v], im = accI[i][j][v];
+        if (TERMS == 3) { re += dR[TERMS == 3 ? i : 0][TERMS == 3 ? j : 0][v]; im += dI[TERMS == 3 ? i : 0][TERMS == 3 ? j : 0][v]; }
+        epilogue_store(P, m, n, re, im);
       }
 }
 
-int gemm_bf16planes(const GemmProb* probs_dev, int n, int total_tiles, int terms, hipStream_t s) {
-  if (total_tiles <= 0) return CSX_OK;
-  if (terms == 2)
-    hipLaunchKernelGGL(k_gemm_planes<2>, dim3(total_tiles), dim3(256), 0, s, probs_dev, n);
-  else if (terms == 3)
-    hipLaunchKernelGGL(k_gemm_planes<3>, dim3(total_tiles), dim3(256), 0, s, probs_dev, n);
-  else
-    return fail(CSX_ERR_ARG, "gemm_bf16planes: terms must be 2 or 3");
-  CSX_HIP(hipGetLastError());
+// Tile variants of each term count, in the order the tuner tries them.
+struct PlanesVariant {
+  int bm, bn, threads;
+  void (*kernel)(const GemmProb*, int);
+};
+// (the last template argument is the minimum waves per SIMD the registers must allow)
+static const PlanesVariant kVariants2[] = {
+    {64, 64, 256, k_gemm_planes<2, 64, 64, 2, 2, false, 3, 32>},
+    {128, 128, 256, k_gemm_planes<2, 128, 128, 2, 2, true, 1, 32>},
+    {128, 128, 512, k_gemm_planes<2, 128, 128, 2, 4, true, 2, 32>},
+    {128, 64, 256, k_gemm_planes<2, 128, 64, 2, 2, false, 2, 32>},
+    {64, 64, 256, k_gemm_planes<2, 64, 64, 2, 2, true, 2, 32>},
+    {64, 64, 256, k_gemm_planes<2, 64, 64, 2, 2, false, 2, 64>},
+    {64, 64, 256, k_gemm_planes<2, 64, 64, 2, 2, true, 2, 64>},
+};
+static const PlanesVariant kVariants3[] = {
+    {64, 64, 256, k_gemm_planes<3, 64, 64, 2, 2, false, 1, 32>},
+    {128, 64, 512, k_gemm_planes<3, 128, 64, 4, 2, false, 2, 32>},
+    {64, 64, 256, k_gemm_planes<3, 64, 64, 2, 2, true, 1, 32>},
+};
+
+int planes_variant_count(int terms) {
+  return terms == 2 ? (int)(sizeof(kVariants2) / sizeof(kVariants2[0]))
+                    : (terms == 3 ? (int)(sizeof(kVariants3) / sizeof(kVariants3[0])) : 0);
+}
+int planes_variant_tile(int terms, int v, int* bm, int* bn) {
+  if (v < 0 || v >= planes_variant_count(terms)) return fail(CSX_ERR_ARG, "planes variant out of range");
+  const PlanesVariant& pv = terms == 2 ? kVariants2[v] : kVariants3[v];
+  *bm = pv.bm; *bn = pv.bn;
   return CSX_OK;
 }
-
-int gemm_bf16planes_big(const GemmProb* probs_dev, int n, int total_tiles, hipStream_t s) {
+int gemm_bf16planes(const GemmProb* probs_dev, int n, int total_tiles, int terms, int variant, hipStream_t s) {
   if (total_tiles <= 0) return CSX_OK;
-  hipLaunchKernelGGL(k_gemm_planes_big, dim3(total_tiles), dim3(256), 0, s, probs_dev, n);
+  if (variant < 0 || variant >= planes_variant_count(terms)) return fail(CSX_ERR_ARG, "gemm_bf16planes: bad variant");
+  const PlanesVariant& pv = terms == 2 ? kVariants2[variant] : kVariants3[variant];
+  hipLaunchKernelGGL(pv.kernel, dim3(total_tiles), dim3(pv.threads), 0, s, probs_dev, n);
   CSX_HIP(hipGetLastError());
   return CSX_OK;
 }

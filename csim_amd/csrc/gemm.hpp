@@ -41,12 +41,13 @@ int plan_tiles(GemmProb* probs, int n, int tile_m, int tile_n);
 // probs_dev: device copy of the problem array (n entries), total_tiles from plan_tiles.
 int gemm_f64(const GemmProb* probs_dev, int n, int total_tiles, hipStream_t s);
 int gemm_f32(const GemmProb* probs_dev, int n, int total_tiles, hipStream_t s);
-// Operands already split into BF16 planes (A, B K-contiguous per plane, lda/ldb and
-// plane strides multiples of 8); output complex128/complex64 or planes (OUT_PLANES).
-int gemm_bf16planes(const GemmProb* probs_dev, int n, int total_tiles, int terms, hipStream_t s);
-// Two-term planes on a 128 x 128 tile (tiles planned with kBigTileM/N); no transC.
-constexpr int kBigTileM = 128, kBigTileN = 128;
-int gemm_bf16planes_big(const GemmProb* probs_dev, int n, int total_tiles, hipStream_t s);
+// Operands already split into BF16 planes (A, B K-contiguous per plane, K, lda/ldb
+// and plane strides multiples of 8); output complex128/complex64 or planes
+// (OUT_PLANES).  `variant` selects a tile shape (planes_variant_tile gives the
+// tile the problem list must be planned with).
+int planes_variant_count(int terms);
+int planes_variant_tile(int terms, int variant, int* bm, int* bn);
+int gemm_bf16planes(const GemmProb* probs_dev, int n, int total_tiles, int terms, int variant, hipStream_t s);
 // complex128 -> `terms` BF16 planes per complex element (hi, mid, lo of re and im),
 // field f's planes at out + f * 2 * terms * npix, plane q at + q * npix.
 int split_c128_to_planes(const void* in, void* out, long npix, int nfield, int terms, hipStream_t s);

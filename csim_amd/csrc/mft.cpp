@@ -12,25 +12,27 @@ namespace csx {
 
 int terms_of(csx_precision p) { return p == CSX_BF16X3 ? 2 : (p == CSX_BF16X6 ? 3 : 0); }
 
-int launch_gemm(const GemmProb* probs_dev, int n, int tiles, csx_precision p, hipStream_t s, int big) {
+int launch_gemm(const GemmProb* probs_dev, int n, int tiles, csx_precision p, hipStream_t s, int variant) {
   switch (p) {
     case CSX_FP64: return gemm_f64(probs_dev, n, tiles, s);
     case CSX_FP32: return gemm_f32(probs_dev, n, tiles, s);
-    case CSX_BF16X3: return big ? gemm_bf16planes_big(probs_dev, n, tiles, s) : gemm_bf16planes(probs_dev, n, tiles, 2, s);
-    case CSX_BF16X6: return gemm_bf16planes(probs_dev, n, tiles, 3, s);
+    case CSX_BF16X3: return gemm_bf16planes(probs_dev, n, tiles, 2, variant, s);
+    case CSX_BF16X6: return gemm_bf16planes(probs_dev, n, tiles, 3, variant, s);
   }
   return fail(CSX_ERR_ARG, "unknown precision");
 }
 
-// Tile variants a precision can launch: index 0 = 64 x 64, 1 = 128 x 128.
-static int n_variants(csx_precision p) { return p == CSX_BF16X3 ? 2 : 1; }
+// Tile variants a precision can launch (variant 0: 64 x 64).
+static int n_variants(csx_precision p) {
+  return p == CSX_BF16X3 || p == CSX_BF16X6 ? planes_variant_count(terms_of(p)) : 1;
+}
 
-// Forced variant from CSX_GEMM_TILE (64 or 128), else -1 (choose by timing).
-static int forced_variant() {
-  const char* e = std::getenv("CSX_GEMM_TILE");
+// Forced variant from CSX_GEMM_VARIANT (an index), else -1 (choose by timing).
+static int forced_variant(csx_precision p) {
+  const char* e = std::getenv("CSX_GEMM_VARIANT");
   if (!e || !*e) return -1;
   int v = std::atoi(e);
-  return v == 128 ? 1 : (v == 64 ? 0 : -1);
+  return v >= 0 && v < n_variants(p) ? v : -1;
 }
 
 // Autotuner cache: problem-list signature -> chosen variant (process lifetime).
@@ -49,8 +51,9 @@ static std::string signature(const std::vector<GemmProb>& v, csx_precision p) {
 }
 
 // Uploads the problem list planned for `variant` into `dev`; returns the tile count.
-static int upload(std::vector<GemmProb>& v, int variant, DevBuf& dev, hipStream_t s, int* tiles) {
-  int tm = variant ? kBigTileM : kTileM, tn = variant ? kBigTileN : kTileN;
+static int upload(std::vector<GemmProb>& v, csx_precision p, int variant, DevBuf& dev, hipStream_t s, int* tiles) {
+  int tm = kTileM, tn = kTileN;
+  if (p == CSX_BF16X3 || p == CSX_BF16X6) CSX_TRY(planes_variant_tile(terms_of(p), variant, &tm, &tn));
   *tiles = plan_tiles(v.data(), (int)v.size(), tm, tn);
   CSX_TRY(dev.reserve(v.size() * sizeof(GemmProb)));
   CSX_HIP(hipMemcpyAsync(dev.p, v.data(), v.size() * sizeof(GemmProb), hipMemcpyHostToDevice, s));
@@ -64,16 +67,16 @@ static int upload(std::vector<GemmProb>& v, int variant, DevBuf& dev, hipStream_
 static int choose_variant(std::vector<GemmProb>& v, csx_precision p, DevBuf& dev, hipStream_t s, int* variant,
                           int* tiles) {
   const int nv = n_variants(p);
-  int forced = forced_variant();
+  int forced = forced_variant(p);
   if (nv == 1 || forced >= 0) {
     *variant = nv == 1 ? 0 : forced;
-    return upload(v, *variant, dev, s, tiles);
+    return upload(v, p, *variant, dev, s, tiles);
   }
   const std::string key = signature(v, p);
   auto it = tune_cache().find(key);
   if (it != tune_cache().end()) {
     *variant = it->second;
-    return upload(v, *variant, dev, s, tiles);
+    return upload(v, p, *variant, dev, s, tiles);
   }
   hipEvent_t e0, e1;
   CSX_HIP(hipEventCreate(&e0));
@@ -83,7 +86,7 @@ static int choose_variant(std::vector<GemmProb>& v, csx_precision p, DevBuf& dev
   int rc = CSX_OK;
   for (int var = 0; var < nv && rc == CSX_OK; var++) {
     int nt = 0;
-    if ((rc = upload(v, var, dev, s, &nt)) != CSX_OK) break;
+    if ((rc = upload(v, p, var, dev, s, &nt)) != CSX_OK) break;
     std::vector<float> ms;
     for (int r = 0; r < 4 && rc == CSX_OK; r++) {
       if (hipEventRecord(e0, s) != hipSuccess) { rc = fail(CSX_ERR_HIP, "autotune: event record"); break; }
@@ -105,7 +108,7 @@ static int choose_variant(std::vector<GemmProb>& v, csx_precision p, DevBuf& dev
   if (rc != CSX_OK) return rc;
   tune_cache()[key] = best_v;
   *variant = best_v;
-  return upload(v, best_v, dev, s, tiles);
+  return upload(v, p, best_v, dev, s, tiles);
 }
 
 namespace {

@@ -73,10 +73,10 @@ class MftPlan {
   csx_precision prec_ = CSX_FP64;
   DevBuf dft_, tbuf_, probs1_, probs2_, specs_;
   int nf_ = 0, tiles1_ = 0, tiles2_ = 0;
-  int var1_ = 0, var2_ = 0;   // tile variant of each stage (0: 64 x 64, 1: 128 x 128)
+  int var1_ = 0, var2_ = 0;   // tile variant of each stage (index into the precision's list)
   double flops_ = 0, flops1_ = 0, flops2_ = 0, bytes1_ = 0, bytes2_ = 0;
 };
 
-int launch_gemm(const GemmProb* probs_dev, int n, int tiles, csx_precision p, hipStream_t s, int big = 0);
+int launch_gemm(const GemmProb* probs_dev, int n, int tiles, csx_precision p, hipStream_t s, int variant = 0);
 
 }  // namespace csx

@@ -157,6 +157,24 @@ __global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float
   }
 }
 
+// Surrogate input embedding: row (c, l) = [cos phi_h, sin phi_h (h < nhex), (lambda_l - lc) / ls],
+// phi_h = 2 pi (n_index - 1) t[c][h] / lambda_l in FP64.
+__global__ void k_phasor_features(const double* __restrict__ t, int ncand, int nhex, const double* __restrict__ lam,
+                                  int nlam, double n_index, double lc, double ls, float* __restrict__ out) {
+  const int row = blockIdx.x;            // c * nlam + l
+  const int c = row / nlam, l = row - c * nlam;
+  const double k = 2.0 * M_PI * (n_index - 1.0) / lam[l];
+  const int w = 2 * nhex + 1;
+  float* o = out + (long)row * w;
+  for (int h = threadIdx.x; h < nhex; h += blockDim.x) {
+    double sn, cs;
+    sincos(k * t[(long)c * nhex + h], &sn, &cs);
+    o[h] = (float)cs;
+    o[nhex + h] = (float)sn;
+  }
+  if (threadIdx.x == 0) o[2 * nhex] = (float)((lam[l] - lc) / ls);
+}
+
 }  // namespace csx
 
 using namespace csx;
@@ -250,6 +268,21 @@ static int mlp_bwd(csx_mlp* m, const float* x, const float* t, int n, double* lo
 }
 
 extern "C" {
+
+int csx_hex_phasor_features(csx_ctx* ctx, const double* d_thickness, int ncand, int nhex, const double* lambdas,
+                            int nlam, double n_index, double lam_center, double lam_scale, float* d_out) {
+  if (!ctx || !d_thickness || !lambdas || !d_out || ncand <= 0 || nhex <= 0 || nlam <= 0 || lam_scale == 0)
+    return fail(CSX_ERR_ARG, "csx_hex_phasor_features: bad arguments");
+  CSX_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  CSX_TRY(ctx->small.reserve(sizeof(double) * nlam));
+  CSX_HIP(hipMemcpyAsync(ctx->small.p, lambdas, sizeof(double) * nlam, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_phasor_features, dim3(ncand * nlam), dim3(256), 0, s, d_thickness, ncand, nhex,
+                     ctx->small.as<double>(), nlam, n_index, lam_center, lam_scale, d_out);
+  CSX_HIP(hipGetLastError());
+  CSX_HIP(hipStreamSynchronize(s));
+  return CSX_OK;
+}
 
 int csx_mlp_create(csx_ctx* ctx, int n_in, int n_h1, int n_h2, int n_out, uint64_t seed, csx_mlp** out) {
   if (!ctx || !out || n_in <= 0 || n_h1 <= 0 || n_h2 <= 0 || n_out <= 0) return fail(CSX_ERR_ARG, "csx_mlp_create: bad arguments");

@@ -1,6 +1,7 @@
 // Pointwise, FFT, resampling, hex-mask and reduction kernels (gfx950).
 // Fields are complex128, column-major (rows = y), batches field-major.
 #include "ops.hpp"
+#include "planes.hpp"
 
 #include <algorithm>
 
@@ -91,50 +92,70 @@ int k_point_sources(double2* E, int rows, int cols, int nfield, const double* pa
   return CSX_OK;
 }
 
-__global__ void k_pointwise(double2* __restrict__ E, int nfield, PwChain pc) {
+// One pass: point-source generation and/or element factors, one field per
+// blockIdx.y (per-field constants hoisted), output in place (complex128) or at
+// the working precision of a following MFT (complex64 or BF16 planes).
+__global__ void k_pointwise(double2* __restrict__ E, PwChain pc) {
 #pragma clang fp contract(off)
-  const long npix = (long)pc.rows * pc.cols, total = npix * nfield;
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
-    long f = i / npix, p = i - f * npix;
-    double2 e;
-    int first = 0;
-    if (pc.nops > 0 && pc.op[0].kind == PW_SOURCE) {
+  const int f = blockIdx.y;
+  const long npix = (long)pc.rows * pc.cols;
+  const bool gen = pc.nops > 0 && pc.op[0].kind == PW_SOURCE;
+  const int first = gen ? 1 : 0;
+  double flux = 0, tx = 0, ty = 0;
+  if (gen) { const double* prm = pc.src_params + 3 * f; flux = prm[0]; tx = prm[1]; ty = prm[2]; }
+  const double ps = pc.pixel_scale, Lx = ps * pc.cols, Ly = ps * pc.rows;
+  double mir[kMaxPwOps];                    // mirror phase factor per op (mirror.cpp:66-72)
+#pragma unroll
+  for (int k = 0; k < kMaxPwOps; k++)
+    mir[k] = (k < pc.nops && pc.op[k].kind == PW_MIRROR) ? -pc.op[k].sign * 2 * 2 * M_PI / pc.lambdas[f] : 0.0;
+  GLOBAL f64x2* Ef = gout<f64x2>(E) + (long)f * npix;
+  const int np32 = (int)npix;                // fields are far below 2^31 pixels
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np32; p += gridDim.x * blockDim.x) {
+    double er, ei;
+    if (gen) {
       // efield::init_point_sources (src/data/efield.cpp:179-206)
-      int c = (int)(p / pc.rows), r = (int)(p - (long)c * pc.rows);
-      double ps = pc.pixel_scale, Lx = ps * pc.cols, Ly = ps * pc.rows;
+      int c = p / pc.rows, r = p - c * pc.rows;
       double x = -(Lx - ps) / 2.0 + c * ps, y = -(Ly - ps) / 2.0 + r * ps;
-      const double* prm = pc.src_params + 3 * f;
-      double v = 2 * x * prm[1] / Lx + 2 * y * prm[2] / Ly;
+      double v = 2 * x * tx / Lx + 2 * y * ty / Ly;
       double sn, cs;
       sincos(2.0 * M_PI * v, &sn, &cs);
-      e = make_double2(prm[0] * cs, prm[0] * sn);
-      first = 1;
+      er = flux * cs; ei = flux * sn;
     } else {
-      e = E[i];
+      f64x2 e = Ef[p];
+      er = e.x; ei = e.y;
     }
     for (int k = first; k < pc.nops; k++) {
       const PwOp& o = pc.op[k];
       if (o.kind == PW_MASK_1PI) {           // mask.cpp: E *= (1+i) m
-        double w = static_cast<const double*>(o.p)[p];
-        e = make_double2(e.x * w - e.y * w, e.x * w + e.y * w);
-      } else if (o.kind == PW_MIRROR) {      // mirror.cpp:66-72
-        double a = -o.sign * 2 * 2 * M_PI / pc.lambdas[f];
-        double ph = a * static_cast<const double*>(o.p)[p];
+        double w = gin<double>(o.p)[p];
+        double tr = er * w - ei * w;
+        ei = er * w + ei * w;
+        er = tr;
+      } else if (o.kind == PW_MIRROR) {
+        double ph = mir[k] * gin<double>(o.p)[p];
         double sn, cs;
         sincos(ph, &sn, &cs);
-        e = make_double2(e.x * cs - e.y * sn, e.x * sn + e.y * cs);
+        double tr = er * cs - ei * sn;
+        ei = er * sn + ei * cs;
+        er = tr;
       } else if (o.kind == PW_CFACTOR) {
-        double2 w = static_cast<const double2*>(o.p)[(f / o.per) * npix + p];
-        e = make_double2(e.x * w.x - e.y * w.y, e.x * w.y + e.y * w.x);
+        f64x2 w = gin<f64x2>(o.p)[(long)(f / o.per) * npix + p];
+        double tr = er * w.x - ei * w.y;
+        ei = er * w.y + ei * w.x;
+        er = tr;
       }
     }
-    E[i] = e;
+    if (pc.out_kind == PW_OUT_C128) Ef[p] = f64x2{er, ei};
+    else if (pc.out_kind == PW_OUT_C64) gout<f32x2>(pc.out)[(long)f * npix + p] = f32x2{(float)er, (float)ei};
+    else store_planes(gout<unsigned short>(pc.out) + (long)f * 2 * pc.terms * npix, p, npix, pc.terms, er, ei);
   }
 }
 int pointwise_chain(double2* E, int nfield, const PwChain& pc, hipStream_t s) {
-  if (pc.nops <= 0) return CSX_OK;
-  long total = (long)pc.rows * pc.cols * nfield;
-  hipLaunchKernelGGL(k_pointwise, dim3(grid_for(total)), dim3(256), 0, s, E, nfield, pc);
+  if (pc.nops <= 0 || nfield <= 0) return CSX_OK;
+  if (pc.out_kind != PW_OUT_C128 && !pc.out) return fail(CSX_ERR_ARG, "pointwise: no output buffer");
+  long npix = (long)pc.rows * pc.cols;
+  int bx = (int)std::min<long>(std::max<long>(1, 4096 / nfield), (npix + 255) / 256);
+  hipLaunchKernelGGL(k_pointwise, dim3(bx, nfield), dim3(256), 0, s, E, pc);
   CSX_HIP(hipGetLastError());
   return CSX_OK;
 }

@@ -20,6 +20,7 @@ int k_point_sources(double2* E, int rows, int cols, int nfield, const double* pa
 enum PwKind : int { PW_SOURCE = 0, PW_MASK_1PI = 1, PW_MIRROR = 2, PW_CFACTOR = 3 };
 constexpr int kMaxPwOps = 6;
 struct PwOp { int kind; const void* p; double sign; int per; };   // per: fields sharing a PW_CFACTOR plane
+enum PwOut : int { PW_OUT_C128 = 0, PW_OUT_C64 = 1, PW_OUT_PLANES = 2 };
 struct PwChain {
   PwOp op[kMaxPwOps];
   int nops;
@@ -27,6 +28,9 @@ struct PwChain {
   double pixel_scale;               // PW_SOURCE grid
   const double* src_params;         // PW_SOURCE: flux, tipX, tiltY per field
   const double* lambdas;            // PW_MIRROR: per field
+  int out_kind;                     // PwOut: in place, or to `out` at a working precision
+  void* out;                        // complex64 fields or BF16 planes (field f at f * 2 terms npix)
+  int terms;
 };
 int pointwise_chain(double2* E, int nfield, const PwChain& pc, hipStream_t s);
 

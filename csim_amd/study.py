@@ -32,7 +32,7 @@ import numpy as np
 from . import workloads as W
 from .chain import Chain
 from .de import DEConfig, DifferentialEvolution
-from .surrogate import MLP, N_BINS, annulus_bins, features, objective_from_profiles
+from .surrogate import MLP, N_BINS, annulus_bins, features_device, objective_from_profiles
 
 
 class ExactObjective:
@@ -104,7 +104,7 @@ class ScreenedSurrogate:
         self.errors = []                # per generation: Spearman(pred, exact) of the survivors
 
     def predict(self, thickness):
-        y = self.mlp.predict(features(thickness, self.lambdas)).astype(np.float64)
+        y = self.mlp.forward(features_device(self.mlp.ctx, thickness, self.lambdas)).cpu().numpy().astype(np.float64)
         C = 10.0 ** y.reshape(thickness.shape[0], len(self.lambdas), -1)
         return objective_from_profiles(C, self.counts)
 
@@ -113,7 +113,7 @@ class ScreenedSurrogate:
         n = X.shape[0]
         idx = (self.pos + np.arange(n)) % self.cap
         it = torch.from_numpy(idx).cuda()
-        self.Xd.index_copy_(0, it, torch.from_numpy(X).cuda())
+        self.Xd.index_copy_(0, it, X)
         self.Yd.index_copy_(0, it, torch.from_numpy(Y).cuda())
         self.pos = (self.pos + n) % self.cap
         self.filled = min(self.cap, self.filled + n)
@@ -127,7 +127,7 @@ class ScreenedSurrogate:
         vals, prof = self.exact.evaluate(t[idx], with_profiles=True)
         if idx.size > 2:
             self.errors.append(spearman(pred[idx], vals))
-        self._push(features(t[idx], self.lambdas), log_labels(prof).reshape(-1, N_BINS))
+        self._push(features_device(self.mlp.ctx, t[idx], self.lambdas), log_labels(prof).reshape(-1, N_BINS))
         nb = min(1024, self.filled)
         for _ in range(self.tune_steps):
             sel = torch.randint(0, self.filled, (nb,), device="cuda", generator=self.gen)
@@ -162,8 +162,8 @@ def run(n=512, n_train=22528, seconds=20.0, n_lambda=9, plain_pop=64, screen_pop
 
     # ---- train (hold out the last 10 %)
     nh = n_train // 10
-    X = features(T, lambdas)
-    Y = log_labels(prof).reshape(-1, N_BINS)
+    X = features_device(ctx, T, lambdas)
+    Y = torch.from_numpy(log_labels(prof).reshape(-1, N_BINS)).cuda()
     ntr = (n_train - nh) * n_lambda
     mlp = MLP(ctx, X.shape[1], 512, 512, N_BINS, seed=seed & 0xffff)
     t0 = time.perf_counter()

@@ -65,6 +65,19 @@ def features(thickness, lambdas):
     return X.reshape(nc * nl, 2 * nh + 1)
 
 
+def features_device(ctx, thickness, lambdas):
+    """`features` computed on the GPU (csx_hex_phasor_features) -> float32 device tensor."""
+    import torch
+    t = torch.as_tensor(np.ascontiguousarray(thickness, dtype=np.float64)).cuda()
+    lam = np.ascontiguousarray(lambdas, dtype=np.float64)
+    nc, nh = t.shape
+    out = torch.empty((nc * lam.size, 2 * nh + 1), dtype=torch.float32, device=t.device)
+    check(lib().csx_hex_phasor_features(ctx.handle, C.c_void_p(t.data_ptr()), nc, nh,
+                                        lam.ctypes.data_as(C.c_void_p), lam.size, N_INDEX, LAM_CENTER,
+                                        LAM_SCALE, C.c_void_p(out.data_ptr())))
+    return out
+
+
 def objective_from_profiles(C_lam_bins, counts):
     """Dark-hole mean of sum_lambda I / peak from per-lambda annulus means."""
     w = counts / counts.sum()
@@ -107,11 +120,12 @@ class MLP:
         return y
 
     def fit(self, X, Y, epochs=20, batch=1024, lr=1e-3, seed=0, log=None):
-        """X (n, n_in) float32, Y (n, n_out) float32 targets (log10 contrast)."""
+        """X (n, n_in), Y (n, n_out) float32 (numpy or device tensors), targets log10 contrast."""
         import torch
         rng = np.random.Generator(np.random.Philox(seed))
-        Xd = torch.from_numpy(np.ascontiguousarray(X, dtype=np.float32)).cuda()
-        Yd = torch.from_numpy(np.ascontiguousarray(Y, dtype=np.float32)).cuda()
+        Xd = X if torch.is_tensor(X) else torch.from_numpy(np.ascontiguousarray(X, dtype=np.float32))
+        Yd = Y if torch.is_tensor(Y) else torch.from_numpy(np.ascontiguousarray(Y, dtype=np.float32))
+        Xd, Yd = Xd.cuda().contiguous(), Yd.cuda().contiguous()
         n = X.shape[0]
         hist = []
         for ep in range(epochs):
@@ -145,6 +159,6 @@ class Surrogate:
 
     def predict_objective(self, thickness):
         nc = thickness.shape[0]
-        y = self.mlp.predict(features(thickness, self.lambdas)).astype(np.float64)
+        y = self.mlp.forward(features_device(self.mlp.ctx, thickness, self.lambdas)).cpu().numpy().astype(np.float64)
         C = 10.0 ** y.reshape(nc, len(self.lambdas), -1)
         return objective_from_profiles(C, self.counts)

@@ -90,7 +90,8 @@ int csx_chain_get_opt_data(csx_chain* chain, const char* element_name, const cha
                            double* v, size_t* n);
 
 /* Candidate batch axis: n_cand sag vectors (n_hex each, candidate-major) for the
- * hex FPM named element_name.  Every propagation then runs
+ * hex FPM named element_name (doubleFraunhoferInterpFPM, or fpmPupToLyot with an
+ * fpmIntHexCMCForPupToLyot mask).  Every propagation then runs
  * n_cand x n_sources x n_lambda fields. n_cand = 0 returns to the single sag set. */
 int csx_chain_set_candidates(csx_chain* chain, const char* element_name,
                              const double* sags, int n_hex, int n_cand);
@@ -233,6 +234,14 @@ void csx_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 /* 218 -> 512 -> 512 -> 32 GELU perceptron trained by the library's own HIP
  * kernels (BF16 weights, FP32 accumulation, FP32 master weights, Adam).  No
  * counterpart in the reference. */
+/* Surrogate input embedding (no reference counterpart): for every candidate c
+ * (d_thickness: ncand x nhex, metres) and wavelength l, one float row
+ * [cos phi_h, sin phi_h, (lambda_l - lam_center) / lam_scale], h < nhex,
+ * phi_h = 2 pi (n_index - 1) t_h / lambda_l; d_out: (ncand nlam) x (2 nhex + 1). */
+int csx_hex_phasor_features(csx_ctx* ctx, const double* d_thickness, int ncand, int nhex,
+                            const double* lambdas, int nlam, double n_index, double lam_center,
+                            double lam_scale, float* d_out);
+
 typedef struct csx_mlp csx_mlp;
 int csx_mlp_create(csx_ctx* ctx, int n_in, int n_h1, int n_h2, int n_out, uint64_t seed,
                    csx_mlp** out);

@@ -83,27 +83,31 @@ def test_mft_mixed_error_level(ctx, prec, tol):
     assert e < tol
 
 
+VARIANTS = {"bf16x3": 7, "bf16x6": 3}
+
+
+@pytest.mark.parametrize("prec,tol", [("bf16x3", 5e-5), ("bf16x6", 5e-6)])
 @pytest.mark.parametrize("nin,nout,batch", [(1024, 256, 2), (200, 72, 3), (96, 136, 2)])
-def test_mft_bf16x3_tile_variants(ctx, monkeypatch, nin, nout, batch):
-    """The 64x64 and 128x128 two-term kernels take the same products in the same
-    order; both stay at the split-BF16 error level against FP64 on ragged sizes."""
+def test_mft_planes_tile_variants(ctx, monkeypatch, prec, tol, nin, nout, batch):
+    """Every tile variant of the split-BF16 kernels takes the same products in the same
+    order per K slab; all stay at the split-BF16 error level against FP64 on ragged sizes."""
     rng = np.random.default_rng(nin + 7 * nout)
     E = np.stack([rand_field(rng, nin, nin) for _ in range(batch)])
     dx, lf = 1e-5, 650e-9
     dxo = lf / (nin * dx) / 2
     gi, go = sym(nin, dx), sym(nout, dxo)
     ref = ops.to_host(ops.mft(ctx, ops.to_device(E), gi, gi, go, go, lf, "fp64"))
-    outs = {}
-    for tile in ("64", "128"):
-        monkeypatch.setenv("CSX_GEMM_TILE", tile)
-        outs[tile] = ops.to_host(ops.mft(ctx, ops.to_device(E), gi, gi, go, go, lf, "bf16x3"))
-    monkeypatch.delenv("CSX_GEMM_TILE")
-    for b in range(batch):
-        e64, e128 = rel_rms(outs["64"][b], ref[b]), rel_rms(outs["128"][b], ref[b])
-        d = rel_rms(outs["128"][b], outs["64"][b])
-        print(f"{nin}->{nout} field {b}: 64 {e64:.3e}  128 {e128:.3e}  between {d:.3e}")
-        assert e64 < 5e-5 and e128 < 5e-5
-        assert d < 1e-6
+    outs = []
+    for v in range(VARIANTS[prec]):
+        monkeypatch.setenv("CSX_GEMM_VARIANT", str(v))
+        outs.append(ops.to_host(ops.mft(ctx, ops.to_device(E), gi, gi, go, go, lf, prec)))
+    monkeypatch.delenv("CSX_GEMM_VARIANT")
+    for v, o in enumerate(outs):
+        for b in range(batch):
+            e, d = rel_rms(o[b], ref[b]), rel_rms(o[b], outs[0][b])
+            print(f"{prec} {nin}->{nout} variant {v} field {b}: vs fp64 {e:.3e}, vs variant 0 {d:.3e}")
+            assert e < tol
+            assert d < 1e-6
 
 
 @pytest.mark.parametrize("n", [8, 64, 256, 2048])
