@@ -109,7 +109,7 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--precision", default="bf16x6", choices=["fp32", "bf16x3", "bf16x6"])
+    ap.add_argument("--precision", default="bf16x3", choices=["fp32", "bf16x3", "bf16x6"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     a = ap.parse_args(argv)

@@ -41,8 +41,8 @@ const char* csx_last_error(void) { return g_err.c_str(); }
 const char* csx_version(void) { return "csx 0.1 (gfx950)"; }
 
 int csx_ctx_create(int device, unsigned flags, csx_ctx** out) {
-  (void)flags;
   if (!out) return fail(CSX_ERR_ARG, "null out");
+  if (flags & ~CSX_CTX_DEFAULT_STREAM) return fail(CSX_ERR_ARG, "csx_ctx_create: unknown flags");
   *out = nullptr;
   int n = 0;
   CSX_HIP(hipGetDeviceCount(&n));
@@ -51,9 +51,14 @@ int csx_ctx_create(int device, unsigned flags, csx_ctx** out) {
   csx_ctx* c = new (std::nothrow) csx_ctx();
   if (!c) return fail(CSX_ERR_NOMEM, "context allocation failed");
   c->device = device;
-  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-  if (e != hipSuccess) { delete c; return fail(CSX_ERR_HIP, hipGetErrorString(e)); }
-  c->own_stream = true;
+  if (flags & CSX_CTX_DEFAULT_STREAM) {
+    c->stream = nullptr;   // the null stream
+    c->own_stream = false;
+  } else {
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { delete c; return fail(CSX_ERR_HIP, hipGetErrorString(e)); }
+    c->own_stream = true;
+  }
   *out = c;
   return CSX_OK;
 }
@@ -61,7 +66,7 @@ int csx_ctx_create(int device, unsigned flags, csx_ctx** out) {
 void csx_ctx_destroy(csx_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
-  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  (void)hipStreamSynchronize(ctx->stream);
   if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }

@@ -258,12 +258,13 @@ static int mlp_bwd(csx_mlp* m, const float* x, const float* t, int n, double* lo
   CSX_TRY(launch(g, m->mode, s));
   hipLaunchKernelGGL(k_colsum, dim3((m->nh1 + 255) / 256), dim3(256), 0, s, d1, n, m->nh1, m->g(1));
   CSX_HIP(hipGetLastError());
+  if (!loss) return CSX_OK;   // no read-back: the step stays asynchronous
   std::vector<double> part(blocks);
   CSX_HIP(hipMemcpyAsync(part.data(), m->partial.p, sizeof(double) * blocks, hipMemcpyDeviceToHost, s));
   CSX_HIP(hipStreamSynchronize(s));
   double L = 0;
   for (double v : part) L += v;
-  if (loss) *loss = L;
+  *loss = L;
   return CSX_OK;
 }
 

@@ -24,13 +24,24 @@ def _torch():
 _live_contexts = weakref.WeakSet()
 
 
+CTX_DEFAULT_STREAM = 1     # csx.h CSX_CTX_DEFAULT_STREAM
+
+
 class Context:
     """One HIP device and stream (csx_ctx).  Objects created on a context
     (chains, surrogates) register with it and are closed before it."""
 
     def __init__(self, device: int = 0):
+        # The library's work is ordered with torch's on torch's current stream of
+        # the device (tensors torch produces asynchronously are then complete
+        # when a library kernel reads them, and the reverse).
+        torch = _torch()
+        with torch.cuda.device(device):
+            ts = torch.cuda.current_stream().cuda_stream
         p = C.c_void_p()
-        check(lib().csx_ctx_create(device, 0, C.byref(p)))
+        check(lib().csx_ctx_create(device, CTX_DEFAULT_STREAM if ts == 0 else 0, C.byref(p)))
+        if ts != 0:
+            check(lib().csx_ctx_set_stream(p, C.c_void_p(ts)))
         self._p = p
         self.device = device
         self._children = weakref.WeakSet()

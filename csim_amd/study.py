@@ -104,9 +104,12 @@ class ScreenedSurrogate:
         self.errors = []                # per generation: Spearman(pred, exact) of the survivors
 
     def predict(self, thickness):
-        y = self.mlp.forward(features_device(self.mlp.ctx, thickness, self.lambdas)).cpu().numpy().astype(np.float64)
-        C = 10.0 ** y.reshape(thickness.shape[0], len(self.lambdas), -1)
-        return objective_from_profiles(C, self.counts)
+        import torch
+        if not hasattr(self, "_w"):
+            self._w = torch.from_numpy(self.counts / self.counts.sum()).float().cuda()
+        y = self.mlp.forward(features_device(self.mlp.ctx, thickness, self.lambdas))
+        C = torch.pow(10.0, y).reshape(thickness.shape[0], len(self.lambdas), -1)
+        return (C.sum(dim=1) @ self._w).double().cpu().numpy()
 
     def _push(self, X, Y):
         import torch
@@ -131,20 +134,16 @@ class ScreenedSurrogate:
         nb = min(1024, self.filled)
         for _ in range(self.tune_steps):
             sel = torch.randint(0, self.filled, (nb,), device="cuda", generator=self.gen)
-            self.mlp.train_step(self.Xd.index_select(0, sel), self.Yd.index_select(0, sel), self.lr)
+            self.mlp.train_step(self.Xd.index_select(0, sel), self.Yd.index_select(0, sel), self.lr, want_loss=False)
         return idx, vals
 
 
-def run(n=512, n_train=22528, seconds=20.0, n_lambda=9, plain_pop=64, screen_pop=512, keep=32, epochs=30,
-        precision="bf16x3", seed=20261207, log=print):
+def train_surrogate(ctx, w, n_train=22528, epochs=20, precision="bf16x3", seed=20261207, keep=32, log=print):
+    """Propagates n_train random thickness vectors, trains the MLP on 90 % of them
+    and measures it on the rest.  Returns (ScreenedSurrogate, report dict)."""
     import torch
-    from .ops import Context
-    ctx = Context(0)
-    w = W.workload3(n=n, n_lambda=n_lambda)
-    nhex = w.meta["n_hex"]
-    out = {"pupil": n, "n_lambda": n_lambda, "n_hex": nhex, "precision": precision}
-
-    # ---- training data: random thickness vectors, exact propagations
+    nhex, n_lambda = w.meta["n_hex"], len(w.meta["lambdas"])
+    out = {}
     gen = ExactObjective(ctx, w, 256, precision)
     lambdas = gen.ch.lambdas()
     rng = np.random.Generator(np.random.Philox(seed))
@@ -159,8 +158,6 @@ def run(n=512, n_train=22528, seconds=20.0, n_lambda=9, plain_pop=64, screen_pop
     # profiles reproduce the exact objective (consistency of the labels)
     rebuilt = objective_from_profiles(prof, gen.counts)
     out["label_consistency_max_rel"] = float(np.max(np.abs(rebuilt - vals) / vals))
-
-    # ---- train (hold out the last 10 %)
     nh = n_train // 10
     X = features_device(ctx, T, lambdas)
     Y = torch.from_numpy(log_labels(prof).reshape(-1, N_BINS)).cuda()
@@ -173,41 +170,69 @@ def run(n=512, n_train=22528, seconds=20.0, n_lambda=9, plain_pop=64, screen_pop
     out["train_loss"] = [float(hist[0]), float(hist[-1])]
     sur = ScreenedSurrogate(mlp, lambdas, gen.counts, None, keep=keep)
     pred = sur.predict(T[n_train - nh:])
-    out["holdout_spearman"] = spearman(pred, vals[n_train - nh:])
-    out["holdout_rel_err_median"] = float(np.median(np.abs(pred - vals[n_train - nh:]) / vals[n_train - nh:]))
+    held = vals[n_train - nh:]
+    out["holdout_spearman"] = spearman(pred, held)
+    out["holdout_rel_err_median"] = float(np.median(np.abs(pred - held) / held))
+    # screening gain on random candidates: mean exact contrast of the surrogate's
+    # top 5 % against the mean of all
+    top = np.argsort(pred)[:max(1, nh // 20)]
+    out["holdout_top5pct_mean_over_mean"] = float(held[top].mean() / held.mean())
     log(f"train {out['train_seconds']:.2f} s, loss {hist[0]:.4f} -> {hist[-1]:.4f}, "
         f"held-out Spearman {out['holdout_spearman']:.3f}")
     gen.ch.close()
+    return sur, out
 
-    # ---- plain DE (every mutant propagated)
+
+def compare_de(ctx, w, sur, seconds=None, eval_budget=None, plain_pop=64, screen_pop=512, precision="bf16x3",
+               seed=20261207, log=print):
+    """Plain DE, then screened DE (stopping at the plain run's best value when a time
+    budget is given).  Budgets: wall seconds or exact candidate evaluations."""
+    import torch
+    nhex = w.meta["n_hex"]
+    out = {}
     cfg = DEConfig(population_size=plain_pop, global_bounds=(0.0, 1e-6), init_bounds=(0.0, 1e-6),
                    max_iterations=10 ** 9, seed=seed + 1)
     exact_p = ExactObjective(ctx, w, plain_pop, precision)
     de = DifferentialEvolution(exact_p, nhex, cfg)
     torch.cuda.synchronize()
-    rp = de.run(time_budget=seconds)
+    rp = de.run(time_budget=seconds, eval_budget=eval_budget)
     out["plain"] = {"population": plain_pop, "best": rp.best_value, "seconds": rp.seconds,
                     "evaluations": rp.evaluations, "generations": len(rp.history) - 1}
     log(f"plain DE: best {rp.best_value:.4e} after {rp.seconds:.1f} s, {rp.evaluations} candidates")
     exact_p.ch.close()
-
-    # ---- screened DE (512 mutants ranked, top `keep` propagated)
-    exact_s = ExactObjective(ctx, w, keep, precision)
+    exact_s = ExactObjective(ctx, w, sur.keep, precision)
     sur.exact = exact_s
     cfg_s = DEConfig(population_size=screen_pop, global_bounds=(0.0, 1e-6), init_bounds=(0.0, 1e-6),
                      max_iterations=10 ** 9, seed=seed + 2)
     de_s = DifferentialEvolution(exact_s, nhex, cfg_s, screen=sur.screen_and_eval)
-    rs = de_s.run(time_budget=seconds, target=rp.best_value)
-    reach = next((h[1] for h in rs.history if h[2] <= rp.best_value), None)
-    out["screened"] = {"population": screen_pop, "keep": keep, "best": rs.best_value, "seconds": rs.seconds,
+    rs = de_s.run(time_budget=seconds, target=rp.best_value if seconds is not None else None,
+                  eval_budget=eval_budget)
+    reach = next((h for h in rs.history if h[2] <= rp.best_value), None)
+    out["screened"] = {"population": screen_pop, "keep": sur.keep, "best": rs.best_value, "seconds": rs.seconds,
                        "evaluations": rs.evaluations, "generations": len(rs.history) - 1,
-                       "seconds_to_plain_best": reach,
+                       "seconds_to_plain_best": reach[1] if reach else None,
+                       "evaluations_to_plain_best": reach[3] if reach else None,
                        "survivor_spearman_median": float(np.median(sur.errors)) if sur.errors else None}
-    out["wall_ratio_vs_plain"] = (reach / rp.seconds) if reach else None
-    out["wall_ratio_incl_training"] = ((reach + out["data_seconds"] + out["train_seconds"]) / rp.seconds
-                                       if reach else None)
-    log(f"screened DE: best {rs.best_value:.4e} after {rs.seconds:.1f} s; reached plain best at {reach}")
+    out["wall_ratio_vs_plain"] = (reach[1] / rp.seconds) if reach else None
+    log(f"screened DE: best {rs.best_value:.4e} after {rs.seconds:.1f} s; reached plain best: "
+        f"{(reach[1], reach[3]) if reach else None}")
     exact_s.ch.close()
+    return out
+
+
+def run(n=512, n_train=22528, seconds=20.0, n_lambda=9, plain_pop=64, screen_pop=512, keep=32, epochs=20,
+        precision="bf16x3", seed=20261207, log=print):
+    from .ops import Context
+    ctx = Context(0)
+    w = W.workload3(n=n, n_lambda=n_lambda)
+    out = {"pupil": n, "n_lambda": n_lambda, "n_hex": w.meta["n_hex"], "precision": precision}
+    sur, tr = train_surrogate(ctx, w, n_train, epochs, precision, seed, keep, log)
+    out.update(tr)
+    out.update(compare_de(ctx, w, sur, seconds=seconds, plain_pop=plain_pop, screen_pop=screen_pop,
+                          precision=precision, seed=seed, log=log))
+    if out["wall_ratio_vs_plain"] is not None:
+        out["wall_ratio_incl_training"] = (out["screened"]["seconds_to_plain_best"] + out["data_seconds"] +
+                                           out["train_seconds"]) / out["plain"]["seconds"]
     return out
 
 
@@ -217,7 +242,7 @@ def main():
     ap.add_argument("--train", type=int, default=22528)
     ap.add_argument("--seconds", type=float, default=20.0)
     ap.add_argument("--lambdas", type=int, default=9)
-    ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--precision", default="bf16x3")
     a = ap.parse_args()
     res = run(n=a.n, n_train=a.train, seconds=a.seconds, n_lambda=a.lambdas, epochs=a.epochs, precision=a.precision,

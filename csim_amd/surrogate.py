@@ -106,11 +106,11 @@ class MLP:
         except Exception:
             pass
 
-    def train_step(self, x_dev, t_dev, lr):
+    def train_step(self, x_dev, t_dev, lr, want_loss=True):
         loss = C.c_double()
         check(lib().csx_mlp_train_step(self._p, C.c_void_p(x_dev.data_ptr()), C.c_void_p(t_dev.data_ptr()),
-                                       x_dev.shape[0], float(lr), C.byref(loss)))
-        return loss.value
+                                       x_dev.shape[0], float(lr), C.byref(loss) if want_loss else None))
+        return loss.value if want_loss else None
 
     def forward(self, x_dev):
         import torch

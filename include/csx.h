@@ -52,10 +52,14 @@ typedef struct csx_chain csx_chain;
 /* ---------------------------------------------------------------- context */
 /* Replaces csim's global state set up in src/csim.cpp:28-50 (init_fft_lib,
  * globalTelescope/globalCoronagraph singletons) with an explicit context bound
- * to one HIP device. flags: reserved, pass 0. */
+ * to one HIP device.  flags: 0 = the context creates its own non-blocking
+ * stream; CSX_CTX_DEFAULT_STREAM = all work goes to the device's default (null)
+ * stream, ordered with a framework that allocates and computes there. */
+#define CSX_CTX_DEFAULT_STREAM 1u
 int csx_ctx_create(int device, unsigned flags, csx_ctx** out);
 void csx_ctx_destroy(csx_ctx* ctx);
-/* Stream for all subsequent work of this context (NULL = the context's own). */
+/* Stream for all subsequent work of this context (NULL = the context's own).
+ * Device buffers produced on another stream must be complete before use. */
 int csx_ctx_set_stream(csx_ctx* ctx, void* hip_stream);
 int csx_ctx_synchronize(csx_ctx* ctx);
 const char* csx_last_error(void);
@@ -255,7 +259,8 @@ int csx_mlp_forward(csx_mlp* mlp, const float* d_x, int n, float* d_y);
 /* Loss sum (y-t)^2/(n n_out) and its gradient (host, same order as params). */
 int csx_mlp_loss_grad(csx_mlp* mlp, const float* d_x, const float* d_t, int n, double* loss,
                       double* grad_flat);
-/* One Adam step on a device minibatch; returns the loss before the step. */
+/* One Adam step on a device minibatch; returns the loss before the step (loss
+ * NULL: no read-back, the step is only enqueued on the context's stream). */
 int csx_mlp_train_step(csx_mlp* mlp, const float* d_x, const float* d_t, int n, double lr,
                        double* loss);
 
