@@ -65,6 +65,12 @@ SIGNATURES = {
     "csx_chain_get_normalized_image": (_i, [_p, _i, _p]),
     "csx_chain_field_profiles": (_i, [_p, _p, _i, _p]),
     "csx_chain_set_profiling": (_i, [_p, _i]),
+    "csx_chain_set_graph": (_i, [_p, _i]),
+    "csx_group_create": (_i, [_p, _i, C.POINTER(_p)]),
+    "csx_group_destroy": (None, [_p]),
+    "csx_group_candidates": (_i, [_p]),
+    "csx_group_contrast": (_i, [_p, C.POINTER(Region), _p]),
+    "csx_group_merged_launches": (_i, [_p]),
     "csx_chain_profile_json": (_i, [_p, C.c_char_p, _sz]),
     "csx_mft_workspace_size": (_sz, [Grid, Grid, Grid, Grid, _i, _i]),
     "csx_mft": (_i, [_p, Grid, Grid, Grid, Grid, _p, _i, _p, _p, _i, _p]),

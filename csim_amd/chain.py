@@ -127,6 +127,10 @@ class Chain:
         check(lib().csx_chain_field_profiles(self._p, _ptr(b), int(nbins), _ptr(out)))
         return out
 
+    def set_graph(self, on: bool):
+        """Replay every propagation after the first as one HIP graph."""
+        check(lib().csx_chain_set_graph(self._p, int(bool(on))))
+
     def set_profiling(self, on: bool):
         check(lib().csx_chain_set_profiling(self._p, int(bool(on))))
 
@@ -142,3 +146,41 @@ class Chain:
         out = np.zeros((inf["out_cols"], inf["out_rows"]))
         check(lib().csx_chain_get_normalized_image(self._p, cand, _ptr(out)))
         return out.T.copy()
+
+
+class ChainGroup:
+    """Chains with identical element lists (e.g. one script at several pupil sizes)
+    propagated together: each contraction stage is one grouped launch."""
+
+    def __init__(self, chains):
+        self.chains = list(chains)
+        arr = (C.c_void_p * len(self.chains))(*[c._p.value for c in self.chains])
+        p = C.c_void_p()
+        check(lib().csx_group_create(arr, len(self.chains), C.byref(p)))
+        self._p = p
+        self.chains[0].ctx.adopt(self)
+
+    def close(self):
+        if self._p:
+            lib().csx_group_destroy(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def merged_launches(self):
+        return check(lib().csx_group_merged_launches(self._p))
+
+    def contrast(self, radii=None, angles=(-180.0, 180.0), xlim=(-1e16, 1e16), reference_lambda=-1.0):
+        n = check(lib().csx_group_candidates(self._p))
+        out = np.zeros(n)
+        if radii is None:
+            check(lib().csx_group_contrast(self._p, None, _ptr(out)))
+        else:
+            r = Region(float(radii[0]), float(radii[1]), float(angles[0]), float(angles[1]),
+                       float(xlim[0]), float(xlim[1]), float(reference_lambda))
+            check(lib().csx_group_contrast(self._p, C.byref(r), _ptr(out)))
+        return out

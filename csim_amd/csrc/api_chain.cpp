@@ -14,6 +14,11 @@ struct csx_chain {
   explicit csx_chain(csx_ctx* c) : impl(c) {}
 };
 
+struct csx_group {
+  ChainGroup impl;
+  explicit csx_group(std::vector<Chain*> c) : impl(std::move(c)) {}
+};
+
 #define CSX_GUARD_BEGIN try {
 #define CSX_GUARD_END                                                               \
   }                                                                                 \
@@ -170,6 +175,44 @@ int csx_chain_field_profiles(csx_chain* c, const int32_t* bin_of_pixel, int nbin
 int csx_chain_set_profiling(csx_chain* c, int enable) {
   CSX_TRY(chk(c));
   return c->impl.set_profiling(enable != 0);
+}
+
+int csx_group_create(csx_chain* const* chains, int n, csx_group** out) {
+  CSX_GUARD_BEGIN
+  if (!chains || n <= 0 || !out) return fail(CSX_ERR_ARG, "csx_group_create: bad arguments");
+  *out = nullptr;
+  std::vector<Chain*> v;
+  for (int i = 0; i < n; i++) {
+    if (!chains[i]) return fail(CSX_ERR_ARG, "csx_group_create: null chain");
+    v.push_back(&chains[i]->impl);
+  }
+  *out = new csx_group(std::move(v));
+  return CSX_OK;
+  CSX_GUARD_END
+}
+
+void csx_group_destroy(csx_group* g) { delete g; }
+
+int csx_group_candidates(csx_group* g) {
+  if (!g) return fail(CSX_ERR_ARG, "null group");
+  return g->impl.n_candidates();
+}
+
+int csx_group_contrast(csx_group* g, const csx_region* region, double* out) {
+  CSX_GUARD_BEGIN
+  if (!g || !out) return fail(CSX_ERR_ARG, "csx_group_contrast: null argument");
+  return g->impl.contrast(region, out);
+  CSX_GUARD_END
+}
+
+int csx_group_merged_launches(csx_group* g) {
+  if (!g) return fail(CSX_ERR_ARG, "null group");
+  return g->impl.merged_launches();
+}
+
+int csx_chain_set_graph(csx_chain* c, int enable) {
+  CSX_TRY(chk(c));
+  return c->impl.set_graph(enable != 0);
 }
 
 int csx_chain_profile_json(csx_chain* c, char* buf, size_t cap) {

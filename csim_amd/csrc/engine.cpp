@@ -564,19 +564,24 @@ static int compile_mft(Chain& ch, Compile& c, const std::vector<Geom>& gout_per_
   size_t nconv = (size_t)nf * gin.npix();
   long npix_in = gin.npix();
   Chain* chp = &ch;
-  c.stages.push_back([=](hipStream_t s) {
-    KernelTimer* t = chp->timer();
-    if (!from_conv && c64) {
-      if (t) CSX_TRY(t->begin("convert_c128_to_c64", 0.0, 24.0 * nconv, s));
-      CSX_TRY(convert_c128_to_c64(in, conv, nconv, s));
+  if (!from_conv && (c64 || T)) {   // operands to the working precision (no fused producer)
+    c.stages.push_back([=](hipStream_t s) {
+      KernelTimer* t = chp->timer();
+      if (c64) {
+        if (t) CSX_TRY(t->begin("convert_c128_to_c64", 0.0, 24.0 * nconv, s));
+        CSX_TRY(convert_c128_to_c64(in, conv, nconv, s));
+      } else {
+        if (t) CSX_TRY(t->begin("split_c128_to_planes", 0.0, (16.0 + 4.0 * T) * nconv, s));
+        CSX_TRY(split_c128_to_planes(in, conv, npix_in, nf, T, s));
+      }
       if (t) CSX_TRY(t->end(s));
-    } else if (!from_conv && T) {
-      if (t) CSX_TRY(t->begin("split_c128_to_planes", 0.0, (16.0 + 4.0 * T) * nconv, s));
-      CSX_TRY(split_c128_to_planes(in, conv, npix_in, nf, T, s));
-      if (t) CSX_TRY(t->end(s));
-    }
-    return pp->run(s, t);
-  });
+      return CSX_OK;
+    });
+    c.stage_names.push_back(label + " operands");
+  }
+  // the contraction itself; its field list lets a chain group merge this launch
+  c.mft_at[c.stages.size()] = F;
+  c.stages.push_back([=](hipStream_t s) { return pp->run(s, chp->timer()); });
   c.stage_names.push_back(label);
   if (out_conv) {
     c.in_conv = true;
@@ -1454,6 +1459,7 @@ int Chain::compile_one(bool calibrating, Compile& c) {
   c.stages.clear();
   c.stage_names.clear();
   c.plans.clear();
+  c.mft_at.clear();
   c.geom = geom_symmetric(ef_.cols, ef_.rows, ef_.pixelScale, ef_.pixelScale);
   c.buf = 0;
   // initial fields (efield::init_point_sources / init / filename slices)
@@ -1552,8 +1558,31 @@ int Chain::compile_all() {
   CSX_TRY(compile_one(true, plan_cal_));
   out_geom_cal_ = plan_cal_.geom; out_buf_cal_ = plan_cal_.buf;
   compiled_ = true;
+  generation_++;
+  region_key_.clear();
   have_full_ = have_cal_ = false;
   calib_valid_ = false;
+  return CSX_OK;
+}
+
+Chain::~Chain() {
+  drop_graphs();
+  if (cap_stream_) (void)hipStreamDestroy(cap_stream_);
+}
+
+void Chain::drop_graphs() {
+  for (GraphState& g : graph_) {
+    if (g.e) (void)hipGraphExecDestroy(g.e);
+    if (g.g) (void)hipGraphDestroy(g.g);
+    g = GraphState();
+  }
+}
+
+int Chain::run_stages(Compile& c, hipStream_t s) {
+  for (size_t i = 0; i < c.stages.size(); i++) {
+    int rc = c.stages[i](s);
+    if (rc != CSX_OK) return fail(rc, "stage '" + c.stage_names[i] + "': " + csx_last_error());
+  }
   return CSX_OK;
 }
 
@@ -1561,9 +1590,26 @@ int Chain::propagate(bool calibrating) {
   CSX_TRY(compile_all());
   CSX_HIP(hipSetDevice(ctx_->device));
   Compile& c = calibrating ? plan_cal_ : plan_full_;
-  for (size_t i = 0; i < c.stages.size(); i++) {
-    int rc = c.stages[i](stream());
-    if (rc != CSX_OK) return fail(rc, "stage '" + c.stage_names[i] + "': " + csx_last_error());
+  GraphState& gs = graph_[calibrating ? 1 : 0];
+  const bool graphable = use_graph_ && !timer_.enabled();
+  if (graphable && !gs.e && gs.ran_eager) {
+    // capture after one eager run (lazily created tables exist by then) on a
+    // private stream: the legacy default stream cannot be captured
+    if (!cap_stream_) CSX_HIP(hipStreamCreateWithFlags(&cap_stream_, hipStreamNonBlocking));
+    CSX_HIP(hipStreamBeginCapture(cap_stream_, hipStreamCaptureModeRelaxed));
+    int rc = run_stages(c, cap_stream_);
+    hipGraph_t g = nullptr;
+    hipError_t ec = hipStreamEndCapture(cap_stream_, &g);
+    if (rc != CSX_OK) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (ec != hipSuccess) return fail(CSX_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ec));
+    gs.g = g;
+    CSX_HIP(hipGraphInstantiate(&gs.e, gs.g, nullptr, nullptr, 0));
+  }
+  if (graphable && gs.e) {
+    CSX_HIP(hipGraphLaunch(gs.e, stream()));
+  } else {
+    CSX_TRY(run_stages(c, stream()));
+    gs.ran_eager = true;
   }
   CSX_HIP(hipStreamSynchronize(stream()));
   if (timer_.enabled()) CSX_TRY(timer_.collect());
@@ -1600,65 +1646,84 @@ int Chain::get_intensity(int cand, double* out) {
   return CSX_OK;
 }
 
-int Chain::contrast(const csx_region* r, double* out, csx_report* rep) {
+// Calibration (unocculted) run and its peak, cached until the chain changes.
+int Chain::ensure_calibration() {
+  CSX_TRY(compile_all());
+  if (calib_valid_) return CSX_OK;
+  const int per = nsrc() * nlam();
+  CSX_TRY(propagate(true));
+  const Geom gc = out_geom_cal_;
+  CSX_TRY(inten_cal_.reserve(sizeof(double) * gc.npix()));
+  CSX_TRY(peak_.reserve(sizeof(double)));
+  CSX_TRY(intensity_groups(field_buf(out_buf_cal_), gc.npix(), 1, per, inten_cal_.as<double>(), stream()));
+  CSX_TRY(max_groups(inten_cal_.as<double>(), gc.npix(), 1, peak_.as<double>(), stream()));
+  CSX_HIP(hipMemcpyAsync(&calib_peak_, peak_.p, sizeof(double), hipMemcpyDeviceToHost, stream()));
+  CSX_HIP(hipStreamSynchronize(stream()));
+  calib_valid_ = true;
+  return CSX_OK;
+}
+
+// Region mean of sum_{sources, lambdas} |E|^2 / peak per candidate from the fields
+// of the last full propagation (regionContrast::compute_contrast, :145-205).
+int Chain::reduce_contrast(const csx_region* r, double* out) {
   Region R = region_;
   if (r) {
     R.r1 = r->radius1; R.r2 = r->radius2; R.a1 = r->angle1; R.a2 = r->angle2;
     R.x1 = r->xlim1; R.x2 = r->xlim2; R.refLambda = r->reference_lambda > 0 ? r->reference_lambda : -1.0;
   }
-  CSX_TRY(compile_all());
-  int per = nsrc() * nlam();
-  auto t0 = std::chrono::steady_clock::now();
-  if (!calib_valid_) {   // calibration (unocculted) run, cached until the chain changes
-    CSX_TRY(propagate(true));
-    const Geom gc = out_geom_cal_;
-    CSX_TRY(inten_cal_.reserve(sizeof(double) * gc.npix()));
-    CSX_TRY(peak_.reserve(sizeof(double)));
-    CSX_TRY(intensity_groups(field_buf(out_buf_cal_), gc.npix(), 1, per, inten_cal_.as<double>(), stream()));
-    CSX_TRY(max_groups(inten_cal_.as<double>(), gc.npix(), 1, peak_.as<double>(), stream()));
-    CSX_HIP(hipMemcpyAsync(&calib_peak_, peak_.p, sizeof(double), hipMemcpyDeviceToHost, stream()));
-    CSX_HIP(hipStreamSynchronize(stream()));
-    calib_valid_ = true;
-  }
-  auto t1 = std::chrono::steady_clock::now();
-  auto t2 = t1;
-  CSX_TRY(propagate(false));
-  auto t3 = std::chrono::steady_clock::now();
+  const int per = nsrc() * nlam();
   const Geom g = out_geom_;
   int nc = ncand();
   CSX_TRY(inten_.reserve(sizeof(double) * g.npix() * nc));
   CSX_TRY(intensity_groups(field_buf(out_buf_), g.npix(), nc, per, inten_.as<double>(), stream()));
-  // region pixels (regionContrast::get_region_pixels, :89-101), column-major order
+  // region pixels (regionContrast::get_region_pixels, :89-101), column-major order;
+  // recomputed only when the region or the output grid changes
   double refl = R.refLambda;
   if (refl == -1.0) {
-    double s = 0;
-    for (double l : ef_.lambdas) s += l;
-    refl = s / nlam();
+    double sum = 0;
+    for (double l : ef_.lambdas) sum += l;
+    refl = sum / nlam();
   }
-  double loD = tel_.loD(refl);
-  std::vector<int32_t> idx;
-  for (int c = 0; c < g.nx; c++)
-    for (int rr = 0; rr < g.ny; rr++) {
-      double XX = g.x0 + c * g.dx, YY = g.y0 + rr * g.dy;
-      double RR = std::sqrt(std::pow(XX, 2) + std::pow(YY, 2)), TT = std::atan2(YY, XX);
-      if ((XX / loD >= R.x1) && (XX / loD <= R.x2) && (RR / loD >= R.r1) && (RR / loD <= R.r2) &&
-          (TT > R.a1 * M_PI / 180) && (TT < R.a2 * M_PI / 180))
-        idx.push_back(c * g.ny + rr);
-    }
-  n_region_ = (int)idx.size();
-  if (idx.empty()) return fail(CSX_ERR_ARG, "contrast region contains no pixel");
-  CSX_TRY(region_idx_.reserve(sizeof(int32_t) * idx.size()));
-  CSX_HIP(hipMemcpyAsync(region_idx_.p, idx.data(), sizeof(int32_t) * idx.size(), hipMemcpyHostToDevice, stream()));
+  const double loD = tel_.loD(refl);
+  const std::vector<double> key = {R.r1, R.r2, R.a1, R.a2, R.x1, R.x2, loD, (double)g.nx, (double)g.ny, g.x0, g.y0,
+                                   g.dx, g.dy};
+  if (key != region_key_) {
+    std::vector<int32_t> idx;
+    for (int c = 0; c < g.nx; c++)
+      for (int rr = 0; rr < g.ny; rr++) {
+        double XX = g.x0 + c * g.dx, YY = g.y0 + rr * g.dy;
+        double RR = std::sqrt(std::pow(XX, 2) + std::pow(YY, 2)), TT = std::atan2(YY, XX);
+        if ((XX / loD >= R.x1) && (XX / loD <= R.x2) && (RR / loD >= R.r1) && (RR / loD <= R.r2) &&
+            (TT > R.a1 * M_PI / 180) && (TT < R.a2 * M_PI / 180))
+          idx.push_back(c * g.ny + rr);
+      }
+    if (idx.empty()) return fail(CSX_ERR_ARG, "contrast region contains no pixel");
+    CSX_TRY(region_idx_.reserve(sizeof(int32_t) * idx.size()));
+    CSX_HIP(hipMemcpy(region_idx_.p, idx.data(), sizeof(int32_t) * idx.size(), hipMemcpyHostToDevice));
+    n_region_ = (int)idx.size();
+    region_key_ = key;
+  }
   CSX_TRY(contrast_.reserve(sizeof(double) * nc));
-  CSX_TRY(region_mean_groups(inten_.as<double>(), g.npix(), nc, region_idx_.as<int32_t>(), (int)idx.size(),
+  CSX_TRY(region_mean_groups(inten_.as<double>(), g.npix(), nc, region_idx_.as<int32_t>(), n_region_,
                              peak_.as<double>(), 0.0, contrast_.as<double>(), stream()));
   CSX_HIP(hipMemcpyAsync(out, contrast_.p, sizeof(double) * nc, hipMemcpyDeviceToHost, stream()));
   CSX_HIP(hipStreamSynchronize(stream()));
+  return CSX_OK;
+}
+
+int Chain::contrast(const csx_region* r, double* out, csx_report* rep) {
+  CSX_TRY(compile_all());
+  auto t0 = std::chrono::steady_clock::now();
+  CSX_TRY(ensure_calibration());
+  auto t1 = std::chrono::steady_clock::now();
+  CSX_TRY(propagate(false));
+  auto t2 = std::chrono::steady_clock::now();
+  CSX_TRY(reduce_contrast(r, out));
   if (rep) {
     rep->calib_peak = calib_peak_;
     rep->n_region_pixels = n_region_;
     rep->seconds_calibration = std::chrono::duration<double>(t1 - t0).count();
-    rep->seconds_propagation = std::chrono::duration<double>(t3 - t2).count();
+    rep->seconds_propagation = std::chrono::duration<double>(t2 - t1).count();
   }
   return CSX_OK;
 }
@@ -1703,6 +1768,88 @@ int Chain::info(csx_chain_info* o) {
   double refl = region_.refLambda;
   if (refl == -1.0) { double s = 0; for (double l : ef_.lambdas) s += l; refl = s / nlam(); }
   o->lambda_over_d = tel_.loD(refl);
+  return CSX_OK;
+}
+
+
+// ============================================================ chain group ==
+int ChainGroup::n_candidates() const {
+  int n = 0;
+  for (Chain* c : ch_) n += c->ncand();
+  return n;
+}
+
+int ChainGroup::build() {
+  if (ch_.empty()) return fail(CSX_ERR_ARG, "chain group: no chains");
+  for (Chain* c : ch_) CSX_TRY(c->compile());
+  bool same = gen_.size() == ch_.size();
+  for (size_t k = 0; same && k < ch_.size(); k++) same = gen_[k] == ch_[k]->generation();
+  if (same && !stages_.empty()) return CSX_OK;
+  Chain* c0 = ch_[0];
+  const Compile& p0 = c0->plan(false);
+  for (Chain* c : ch_) {
+    const Compile& pk = c->plan(false);
+    if (c->ctx() != c0->ctx()) return fail(CSX_ERR_ARG, "chain group: chains on different contexts");
+    if (c->precision() != c0->precision()) return fail(CSX_ERR_ARG, "chain group: precisions differ");
+    if (pk.stages.size() != p0.stages.size()) return fail(CSX_ERR_ARG, "chain group: element lists differ");
+    for (size_t i = 0; i < p0.stages.size(); i++)
+      if ((pk.mft_at.count(i) != 0) != (p0.mft_at.count(i) != 0))
+        return fail(CSX_ERR_ARG, "chain group: element lists differ at stage " + std::to_string(i));
+  }
+  stages_.clear();
+  plans_.clear();
+  merged_ = 0;
+  for (size_t i = 0; i < p0.stages.size(); i++) {
+    if (p0.mft_at.count(i)) {
+      std::vector<MftField> F;
+      for (Chain* c : ch_) {
+        const auto& f = c->plan(false).mft_at.at(i);
+        F.insert(F.end(), f.begin(), f.end());
+      }
+      auto plan = std::make_unique<MftPlan>();
+      CSX_TRY(plan->build(F, c0->precision(), c0->stream()));
+      const MftPlan* pp = plan.get();
+      plans_.push_back(std::move(plan));
+      Chain* tc = c0;
+      stages_.push_back([pp, tc](hipStream_t s) { return pp->run(s, tc->timer()); });
+      merged_++;
+    } else {
+      std::vector<std::function<int(hipStream_t)>> fns;
+      for (Chain* c : ch_) fns.push_back(c->plan(false).stages[i]);
+      stages_.push_back([fns](hipStream_t s) {
+        for (const auto& f : fns) CSX_TRY(f(s));
+        return CSX_OK;
+      });
+    }
+  }
+  gen_.clear();
+  for (Chain* c : ch_) gen_.push_back(c->generation());
+  return CSX_OK;
+}
+
+int ChainGroup::propagate() {
+  CSX_TRY(build());
+  hipStream_t s = ch_[0]->stream();
+  for (size_t i = 0; i < stages_.size(); i++) {
+    int rc = stages_[i](s);
+    if (rc != CSX_OK) return fail(rc, "group stage " + std::to_string(i) + ": " + csx_last_error());
+  }
+  CSX_HIP(hipStreamSynchronize(s));
+  for (Chain* c : ch_) {
+    c->mark_propagated(false);
+    if (c->timer()) CSX_TRY(c->timer()->collect());
+  }
+  return CSX_OK;
+}
+
+int ChainGroup::contrast(const csx_region* r, double* out) {
+  for (Chain* c : ch_) CSX_TRY(c->ensure_calibration());   // each chain's own (cached) calibration
+  CSX_TRY(propagate());
+  size_t off = 0;
+  for (Chain* c : ch_) {
+    CSX_TRY(c->reduce_contrast(r, out + off));
+    off += c->ncand();
+  }
   return CSX_OK;
 }
 

@@ -76,6 +76,7 @@ struct Compile {
   std::vector<std::function<int(hipStream_t)>> stages;
   std::vector<std::string> stage_names;
   std::vector<std::unique_ptr<MftPlan>> plans;
+  std::map<size_t, std::vector<MftField>> mft_at;   // stage index -> fields of a plain MFT launch
 };
 
 // ---------------------------------------------------------------- elements --
@@ -105,6 +106,9 @@ struct Element {
 class Chain {
  public:
   explicit Chain(csx_ctx* ctx) : ctx_(ctx) {}
+  ~Chain();
+  Chain(const Chain&) = delete;
+  Chain& operator=(const Chain&) = delete;
   int load_script(const std::string& text);
   int set_array(const std::string& elem, const std::string& key, const double* d, int rows, int cols);
   int set_complex_array(const std::string& elem, const std::string& key, const double* d, int rows, int cols,
@@ -117,10 +121,22 @@ class Chain {
   int get_field(int cand, int src, int lam, double* out);
   int get_intensity(int cand, double* out);
   int contrast(const csx_region* r, double* out, csx_report* rep);
+  int ensure_calibration();
+  int reduce_contrast(const csx_region* r, double* out);
+  // ---- chain groups (one launch per contraction over several chains)
+  Compile& plan(bool calibrating) { return calibrating ? plan_cal_ : plan_full_; }
+  int compile() { return compile_all(); }
+  long generation() const { return generation_; }     // bumps on every recompilation
+  void mark_propagated(bool calibrating) {
+    if (calibrating) { have_cal_ = true; have_full_ = false; } else { have_full_ = true; have_cal_ = false; }
+  }
+  double calib_peak() const { return calib_peak_; }
+  int n_region_pixels() const { return n_region_; }
   int get_normalized_image(int cand, double* out);
   int field_profiles(const int32_t* bins, int nbins, double* out);
   int info(csx_chain_info* out);
   int set_profiling(bool on) { timer_.enable(on); return CSX_OK; }
+  int set_graph(bool on) { use_graph_ = on; if (!on) drop_graphs(); return CSX_OK; }
   std::string profile_json() const { return timer_.json(); }
   KernelTimer* timer() { return timer_.enabled() ? &timer_ : nullptr; }
 
@@ -142,7 +158,7 @@ class Chain {
   Element* elem_by_name(const std::string& n);
   Element* elem_at(int i) { return (i >= 0 && i < (int)elems_.size()) ? elems_[i].get() : nullptr; }
   int n_elems() const { return (int)elems_.size(); }
-  void invalidate() { compiled_ = false; calib_valid_ = false; }
+  void invalidate() { compiled_ = false; calib_valid_ = false; drop_graphs(); }
   void invalidate_calibration() { calib_valid_ = false; }
   DevBuf& scratch(const std::string& key) { return scratch_[key]; }
   int candidate_sags(const std::string& elem, const double** dev, int* nhex, int* ncand);
@@ -150,6 +166,8 @@ class Chain {
  private:
   int compile_all();
   int compile_one(bool calibrating, Compile& c);
+  int run_stages(Compile& c, hipStream_t s);
+  void drop_graphs();
   int parse_top(const std::vector<Block>& blocks);
   int make_element(const Block& b, std::unique_ptr<Element>& out);
 
@@ -176,7 +194,36 @@ class Chain {
   double calib_peak_ = 0;
   bool calib_valid_ = false;
   int n_region_ = 0;
+  std::vector<double> region_key_;
+  long generation_ = 0;
   KernelTimer timer_;
+  // graph replay of each plan ([0] full, [1] calibration)
+  struct GraphState { hipGraph_t g = nullptr; hipGraphExec_t e = nullptr; bool ran_eager = false; };
+  bool use_graph_ = false;
+  GraphState graph_[2];
+  hipStream_t cap_stream_ = nullptr;
+};
+
+
+// Several compiled chains with identical element lists (the same script at
+// different pupil sizes, e.g. 512^2 and 1024^2 DE candidates) propagated together:
+// every contraction stage of all chains is one grouped GEMM launch over uneven
+// problem sizes; the other stages run chain by chain.
+class ChainGroup {
+ public:
+  explicit ChainGroup(std::vector<Chain*> chains) : ch_(std::move(chains)) {}
+  int propagate();
+  int contrast(const csx_region* r, double* out);   // out: candidates of every chain, chain by chain
+  int n_candidates() const;
+  int merged_launches() const { return merged_; }
+
+ private:
+  int build();
+  std::vector<Chain*> ch_;
+  std::vector<long> gen_;
+  std::vector<std::function<int(hipStream_t)>> stages_;
+  std::vector<std::unique_ptr<MftPlan>> plans_;
+  int merged_ = 0;
 };
 
 }  // namespace csx

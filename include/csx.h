@@ -155,6 +155,29 @@ int csx_chain_field_profiles(csx_chain* chain, const int32_t* bin_of_pixel, int 
 int csx_chain_set_profiling(csx_chain* chain, int enable);
 int csx_chain_profile_json(csx_chain* chain, char* buf, size_t cap);
 
+/* Graph replay (no reference counterpart; csim runs elements one by one on the
+ * CPU): with graphs on, the second and later propagations of a compiled chain
+ * replay one HIP graph holding every stage (captured on a private stream,
+ * launched on the chain's stream).  Device data the stages read (sags,
+ * candidate batches, bound arrays updated in place) are read at replay time;
+ * recompilation discards the graphs.  Ignored while profiling is on. */
+int csx_chain_set_graph(csx_chain* chain, int enable);
+
+/* Chain group (no reference counterpart; csim runs one configuration at a time):
+ * several chains with identical element lists, e.g. the same script at 512^2 and
+ * 1024^2 pupils, propagated together.  Every contraction stage of all chains is
+ * one grouped launch over uneven problem sizes; other stages run chain by chain.
+ * The chains must share the context and precision and outlive the group; a
+ * chain recompiled since the last call is picked up. */
+typedef struct csx_group csx_group;
+int csx_group_create(csx_chain* const* chains, int n, csx_group** out);
+void csx_group_destroy(csx_group* group);
+int csx_group_candidates(csx_group* group);          /* sum of the chains' candidate counts */
+/* Region contrast (regionContrast::compute_contrast) of every candidate of every
+ * chain, chain by chain; each chain's calibration is its own (cached). */
+int csx_group_contrast(csx_group* group, const csx_region* region, double* out);
+int csx_group_merged_launches(csx_group* group);     /* grouped contraction launches per propagation */
+
 /* ------------------------------------------------------ operator API (device) */
 /* Uniform 1-D grid: x_i = x0 + i*dx, n points. */
 typedef struct { int n; double x0; double dx; } csx_grid;
