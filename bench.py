@@ -104,6 +104,49 @@ def cpu_baseline(w, budget_s=12.0):
                       f"{dt:.1f} s"}
 
 
+def de_batch_uneven(ctx, precision, steps=5):
+    """DE generation with uneven pupils: 64 candidates x 9 wavelengths, 32 on 512^2
+    and 32 on 1024^2 pupils (radii halved), one grouped launch per contraction
+    stage; beside it, the two chains run one after the other."""
+    import torch
+    from csim_amd import workloads as W
+    from csim_amd.chain import ChainGroup
+    rng = np.random.Generator(np.random.Philox(20261203 + 3))
+    chains, flops = [], 0.0
+    for n in (512, 1024):
+        w = W.workload3(n=n, n_lambda=9)
+        t = rng.random((32, w.meta["n_hex"])) * 1e-6
+        ch = build_chain(ctx, w, precision)
+        ch.set_candidates("fpm", W.thickness_to_sag(t))
+        ch.contrast()                        # compile, tune, calibrate
+        chains.append(ch)
+    g = ChainGroup(chains)
+    g.contrast()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.contrast()
+    torch.cuda.synchronize()
+    dt_g = (time.perf_counter() - t0) / steps
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for ch in chains:
+            ch.contrast()
+    torch.cuda.synchronize()
+    dt_s = (time.perf_counter() - t0) / steps
+    fields = 2 * 32 * 9
+    # useful flops: 1024^2 propagation 6,576,668,672; 512^2 1,946,157,056 (8 per complex MAC)
+    flops = 32 * 9 * (6576668672 + 1946157056)
+    res = {"fields_per_generation": fields, "ms_per_generation": round(dt_g * 1e3, 3),
+           "propagations_per_s": round(fields / dt_g, 1), "useful_tflops": round(flops / dt_g / 1e12, 1),
+           "grouped_launches_per_generation": g.merged_launches(),
+           "ms_per_generation_two_chains_separately": round(dt_s * 1e3, 3)}
+    g.close()
+    for ch in chains:
+        ch.close()
+    return res
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,6 +155,7 @@ def main(argv=None):
     ap.add_argument("--precision", default="bf16x3", choices=["fp32", "bf16x3", "bf16x6"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-uneven", action="store_true")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print("bench.py measures one GPU: every field shards independently, so the whole-job rate at N GPUs is "
@@ -173,6 +217,8 @@ def main(argv=None):
         out["contrast_error"] = err
         for ch in (dk64, dkm, plain):
             ch.close()
+    if not a.no_uneven:
+        out["de_batch_uneven_512_1024"] = de_batch_uneven(ctx, a.precision)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w)
     print(json.dumps(out))

@@ -186,3 +186,26 @@ def test_graph_replay_matches_eager(ctx):
     for s in range(2):
         for l in range(3):
             assert np.array_equal(g.field(0, s, l), eager.field(0, s, l))
+
+
+def test_chain_group_uneven_pupils_matches_chains(ctx):
+    """A 512^2 and a 1024^2 DE batch (same script, radii halved) in one group: every
+    contraction stage is one grouped launch over both sizes, and the contrasts equal
+    those of the chains run alone."""
+    from csim_amd.chain import ChainGroup
+    rng = np.random.default_rng(31)
+    chains, alone = [], []
+    for n in (512, 1024):
+        w = W.workload3(n=n, n_lambda=3)
+        t = rng.random((4, w.meta["n_hex"])) * 1e-6
+        ch = build(ctx, w, "bf16x3")
+        ch.set_candidates("fpm", W.thickness_to_sag(t))
+        c, _ = ch.contrast()
+        alone.append(c)
+        chains.append(ch)
+    g = ChainGroup(chains)
+    got = g.contrast()
+    print("group merged launches", g.merged_launches(), "contrasts", got)
+    assert g.merged_launches() == 6
+    ref = np.concatenate(alone)
+    assert np.max(np.abs(got - ref) / ref) < 1e-12
