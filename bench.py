@@ -139,7 +139,7 @@ def de_batch_uneven(ctx, precision, steps=5):
     flops = 32 * 9 * (6576668672 + 1946157056)
     res = {"fields_per_generation": fields, "ms_per_generation": round(dt_g * 1e3, 3),
            "propagations_per_s": round(fields / dt_g, 1), "useful_tflops": round(flops / dt_g / 1e12, 1),
-           "grouped_launches_per_generation": g.merged_launches(),
+           "grouped_mft_stages_per_generation": g.merged_launches(),
            "ms_per_generation_two_chains_separately": round(dt_s * 1e3, 3)}
     g.close()
     for ch in chains:

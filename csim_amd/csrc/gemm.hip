@@ -271,10 +271,13 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
   }
   u32x4 ra[ITA], rb[ITB];
   unsigned vA = 0, vB = 0;                        // valid bits of the slab in registers
+  const bool tile_full = m0 + BM <= M && n0 + BN <= N;
+  bool full = false;                              // block-uniform: no chunk of the slab needs zeroing
   // K is a multiple of 8: a chunk is wholly inside or outside.  Loads are
   // unconditional (clamped addresses); the data are zeroed only when stored to
   // LDS, so the loads stay in flight meanwhile.
   auto gload = [&](int k0) {
+    full = tile_full && k0 + BK <= K;
     if (k0 + BK <= K) {
 #pragma unroll
       for (int it = 0; it < ITA; it++) ra[it] = *reinterpret_cast<const GLOBAL u32x4*>(A + offA[it] + k0);
@@ -295,6 +298,19 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
     unsigned short* base = lds + buf * STAGE;
     const u32x4 z = u32x4{0, 0, 0, 0};
     const int c = t % CH;
+    if (full) {
+#pragma unroll
+      for (int it = 0; it < ITA; it++) {
+        const int e = it * NT + t, q = e / (BM * CH), r = (e % (BM * CH)) / CH;
+        *reinterpret_cast<u32x4*>(base + q * PA + swz<BK>(r, c)) = ra[it];
+      }
+#pragma unroll
+      for (int it = 0; it < ITB; it++) {
+        const int e = it * NT + t, q = e / (BN * CH), r = (e % (BN * CH)) / CH;
+        *reinterpret_cast<u32x4*>(base + NP * PA + q * PB + swz<BK>(r, c)) = rb[it];
+      }
+      return;
+    }
 #pragma unroll
     for (int it = 0; it < ITA; it++) {
       const int e = it * NT + t, q = e / (BM * CH), r = (e % (BM * CH)) / CH;

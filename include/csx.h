@@ -176,7 +176,7 @@ int csx_group_candidates(csx_group* group);          /* sum of the chains' candi
 /* Region contrast (regionContrast::compute_contrast) of every candidate of every
  * chain, chain by chain; each chain's calibration is its own (cached). */
 int csx_group_contrast(csx_group* group, const csx_region* region, double* out);
-int csx_group_merged_launches(csx_group* group);     /* grouped contraction launches per propagation */
+int csx_group_merged_launches(csx_group* group);     /* grouped MFT stages per propagation (2 GEMM launches each) */
 
 /* ------------------------------------------------------ operator API (device) */
 /* Uniform 1-D grid: x_i = x0 + i*dx, n points. */
