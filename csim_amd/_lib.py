@@ -66,6 +66,8 @@ SIGNATURES = {
     "csx_chain_field_profiles": (_i, [_p, _p, _i, _p]),
     "csx_chain_set_profiling": (_i, [_p, _i]),
     "csx_chain_set_graph": (_i, [_p, _i]),
+    "csx_chain_hex_response": (_i, [_p, C.c_char_p, C.POINTER(Region), _p, C.POINTER(_i), C.POINTER(_i),
+                                    C.POINTER(_i), C.POINTER(_d)]),
     "csx_group_create": (_i, [_p, _i, C.POINTER(_p)]),
     "csx_group_destroy": (None, [_p]),
     "csx_group_candidates": (_i, [_p]),

@@ -45,17 +45,20 @@ class Chain:
 
     # -------------------------------------------------------------- setup --
     def set_array(self, element: str, key: str, a):
+        self._info = None
         a = _colmajor(a, np.float64)
         if a.ndim == 1:
             a = a.reshape(-1, 1, order="F")
         check(lib().csx_chain_set_array(self._p, element.encode(), key.encode(), _ptr(a), a.shape[0], a.shape[1]))
 
     def set_complex_array(self, element: str, key: str, a, slice_index: int = 0):
+        self._info = None
         a = _colmajor(a, np.complex128)
         check(lib().csx_chain_set_complex_array(self._p, element.encode(), key.encode(), _ptr(a),
                                                 a.shape[0], a.shape[1], int(slice_index)))
 
     def set_opt_data(self, element: str, name: str, v):
+        self._info = None
         v = np.ascontiguousarray(v, dtype=np.float64).ravel()
         check(lib().csx_chain_set_opt_data(self._p, element.encode(), name.encode(), _ptr(v), v.size))
 
@@ -68,6 +71,7 @@ class Chain:
 
     def set_candidates(self, element: str, sags):
         """sags: (n_cand, n_hex) array, or None to return to the single sag set."""
+        self._info = None
         if sags is None:
             check(lib().csx_chain_set_candidates(self._p, element.encode(), None, 0, 0))
             return
@@ -75,12 +79,16 @@ class Chain:
         check(lib().csx_chain_set_candidates(self._p, element.encode(), _ptr(s), s.shape[1], s.shape[0]))
 
     def set_precision(self, precision: str):
+        self._info = None
         check(lib().csx_chain_set_precision(self._p, PRECISIONS[precision]))
 
     def info(self) -> dict:
-        i = ChainInfo()
-        check(lib().csx_chain_get_info(self._p, C.byref(i)))
-        return {f: getattr(i, f) for f, _ in ChainInfo._fields_}
+        """Geometry and batch shape; cached until a setter changes the chain."""
+        if getattr(self, "_info", None) is None:
+            i = ChainInfo()
+            check(lib().csx_chain_get_info(self._p, C.byref(i)))
+            self._info = {f: getattr(i, f) for f, _ in ChainInfo._fields_}
+        return dict(self._info)
 
     def lambdas(self):
         n = check(lib().csx_chain_get_lambdas(self._p, None, 0))
@@ -126,6 +134,25 @@ class Chain:
         out = np.zeros((nf, nbins))
         check(lib().csx_chain_field_profiles(self._p, _ptr(b), int(nbins), _ptr(out)))
         return out
+
+    def hex_response(self, element: str, radii=None, angles=(-180.0, 180.0), xlim=(-1e16, 1e16),
+                     reference_lambda=-1.0):
+        """Response matrix of the hex FPM (makeHexCFpmResponse): device complex128
+        tensor (n_slices, n_hex + 1, n_pix) (each slice column-major npix x ncol),
+        and the calibration peak."""
+        import torch
+        r = None
+        if radii is not None:
+            r = C.byref(Region(float(radii[0]), float(radii[1]), float(angles[0]), float(angles[1]),
+                               float(xlim[0]), float(xlim[1]), float(reference_lambda)))
+        npix, ncol, nsl, calib = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        check(lib().csx_chain_hex_response(self._p, element.encode(), r, None, C.byref(npix), C.byref(ncol),
+                                           C.byref(nsl), C.byref(calib)))
+        M = torch.empty((nsl.value, ncol.value, npix.value), dtype=torch.complex128, device="cuda")
+        check(lib().csx_chain_hex_response(self._p, element.encode(), r, C.c_void_p(M.data_ptr()), C.byref(npix),
+                                           C.byref(ncol), C.byref(nsl), C.byref(calib)))
+        self._info = None
+        return M, calib.value
 
     def set_graph(self, on: bool):
         """Replay every propagation after the first as one HIP graph."""

@@ -177,6 +177,15 @@ int csx_chain_set_profiling(csx_chain* c, int enable) {
   return c->impl.set_profiling(enable != 0);
 }
 
+int csx_chain_hex_response(csx_chain* c, const char* element_name, const csx_region* region, void* d_M, int* npix,
+                           int* ncol, int* nslices, double* calib_peak) {
+  CSX_GUARD_BEGIN
+  CSX_TRY(chk(c));
+  if (!element_name) return fail(CSX_ERR_ARG, "null element name");
+  return c->impl.hex_response(element_name, region, static_cast<double2*>(d_M), npix, ncol, nslices, calib_peak);
+  CSX_GUARD_END
+}
+
 int csx_group_create(csx_chain* const* chains, int n, csx_group** out) {
   CSX_GUARD_BEGIN
   if (!chains || n <= 0 || !out) return fail(CSX_ERR_ARG, "csx_group_create: bad arguments");

@@ -2,6 +2,7 @@
 #include "engine.hpp"
 
 #include <complex>
+#include <limits>
 
 #include <algorithm>
 #include <chrono>
@@ -876,9 +877,12 @@ struct DoubleFraunhoferElem : Element {
         int l = f % nl, cand = f / (nf / ncand);
         if (cand >= ncm) cand = ncm - 1;
         MftField& m = mk[f];
-        m.mask_kind = MASK_COMPLEX;
+        // response batch: a single-hex candidate passes nothing outside the mask
+        // rectangle (that light belongs to the last, outside-the-hexes column)
+        m.mask_kind = (use == kUseOnlyPerCandidate && f / (nf / ncand) < hex.raster.nhex) ? MASK_COMPLEX_CLIP
+                                                                                          : MASK_COMPLEX;
         m.mask = dm + ((size_t)cand * nl + l) * plane;
-        // the hex mask is fliplr'd? no: applied as computed (doubleFraunhoferInterpFPM)
+        // applied as computed: doubleFraunhoferInterpFPM does not mirror the hex mask
         m.mrows = S; m.mcols = S;
         m.row0 = gf[l].ny / 2 - S / 2; m.col0 = gf[l].nx / 2 - S / 2;
       }
@@ -947,7 +951,7 @@ struct FpmPupToLyotElem : Element {
   bool calibration_aware() const override { return true; }   // the binary mask has a calibration mask
   int mtype = MT_NONE;
   double focalRatio = -1, fRatioSign = 1, zoomFftSign = 1;
-  bool babinet = false;
+  int babinet_set = -1;   // setBabinet (fpmPupToLyot.cpp:452-453); -1: the mask's own default
   // fpmCMCForPupToLyot
   std::vector<double2> cube;
   int mrows = 0, mcols = 0, mslices = 0, maskIndex = -1;
@@ -1029,7 +1033,7 @@ struct FpmPupToLyotElem : Element {
   }
   int set_opt(Chain& ch, const std::string& n, const std::vector<double>& v) override {
     if (n == "reinit") return CSX_OK;   // spectra are rebuilt on every propagation
-    if (mtype == MT_HEX && n == "setBabinet" && !v.empty()) { babinet = v[0] != 0; ch.invalidate(); return CSX_OK; }
+    if (mtype == MT_HEX && n == "setBabinet" && !v.empty()) { babinet_set = v[0] != 0; ch.invalidate(); return CSX_OK; }
     if (mtype == MT_HEX && n == "sags") {
       hex.sags = v;
       ch.invalidate_calibration();
@@ -1084,7 +1088,11 @@ struct FpmPupToLyotElem : Element {
   }
   int compile(Chain& ch, Compile& c, int) override {
     const bool binary = mtype == MT_BINARY;
-    if (skipped(c) && !binary) return CSX_OK;   // Babinet masks: K = exp(0) - 1 = 0, E unchanged
+    // doBabinet: on for the complex-cube and hex masks unless set off (set_babinet(true)
+    // in their constructors); it both subtracts 1 from the mask and adds E back
+    const bool babinet = babinet_set >= 0 ? babinet_set != 0 : !binary;
+    if (skipped(c) && !binary && babinet) return CSX_OK;   // calibration K = exp(0) - 1 = 0: E unchanged
+    const bool calib_ones = skipped(c) && !binary;         // Babinet off: calibration K = exp(0) = 1
     const bool use_calib = binary && skipped(c);
     if (use_calib && calibMat.empty()) return fail(CSX_ERR_ARG, "fpmBinaryForPupToLyot: calibFilename unset");
     if (use_calib && (crows != brows || ccols != bcols)) return fail(CSX_ERR_ARG, "fpmBinaryForPupToLyot: calib size");
@@ -1106,7 +1114,12 @@ struct FpmPupToLyotElem : Element {
     long src_stride = 0;
     int fl = 0, minus1 = 0;
     std::function<int(hipStream_t)> hex_stage;
-    if (mtype == MT_HEX) {
+    if (calib_ones) {
+      std::vector<double2> m((size_t)S2, make_double2(1.0, 0.0));
+      CSX_TRY(d_src[which].reserve(sizeof(double2) * S2));
+      CSX_HIP(hipMemcpy(d_src[which].p, m.data(), sizeof(double2) * S2, hipMemcpyHostToDevice));
+      src = d_src[which].as<double2>(); src_stride = 0;
+    } else if (mtype == MT_HEX) {
       const int S = hex.raster.S;
       const double* sags_dev = nullptr;
       int nhex = hex.raster.nhex, ncd = 1;
@@ -1144,7 +1157,7 @@ struct FpmPupToLyotElem : Element {
       CSX_HIP(hipMemcpy(d_src[which].p, m.data(), sizeof(double2) * S2, hipMemcpyHostToDevice));
       src = d_src[which].as<double2>(); src_stride = 0;   // the same mask for every slice
     }
-    babinet = mtype != MT_BINARY;
+    if (mtype != MT_BINARY) minus1 = babinet ? 1 : 0;
     // kernel inputs K (one per (mask candidate, lambda)), their transforms H
     const int nk = ncm * nl;
     CSX_TRY(d_K[which].reserve(sizeof(double2) * S2 * nk));
@@ -1175,7 +1188,13 @@ struct FpmPupToLyotElem : Element {
     CSX_TRY(plan->build(F, CSX_FP64, ch.stream()));
     const MftPlan* pp = plan.get();
     c.plans.push_back(std::move(plan));
-    const int rows = mg.ny, cols = mg.nx, fli = fl, mi1 = minus1;
+    // Babinet rows: every mask, or in the response batch (candidate c uses only hex c)
+    // only the last candidate, whose mask is the area outside the hexes
+    // (makeHexCFpmResponse.cpp:150-162: setBabinet for the last column only)
+    const bool resp = mtype == MT_HEX && hex.useOnlyThisHex == kUseOnlyPerCandidate;
+    const int big = std::numeric_limits<int>::max();
+    const int rows = mg.ny, cols = mg.nx, fli = fl;
+    const int mi1 = !minus1 ? big : (resp ? hex.raster.nhex * nl : 0);
     const double zs = zoomFftSign;
     Chain* chp = &ch;
     c.stages.push_back([=](hipStream_t s) {
@@ -1189,7 +1208,8 @@ struct FpmPupToLyotElem : Element {
     double2* Ef = ch.field_buf(c.buf);
     double2* P = d_P[which].as<double2>();
     const double2* dsc = d_scale[which].as<double2>();
-    const int bab = babinet ? 1 : 0, per_cand = nsrc * nl, ncmc = ncm;
+    const int bab = !babinet ? big : (resp ? hex.raster.nhex * nsrc * nl : 0);
+    const int per_cand = nsrc * nl, ncmc = ncm;
     c.stages.push_back([=](hipStream_t s) {
       CSX_TRY(embed_center(Ef, n, n, P, n2, n2, nf, 0, s));
       CSX_TRY(fft2_batched(P, n2, nf, -1, s));
@@ -1663,6 +1683,35 @@ int Chain::ensure_calibration() {
   return CSX_OK;
 }
 
+// Output pixels of the contrast region (regionContrast::get_region_pixels, :89-101),
+// column-major (arma::find) order; r overrides the script's regionContrast block.
+int Chain::region_pixels(const csx_region* r, std::vector<int32_t>& idx) {
+  Region R = region_;
+  if (r) {
+    R.r1 = r->radius1; R.r2 = r->radius2; R.a1 = r->angle1; R.a2 = r->angle2;
+    R.x1 = r->xlim1; R.x2 = r->xlim2; R.refLambda = r->reference_lambda > 0 ? r->reference_lambda : -1.0;
+  }
+  double refl = R.refLambda;
+  if (refl == -1.0) {
+    double sum = 0;
+    for (double l : ef_.lambdas) sum += l;
+    refl = sum / nlam();
+  }
+  const double loD = tel_.loD(refl);
+  const Geom g = out_geom_;
+  idx.clear();
+  for (int c = 0; c < g.nx; c++)
+    for (int rr = 0; rr < g.ny; rr++) {
+      double XX = g.x0 + c * g.dx, YY = g.y0 + rr * g.dy;
+      double RR = std::sqrt(std::pow(XX, 2) + std::pow(YY, 2)), TT = std::atan2(YY, XX);
+      if ((XX / loD >= R.x1) && (XX / loD <= R.x2) && (RR / loD >= R.r1) && (RR / loD <= R.r2) &&
+          (TT > R.a1 * M_PI / 180) && (TT < R.a2 * M_PI / 180))
+        idx.push_back(c * g.ny + rr);
+    }
+  if (idx.empty()) return fail(CSX_ERR_ARG, "contrast region contains no pixel");
+  return CSX_OK;
+}
+
 // Region mean of sum_{sources, lambdas} |E|^2 / peak per candidate from the fields
 // of the last full propagation (regionContrast::compute_contrast, :145-205).
 int Chain::reduce_contrast(const csx_region* r, double* out) {
@@ -1689,15 +1738,7 @@ int Chain::reduce_contrast(const csx_region* r, double* out) {
                                    g.dx, g.dy};
   if (key != region_key_) {
     std::vector<int32_t> idx;
-    for (int c = 0; c < g.nx; c++)
-      for (int rr = 0; rr < g.ny; rr++) {
-        double XX = g.x0 + c * g.dx, YY = g.y0 + rr * g.dy;
-        double RR = std::sqrt(std::pow(XX, 2) + std::pow(YY, 2)), TT = std::atan2(YY, XX);
-        if ((XX / loD >= R.x1) && (XX / loD <= R.x2) && (RR / loD >= R.r1) && (RR / loD <= R.r2) &&
-            (TT > R.a1 * M_PI / 180) && (TT < R.a2 * M_PI / 180))
-          idx.push_back(c * g.ny + rr);
-      }
-    if (idx.empty()) return fail(CSX_ERR_ARG, "contrast region contains no pixel");
+    CSX_TRY(region_pixels(r, idx));
     CSX_TRY(region_idx_.reserve(sizeof(int32_t) * idx.size()));
     CSX_HIP(hipMemcpy(region_idx_.p, idx.data(), sizeof(int32_t) * idx.size(), hipMemcpyHostToDevice));
     n_region_ = (int)idx.size();
@@ -1771,6 +1812,62 @@ int Chain::info(csx_chain_info* o) {
   return CSX_OK;
 }
 
+
+// ======================================================= response matrix ==
+// makeHexCFpmResponse::compute_response (src/tools/makeHexCFpmResponse.cpp:91-190) as
+// one batch: with every sag zero, candidate c < n_hex transmits only hex c and the
+// last candidate only the area outside the hexes (with the Babinet term for the
+// convolution FPM); the region pixels of each field form column c of M.  The
+// objective of any sag vector is then |M [exp(i phaseSign 4 pi s / lambda); 1]|^2
+// (optHexRespObjective, src/optimization/optObjective.cpp:24-69).
+int Chain::hex_response(const std::string& elem, const csx_region* r, double2* d_M, int* npix, int* ncol,
+                        int* nslice, double* calib) {
+  Element* e = elem_by_name(elem);
+  const HexFpm* hx = nullptr;
+  if (auto* d = dynamic_cast<DoubleFraunhoferElem*>(e)) hx = d->has_hex ? &d->hex : nullptr;
+  else if (auto* p = dynamic_cast<FpmPupToLyotElem*>(e)) hx = p->mtype == FpmPupToLyotElem::MT_HEX ? &p->hex : nullptr;
+  if (!hx) return fail(CSX_ERR_ARG, "no hex-FPM element named " + elem);
+  const int nhex = hx->raster.nhex;
+  CSX_TRY(compile_all());
+  std::vector<int32_t> idx;
+  CSX_TRY(region_pixels(r, idx));
+  if (npix) *npix = (int)idx.size();
+  if (ncol) *ncol = nhex + 1;
+  if (nslice) *nslice = nsrc() * nlam();
+  if (!d_M) return CSX_OK;
+  std::vector<double> keep_use;
+  CSX_TRY(e->get_opt(*this, "useOnlyThisHex", keep_use));
+  const int keep_ncand = ncand_, keep_nhex = cand_nhex_;
+  std::vector<double> keep_sags;
+  if (keep_ncand > 0) {
+    keep_sags.resize((size_t)keep_nhex * keep_ncand);
+    CSX_HIP(hipMemcpy(keep_sags.data(), cand_sags_.p, sizeof(double) * keep_sags.size(), hipMemcpyDeviceToHost));
+  }
+  const std::string keep_elem = cand_elem_;
+  std::vector<double> zeros((size_t)nhex * (nhex + 1), 0.0);
+  int rc = e->set_opt(*this, "useOnlyThisHex", std::vector<double>{(double)kUseOnlyPerCandidate});
+  if (rc == CSX_OK) rc = set_candidates(elem, zeros.data(), nhex, nhex + 1);
+  if (rc == CSX_OK) rc = ensure_calibration();
+  if (rc == CSX_OK) rc = propagate(false);
+  if (rc == CSX_OK) {
+    DevBuf& di = scratch("response_idx");
+    rc = di.reserve(sizeof(int32_t) * idx.size());
+    if (rc == CSX_OK && hipMemcpy(di.p, idx.data(), sizeof(int32_t) * idx.size(), hipMemcpyHostToDevice) != hipSuccess)
+      rc = fail(CSX_ERR_HIP, "response: index upload");
+    if (rc == CSX_OK)
+      rc = gather_response(field_buf(out_buf_), out_geom_.npix(), di.as<int32_t>(), (int)idx.size(), nhex + 1,
+                           nsrc() * nlam(), d_M, stream());
+    if (rc == CSX_OK && hipStreamSynchronize(stream()) != hipSuccess) rc = fail(CSX_ERR_HIP, "response: sync");
+  }
+  if (calib) *calib = calib_peak_;
+  // restore the element's hex selection and the candidate batch
+  int rc2 = e->set_opt(*this, "useOnlyThisHex", keep_use);
+  int rc3 = keep_ncand > 0 ? set_candidates(keep_elem, keep_sags.data(), keep_nhex, keep_ncand)
+                           : set_candidates(elem, nullptr, 0, 0);
+  if (rc != CSX_OK) return rc;
+  if (rc2 != CSX_OK) return rc2;
+  return rc3;
+}
 
 // ============================================================ chain group ==
 int ChainGroup::n_candidates() const {

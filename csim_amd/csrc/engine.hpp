@@ -123,6 +123,9 @@ class Chain {
   int contrast(const csx_region* r, double* out, csx_report* rep);
   int ensure_calibration();
   int reduce_contrast(const csx_region* r, double* out);
+  int region_pixels(const csx_region* r, std::vector<int32_t>& idx);
+  int hex_response(const std::string& elem, const csx_region* r, double2* d_M, int* npix, int* ncol, int* nslice,
+                   double* calib);
   // ---- chain groups (one launch per contraction over several chains)
   Compile& plan(bool calibrating) { return calibrating ? plan_cal_ : plan_full_; }
   int compile() { return compile_all(); }

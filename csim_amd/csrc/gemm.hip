@@ -53,7 +53,7 @@ __device__ __forceinline__ void epilogue_value(const GemmProb& P, int m, int n, 
     int mr = m - P.mm0, nc = n - P.mn0;
     if (mr >= 0 && mr < P.mM && nc >= 0 && nc < P.mN) {
       long off = (long)mr * P.ldm + nc;
-      if (P.mask_kind == MASK_COMPLEX) {
+      if (P.mask_kind == MASK_COMPLEX || P.mask_kind == MASK_COMPLEX_CLIP) {
         f64x2 w = gin<f64x2>(P.mask)[off];
         double tr = vr * w.x - vi * w.y;
         vi = vr * w.y + vi * w.x;
@@ -68,6 +68,8 @@ __device__ __forceinline__ void epilogue_value(const GemmProb& P, int m, int n, 
           vr *= w; vi *= w;
         }
       }
+    } else if (P.mask_kind == MASK_COMPLEX_CLIP) {
+      vr = 0; vi = 0;
     }
   }
   out_r = vr;

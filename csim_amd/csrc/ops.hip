@@ -430,14 +430,15 @@ __global__ void k_hex(const int32_t* __restrict__ nsub, const int32_t* __restric
     int cand = (int)(cl / nlam), li = (int)(cl - (long)cand * nlam);
     double lam = lambdas[li];
     const double* sg = sags + (long)cand * nhex;
+    const int only = use_only == kUseOnlyPerCandidate ? cand : use_only;
     double ar = 0, ai = 0;
 #pragma unroll
     for (int s = 0; s < 3; s++) {
       int h = hexnum[p + s * plane];
       double ns = (double)nsub[p + s * plane];
-      if (use_only > -1) {
-        if (h == use_only) ;
-        else if (use_only == nhex && h == -1) ;
+      if (only > -1) {
+        if (h == only) ;
+        else if (only == nhex && h == -1) ;
         else ns = 0;
       }
       double sag = h > -1 ? sg[h] : 0.0;
@@ -602,23 +603,23 @@ int embed_center(const double2* src, int srows, int scols, double2* dst, int dro
 // fpmPupToLyot (src/elements/fpmPupToLyot.cpp:24-167).  Kernel input of the mask
 // transform: out[k] = factor * (src[k] - minus1), columns mirrored for fliplr.
 __global__ void k_fpm_prep(const double2* __restrict__ src, long src_stride, int rows, int cols, int n,
-                           int fliplr, int minus1, double factor, double2* __restrict__ out) {
+                           int fliplr, int minus1_from, double factor, double2* __restrict__ out) {
   long np = (long)rows * cols, total = np * n;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
     long k = i / np, p = i - k * np;
     int c = (int)(p / rows), r = (int)(p - (long)c * rows);
     int sc = fliplr ? cols - 1 - c : c;
     double2 v = src[k * src_stride + (long)sc * rows + r];
-    if (minus1) v.x -= 1.0;
+    if (k >= minus1_from) v.x -= 1.0;
     out[i] = make_double2(factor * v.x, factor * v.y);
   }
 }
-int fpm_prep(const double2* src, long src_stride, int rows, int cols, int n, int fliplr, int minus1, double factor,
-             double2* out, hipStream_t s) {
+int fpm_prep(const double2* src, long src_stride, int rows, int cols, int n, int fliplr, int minus1_from,
+             double factor, double2* out, hipStream_t s) {
   long total = (long)rows * cols * n;
   if (total <= 0) return CSX_OK;
   hipLaunchKernelGGL(k_fpm_prep, dim3(grid_for(total)), dim3(256), 0, s, src, src_stride, rows, cols, n, fliplr,
-                     minus1, factor, out);
+                     minus1_from, factor, out);
   CSX_HIP(hipGetLastError());
   return CSX_OK;
 }
@@ -653,7 +654,7 @@ int spectrum_mul(double2* P, int n2, int nfield, const double2* H, int per_cand,
 
 // E[f] = scale[l] * P[f](centre) + (babinet ? E[f] : 0), centre rows/cols n/2 .. 3n/2 - 1.
 __global__ void k_crop_add(double2* __restrict__ E, int n, const double2* __restrict__ P, int n2, int nfield, int nl,
-                           const double2* __restrict__ scale, int babinet) {
+                           const double2* __restrict__ scale, int babinet_from) {
   long np = (long)n * n, total = np * nfield;
   int o = n / 2;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
@@ -662,15 +663,39 @@ __global__ void k_crop_add(double2* __restrict__ E, int n, const double2* __rest
     double2 a = P[f * (long)n2 * n2 + (long)(c + o) * n2 + (r + o)];
     double2 sc = scale[f % nl];
     double2 v = make_double2(a.x * sc.x - a.y * sc.y, a.x * sc.y + a.y * sc.x);
-    if (babinet) { double2 e = E[i]; v.x += e.x; v.y += e.y; }
+    if (f >= babinet_from) { double2 e = E[i]; v.x += e.x; v.y += e.y; }
     E[i] = v;
   }
 }
-int crop_add(double2* E, int n, const double2* P, int n2, int nfield, int nl, const double2* scale_dev, int babinet,
-             hipStream_t s) {
+int crop_add(double2* E, int n, const double2* P, int n2, int nfield, int nl, const double2* scale_dev,
+             int babinet_from, hipStream_t s) {
   long total = (long)n * n * nfield;
   if (total <= 0) return CSX_OK;
-  hipLaunchKernelGGL(k_crop_add, dim3(grid_for(total)), dim3(256), 0, s, E, n, P, n2, nfield, nl, scale_dev, babinet);
+  hipLaunchKernelGGL(k_crop_add, dim3(grid_for(total)), dim3(256), 0, s, E, n, P, n2, nfield, nl, scale_dev,
+                     babinet_from);
+  CSX_HIP(hipGetLastError());
+  return CSX_OK;
+}
+
+
+// Response-matrix gather (makeHexCFpmResponse.cpp:144-184): field f = (c * nsrc + s) *
+// nl + l, region pixel j -> M[slice = s * nl + l][column c][j] (column-major npix x ncol).
+__global__ void k_gather_response(const double2* __restrict__ E, long npix_field, const int32_t* __restrict__ idx,
+                                  int nidx, int ncol, int nslice, double2* __restrict__ M) {
+  long total = (long)nidx * ncol * nslice;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    long j = i % nidx, rest = i / nidx;
+    int c = (int)(rest % ncol), sl = (int)(rest / ncol);
+    long f = (long)c * nslice + sl;
+    M[i] = E[f * npix_field + idx[j]];
+  }
+}
+int gather_response(const double2* E, long npix_field, const int32_t* idx, int nidx, int ncol, int nslice,
+                    double2* M, hipStream_t s) {
+  long total = (long)nidx * ncol * nslice;
+  if (total <= 0) return CSX_OK;
+  hipLaunchKernelGGL(k_gather_response, dim3(grid_for(total)), dim3(256), 0, s, E, npix_field, idx, nidx, ncol,
+                     nslice, M);
   CSX_HIP(hipGetLastError());
   return CSX_OK;
 }

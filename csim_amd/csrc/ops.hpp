@@ -46,6 +46,9 @@ int resample_batched(const double2* in, int rows, int cols, int nfield, double2*
                      int ocols, const InterpTable* ty_dev, const InterpTable* tx_dev, hipStream_t s);
 
 // Hex FPM masks for (candidate, lambda) pairs (complexHexMaskFPM.cpp:44-99).
+// use_only: -1 every hex, h one hex (h == nhex: only the area outside the hexes),
+// kUseOnlyPerCandidate: candidate c uses only hex c (the response batch).
+constexpr int kUseOnlyPerCandidate = -2;
 int hex_masks(const int32_t* nsub, const int32_t* hexnum, int S, int nsubpix, const double* sags,
               int nhex, int ncand, const double* lambdas_dev, int nlam, double phase_sign,
               int use_only, double2* out, hipStream_t s);
@@ -67,14 +70,17 @@ int embed_center(const double2* src, int srows, int scols, double2* dst, int dro
                  int nfield, int crop, hipStream_t s);
 
 // Convolution-form FPM (fpmPupToLyot.cpp:24-167), square even fields, 2n padding.
-// out[k] = factor (src[k] - minus1) over n masks of rows x cols (columns mirrored if fliplr)
-int fpm_prep(const double2* src, long src_stride, int rows, int cols, int n, int fliplr, int minus1, double factor,
-             double2* out, hipStream_t s);
+// out[k] = factor (src[k] - (k >= minus1_from)) over n masks of rows x cols (columns mirrored if fliplr)
+int fpm_prep(const double2* src, long src_stride, int rows, int cols, int n, int fliplr, int minus1_from,
+             double factor, double2* out, hipStream_t s);
 // P[f] *= H[cand(f) * nl + l(f)] (-1)^(r+c), cand = f / per_cand (clamped), l = f % nl
 int spectrum_mul(double2* P, int n2, int nfield, const double2* H, int per_cand, int nl, int nmask_cand,
                  hipStream_t s);
-// E[f] = scale[f % nl] * centre(P[f]) + (babinet ? E[f] : 0)
-int crop_add(double2* E, int n, const double2* P, int n2, int nfield, int nl, const double2* scale_dev, int babinet,
-             hipStream_t s);
+// E[f] = scale[f % nl] * centre(P[f]) + (f >= babinet_from ? E[f] : 0)
+int crop_add(double2* E, int n, const double2* P, int n2, int nfield, int nl, const double2* scale_dev,
+             int babinet_from, hipStream_t s);
+// M[s * nl + l][c][j] = E[(c * nslice + s * nl + l)][idx[j]], nslice = nsrc * nl
+int gather_response(const double2* E, long npix_field, const int32_t* idx, int nidx, int ncol, int nslice,
+                    double2* M, hipStream_t s);
 
 }  // namespace csx

@@ -92,7 +92,7 @@ class ScreenedSurrogate:
     exact values of the survivors: their labels go into a device ring buffer and a few
     Adam steps on random batches of it follow every generation."""
 
-    def __init__(self, mlp, lambdas, counts, exact, keep=32, tune_steps=4, lr=3e-4, capacity=32768, seed=5):
+    def __init__(self, mlp, lambdas, counts, exact, keep=32, tune_steps=2, lr=3e-4, capacity=32768, seed=5):
         import torch
         self.mlp, self.lambdas, self.counts, self.exact = mlp, list(lambdas), counts, exact
         self.keep, self.tune_steps, self.lr = keep, tune_steps, lr

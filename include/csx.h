@@ -163,6 +163,18 @@ int csx_chain_profile_json(csx_chain* chain, char* buf, size_t cap);
  * recompilation discards the graphs.  Ignored while profiling is on. */
 int csx_chain_set_graph(csx_chain* chain, int enable);
 
+/* Hex-FPM response matrix: makeHexCFpmResponse::compute_response
+ * (src/tools/makeHexCFpmResponse.cpp:91-190) as one batch of n_hex + 1
+ * candidates.  d_M (device, complex128): nslices = n_sources * n_lambdas slices
+ * (slice s * n_lambdas + l), each npix x ncol column-major, ncol = n_hex + 1;
+ * column h < n_hex is the region field of hex h alone, column n_hex the area
+ * outside the hexes (with the Babinet term for fpmPupToLyot), so that
+ * csx_hex_resp_objective(d_M, ...) gives the region contrast of any sag vector.
+ * d_M NULL: only the sizes.  The element's hex selection and the chain's
+ * candidate batch are restored afterwards.  calib_peak: the calibration peak. */
+int csx_chain_hex_response(csx_chain* chain, const char* element_name, const csx_region* region, void* d_M,
+                           int* npix, int* ncol, int* nslices, double* calib_peak);
+
 /* Chain group (no reference counterpart; csim runs one configuration at a time):
  * several chains with identical element lists, e.g. the same script at 512^2 and
  * 1024^2 pupils, propagated together.  Every contraction stage of all chains is

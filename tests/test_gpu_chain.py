@@ -209,3 +209,37 @@ def test_chain_group_uneven_pupils_matches_chains(ctx):
     assert g.merged_launches() == 3         # pupil->FPM, FPM->Lyot, Lyot->image: two GEMM launches each
     ref = np.concatenate(alone)
     assert np.max(np.abs(got - ref) / ref) < 1e-12
+
+
+@pytest.mark.parametrize("kind", ["doubleFraunhoferInterpFPM", "fpmPupToLyot"])
+def test_hex_response_matrix_reproduces_propagation(ctx, kind):
+    """makeHexCFpmResponse as one batch: the response matrix M and the reference's
+    linear objective |M [exp(i 4 pi s / lambda); 1]|^2 (csx_hex_resp_objective) give
+    the region contrast of full FP64 propagations of random sag vectors."""
+    import ctypes as C
+    import torch
+    from csim_amd._lib import check, lib
+    if kind == "fpmPupToLyot":
+        w = W.workload_p2l("hex")
+    else:
+        w = W.workload2(n=256, n_lambda=2, offsets=(0.0, 2.0))
+    ch = build(ctx, w, "fp64")
+    M, calib = ch.hex_response("fpm")
+    nsl, ncol, npix = M.shape
+    nhex = ncol - 1
+    rng = np.random.default_rng(41)
+    ncand = 5
+    sags = W.thickness_to_sag(rng.random((ncand, nhex)) * 1e-6)
+    ch.set_candidates("fpm", sags)
+    c, rep = ch.contrast()
+    assert abs(rep["calib_peak"] - calib) <= 1e-14 * calib
+    nsrc = nsl // len(ch.lambdas())
+    lam = np.ascontiguousarray(np.tile(ch.lambdas(), nsrc))
+    d_X = torch.from_numpy(np.ascontiguousarray(sags)).cuda()
+    d_o = torch.zeros(ncand, dtype=torch.float64, device="cuda")
+    check(lib().csx_hex_resp_objective(ctx.handle, C.c_void_p(M.data_ptr()), npix, ncol, nsl,
+                                       lam.ctypes.data_as(C.c_void_p), 1.0, float(calib),
+                                       C.c_void_p(d_X.data_ptr()), nhex, ncand, 0, C.c_void_p(d_o.data_ptr()), 0))
+    lin = d_o.cpu().numpy()
+    print(kind, "propagated", c, "linear", lin)
+    assert np.max(np.abs(lin - c) / c) < 1e-11
