@@ -228,11 +228,44 @@ __device__ __forceinline__ int swz(int row, int chunk) {
   return BK == 32 ? row * 32 + ((chunk ^ ((row >> 1) & 3)) << 3) : row * 64 + ((chunk ^ (row & 7)) << 3);
 }
 
-template <int TERMS, int BM, int BN, int WM, int WN, bool DB, int WPE, int BK>
+// BF16 MFMA shapes (layouts measured on gfx950).  16x16x32: lane l holds row l%16,
+// k = 8 (l/16) .. +7 of each operand; D value v at [4 (l/16) + v][l%16].
+// 32x32x16: lane l holds row l%32, k = 8 (l/32) .. +7; D value v at
+// [4 (l/32) + (v%4) + 8 (v/4)][l%32].
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+template <int MF> struct MfmaBf16;
+template <> struct MfmaBf16<16> {
+  typedef f32x4 acc_t;
+  static constexpr int NV = 4, KSTEP = 32;
+  static __device__ __forceinline__ acc_t mma(bf16x8 a, bf16x8 b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int frow(int lane) { return lane & 15; }
+  static __device__ __forceinline__ int fchunk(int lane, int ks) { return ks * 4 + (lane >> 4); }
+  static __device__ __forceinline__ int drow(int lane, int v) { return 4 * (lane >> 4) + v; }
+  static __device__ __forceinline__ int dcol(int lane) { return lane & 15; }
+};
+template <> struct MfmaBf16<32> {
+  typedef f32x16 acc_t;
+  static constexpr int NV = 16, KSTEP = 16;
+  static __device__ __forceinline__ acc_t mma(bf16x8 a, bf16x8 b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int frow(int lane) { return lane & 31; }
+  static __device__ __forceinline__ int fchunk(int lane, int ks) { return ks * 2 + (lane >> 5); }
+  static __device__ __forceinline__ int drow(int lane, int v) { return 4 * (lane >> 5) + (v & 3) + 8 * (v >> 2); }
+  static __device__ __forceinline__ int dcol(int lane) { return lane & 31; }
+};
+
+template <int TERMS, int BM, int BN, int WM, int WN, bool DB, int WPE, int BK, int MF>
 __global__ __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(WPE)))
 void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
+  typedef MfmaBf16<MF> Mf;
+  typedef typename Mf::acc_t Acc;
+  constexpr int NV = Mf::NV;
   constexpr int NT = 64 * WM * WN, NP = 2 * TERMS, CH = BK / 8;   // 16-byte chunks per LDS row
-  constexpr int IM = BM / WM / 16, JN = BN / WN / 16;
+  constexpr int IM = BM / WM / MF, JN = BN / WN / MF;
+  static_assert(IM >= 1 && JN >= 1 && (BM / WM) % MF == 0 && (BN / WN) % MF == 0, "wave tile");
   constexpr int PA = BM * BK, PB = BN * BK;        // bf16 elements of one plane of each operand tile
   constexpr int STAGE = NP * (PA + PB);
   constexpr int ITA = NP * BM * CH / NT, ITB = NP * BN * CH / NT;
@@ -325,38 +358,41 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
     }
   };
 
-  f32x4 accR[IM][JN], accI[IM][JN];
-  double dR[TERMS == 3 ? IM : 1][TERMS == 3 ? JN : 1][4], dI[TERMS == 3 ? IM : 1][TERMS == 3 ? JN : 1][4];
+  Acc accR[IM][JN], accI[IM][JN];
+  double dR[TERMS == 3 ? IM : 1][TERMS == 3 ? JN : 1][TERMS == 3 ? NV : 1];
+  double dI[TERMS == 3 ? IM : 1][TERMS == 3 ? JN : 1][TERMS == 3 ? NV : 1];
 #pragma unroll
   for (int i = 0; i < IM; i++)
 #pragma unroll
-    for (int j = 0; j < JN; j++) { accR[i][j] = f32x4{0, 0, 0, 0}; accI[i][j] = f32x4{0, 0, 0, 0}; }
+    for (int j = 0; j < JN; j++)
+#pragma unroll
+      for (int v = 0; v < NV; v++) { accR[i][j][v] = 0.f; accI[i][j][v] = 0.f; }
   if (TERMS == 3) {
 #pragma unroll
     for (int i = 0; i < (TERMS == 3 ? IM : 1); i++)
 #pragma unroll
       for (int j = 0; j < (TERMS == 3 ? JN : 1); j++)
 #pragma unroll
-        for (int v = 0; v < 4; v++) { dR[i][j][v] = 0; dI[i][j][v] = 0; }
+        for (int v = 0; v < (TERMS == 3 ? NV : 1); v++) { dR[i][j][v] = 0; dI[i][j][v] = 0; }
   }
 
-  const int fr = lane & 15, fc = lane >> 4;
+  const int fr = Mf::frow(lane);
   const u32x4 sgn = u32x4{0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
   auto compute = [&](const unsigned short* base) {
 #pragma unroll
-   for (int kk = 0; kk < BK / 32; kk++) {
-    const int fk = kk * 4 + fc;                  // 16-byte chunk of this lane's 8 k values
+   for (int kk = 0; kk < BK / Mf::KSTEP; kk++) {
+    const int fk = Mf::fchunk(lane, kk);         // 16-byte chunk of this lane's 8 k values
     u32x4 b[NP][JN];
 #pragma unroll
     for (int q = 0; q < NP; q++)
 #pragma unroll
       for (int j = 0; j < JN; j++)
-        b[q][j] = *reinterpret_cast<const u32x4*>(base + NP * PA + q * PB + swz<BK>(wn * (BN / WN) + j * 16 + fr, fk));
+        b[q][j] = *reinterpret_cast<const u32x4*>(base + NP * PA + q * PB + swz<BK>(wn * (BN / WN) + j * MF + fr, fk));
 #pragma unroll
     for (int i = 0; i < IM; i++) {
       u32x4 a[NP];
 #pragma unroll
-      for (int q = 0; q < NP; q++) a[q] = *reinterpret_cast<const u32x4*>(base + q * PA + swz<BK>(wm * (BM / WM) + i * 16 + fr, fk));
+      for (int q = 0; q < NP; q++) a[q] = *reinterpret_cast<const u32x4*>(base + q * PA + swz<BK>(wm * (BM / WM) + i * MF + fr, fk));
 #pragma unroll
       for (int ta = 0; ta < TERMS; ta++) {
         const bf16x8 ar = __builtin_bit_cast(bf16x8, a[2 * ta]);
@@ -369,15 +405,15 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
           for (int j = 0; j < JN; j++) {
             const bf16x8 br = __builtin_bit_cast(bf16x8, b[2 * tb][j]);
             const bf16x8 bi = __builtin_bit_cast(bf16x8, b[2 * tb + 1][j]);
-            accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ar, br, accR[i][j], 0, 0, 0);
-            accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ar, bi, accI[i][j], 0, 0, 0);
+            accR[i][j] = Mf::mma(ar, br, accR[i][j]);
+            accI[i][j] = Mf::mma(ar, bi, accI[i][j]);
           }
 #pragma unroll
           for (int j = 0; j < JN; j++) {
             const bf16x8 br = __builtin_bit_cast(bf16x8, b[2 * tb][j]);
             const bf16x8 bi = __builtin_bit_cast(bf16x8, b[2 * tb + 1][j]);
-            accR[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ain, bi, accR[i][j], 0, 0, 0);
-            accI[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ai, br, accI[i][j], 0, 0, 0);
+            accR[i][j] = Mf::mma(ain, bi, accR[i][j]);
+            accI[i][j] = Mf::mma(ai, br, accI[i][j]);
           }
         }
       }
@@ -388,8 +424,10 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
 #pragma unroll
         for (int j = 0; j < (TERMS == 3 ? JN : 1); j++) {
 #pragma unroll
-          for (int v = 0; v < 4; v++) { dR[i][j][v] += accR[i][j][v]; dI[i][j][v] += accI[i][j][v]; }
-          accR[i][j] = f32x4{0, 0, 0, 0}; accI[i][j] = f32x4{0, 0, 0, 0};
+          for (int v = 0; v < (TERMS == 3 ? NV : 1); v++) {
+            dR[i][j][v] += accR[i][j][v]; dI[i][j][v] += accI[i][j][v];
+            accR[i][j][v] = 0.f; accI[i][j][v] = 0.f;
+          }
         }
     }
    }
@@ -429,9 +467,9 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
 #pragma unroll
       for (int j = 0; j < JN; j++)
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
-          const int ml = wm * (BM / WM) + i * 16 + 4 * (lane >> 4) + v;
-          const int nl = wn * (BN / WN) + j * 16 + (lane & 15);
+        for (int v = 0; v < NV; v++) {
+          const int ml = wm * (BM / WM) + i * MF + Mf::drow(lane, v);
+          const int nl = wn * (BN / WN) + j * MF + Mf::dcol(lane);
           double re = accR[i][j][v], im = accI[i][j][v];
           if (TERMS == 3) { re += dR[TERMS == 3 ? i : 0][TERMS == 3 ? j : 0][v]; im += dI[TERMS == 3 ? i : 0][TERMS == 3 ? j : 0][v]; }
           double vr, vi;
@@ -464,9 +502,9 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
 #pragma unroll
     for (int j = 0; j < JN; j++)
 #pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const int m = m0 + wm * (BM / WM) + i * 16 + 4 * (lane >> 4) + v;
-        const int n = n0 + wn * (BN / WN) + j * 16 + (lane & 15);
+      for (int v = 0; v < NV; v++) {
+        const int m = m0 + wm * (BM / WM) + i * MF + Mf::drow(lane, v);
+        const int n = n0 + wn * (BN / WN) + j * MF + Mf::dcol(lane);
         double re = accR[i][j][v], im = accI[i][j][v];
         if (TERMS == 3) { re += dR[TERMS == 3 ? i : 0][TERMS == 3 ? j : 0][v]; im += dI[TERMS == 3 ? i : 0][TERMS == 3 ? j : 0][v]; }
         epilogue_store(P, m, n, re, im);
@@ -480,18 +518,22 @@ struct PlanesVariant {
 };
 // (the last template argument is the minimum waves per SIMD the registers must allow)
 static const PlanesVariant kVariants2[] = {
-    {64, 64, 256, k_gemm_planes<2, 64, 64, 2, 2, false, 3, 32>},
-    {128, 128, 256, k_gemm_planes<2, 128, 128, 2, 2, true, 1, 32>},
-    {128, 128, 512, k_gemm_planes<2, 128, 128, 2, 4, true, 2, 32>},
-    {128, 64, 256, k_gemm_planes<2, 128, 64, 2, 2, false, 2, 32>},
-    {64, 64, 256, k_gemm_planes<2, 64, 64, 2, 2, true, 2, 32>},
-    {64, 64, 256, k_gemm_planes<2, 64, 64, 2, 2, false, 2, 64>},
-    {64, 64, 256, k_gemm_planes<2, 64, 64, 2, 2, true, 2, 64>},
+    {64, 64, 256, k_gemm_planes<2, 64, 64, 2, 2, false, 3, 32, 16>},
+    {128, 128, 256, k_gemm_planes<2, 128, 128, 2, 2, true, 1, 32, 16>},
+    {128, 128, 512, k_gemm_planes<2, 128, 128, 2, 4, true, 2, 32, 16>},
+    {128, 64, 256, k_gemm_planes<2, 128, 64, 2, 2, false, 2, 32, 16>},
+    {64, 64, 256, k_gemm_planes<2, 64, 64, 2, 2, true, 2, 32, 16>},
+    {64, 64, 256, k_gemm_planes<2, 64, 64, 2, 2, false, 2, 64, 16>},
+    {64, 64, 256, k_gemm_planes<2, 64, 64, 2, 2, true, 2, 64, 16>},
+    {128, 128, 256, k_gemm_planes<2, 128, 128, 2, 2, false, 1, 32, 32>},
+    {128, 128, 512, k_gemm_planes<2, 128, 128, 2, 4, true, 2, 32, 32>},
+    {128, 64, 256, k_gemm_planes<2, 128, 64, 2, 2, false, 2, 32, 32>},
+    {64, 128, 256, k_gemm_planes<2, 64, 128, 2, 2, false, 2, 32, 32>},
 };
 static const PlanesVariant kVariants3[] = {
-    {64, 64, 256, k_gemm_planes<3, 64, 64, 2, 2, false, 1, 32>},
-    {128, 64, 512, k_gemm_planes<3, 128, 64, 4, 2, false, 2, 32>},
-    {64, 64, 256, k_gemm_planes<3, 64, 64, 2, 2, true, 1, 32>},
+    {64, 64, 256, k_gemm_planes<3, 64, 64, 2, 2, false, 1, 32, 16>},
+    {128, 64, 512, k_gemm_planes<3, 128, 64, 4, 2, false, 2, 32, 16>},
+    {64, 64, 256, k_gemm_planes<3, 64, 64, 2, 2, true, 1, 32, 16>},
 };
 
 int planes_variant_count(int terms) {

@@ -15,7 +15,7 @@ src/tools/regionContrast.cpp:145-205) run two ways on the batched GPU chain:
 The surrogate is trained first on propagations of random thickness vectors
 (t ~ U(0, 1 um)) made by the same chain.  Usage:
 
-    python -m csim_amd.study [--n 512] [--train 22528] [--seconds 20]
+    python -m csim_amd.study [--n 512] [--train 90112] [--epochs 10] [--seconds 20]
 
 prints one JSON object (data-generation and training times, held-out rank
 correlation, and both DE traces with the time each reached the plain run's
@@ -102,6 +102,7 @@ class ScreenedSurrogate:
         self.gen = torch.Generator(device="cuda")
         self.gen.manual_seed(seed)
         self.errors = []                # per generation: Spearman(pred, exact) of the survivors
+        self.seconds = {"predict": 0.0, "exact": 0.0, "learn": 0.0}   # host wall time per phase
 
     def predict(self, thickness):
         import torch
@@ -124,10 +125,13 @@ class ScreenedSurrogate:
     def screen_and_eval(self, pop):
         """DE hook: pop (D, P) -> (indices evaluated, their exact values)."""
         import torch
+        t0 = time.perf_counter()
         t = np.ascontiguousarray(pop.T)
         pred = self.predict(t)
         idx = np.argsort(pred)[:self.keep]
+        t1 = time.perf_counter()
         vals, prof = self.exact.evaluate(t[idx], with_profiles=True)
+        t2 = time.perf_counter()
         if idx.size > 2:
             self.errors.append(spearman(pred[idx], vals))
         self._push(features_device(self.mlp.ctx, t[idx], self.lambdas), log_labels(prof).reshape(-1, N_BINS))
@@ -135,6 +139,10 @@ class ScreenedSurrogate:
         for _ in range(self.tune_steps):
             sel = torch.randint(0, self.filled, (nb,), device="cuda", generator=self.gen)
             self.mlp.train_step(self.Xd.index_select(0, sel), self.Yd.index_select(0, sel), self.lr, want_loss=False)
+        t3 = time.perf_counter()
+        self.seconds["predict"] += t1 - t0
+        self.seconds["exact"] += t2 - t1
+        self.seconds["learn"] += t3 - t2
         return idx, vals
 
 
@@ -212,7 +220,8 @@ def compare_de(ctx, w, sur, seconds=None, eval_budget=None, plain_pop=64, screen
                        "evaluations": rs.evaluations, "generations": len(rs.history) - 1,
                        "seconds_to_plain_best": reach[1] if reach else None,
                        "evaluations_to_plain_best": reach[3] if reach else None,
-                       "survivor_spearman_median": float(np.median(sur.errors)) if sur.errors else None}
+                       "survivor_spearman_median": float(np.median(sur.errors)) if sur.errors else None,
+                       "seconds_by_phase": {k: round(v, 3) for k, v in sur.seconds.items()}}
     out["wall_ratio_vs_plain"] = (reach[1] / rp.seconds) if reach else None
     log(f"screened DE: best {rs.best_value:.4e} after {rs.seconds:.1f} s; reached plain best: "
         f"{(reach[1], reach[3]) if reach else None}")
@@ -220,7 +229,7 @@ def compare_de(ctx, w, sur, seconds=None, eval_budget=None, plain_pop=64, screen
     return out
 
 
-def run(n=512, n_train=22528, seconds=20.0, n_lambda=9, plain_pop=64, screen_pop=512, keep=32, epochs=20,
+def run(n=512, n_train=90112, seconds=20.0, n_lambda=9, plain_pop=64, screen_pop=512, keep=32, epochs=10,
         precision="bf16x3", seed=20261207, log=print):
     from .ops import Context
     ctx = Context(0)
@@ -239,10 +248,10 @@ def run(n=512, n_train=22528, seconds=20.0, n_lambda=9, plain_pop=64, screen_pop
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--n", type=int, default=512)
-    ap.add_argument("--train", type=int, default=22528)
+    ap.add_argument("--train", type=int, default=90112)
     ap.add_argument("--seconds", type=float, default=20.0)
     ap.add_argument("--lambdas", type=int, default=9)
-    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--epochs", type=int, default=10)
     ap.add_argument("--precision", default="bf16x3")
     a = ap.parse_args()
     res = run(n=a.n, n_train=a.train, seconds=a.seconds, n_lambda=a.lambdas, epochs=a.epochs, precision=a.precision,

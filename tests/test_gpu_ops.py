@@ -83,7 +83,7 @@ def test_mft_mixed_error_level(ctx, prec, tol):
     assert e < tol
 
 
-VARIANTS = {"bf16x3": 7, "bf16x6": 3}
+VARIANTS = {"bf16x3": 11, "bf16x6": 3}
 
 
 @pytest.mark.parametrize("prec,tol", [("bf16x3", 5e-5), ("bf16x6", 5e-6)])
