@@ -96,12 +96,12 @@ def cpu_baseline(w, budget_s=12.0):
         s, li = divmod(n % (oc.m.nsrc * nl), nl)
         oc.field(s, li)
         n += 1
-        if time.perf_counter() - t0 > budget_s or n >= oc.m.nsrc * nl:
+        if time.perf_counter() - t0 > budget_s:
             break
     dt = time.perf_counter() - t0
     return {"value": round(n / dt, 4), "unit": "propagations/s", "cores": oracle.num_threads(), "kind": "port",
-            "sample": f"{n} of the workload's fields (pupil->FPM->Lyot->image), Bluestein chirp-z zoomFft, "
-                      f"{dt:.1f} s"}
+            "sample": f"{n} propagations (pupil->FPM->Lyot->image) cycling over the workload's "
+                      f"{oc.m.nsrc * nl} fields, Bluestein chirp-z zoomFft, {dt:.1f} s"}
 
 
 def de_batch_uneven(ctx, precision, steps=5):
