@@ -58,6 +58,19 @@ def time_chain(ch, steps, warmup):
     return dt, c, prof
 
 
+def measured_traffic(key):
+    """HBM bytes per launch of kernel `key` from the committed FETCH_SIZE / WRITE_SIZE
+    summary of this workload (rocprofv3 --pmc passes of `bench.py --pmc-probe`)."""
+    import re
+    path = os.path.join(HERE, "profiles", "gemm_bf16x3_workload2_hbm_traffic.md")
+    try:
+        txt = open(path).read()
+    except OSError:
+        return None
+    m = re.search(r"^traffic_bytes_per_launch\[" + re.escape(key) + r"\]: (\d+)", txt, re.M)
+    return int(m.group(1)) if m else None
+
+
 def gemm_roofline(prof, precision):
     key = f"gemm_{precision}"
     st = prof.get(key)
@@ -67,8 +80,11 @@ def gemm_roofline(prof, precision):
     avg_s = st["ms_total"] / st["launches"] / 1e3
     achieved = per_launch_flops / avg_s / 1e12
     peak = PEAKS[precision]
+    traffic = measured_traffic(key)
     return {"bound": "mfma", "kernel": key, "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-            "frac": round(achieved / peak, 4), "traffic": None,
+            "frac": round(achieved / peak, 4), "traffic": traffic,
+            "traffic_source": "profiles/gemm_bf16x3_workload2_hbm_traffic.md" if traffic else None,
+            "algorithmic_bytes_per_launch": st["bytes_total"] / st["launches"],
             "avg_launch_ms": round(st["ms_total"] / st["launches"], 4),
             "flops_per_launch": per_launch_flops, "mfma_products": PRODUCTS[precision],
             "issued_frac": round(achieved * PRODUCTS[precision] / peak, 4)}
@@ -135,16 +151,43 @@ def de_batch_uneven(ctx, precision, steps=5):
     torch.cuda.synchronize()
     dt_s = (time.perf_counter() - t0) / steps
     fields = 2 * 32 * 9
-    # useful flops: 1024^2 propagation 6,576,668,672; 512^2 1,946,157,056 (8 per complex MAC)
+    # csim's work per propagation: 1024^2 6,576,668,672 flop, 512^2 1,946,157,056 (8 per complex
+    # MAC); the candidates share their pupil-plane fields, whose first contraction runs once
     flops = 32 * 9 * (6576668672 + 1946157056)
     res = {"fields_per_generation": fields, "ms_per_generation": round(dt_g * 1e3, 3),
-           "propagations_per_s": round(fields / dt_g, 1), "useful_tflops": round(flops / dt_g / 1e12, 1),
+           "propagations_per_s": round(fields / dt_g, 1), "effective_tflops": round(flops / dt_g / 1e12, 1),
            "grouped_mft_stages_per_generation": g.merged_launches(),
            "ms_per_generation_two_chains_separately": round(dt_s * 1e3, 3)}
     g.close()
     for ch in chains:
         ch.close()
     return res
+
+
+def pmc_probe(precision):
+    """Short run for hardware-counter passes (rocprofv3 --pmc): a plain device copy
+    of known size (512 MiB read, 512 MiB written) that calibrates the byte counters,
+    then the workload's steps (graph replay off, so every kernel is its own dispatch)."""
+    import torch
+    from csim_amd import workloads as W
+    from csim_amd.ops import Context
+    x = torch.empty(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    x.fill_(1.0)
+    torch.cuda.synchronize()
+    for _ in range(2):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    del x, y
+    ctx = Context(0)
+    ch = build_chain(ctx, W.workload2(), precision)
+    ch.set_graph(False)
+    for _ in range(3):
+        ch.contrast()
+    torch.cuda.synchronize()
+    ch.close()
+    print("pmc probe done", flush=True)
+    return 0
 
 
 def main(argv=None):
@@ -156,6 +199,7 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-uneven", action="store_true")
+    ap.add_argument("--pmc-probe", action="store_true", help="short run for rocprofv3 --pmc counter passes")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print("bench.py measures one GPU: every field shards independently, so the whole-job rate at N GPUs is "
@@ -165,6 +209,8 @@ def main(argv=None):
     if not torch.cuda.is_available():
         print("bench.py needs a HIP device", file=sys.stderr)
         return 1
+    if a.pmc_probe:
+        return pmc_probe(a.precision)
     from csim_amd import workloads as W
     from csim_amd.ops import Context
     ctx = Context(0)

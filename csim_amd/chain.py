@@ -174,6 +174,15 @@ class Chain:
         check(lib().csx_chain_get_normalized_image(self._p, cand, _ptr(out)))
         return out.T.copy()
 
+    def contrast_curve(self, cand: int = 0):
+        """(radius in lambda/D, mean normalised intensity) rows of csim's contrast
+        curve (contrastCurve::make_contrast_curve) for one candidate."""
+        n = check(lib().csx_chain_contrast_curve(self._p, cand, None, None, 0))
+        pos = np.zeros(n)
+        avg = np.zeros(n)
+        check(lib().csx_chain_contrast_curve(self._p, cand, _ptr(pos), _ptr(avg), n))
+        return pos, avg
+
 
 class ChainGroup:
     """Chains with identical element lists (e.g. one script at several pupil sizes)

@@ -164,6 +164,16 @@ int csx_chain_get_normalized_image(csx_chain* c, int cand, double* out) {
   CSX_GUARD_END
 }
 
+int csx_chain_contrast_curve(csx_chain* c, int cand, double* pos, double* avg, int capacity) {
+  CSX_GUARD_BEGIN
+  CSX_TRY(chk(c));
+  if ((pos == nullptr) != (avg == nullptr)) return fail(CSX_ERR_ARG, "contrast curve: pos and avg both or neither");
+  int rows = 0;
+  CSX_TRY(c->impl.contrast_curve(cand, pos, avg, capacity, &rows));
+  return rows;
+  CSX_GUARD_END
+}
+
 int csx_chain_field_profiles(csx_chain* c, const int32_t* bin_of_pixel, int nbins, double* out) {
   CSX_GUARD_BEGIN
   CSX_TRY(chk(c));

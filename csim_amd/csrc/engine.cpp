@@ -200,10 +200,33 @@ static int flush_pointwise(Chain& ch, Compile& c, bool to_conv = false) {
   pc.lambdas = c.calibrating ? ch.scratch("lam_cal").as<double>() : ch.lambdas_dev();
   double2* E = ch.field_buf(c.buf);
   int nf = ch.nfield_for(c);
+  // fields are (candidate, source, lambda) with lambda fastest: a generating pass
+  // over masks and mirrors only runs pixel-major (one phasor per wavelength and source)
+  bool pm = pc.op[0].kind == PW_SOURCE && ch.nlam() <= kMaxPwLam && c.geom.npix() % 2 == 0;
+  for (int i = 1; i < pc.nops; i++) pm = pm && (pc.op[i].kind == PW_MASK_1PI || pc.op[i].kind == PW_MIRROR);
+  pc.nl = pm ? ch.nlam() : 0;
   std::string name = "pointwise x" + std::to_string(pc.nops) + (pc.out_kind != PW_OUT_C128 ? " -> working precision" : "");
   c.stages.push_back([=](hipStream_t s) { return pointwise_chain(E, nf, pc, s); });
   c.stage_names.push_back(name);
   c.pending.clear();
+  return CSX_OK;
+}
+
+// Copies the shared fields (one candidate's) to every candidate's slots of the
+// current complex128 buffer; afterwards the buffers hold every candidate.
+static int expand_candidates(Chain& ch, Compile& c) {
+  const int all = ch.ncand_for(c);
+  if (c.live <= 0 || c.live >= all) { c.live = 0; return CSX_OK; }
+  CSX_TRY(flush_pointwise(ch, c));
+  if (c.in_conv) return fail(CSX_ERR_STATE, "candidate expansion of converted fields");
+  const size_t bytes = sizeof(double2) * (size_t)ch.nfield_for(c) * c.geom.npix();
+  char* E = reinterpret_cast<char*>(ch.field_buf(c.buf));
+  c.stages.push_back([=](hipStream_t s) {
+    for (int k = 1; k < all; k++) CSX_HIP(hipMemcpyAsync(E + k * bytes, E, bytes, hipMemcpyDeviceToDevice, s));
+    return CSX_OK;
+  });
+  c.stage_names.push_back("candidate copies");
+  c.live = 0;
   return CSX_OK;
 }
 
@@ -517,9 +540,13 @@ struct DownsampleElem : Element {
 static int compile_mft(Chain& ch, Compile& c, const std::vector<Geom>& gout_per_lam,
                        const std::vector<double>& lf_per_lam, const std::string& label,
                        const std::vector<MftField>* masks = nullptr, bool out_conv = false,
-                       const double* epi_mask_1pi = nullptr) {
+                       const double* epi_mask_1pi = nullptr, bool expand = false) {
   CSX_TRY(flush_pointwise(ch, c, /*to_conv=*/true));
-  const int nf = ch.nfield_for(c), nl = ch.nlam();
+  // expand: the fields are one candidate's copy; the MFT writes every candidate's
+  // output (its own mask each) from that copy, whose first contraction the plan
+  // then computes once
+  const int nf_in = ch.nfield_for(c), nl = ch.nlam();
+  const int nf = expand ? ch.nfield_all(c) : nf_in;
   const Geom gin = c.geom;
   const int T = terms_of(ch.precision());
   const bool c64 = ch.precision() == CSX_FP32;
@@ -538,7 +565,8 @@ static int compile_mft(Chain& ch, Compile& c, const std::vector<Geom>& gout_per_
     const Geom& go = gout_per_lam[l];
     if (go.nx != g0.nx || go.ny != g0.ny) return fail(CSX_ERR_UNSUPPORTED, label + ": output size varies with lambda");
     MftField& m = F[f];
-    m.in = mixed ? (const void*)(conv + (size_t)f * gin.npix() * ein) : (const void*)(in + (size_t)f * gin.npix());
+    const size_t fi = (size_t)(f % nf_in);
+    m.in = mixed ? (const void*)(conv + fi * gin.npix() * ein) : (const void*)(in + fi * gin.npix());
     m.in_kind = wkind;
     if (out_conv) {
       m.out = conv + (size_t)f * go.npix() * ein;
@@ -562,8 +590,9 @@ static int compile_mft(Chain& ch, Compile& c, const std::vector<Geom>& gout_per_
   CSX_TRY(plan->build(F, ch.precision(), ch.stream()));
   const MftPlan* pp = plan.get();
   c.plans.push_back(std::move(plan));
-  size_t nconv = (size_t)nf * gin.npix();
+  size_t nconv = (size_t)nf_in * gin.npix();
   long npix_in = gin.npix();
+  if (expand) c.live = 0;
   Chain* chp = &ch;
   if (!from_conv && (c64 || T)) {   // operands to the working precision (no fused producer)
     c.stages.push_back([=](hipStream_t s) {
@@ -573,7 +602,7 @@ static int compile_mft(Chain& ch, Compile& c, const std::vector<Geom>& gout_per_
         CSX_TRY(convert_c128_to_c64(in, conv, nconv, s));
       } else {
         if (t) CSX_TRY(t->begin("split_c128_to_planes", 0.0, (16.0 + 4.0 * T) * nconv, s));
-        CSX_TRY(split_c128_to_planes(in, conv, npix_in, nf, T, s));
+        CSX_TRY(split_c128_to_planes(in, conv, npix_in, nf_in, T, s));
       }
       if (t) CSX_TRY(t->end(s));
       return CSX_OK;
@@ -833,7 +862,11 @@ struct DoubleFraunhoferElem : Element {
   }
   int compile(Chain& ch, Compile& c, int index) override {
     if (skipped(c)) return CSX_OK;
-    const int nl = ch.nlam(), nf = ch.nfield_for(c), ncand = ch.ncand_for(c);
+    // the element holding the candidate sags expands the shared fields to every candidate
+    const double* cs_dev = nullptr;
+    int cs_nhex = 0, cs_ncand = 0;
+    const bool expand = c.live > 0 && ch.candidate_sags(name, &cs_dev, &cs_nhex, &cs_ncand) == CSX_OK;
+    const int nl = ch.nlam(), nf = expand ? ch.nfield_all(c) : ch.nfield_for(c), ncand = ch.ncand_for(c);
     const Geom E = c.geom;
     std::vector<Geom> gf(nl);
     std::vector<double> lf(nl), lfo(nl);
@@ -919,7 +952,7 @@ struct DoubleFraunhoferElem : Element {
     }
     if (mask_stage) { c.stages.push_back(mask_stage); c.stage_names.push_back("hex masks " + name); }
     // pupil -> FPM (mask in the epilogue) -> pupil grid of the input
-    CSX_TRY(compile_mft(ch, c, gf, lf, type + " in " + name, &mk, /*out_conv=*/true));
+    CSX_TRY(compile_mft(ch, c, gf, lf, type + " in " + name, &mk, /*out_conv=*/true, nullptr, expand));
     // a mask element right after folds into the epilogue of the return MFT
     const double* lyot = nullptr;
     int claim = 0;
@@ -964,7 +997,7 @@ struct FpmPupToLyotElem : Element {
   int brows = 0, bcols = 0, crows = 0, ccols = 0;
   double referenceLambda = 0, innerRadiusFld = 0, outerRadiusFld = 0;
   // device buffers, [0] full run, [1] calibration run
-  DevBuf d_src[2], d_K[2], d_H[2], d_P[2], d_scale[2], d_lam;
+  DevBuf d_src[2], d_K[2], d_H[2], d_P[2], d_scale[2], d_lam, d_Pu;
 
   int set(Chain&, const std::string& k, const std::string& a) override {
     if (set_common(k, a) || k == "fpmPupToLyot") return CSX_OK;
@@ -1102,10 +1135,18 @@ struct FpmPupToLyotElem : Element {
     if (n2 > 4096 || (n2 & (n2 - 1))) return fail(CSX_ERR_UNSUPPORTED, "fpmPupToLyot: 2n must be a power of two <= 4096");
     CSX_TRY(flush_pointwise(ch, c));
     if (c.in_conv) return fail(CSX_ERR_STATE, "fpmPupToLyot: fields at working precision");
+    // holding the candidate sags: the shared fields (one candidate's) are transformed
+    // once and every candidate's convolution starts from that spectrum; otherwise a
+    // shared prefix ends here
+    const double* cs_dev = nullptr;
+    int cs_nhex = 0, cs_ncand = 0;
+    const bool expand = c.live > 0 && ch.candidate_sags(name, &cs_dev, &cs_nhex, &cs_ncand) == CSX_OK;
+    if (!expand) CSX_TRY(expand_candidates(ch, c));
+    const int nf_in = ch.nfield_for(c);
     Geom mg;
     std::vector<double> lf;
     CSX_TRY(mask_geom(ch, mg, lf));
-    const int nl = ch.nlam(), nf = ch.nfield_for(c), ncand = ch.ncand_for(c), nsrc = ch.nsrc();
+    const int nl = ch.nlam(), nf = ch.nfield_all(c), ncand = ch.ncand_for(c), nsrc = ch.nsrc();
     const int which = c.calibrating ? 1 : 0;
     const long S2 = (long)mg.nx * mg.ny, P2 = (long)n2 * n2;
     // source masks and the number of mask candidates
@@ -1210,13 +1251,30 @@ struct FpmPupToLyotElem : Element {
     const double2* dsc = d_scale[which].as<double2>();
     const int bab = !babinet ? big : (resp ? hex.raster.nhex * nsrc * nl : 0);
     const int per_cand = nsrc * nl, ncmc = ncm;
-    c.stages.push_back([=](hipStream_t s) {
-      CSX_TRY(embed_center(Ef, n, n, P, n2, n2, nf, 0, s));
-      CSX_TRY(fft2_batched(P, n2, nf, -1, s));
-      CSX_TRY(spectrum_mul(P, n2, nf, dH, per_cand, nl, ncmc, s));
-      CSX_TRY(fft2_batched(P, n2, nf, +1, s));
-      return crop_add(Ef, n, P, n2, nf, nl, dsc, bab, s);
-    });
+    if (expand) {
+      // spectra of the shared fields in their own buffer; every candidate's output goes
+      // to the other field buffer (its Babinet term reads the shared input)
+      CSX_TRY(d_Pu.reserve(sizeof(double2) * P2 * nf_in));
+      double2* Pu = d_Pu.as<double2>();
+      double2* Eo = ch.field_buf(1 - c.buf);
+      c.stages.push_back([=](hipStream_t s) {
+        CSX_TRY(embed_center(Ef, n, n, Pu, n2, n2, nf_in, 0, s));
+        CSX_TRY(fft2_batched(Pu, n2, nf_in, -1, s));
+        CSX_TRY(spectrum_mul(P, Pu, nf_in, n2, nf, dH, per_cand, nl, ncmc, s));
+        CSX_TRY(fft2_batched(P, n2, nf, +1, s));
+        return crop_add(Eo, Ef, nf_in, n, P, n2, nf, nl, dsc, bab, s);
+      });
+      c.buf = 1 - c.buf;
+      c.live = 0;
+    } else {
+      c.stages.push_back([=](hipStream_t s) {
+        CSX_TRY(embed_center(Ef, n, n, P, n2, n2, nf, 0, s));
+        CSX_TRY(fft2_batched(P, n2, nf, -1, s));
+        CSX_TRY(spectrum_mul(P, P, nf, n2, nf, dH, per_cand, nl, ncmc, s));
+        CSX_TRY(fft2_batched(P, n2, nf, +1, s));
+        return crop_add(Ef, Ef, nf, n, P, n2, nf, nl, dsc, bab, s);
+      });
+    }
     c.stage_names.push_back("fpmPupToLyot convolution " + name);
     return CSX_OK;
   }
@@ -1482,6 +1540,12 @@ int Chain::compile_one(bool calibrating, Compile& c) {
   c.mft_at.clear();
   c.geom = geom_symmetric(ef_.cols, ef_.rows, ef_.pixelScale, ef_.pixelScale);
   c.buf = 0;
+  // a candidate batch carries one copy of the fields up to the element holding the
+  // candidate sags (everything before it is the same for every candidate)
+  c.live = 0;
+  if (!calibrating && ncand() > 1)
+    for (auto& e : elems_)
+      if (e->name == cand_elem_ && !e->skipped(c)) c.live = 1;
   // initial fields (efield::init_point_sources / init / filename slices)
   const int nf = nfield_for(c), nl = nlam(), ns = nsrc();
   const long npix = c.geom.npix();
@@ -1526,6 +1590,7 @@ int Chain::compile_one(bool calibrating, Compile& c) {
   }
   CSX_TRY(flush_pointwise(*this, c));
   if (c.in_conv) return fail(CSX_ERR_STATE, "chain ends on converted fields");
+  CSX_TRY(expand_candidates(*this, c));
   return CSX_OK;
 }
 
@@ -1606,11 +1671,13 @@ int Chain::run_stages(Compile& c, hipStream_t s) {
   return CSX_OK;
 }
 
-int Chain::propagate(bool calibrating) {
+int Chain::propagate(bool calibrating, bool sync) {
   CSX_TRY(compile_all());
   CSX_HIP(hipSetDevice(ctx_->device));
   Compile& c = calibrating ? plan_cal_ : plan_full_;
   GraphState& gs = graph_[calibrating ? 1 : 0];
+  // the kernel timer's per-launch events need eager launches (event records
+  // captured into a graph do not keep their place between the kernels)
   const bool graphable = use_graph_ && !timer_.enabled();
   if (graphable && !gs.e && gs.ran_eager) {
     // capture after one eager run (lazily created tables exist by then) on a
@@ -1631,8 +1698,10 @@ int Chain::propagate(bool calibrating) {
     CSX_TRY(run_stages(c, stream()));
     gs.ran_eager = true;
   }
-  CSX_HIP(hipStreamSynchronize(stream()));
-  if (timer_.enabled()) CSX_TRY(timer_.collect());
+  if (sync) {
+    CSX_HIP(hipStreamSynchronize(stream()));
+    CSX_TRY(finish_timing());
+  }
   if (calibrating) { have_cal_ = true; have_full_ = false; } else { have_full_ = true; have_cal_ = false; }
   return CSX_OK;
 }
@@ -1757,15 +1826,55 @@ int Chain::contrast(const csx_region* r, double* out, csx_report* rep) {
   auto t0 = std::chrono::steady_clock::now();
   CSX_TRY(ensure_calibration());
   auto t1 = std::chrono::steady_clock::now();
-  CSX_TRY(propagate(false));
+  CSX_TRY(propagate(false, false));       // the reduction below is queued behind it
+  CSX_TRY(reduce_contrast(r, out));        // ends with a stream synchronisation
   auto t2 = std::chrono::steady_clock::now();
-  CSX_TRY(reduce_contrast(r, out));
+  CSX_TRY(finish_timing());
   if (rep) {
     rep->calib_peak = calib_peak_;
     rep->n_region_pixels = n_region_;
     rep->seconds_calibration = std::chrono::duration<double>(t1 - t0).count();
     rep->seconds_propagation = std::chrono::duration<double>(t2 - t1).count();
   }
+  return CSX_OK;
+}
+
+// contrastCurve::make_contrast_curve (src/tools/contrastCurve.cpp:68-158); its
+// pixelSampling is 1 (contrastCurve.hpp:15) and it reads the middle wavelength
+// lambdaData[(int) round(n_slices/2)] (integer division first).
+int Chain::contrast_curve(int cand, double* pos, double* avg, int capacity, int* rows) {
+  CSX_TRY(compile_all());
+  const Geom g = out_geom_;
+  const int pixel_sampling = 1;
+  const int ns = (g.nx - 2) / pixel_sampling;
+  const int nrow = std::max(ns - 3, 0);
+  *rows = nrow;
+  if (!pos) return CSX_OK;
+  if (nrow == 0) return fail(CSX_ERR_ARG, "contrast curve: output grid too small");
+  if (capacity < nrow) return fail(CSX_ERR_ARG, "contrast curve: capacity below the row count");
+  if (cand < 0 || cand >= ncand()) return fail(CSX_ERR_ARG, "contrast curve: candidate out of range");
+  CSX_TRY(ensure_calibration());
+  CSX_TRY(propagate(false, false));
+  const int per = nsrc() * nlam(), nc = ncand();
+  CSX_TRY(inten_.reserve(sizeof(double) * g.npix() * nc));
+  CSX_TRY(intensity_groups(field_buf(out_buf_), g.npix(), nc, per, inten_.as<double>(), stream()));
+  const double loD = tel_.loD(ef_.lambdas[nlam() / 2]);
+  // arma::linspace(0, max(pixelX / loD), ns)
+  const double pmax = std::max(g.x0, g.x0 + (g.nx - 1) * g.dx) / loD;
+  std::vector<double> smp(ns);
+  if (ns == 1) smp[0] = pmax;
+  for (int i = 0; ns > 1 && i < ns; i++) smp[i] = i == ns - 1 ? pmax : i * (pmax / (ns - 1));
+  DevBuf& ds = scratch("curve_samples");
+  DevBuf& dout = scratch("curve_out");
+  CSX_TRY(ds.reserve(sizeof(double) * ns));
+  CSX_TRY(dout.reserve(sizeof(double) * nrow));
+  CSX_HIP(hipMemcpyAsync(ds.p, smp.data(), sizeof(double) * ns, hipMemcpyHostToDevice, stream()));
+  CSX_TRY(annulus_means(inten_.as<double>() + (size_t)cand * g.npix(), g.nx, g.ny, g.x0, g.dx, g.y0, g.dy, loD,
+                        ds.as<double>() + 1, nrow, peak_.as<double>(), dout.as<double>(), stream()));
+  CSX_HIP(hipMemcpyAsync(avg, dout.p, sizeof(double) * nrow, hipMemcpyDeviceToHost, stream()));
+  CSX_HIP(hipStreamSynchronize(stream()));
+  CSX_TRY(finish_timing());
+  for (int k = 0; k < nrow; k++) pos[k] = smp[k + 2];
   return CSX_OK;
 }
 
@@ -1924,29 +2033,30 @@ int ChainGroup::build() {
   return CSX_OK;
 }
 
-int ChainGroup::propagate() {
+int ChainGroup::propagate(bool sync) {
   CSX_TRY(build());
   hipStream_t s = ch_[0]->stream();
   for (size_t i = 0; i < stages_.size(); i++) {
     int rc = stages_[i](s);
     if (rc != CSX_OK) return fail(rc, "group stage " + std::to_string(i) + ": " + csx_last_error());
   }
-  CSX_HIP(hipStreamSynchronize(s));
+  if (sync) CSX_HIP(hipStreamSynchronize(s));
   for (Chain* c : ch_) {
     c->mark_propagated(false);
-    if (c->timer()) CSX_TRY(c->timer()->collect());
+    if (sync) CSX_TRY(c->finish_timing());
   }
   return CSX_OK;
 }
 
 int ChainGroup::contrast(const csx_region* r, double* out) {
   for (Chain* c : ch_) CSX_TRY(c->ensure_calibration());   // each chain's own (cached) calibration
-  CSX_TRY(propagate());
+  CSX_TRY(propagate(false));               // the reductions are queued behind it on the same stream
   size_t off = 0;
   for (Chain* c : ch_) {
     CSX_TRY(c->reduce_contrast(r, out + off));
     off += c->ncand();
   }
+  for (Chain* c : ch_) CSX_TRY(c->finish_timing());
   return CSX_OK;
 }
 

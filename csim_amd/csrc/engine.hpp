@@ -77,6 +77,10 @@ struct Compile {
   std::vector<std::string> stage_names;
   std::vector<std::unique_ptr<MftPlan>> plans;
   std::map<size_t, std::vector<MftField>> mft_at;   // stage index -> fields of a plain MFT launch
+  // candidate copies the buffers hold (0: all).  Before the element that takes the
+  // candidate sags every candidate's fields are identical, so they are carried once
+  // (live = 1) and that element expands them to every candidate.
+  int live = 0;
 };
 
 // ---------------------------------------------------------------- elements --
@@ -117,7 +121,10 @@ class Chain {
   int get_opt(const std::string& elem, const std::string& name, std::vector<double>& v);
   int set_candidates(const std::string& elem, const double* sags, int nhex, int ncand);
   int set_precision(csx_precision p) { if (p != prec_) { prec_ = p; invalidate(); } return CSX_OK; }
-  int propagate(bool calibrating);
+  // sync = false leaves the work queued on the context's stream (the caller
+  // synchronises, then calls finish_timing)
+  int propagate(bool calibrating, bool sync = true);
+  int finish_timing() { return timer_.enabled() ? timer_.collect() : CSX_OK; }
   int get_field(int cand, int src, int lam, double* out);
   int get_intensity(int cand, double* out);
   int contrast(const csx_region* r, double* out, csx_report* rep);
@@ -136,6 +143,8 @@ class Chain {
   double calib_peak() const { return calib_peak_; }
   int n_region_pixels() const { return n_region_; }
   int get_normalized_image(int cand, double* out);
+  // pos/avg null: only *rows is set (no propagation)
+  int contrast_curve(int cand, double* pos, double* avg, int capacity, int* rows);
   int field_profiles(const int32_t* bins, int nbins, double* out);
   int info(csx_chain_info* out);
   int set_profiling(bool on) { timer_.enable(on); return CSX_OK; }
@@ -151,7 +160,11 @@ class Chain {
   csx_precision precision() const { return prec_; }
   int ncand() const { return std::max(1, ncand_); }
   int ncand_for(const Compile& c) const { return c.calibrating ? 1 : ncand(); }
-  int nfield_for(const Compile& c) const { return ncand_for(c) * ef_.nsources() * (int)ef_.lambdas.size(); }
+  // fields in the buffers at this point of the compile / with every candidate
+  int nfield_for(const Compile& c) const {
+    return (c.live > 0 ? c.live : ncand_for(c)) * ef_.nsources() * (int)ef_.lambdas.size();
+  }
+  int nfield_all(const Compile& c) const { return ncand_for(c) * ef_.nsources() * (int)ef_.lambdas.size(); }
   int nlam() const { return (int)ef_.lambdas.size(); }
   int nsrc() const { return ef_.nsources(); }
   double2* field_buf(int i) const { return bufs_[i].as<double2>(); }
@@ -215,7 +228,7 @@ class Chain {
 class ChainGroup {
  public:
   explicit ChainGroup(std::vector<Chain*> chains) : ch_(std::move(chains)) {}
-  int propagate();
+  int propagate(bool sync = true);
   int contrast(const csx_region* r, double* out);   // out: candidates of every chain, chain by chain
   int n_candidates() const;
   int merged_launches() const { return merged_; }

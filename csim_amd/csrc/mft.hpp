@@ -23,12 +23,11 @@ class KernelTimer {
   std::string json() const;
   struct Stat { long count = 0; double ms = 0, flops = 0, bytes = 0; };
   const std::map<std::string, Stat>& stats() const { return stats_; }
-
  private:
-  struct Rec { std::string name; double flops, bytes; int ev; };
+  struct Rec { std::string name; double flops, bytes; hipEvent_t b, e; };   // a launch between two events
+  int get_event(hipEvent_t* e);
   bool on_ = false;
-  std::vector<hipEvent_t> pool_;
-  int used_ = 0;
+  std::vector<hipEvent_t> free_, all_;
   std::vector<Rec> open_;
   std::map<std::string, Stat> stats_;
 };
@@ -72,7 +71,7 @@ class MftPlan {
  private:
   csx_precision prec_ = CSX_FP64;
   DevBuf dft_, tbuf_, probs1_, probs2_, specs_;
-  int nf_ = 0, tiles1_ = 0, tiles2_ = 0;
+  int nf_ = 0, nt_ = 0, tiles1_ = 0, tiles2_ = 0;   // fields, distinct first contractions
   int var1_ = 0, var2_ = 0;   // tile variant of each stage (index into the precision's list)
   double flops_ = 0, flops1_ = 0, flops2_ = 0, bytes1_ = 0, bytes2_ = 0;
 };

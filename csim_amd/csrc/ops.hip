@@ -150,10 +150,138 @@ __global__ void k_pointwise(double2* __restrict__ E, PwChain pc) {
     else store_planes(gout<unsigned short>(pc.out) + (long)f * 2 * pc.terms * npix, p, npix, pc.terms, er, ei);
   }
 }
+// Pixel-major generation pass: a thread owns 2 consecutive pixels of every field.
+// The factors that do not depend on the field are formed once (the (1+i) m masks),
+// the mirror phasors once per wavelength (the mirror phases summed: one sincos per
+// pixel and wavelength however many mirrors), the source phasor once per source;
+// each field value is then source x factor(lambda), written at the output precision.
+// Differs from applying the elements in sequence only by FP64 rounding.
+template <int NL>
+__global__ __launch_bounds__(256) void k_pointwise_gen(double2* __restrict__ E, PwChain pc, int nouter) {
+#pragma clang fp contract(off)
+  constexpr int nl = NL;
+  const int npix = pc.rows * pc.cols;
+  const int p0 = 2 * (blockIdx.x * blockDim.x + threadIdx.x);
+  if (p0 >= npix) return;
+  double cr[2] = {1.0, 1.0}, ci[2] = {0.0, 0.0};
+  for (int k = 1; k < pc.nops; k++) {
+    if (pc.op[k].kind != PW_MASK_1PI) continue;
+    const f64x2 w = *reinterpret_cast<const GLOBAL f64x2*>(gin<double>(pc.op[k].p) + p0);
+#pragma unroll
+    for (int x = 0; x < 2; x++) {
+      const double tr = cr[x] * w[x] - ci[x] * w[x];
+      ci[x] = cr[x] * w[x] + ci[x] * w[x];
+      cr[x] = tr;
+    }
+  }
+  double Fr[NL][2], Fi[NL][2];
+#pragma unroll
+  for (int li = 0; li < NL; li++) {
+    double ph[2] = {0.0, 0.0};
+    bool any = false;
+    for (int k = 1; k < pc.nops; k++) {
+      if (pc.op[k].kind != PW_MIRROR) continue;
+      // mirror.cpp:66-72: exp((-mirrorSign*2*2*M_PI/lambda) * i * h)
+      const double a = -pc.op[k].sign * 2 * 2 * M_PI / pc.lambdas[li];
+      const f64x2 h = *reinterpret_cast<const GLOBAL f64x2*>(gin<double>(pc.op[k].p) + p0);
+      ph[0] += a * h[0];
+      ph[1] += a * h[1];
+      any = true;
+    }
+#pragma unroll
+    for (int x = 0; x < 2; x++) {
+      if (any) {
+        double sn, cs;
+        sincos(ph[x], &sn, &cs);
+        Fr[li][x] = cr[x] * cs - ci[x] * sn;
+        Fi[li][x] = cr[x] * sn + ci[x] * cs;
+      } else {
+        Fr[li][x] = cr[x];
+        Fi[li][x] = ci[x];
+      }
+    }
+  }
+  const double ps = pc.pixel_scale, Lx = ps * pc.cols, Ly = ps * pc.rows;
+  double sr[2] = {0, 0}, si[2] = {0, 0};
+  double lflux = 0, ltx = 0, lty = 0;
+  for (int o = 0; o < nouter; o++) {
+    const double* prm = pc.src_params + 3 * o * nl;
+    const double flux = prm[0], tx = prm[1], ty = prm[2];
+    if (o == 0 || flux != lflux || tx != ltx || ty != lty) {
+      // efield::init_point_sources (src/data/efield.cpp:179-206)
+#pragma unroll
+      for (int x = 0; x < 2; x++) {
+        const int p = p0 + x, c = p / pc.rows, r = p - c * pc.rows;
+        const double xx = -(Lx - ps) / 2.0 + c * ps, yy = -(Ly - ps) / 2.0 + r * ps;
+        const double v = 2 * xx * tx / Lx + 2 * yy * ty / Ly;
+        double sn, cs;
+        sincos(2.0 * M_PI * v, &sn, &cs);
+        sr[x] = flux * cs;
+        si[x] = flux * sn;
+      }
+      lflux = flux; ltx = tx; lty = ty;
+    }
+#pragma unroll
+    for (int li = 0; li < NL; li++) {
+      const long f = (long)o * nl + li;
+      double er[2], ei[2];
+#pragma unroll
+      for (int x = 0; x < 2; x++) {
+        er[x] = sr[x] * Fr[li][x] - si[x] * Fi[li][x];
+        ei[x] = sr[x] * Fi[li][x] + si[x] * Fr[li][x];
+      }
+      if (pc.out_kind == PW_OUT_C128) {
+        GLOBAL f64x2* d = gout<f64x2>(E) + f * npix + p0;
+        d[0] = f64x2{er[0], ei[0]};
+        d[1] = f64x2{er[1], ei[1]};
+      } else if (pc.out_kind == PW_OUT_C64) {
+        typedef float f32x4 __attribute__((ext_vector_type(4)));
+        *reinterpret_cast<GLOBAL f32x4*>(gout<float>(pc.out) + 2 * (f * npix + p0)) =
+            f32x4{(float)er[0], (float)ei[0], (float)er[1], (float)ei[1]};
+      } else {
+        // BF16 planes of field f: plane q at (2 terms f + q) npix; two pixels = one 32-bit store
+        GLOBAL unsigned* d = reinterpret_cast<GLOBAL unsigned*>(gout<unsigned short>(pc.out) + f * 2 * pc.terms * npix + p0);
+        const int stride = npix / 2;
+        float xr[2] = {(float)er[0], (float)er[1]}, xi[2] = {(float)ei[0], (float)ei[1]};
+        for (int t = 0; t < pc.terms; t++) {
+          unsigned short hr[2], hi[2];
+#pragma unroll
+          for (int x = 0; x < 2; x++) {
+            hr[x] = bf16_rne(xr[x]);
+            hi[x] = bf16_rne(xi[x]);
+            xr[x] -= bf16_val(hr[x]);
+            xi[x] -= bf16_val(hi[x]);
+          }
+          d[(2 * t) * stride] = (unsigned)hr[0] | ((unsigned)hr[1] << 16);
+          d[(2 * t + 1) * stride] = (unsigned)hi[0] | ((unsigned)hi[1] << 16);
+        }
+      }
+    }
+  }
+}
+
 int pointwise_chain(double2* E, int nfield, const PwChain& pc, hipStream_t s) {
   if (pc.nops <= 0 || nfield <= 0) return CSX_OK;
   if (pc.out_kind != PW_OUT_C128 && !pc.out) return fail(CSX_ERR_ARG, "pointwise: no output buffer");
   long npix = (long)pc.rows * pc.cols;
+  if (pc.nl > 0) {
+    if (pc.op[0].kind != PW_SOURCE || pc.nl > kMaxPwLam || npix % 2 || nfield % pc.nl)
+      return fail(CSX_ERR_ARG, "pointwise: pixel-major pass preconditions");
+    for (int k = 1; k < pc.nops; k++)
+      if (pc.op[k].kind != PW_MASK_1PI && pc.op[k].kind != PW_MIRROR)
+        return fail(CSX_ERR_ARG, "pointwise: pixel-major pass takes masks and mirrors only");
+    typedef void (*GenKernel)(double2*, PwChain, int);
+    static const GenKernel kGen[kMaxPwLam] = {
+        k_pointwise_gen<1>, k_pointwise_gen<2>, k_pointwise_gen<3>, k_pointwise_gen<4>,
+        k_pointwise_gen<5>, k_pointwise_gen<6>, k_pointwise_gen<7>, k_pointwise_gen<8>,
+        k_pointwise_gen<9>, k_pointwise_gen<10>, k_pointwise_gen<11>, k_pointwise_gen<12>,
+        k_pointwise_gen<13>, k_pointwise_gen<14>, k_pointwise_gen<15>, k_pointwise_gen<16>};
+    static_assert(kMaxPwLam == 16, "one instance per wavelength count");
+    const int nb = (int)((npix / 2 + 255) / 256);
+    hipLaunchKernelGGL(kGen[pc.nl - 1], dim3(nb), dim3(256), 0, s, E, pc, nfield / pc.nl);
+    CSX_HIP(hipGetLastError());
+    return CSX_OK;
+  }
   int bx = (int)std::min<long>(std::max<long>(1, 4096 / nfield), (npix + 255) / 256);
   hipLaunchKernelGGL(k_pointwise, dim3(bx, nfield), dim3(256), 0, s, E, pc);
   CSX_HIP(hipGetLastError());
@@ -528,6 +656,34 @@ __global__ void k_region_mean(const double* __restrict__ I, long npix, const int
     out[blockIdx.x] = (v / nidx) / pk;
   }
 }
+// Contrast curve (contrastCurve::make_contrast_curve, src/tools/contrastCurve.cpp:124-133):
+// bin b is the mean of I over the pixels whose radius r = sqrt(x^2 + y^2) / loD lies
+// strictly between s[b] and s[b + 2], divided by the peak; NaN for an empty annulus.
+__global__ void k_annulus_means(const double* __restrict__ I, int nx, int ny, double x0, double dx, double y0,
+                                double dy, double loD, const double* __restrict__ smp,
+                                const double* __restrict__ peak_dev, double* __restrict__ out) {
+  const double lo = smp[blockIdx.x], hi = smp[blockIdx.x + 2];
+  double v = 0, n = 0;
+  for (int p = threadIdx.x; p < nx * ny; p += blockDim.x) {
+    const int c = p / ny, r = p - c * ny;
+    const double XX = x0 + c * dx, YY = y0 + r * dy;
+    const double rr = sqrt(XX * XX + YY * YY) / loD;
+    if (rr > lo && rr < hi) { v += I[p]; n += 1; }
+  }
+  v = block_reduce(v, 0);
+  __syncthreads();
+  n = block_reduce(n, 0);
+  if (threadIdx.x == 0) out[blockIdx.x] = n > 0 ? (v / n) / peak_dev[0] : NAN;
+}
+int annulus_means(const double* I, int nx, int ny, double x0, double dx, double y0, double dy, double loD,
+                  const double* samples_dev, int nbins, const double* peak_dev, double* out, hipStream_t s) {
+  if (nbins <= 0) return CSX_OK;
+  hipLaunchKernelGGL(k_annulus_means, dim3(nbins), dim3(256), 0, s, I, nx, ny, x0, dx, y0, dy, loD, samples_dev,
+                     peak_dev, out);
+  CSX_HIP(hipGetLastError());
+  return CSX_OK;
+}
+
 int region_mean_groups(const double* I, long npix, int ngroups, const int32_t* idx, int nidx,
                        const double* peak_dev, double peak, double* out, hipStream_t s) {
   if (nidx <= 0) return fail(CSX_ERR_ARG, "region_mean: empty region");
@@ -626,34 +782,41 @@ int fpm_prep(const double2* src, long src_stride, int rows, int cols, int n, int
 
 // P[f](r, c) *= H[h(f)](r, c) (-1)^(r + c): the transform of fft_shift(maskHat)
 // for an even size is (-1)^(r+c) times the transform of maskHat.
-__global__ void k_spectrum_mul(double2* __restrict__ P, int n2, const double2* __restrict__ H, int per_cand, int nl,
-                               int nmask_cand) {
+// P[f] = H[cand(f), f % nl] * Pin[f % nin] * (-1)^(r+c) (fft_shift of the kernel folded
+// in); in place when Pin == P and nin == the field count.
+__global__ void k_spectrum_mul(double2* __restrict__ P, const double2* __restrict__ Pin, int nin, int n2,
+                               const double2* __restrict__ H, int per_cand, int nl, int nmask_cand) {
   const int f = blockIdx.y;
   int cand = f / per_cand, l = f % nl;
   if (cand >= nmask_cand) cand = nmask_cand - 1;
   const long np = (long)n2 * n2;
   double2* p = P + (long)f * np;
+  const double2* q = Pin + (long)(f % nin) * np;
   const double2* h = H + ((long)cand * nl + l) * np;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < np; i += (long)gridDim.x * blockDim.x) {
     int c = (int)(i / n2), r = (int)(i - (long)c * n2);
-    double2 a = h[i], b = p[i];
+    double2 a = h[i], b = q[i];
     double2 v = make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
     if ((r + c) & 1) { v.x = -v.x; v.y = -v.y; }
     p[i] = v;
   }
 }
-int spectrum_mul(double2* P, int n2, int nfield, const double2* H, int per_cand, int nl, int nmask_cand,
-                 hipStream_t s) {
+int spectrum_mul(double2* P, const double2* Pin, int nin, int n2, int nfield, const double2* H, int per_cand, int nl,
+                 int nmask_cand, hipStream_t s) {
   if (nfield <= 0) return CSX_OK;
+  if (nin <= 0 || (Pin == P && nin != nfield)) return fail(CSX_ERR_ARG, "spectrum_mul: in-place needs every field");
   long np = (long)n2 * n2;
   int bx = (int)std::min<long>(2048, (np + 255) / 256);
-  hipLaunchKernelGGL(k_spectrum_mul, dim3(bx, nfield), dim3(256), 0, s, P, n2, H, per_cand, nl, nmask_cand);
+  hipLaunchKernelGGL(k_spectrum_mul, dim3(bx, nfield), dim3(256), 0, s, P, Pin, nin, n2, H, per_cand, nl,
+                     nmask_cand);
   CSX_HIP(hipGetLastError());
   return CSX_OK;
 }
 
-// E[f] = scale[l] * P[f](centre) + (babinet ? E[f] : 0), centre rows/cols n/2 .. 3n/2 - 1.
-__global__ void k_crop_add(double2* __restrict__ E, int n, const double2* __restrict__ P, int n2, int nfield, int nl,
+// E[f] = scale[l] * P[f](centre) + (babinet ? Ein[f % nin] : 0), centre rows/cols n/2 .. 3n/2 - 1;
+// in place when Ein == E and nin == the field count.
+__global__ void k_crop_add(double2* __restrict__ E, const double2* __restrict__ Ein, int nin, int n,
+                           const double2* __restrict__ P, int n2, int nfield, int nl,
                            const double2* __restrict__ scale, int babinet_from) {
   long np = (long)n * n, total = np * nfield;
   int o = n / 2;
@@ -663,15 +826,16 @@ __global__ void k_crop_add(double2* __restrict__ E, int n, const double2* __rest
     double2 a = P[f * (long)n2 * n2 + (long)(c + o) * n2 + (r + o)];
     double2 sc = scale[f % nl];
     double2 v = make_double2(a.x * sc.x - a.y * sc.y, a.x * sc.y + a.y * sc.x);
-    if (f >= babinet_from) { double2 e = E[i]; v.x += e.x; v.y += e.y; }
+    if (f >= babinet_from) { double2 e = Ein[(f % nin) * np + p]; v.x += e.x; v.y += e.y; }
     E[i] = v;
   }
 }
-int crop_add(double2* E, int n, const double2* P, int n2, int nfield, int nl, const double2* scale_dev,
-             int babinet_from, hipStream_t s) {
+int crop_add(double2* E, const double2* Ein, int nin, int n, const double2* P, int n2, int nfield, int nl,
+             const double2* scale_dev, int babinet_from, hipStream_t s) {
   long total = (long)n * n * nfield;
   if (total <= 0) return CSX_OK;
-  hipLaunchKernelGGL(k_crop_add, dim3(grid_for(total)), dim3(256), 0, s, E, n, P, n2, nfield, nl, scale_dev,
+  if (nin <= 0 || (Ein == E && nin != nfield)) return fail(CSX_ERR_ARG, "crop_add: in-place needs every field");
+  hipLaunchKernelGGL(k_crop_add, dim3(grid_for(total)), dim3(256), 0, s, E, Ein, nin, n, P, n2, nfield, nl, scale_dev,
                      babinet_from);
   CSX_HIP(hipGetLastError());
   return CSX_OK;

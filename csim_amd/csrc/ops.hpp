@@ -31,7 +31,12 @@ struct PwChain {
   int out_kind;                     // PwOut: in place, or to `out` at a working precision
   void* out;                        // complex64 fields or BF16 planes (field f at f * 2 terms npix)
   int terms;
+  // > 0: pixel-major generation pass.  Requires op[0] = PW_SOURCE, no PW_CFACTOR, an
+  // even pixel count, nl <= kMaxPwLam, lambdas[f] = lambdas[f % nl] and source
+  // parameters that depend on f / nl only (the chain's field order).
+  int nl;
 };
+constexpr int kMaxPwLam = 16;
 int pointwise_chain(double2* E, int nfield, const PwChain& pc, hipStream_t s);
 
 // 2-D FFT, csim semantics: dir -1 forward unnormalised, +1 inverse / n^2.
@@ -61,6 +66,11 @@ int max_groups(const double* I, long npix, int ngroups, double* out, hipStream_t
 int region_mean_groups(const double* I, long npix, int ngroups, const int32_t* idx, int nidx,
                        const double* peak_dev, double peak, double* out, hipStream_t s);
 
+// out[b] = mean of I over pixels (column-major ny x nx grid) with s[b] < r / loD < s[b + 2],
+// divided by peak_dev[0]; b < nbins (samples_dev holds nbins + 2 values)
+int annulus_means(const double* I, int nx, int ny, double x0, double dx, double y0, double dy, double loD,
+                  const double* samples_dev, int nbins, const double* peak_dev, double* out, hipStream_t s);
+
 // out[f][b] = sum_{p : bin[p] == b} |E_f[p]|^2 over nfield fields (bins < 0 skipped)
 int bin_sums(const double2* E, long npix, int nfield, const int32_t* bin, int nbins, double* out, hipStream_t s);
 
@@ -73,12 +83,14 @@ int embed_center(const double2* src, int srows, int scols, double2* dst, int dro
 // out[k] = factor (src[k] - (k >= minus1_from)) over n masks of rows x cols (columns mirrored if fliplr)
 int fpm_prep(const double2* src, long src_stride, int rows, int cols, int n, int fliplr, int minus1_from,
              double factor, double2* out, hipStream_t s);
-// P[f] *= H[cand(f) * nl + l(f)] (-1)^(r+c), cand = f / per_cand (clamped), l = f % nl
-int spectrum_mul(double2* P, int n2, int nfield, const double2* H, int per_cand, int nl, int nmask_cand,
-                 hipStream_t s);
-// E[f] = scale[f % nl] * centre(P[f]) + (f >= babinet_from ? E[f] : 0)
-int crop_add(double2* E, int n, const double2* P, int n2, int nfield, int nl, const double2* scale_dev,
-             int babinet_from, hipStream_t s);
+// P[f] = H[cand(f) * nl + l(f)] Pin[f % nin] (-1)^(r+c), cand = f / per_cand (clamped),
+// l = f % nl; Pin == P (with nin == nfield) multiplies in place
+int spectrum_mul(double2* P, const double2* Pin, int nin, int n2, int nfield, const double2* H, int per_cand, int nl,
+                 int nmask_cand, hipStream_t s);
+// E[f] = scale[f % nl] * centre(P[f]) + (f >= babinet_from ? Ein[f % nin] : 0); Ein == E
+// (with nin == nfield) adds in place
+int crop_add(double2* E, const double2* Ein, int nin, int n, const double2* P, int n2, int nfield, int nl,
+             const double2* scale_dev, int babinet_from, hipStream_t s);
 // M[s * nl + l][c][j] = E[(c * nslice + s * nl + l)][idx[j]], nslice = nsrc * nl
 int gather_response(const double2* E, long npix_field, const int32_t* idx, int nidx, int ncol, int nslice,
                     double2* M, hipStream_t s);

@@ -124,7 +124,7 @@ typedef struct {
   double calib_peak;          /* max of the calibration intensity */
   int n_region_pixels;
   double seconds_calibration; /* wall time of the calibration propagation (0 if cached) */
-  double seconds_propagation; /* wall time of the full propagation */
+  double seconds_propagation; /* wall time of the full propagation and the region reduction */
 } csx_report;
 /* regionContrast::compute_contrast (src/tools/regionContrast.cpp:145-205): runs
  * the calibration chain (cached until the chain changes), the full chain, and
@@ -140,6 +140,17 @@ int csx_chain_contrast(csx_chain* chain, const csx_region* region, double* contr
                        csx_report* rep);
 /* Normalised image I / max(I_cal) of one candidate (after csx_chain_contrast). */
 int csx_chain_get_normalized_image(csx_chain* chain, int cand, double* out);
+
+/* Contrast curve of one candidate (contrastCurve::make_contrast_curve,
+ * src/tools/contrastCurve.cpp:68-158): calibration and full propagation, the
+ * source/wavelength intensity sum over the calibration peak, averaged over
+ * overlapping annuli.  With n = out_cols - 2 sample radii s_i = linspace(0,
+ * max x / loD, n) (loD at the middle wavelength, lambdas[n_lambdas / 2]), row k
+ * (k = 0 .. n - 4) is pos[k] = s[k + 2] and avg[k] = mean of I / peak over the
+ * pixels with s[k + 1] < r / loD < s[k + 3] (NaN if none): the rows csim writes
+ * to its curve file.  pos = avg = NULL returns the row count without running;
+ * otherwise returns the row count, or CSX_ERR_ARG when capacity is smaller. */
+int csx_chain_contrast_curve(csx_chain* chain, int cand, double* pos, double* avg, int capacity);
 
 /* Per-field binned intensity of the last propagation: out[f][b] = sum of |E_f|^2
  * over the output pixels p with bin_of_pixel[p] == b (column-major pixel order,

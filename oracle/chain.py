@@ -18,6 +18,7 @@ product code:
   fpmPupToLyot    src/elements/fpmPupToLyot.cpp:24-167, 240-496 (three mask kinds)
   fraunhoferFocal src/elements/fraunhoferFocal.cpp:23-72
   region contrast src/tools/regionContrast.cpp:89-205
+  contrast curve  src/tools/contrastCurve.cpp:68-158
 The additive keys (fpmSamplesPerflD, fpmFOVflD, occulterRadiusFld, hexLayout)
 place the FPM on fraunhoferFocal's grid convention: symmetric grid of pixel
 loD(referenceLambda)/samplesPerflD, lambda*f = lambda * 2 R * F#.
@@ -30,6 +31,24 @@ import numpy as np
 
 from . import (center_mask_multiply, downsample, fpm_pup_to_lyot, fresnel_as, grid_symmetric, hex_centers,
                hex_design_array, hex_interp_data, hex_mask, interp2, mask_apply, mirror_apply, point_source, zoomfft)
+
+
+# ------------------------------------------------------- contrast curve ---
+def annulus_curve(I, x, loD, pixel_sampling=1):
+    """The averaging of contrastCurve::make_contrast_curve
+    (src/tools/contrastCurve.cpp:124-139) on a square image I over grid x (both
+    axes): n = (len(x) - 2) / pixelSampling sample radii s = linspace(0, max(x)/loD, n);
+    bin i (1 <= i <= n - 2) at s[i] is the mean of I over s[i-1] < r/loD < s[i+1];
+    the file gets bins i >= 2."""
+    RR = np.sqrt(x[None, :] ** 2 + x[:, None] ** 2) / loD
+    n = int((len(x) - 2) / pixel_sampling)
+    smp = np.linspace(0.0, (x / loD).max(), n)
+    pos, avg = [], []
+    for i in range(1, n - 1):
+        sel = (RR > smp[i - 1]) & (RR < smp[i + 1])
+        avg.append(I[sel].mean() if sel.any() else np.nan)
+        pos.append(smp[i])
+    return np.array(pos[1:]), np.array(avg[1:])
 
 
 # ------------------------------------------------------------- parsing ---
@@ -330,6 +349,21 @@ class OracleChain:
         sel = ((XX / loD >= r["x1"]) & (XX / loD <= r["x2"]) & (RR / loD >= r["r1"]) & (RR / loD <= r["r2"])
                & (TT > r["a1"] * np.pi / 180) & (TT < r["a2"] * np.pi / 180))
         return sel
+
+    def contrast_curve(self, cand_sags=None, pixel_sampling=1):
+        """contrastCurve::make_contrast_curve (src/tools/contrastCurve.cpp:68-158): the
+        source/wavelength intensity sum over the calibration peak, averaged over the
+        annuli sample[i-1] < r/loD < sample[i+1] (strict) with
+        sample = linspace(0, max(pixelX)/loD, (nX - 2)/pixelSampling) and loD at
+        lambdaData[(int) round(n_slices/2)]; returns the rows the reference writes
+        to its file (i >= 2)."""
+        m = self.m
+        loD = m.loD(m.lambdas[len(m.lambdas) // 2])
+        peak = self.intensity(True).max()
+        I = self.intensity(False, cand_sags) / peak
+        g = self.out_grid
+        x = g[1] + np.arange(g[0]) * g[2]
+        return annulus_curve(I, x, loD, pixel_sampling)
 
     def contrast(self, cand_sags=None):
         Ic = self.intensity(True)

@@ -61,6 +61,22 @@ def test_workload2_reduced_fp64(ctx):
     assert abs(c[0] - c_ref) <= 1e-12 * c_ref
 
 
+def test_contrast_curve_vs_oracle(ctx):
+    """contrastCurve::make_contrast_curve on a 3-wavelength, 2-source chain (the
+    middle wavelength sets lambda/D): sample radii equal, annulus means to 1e-12."""
+    w = W.workload2(n_lambda=3, offsets=(0.0, 2.0))
+    ch = build(ctx, w)
+    pos, avg = ch.contrast_curve()
+    oc = OracleChain(w.script, w.arrays, sags=w.meta["sags"])
+    pos_ref, avg_ref = oc.contrast_curve()
+    assert pos.shape == pos_ref.shape and np.allclose(pos, pos_ref, rtol=1e-15, atol=0)
+    assert np.array_equal(np.isnan(avg), np.isnan(avg_ref))
+    ok = ~np.isnan(avg_ref)
+    err = float(np.max(np.abs(avg[ok] - avg_ref[ok]) / np.abs(avg_ref[ok])))
+    print(f"contrast curve: {pos.size} rows, {ok.sum()} non-empty, max rel err {err:.3e}")
+    assert err < 1e-12
+
+
 def _dark_hole_errors(ctx, w, precision):
     ref = build(ctx, w, "fp64")
     c64, _ = ref.contrast()
@@ -95,8 +111,11 @@ def test_workload2_mixed_relative_error(ctx, precision):
     assert dmean / c < 1e-6
 
 
-def test_candidate_batch_matches_single(ctx):
-    w = W.workload2(n_lambda=2, offsets=(0.0,))
+@pytest.mark.parametrize("kind", ["doubleFraunhoferInterpFPM", "fpmPupToLyot"])
+def test_candidate_batch_matches_single(ctx, kind):
+    """A candidate batch (the pupil-plane fields carried once and expanded by the FPM
+    element) gives every candidate's contrast as a chain run with that candidate alone."""
+    w = W.workload2(n_lambda=2, offsets=(0.0,)) if kind != "fpmPupToLyot" else W.workload_p2l("hex", offsets=(0.0, 2.0))
     ch = build(ctx, w)
     n_hex = w.meta["n_hex"]
     t = W.hex_thickness(w.meta["seed"], 7, n_hex, n_cand=3)

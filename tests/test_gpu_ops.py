@@ -84,13 +84,16 @@ def test_mft_mixed_error_level(ctx, prec, tol):
 
 
 VARIANTS = {"bf16x3": 11, "bf16x6": 3}
+MFMA32 = {"bf16x3": {7, 8, 9, 10}, "bf16x6": set()}   # variants on the 32x32x16 MFMA (gemm.hip)
 
 
 @pytest.mark.parametrize("prec,tol", [("bf16x3", 5e-5), ("bf16x6", 5e-6)])
 @pytest.mark.parametrize("nin,nout,batch", [(1024, 256, 2), (200, 72, 3), (96, 136, 2)])
 def test_mft_planes_tile_variants(ctx, monkeypatch, prec, tol, nin, nout, batch):
-    """Every tile variant of the split-BF16 kernels takes the same products in the same
-    order per K slab; all stay at the split-BF16 error level against FP64 on ragged sizes."""
+    """Every tile variant of the split-BF16 kernels stays at the split-BF16 error level
+    against FP64 on ragged sizes.  Variants on the same MFMA shape as variant 0 take the
+    same products in the same order per K slab (equal to 1e-6); the 32x32x16 ones sum
+    the k values of a slab in another order inside the MFMA (equal to tol / 10)."""
     rng = np.random.default_rng(nin + 7 * nout)
     E = np.stack([rand_field(rng, nin, nin) for _ in range(batch)])
     dx, lf = 1e-5, 650e-9
@@ -107,7 +110,7 @@ def test_mft_planes_tile_variants(ctx, monkeypatch, prec, tol, nin, nout, batch)
             e, d = rel_rms(o[b], ref[b]), rel_rms(o[b], outs[0][b])
             print(f"{prec} {nin}->{nout} variant {v} field {b}: vs fp64 {e:.3e}, vs variant 0 {d:.3e}")
             assert e < tol
-            assert d < 1e-6
+            assert d < (tol / 10 if v in MFMA32[prec] else 1e-6)
 
 
 @pytest.mark.parametrize("n", [8, 64, 256, 2048])

@@ -326,6 +326,36 @@ def test_region_pixels_annulus():
     assert 0 < half.size < idx.size
 
 
+def test_annulus_curve_pins():
+    """csim's contrast-curve averaging (strict annuli of two sample spacings around each
+    sample radius): a constant image gives the constant; a one-pixel image is seen only
+    by the rows whose centre lies within one spacing of its radius; a disk image gives
+    exactly 1 inside and 0 outside; rows start at the third sample."""
+    from oracle.chain import annulus_curve
+    n, d, loD = 128, 0.37, 1.3
+    x = sym(n, d) + np.arange(n) * d
+    m = n - 2
+    smp = np.linspace(0.0, x.max() / loD, m)
+    step = smp[1]
+    pos, avg = annulus_curve(np.full((n, n), 2.5), x, loD)
+    assert pos.size == m - 3 and pos[0] == smp[2] and pos[-1] == smp[m - 2]
+    assert np.all(np.abs(avg - 2.5) <= 1e-15 * 2.5)
+    I = np.zeros((n, n))
+    I[90, 101] = 1.0                                   # row y index 90, column x index 101
+    r = np.hypot(x[101], x[90]) / loD
+    pos, avg = annulus_curve(I, x, loD)
+    hit = avg > 0
+    assert np.array_equal(hit, np.abs(pos - r) < step)
+    assert 1 <= hit.sum() <= 2
+    R0 = 12.0
+    RR = np.hypot(x[None, :], x[:, None]) / loD
+    pos, avg = annulus_curve((RR < R0).astype(float), x, loD)
+    inside = pos + step <= R0
+    outside = pos - step >= R0
+    assert inside.sum() > 5 and outside.sum() > 5
+    assert np.all(avg[inside] == 1.0) and np.all(avg[outside] == 0.0)
+
+
 # ------------------------------------------------------------------ MLP ---
 def test_mlp_gradients_vs_finite_differences():
     rng = np.random.default_rng(9)
