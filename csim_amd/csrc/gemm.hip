@@ -192,9 +192,144 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const GemmProb* __restrict__ pr
       }
 }
 
-int gemm_f64(const GemmProb* probs_dev, int n, int total_tiles, hipStream_t s) {
+// FP64 tile variants: block BM x BN, WM x WN waves of IM x JN 16x16 sub-tiles, K
+// slabs of 16 complex values double-buffered in LDS (one barrier per slab, the next
+// slab's global loads in flight during the current slab's MFMAs).  Within a 4-deep k
+// step the products are issued in two sweeps (real parts of A, then imaginary
+// parts), so two updates of one accumulator are 2 IM JN MFMAs apart.
+template <int BM, int BN, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) void k_gemm_f64db(const GemmProb* __restrict__ probs, int nprob) {
+  typedef Mma<double>::v4 V4;
+  constexpr int NT = 64 * WM * WN, BK = 16, LDK = BK + 1;
+  constexpr int IM = BM / WM / 16, JN = BN / WN / 16;
+  constexpr int CA = BM * BK / 4 / NT, CB = BN * BK / 4 / NT;   // 4-value chunks per thread
+  static_assert(IM >= 1 && JN >= 1 && CA >= 1 && CB >= 1 && (BM * BK / 4) % NT == 0 && (BN * BK / 4) % NT == 0,
+                "tile shape");
+  __shared__ f64x2 As[2][BM][LDK];
+  __shared__ f64x2 Bs[2][BN][LDK];
+
+  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  const GemmProb& P = probs[find_prob(probs, nprob, tile)];
+  const int local = tile - P.tile_start;
+  const int tm = local % P.tiles_m, tn = local / P.tiles_m;
+  const int m0 = tm * BM, n0 = tn * BN;
+  const GLOBAL f64x2* A = gin<f64x2>(P.A);
+  const GLOBAL f64x2* B = gin<f64x2>(P.B);
+  const int M = P.M, N = P.N, K = P.K;
+  const long lda = P.lda, ldb = P.ldb;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm = wave / WN, wn = wave % WN;
+
+  V4 accR[IM][JN], accI[IM][JN];
+#pragma unroll
+  for (int i = 0; i < IM; i++)
+#pragma unroll
+    for (int j = 0; j < JN; j++) { accR[i][j] = V4{0, 0, 0, 0}; accI[i][j] = V4{0, 0, 0, 0}; }
+
+  // chunk e = c * NT + t: row e / 4, k offset (e % 4) * 4 (4 threads cover a 16-deep row)
+  f64x2 ra[CA][4], rb[CB][4];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int c = 0; c < CA; c++) {
+      const int e = c * NT + t, r = e >> 2, kq = (e & 3) * 4;
+      const long row = (long)min(m0 + r, M - 1) * lda;
+#pragma unroll
+      for (int q = 0; q < 4; q++) ra[c][q] = A[row + min(k0 + kq + q, K - 1)];
+    }
+#pragma unroll
+    for (int c = 0; c < CB; c++) {
+      const int e = c * NT + t, r = e >> 2, kq = (e & 3) * 4;
+      const long row = (long)min(n0 + r, N - 1) * ldb;
+#pragma unroll
+      for (int q = 0; q < 4; q++) rb[c][q] = B[row + min(k0 + kq + q, K - 1)];
+    }
+  };
+  // the zeroing of rows / k beyond the matrix is applied when the slab is stored
+  auto sstore = [&](int buf, int k0) {
+    const f64x2 z = f64x2{0.0, 0.0};
+#pragma unroll
+    for (int c = 0; c < CA; c++) {
+      const int e = c * NT + t, r = e >> 2, kq = (e & 3) * 4;
+      const bool rin = m0 + r < M;
+#pragma unroll
+      for (int q = 0; q < 4; q++) As[buf][r][kq + q] = (rin && k0 + kq + q < K) ? ra[c][q] : z;
+    }
+#pragma unroll
+    for (int c = 0; c < CB; c++) {
+      const int e = c * NT + t, r = e >> 2, kq = (e & 3) * 4;
+      const bool rin = n0 + r < N;
+#pragma unroll
+      for (int q = 0; q < 4; q++) Bs[buf][r][kq + q] = (rin && k0 + kq + q < K) ? rb[c][q] : z;
+    }
+  };
+  const int nslab = (K + BK - 1) / BK;
+  gload(0);
+  sstore(0, 0);
+  __syncthreads();
+  for (int sl = 0; sl < nslab; sl++) {
+    const int cur = sl & 1;
+    if (sl + 1 < nslab) gload((sl + 1) * BK);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      f64x2 a[IM], b[JN];
+#pragma unroll
+      for (int i = 0; i < IM; i++) a[i] = As[cur][wm * (BM / WM) + i * 16 + (lane & 15)][kk + (lane >> 4)];
+#pragma unroll
+      for (int j = 0; j < JN; j++) b[j] = Bs[cur][wn * (BN / WN) + j * 16 + (lane & 15)][kk + (lane >> 4)];
+#pragma unroll
+      for (int i = 0; i < IM; i++)
+#pragma unroll
+        for (int j = 0; j < JN; j++) {
+          accR[i][j] = Mma<double>::mma(a[i].x, b[j].x, accR[i][j]);
+          accI[i][j] = Mma<double>::mma(a[i].x, b[j].y, accI[i][j]);
+        }
+#pragma unroll
+      for (int i = 0; i < IM; i++)
+#pragma unroll
+        for (int j = 0; j < JN; j++) {
+          accR[i][j] = Mma<double>::mma(-a[i].y, b[j].y, accR[i][j]);
+          accI[i][j] = Mma<double>::mma(a[i].y, b[j].x, accI[i][j]);
+        }
+    }
+    if (sl + 1 < nslab) sstore(cur ^ 1, (sl + 1) * BK);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < IM; i++)
+#pragma unroll
+    for (int j = 0; j < JN; j++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int m = m0 + wm * (BM / WM) + i * 16 + Mma<double>::row(lane, v);
+        const int n = n0 + wn * (BN / WN) + j * 16 + (lane & 15);
+        epilogue_store(P, m, n, accR[i][j][v], accI[i][j][v]);
+      }
+}
+
+struct F64Variant {
+  int bm, bn, threads;
+  void (*kernel)(const GemmProb*, int);
+};
+static const F64Variant kVariantsF64[] = {
+    {64, 64, 256, k_gemm_nt<double>},
+    {64, 64, 256, k_gemm_f64db<64, 64, 2, 2>},
+    {64, 128, 256, k_gemm_f64db<64, 128, 2, 2>},
+    {128, 64, 256, k_gemm_f64db<128, 64, 2, 2>},
+    {128, 128, 512, k_gemm_f64db<128, 128, 2, 4>},
+};
+int f64_variant_count() { return (int)(sizeof(kVariantsF64) / sizeof(kVariantsF64[0])); }
+int f64_variant_tile(int v, int* bm, int* bn) {
+  if (v < 0 || v >= f64_variant_count()) return fail(CSX_ERR_ARG, "fp64 variant out of range");
+  *bm = kVariantsF64[v].bm;
+  *bn = kVariantsF64[v].bn;
+  return CSX_OK;
+}
+
+int gemm_f64(const GemmProb* probs_dev, int n, int total_tiles, hipStream_t s, int variant) {
   if (total_tiles <= 0) return CSX_OK;
-  hipLaunchKernelGGL(k_gemm_nt<double>, dim3(total_tiles), dim3(256), 0, s, probs_dev, n);
+  if (variant < 0 || variant >= f64_variant_count()) return fail(CSX_ERR_ARG, "gemm_f64: bad variant");
+  const F64Variant& v = kVariantsF64[variant];
+  hipLaunchKernelGGL(v.kernel, dim3(total_tiles), dim3(v.threads), 0, s, probs_dev, n);
   CSX_HIP(hipGetLastError());
   return CSX_OK;
 }

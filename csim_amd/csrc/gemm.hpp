@@ -40,7 +40,10 @@ constexpr int kTileM = 64, kTileN = 64;
 int plan_tiles(GemmProb* probs, int n, int tile_m, int tile_n);
 
 // probs_dev: device copy of the problem array (n entries), total_tiles from plan_tiles.
-int gemm_f64(const GemmProb* probs_dev, int n, int total_tiles, hipStream_t s);
+int gemm_f64(const GemmProb* probs_dev, int n, int total_tiles, hipStream_t s, int variant = 0);
+// FP64 tile variants (variant 0: the 64 x 64 single-buffered kernel)
+int f64_variant_count();
+int f64_variant_tile(int variant, int* bm, int* bn);
 int gemm_f32(const GemmProb* probs_dev, int n, int total_tiles, hipStream_t s);
 // Operands already split into BF16 planes (A, B K-contiguous per plane, K, lda/ldb
 // and plane strides multiples of 8); output complex128/complex64 or planes

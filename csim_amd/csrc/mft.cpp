@@ -15,7 +15,7 @@ int terms_of(csx_precision p) { return p == CSX_BF16X3 ? 2 : (p == CSX_BF16X6 ? 
 
 int launch_gemm(const GemmProb* probs_dev, int n, int tiles, csx_precision p, hipStream_t s, int variant) {
   switch (p) {
-    case CSX_FP64: return gemm_f64(probs_dev, n, tiles, s);
+    case CSX_FP64: return gemm_f64(probs_dev, n, tiles, s, variant);
     case CSX_FP32: return gemm_f32(probs_dev, n, tiles, s);
     case CSX_BF16X3: return gemm_bf16planes(probs_dev, n, tiles, 2, variant, s);
     case CSX_BF16X6: return gemm_bf16planes(probs_dev, n, tiles, 3, variant, s);
@@ -25,6 +25,7 @@ int launch_gemm(const GemmProb* probs_dev, int n, int tiles, csx_precision p, hi
 
 // Tile variants a precision can launch (variant 0: 64 x 64).
 static int n_variants(csx_precision p) {
+  if (p == CSX_FP64) return f64_variant_count();
   return p == CSX_BF16X3 || p == CSX_BF16X6 ? planes_variant_count(terms_of(p)) : 1;
 }
 
@@ -55,6 +56,7 @@ static std::string signature(const std::vector<GemmProb>& v, csx_precision p) {
 static int upload(std::vector<GemmProb>& v, csx_precision p, int variant, DevBuf& dev, hipStream_t s, int* tiles) {
   int tm = kTileM, tn = kTileN;
   if (p == CSX_BF16X3 || p == CSX_BF16X6) CSX_TRY(planes_variant_tile(terms_of(p), variant, &tm, &tn));
+  if (p == CSX_FP64) CSX_TRY(f64_variant_tile(variant, &tm, &tn));
   *tiles = plan_tiles(v.data(), (int)v.size(), tm, tn);
   CSX_TRY(dev.reserve(v.size() * sizeof(GemmProb)));
   CSX_HIP(hipMemcpyAsync(dev.p, v.data(), v.size() * sizeof(GemmProb), hipMemcpyHostToDevice, s));
@@ -62,10 +64,12 @@ static int upload(std::vector<GemmProb>& v, csx_precision p, int variant, DevBuf
   return CSX_OK;
 }
 
-// Picks the fastest tile variant of one grouped launch by timing each (median of
-// 3 launches after one warm-up, HIP events on the launch stream; a variant must
-// beat the best earlier one by 2 %).  The launches
-// write the stage's own output buffers only, which the real run overwrites.
+// Picks the fastest tile variant of one grouped launch by timing: two passes over
+// the variants, each timing 3 back-to-back launches between two HIP events on the
+// launch stream after one warm-up launch; a variant's time is its better pass, and
+// a later variant must beat the best earlier one by 2 % (choices within timing noise
+// stay put).  The launches write the stage's own output buffers only, which the
+// real run overwrites.
 static int choose_variant(std::vector<GemmProb>& v, csx_precision p, DevBuf& dev, hipStream_t s, int* variant,
                           int* tiles) {
   const int nv = n_variants(p);
@@ -83,35 +87,54 @@ static int choose_variant(std::vector<GemmProb>& v, csx_precision p, DevBuf& dev
   hipEvent_t e0, e1;
   CSX_HIP(hipEventCreate(&e0));
   CSX_HIP(hipEventCreate(&e1));
-  float best = 1e30f;
-  int best_v = 0;
+  std::vector<float> tbest(nv, 1e30f);
   int rc = CSX_OK;
-  for (int var = 0; var < nv && rc == CSX_OK; var++) {
-    int nt = 0;
-    if ((rc = upload(v, p, var, dev, s, &nt)) != CSX_OK) break;
-    std::vector<float> ms;
-    for (int r = 0; r < 4 && rc == CSX_OK; r++) {
-      if (hipEventRecord(e0, s) != hipSuccess) { rc = fail(CSX_ERR_HIP, "autotune: event record"); break; }
+  for (int pass = 0; pass < 2 && rc == CSX_OK; pass++) {
+    for (int var = 0; var < nv && rc == CSX_OK; var++) {
+      int nt = 0;
+      if ((rc = upload(v, p, var, dev, s, &nt)) != CSX_OK) break;
       if ((rc = launch_gemm(dev.as<GemmProb>(), (int)v.size(), nt, p, s, var)) != CSX_OK) break;
+      if (hipEventRecord(e0, s) != hipSuccess) { rc = fail(CSX_ERR_HIP, "autotune: event record"); break; }
+      for (int r = 0; r < 3 && rc == CSX_OK; r++) rc = launch_gemm(dev.as<GemmProb>(), (int)v.size(), nt, p, s, var);
+      if (rc != CSX_OK) break;
       if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
         rc = fail(CSX_ERR_HIP, "autotune: event wait");
         break;
       }
       float t = 0;
       (void)hipEventElapsedTime(&t, e0, e1);
-      if (r > 0) ms.push_back(t);
+      tbest[var] = std::min(tbest[var], t / 3);
     }
-    if (rc != CSX_OK) break;
-    std::sort(ms.begin(), ms.end());
-    // a later variant must win by 2 %: choices within timing noise stay put
-    if (ms[1] < best * 0.98f) { best = ms[1]; best_v = var; }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc != CSX_OK) return rc;
+  int best_v = 0;
+  for (int var = 1; var < nv; var++)
+    if (tbest[var] < tbest[best_v] * 0.98f) best_v = var;
   tune_cache()[key] = best_v;
   *variant = best_v;
   return upload(v, p, best_v, dev, s, tiles);
+}
+
+// Problems of different shapes (a chain group: 512^2 and 1024^2 pupils) have tiles of
+// different cost.  Each XCD works a contiguous range of tiles, so the shape classes
+// are spread evenly over the list (each keeps its order): every range gets the same
+// mix of work instead of one XCD taking the small problems and finishing early.
+static void spread_shapes(std::vector<GemmProb>& v) {
+  std::map<std::tuple<int, int, int>, std::vector<int>> cls;
+  for (int i = 0; i < (int)v.size(); i++) cls[std::make_tuple(v[i].M, v[i].N, v[i].K)].push_back(i);
+  if (cls.size() < 2) return;
+  std::vector<std::pair<double, int>> key;
+  for (auto& kv : cls) {
+    const int n = (int)kv.second.size();
+    for (int j = 0; j < n; j++) key.push_back({(j + 0.5) / n, kv.second[j]});
+  }
+  std::stable_sort(key.begin(), key.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+  std::vector<GemmProb> out;
+  out.reserve(v.size());
+  for (const auto& k : key) out.push_back(v[k.second]);
+  v.swap(out);
 }
 
 namespace {
@@ -265,6 +288,8 @@ int MftPlan::build(const std::vector<MftField>& fields, csx_precision p, hipStre
   };
   by_kernel(p1);
   by_kernel(p2);
+  spread_shapes(p1);
+  spread_shapes(p2);
   // stage 2 reads stage 1's output, so stage 1 is tuned (and left written) first
   CSX_TRY(choose_variant(p1, p, probs1_, s, &var1_, &tiles1_));
   CSX_TRY(choose_variant(p2, p, probs2_, s, &var2_, &tiles2_));

@@ -342,6 +342,11 @@ int csx_mlp_set_params(csx_mlp* m, const double* flat) {
   std::vector<float> h(m->nparam);
   for (long i = 0; i < m->nparam; i++) h[i] = (float)flat[i];
   CSX_HIP(hipMemcpy(m->P.p, h.data(), sizeof(float) * m->nparam, hipMemcpyHostToDevice));
+  // new parameters start a new optimisation: Adam moments and step count restart
+  std::fill(h.begin(), h.end(), 0.0f);
+  CSX_HIP(hipMemcpy(m->Mom.p, h.data(), sizeof(float) * m->nparam, hipMemcpyHostToDevice));
+  CSX_HIP(hipMemcpy(m->Vel.p, h.data(), sizeof(float) * m->nparam, hipMemcpyHostToDevice));
+  m->step = 0;
   return CSX_OK;
 }
 

@@ -18,7 +18,7 @@ The surrogate is trained first on propagations of random thickness vectors
     python -m csim_amd.study [--n 512] [--train 90112] [--epochs 10] [--seconds 20]
 
 prints one JSON object (data-generation and training times, held-out rank
-correlation, and both DE traces with the time each reached the plain run's
+correlation, and the DE traces with the time each reached the plain run's
 final contrast).
 """
 from __future__ import annotations
@@ -122,6 +122,14 @@ class ScreenedSurrogate:
         self.pos = (self.pos + n) % self.cap
         self.filled = min(self.cap, self.filled + n)
 
+    def tune(self):
+        """A few Adam steps on random batches of the ring buffer."""
+        import torch
+        nb = min(1024, self.filled)
+        for _ in range(self.tune_steps):
+            sel = torch.randint(0, self.filled, (nb,), device="cuda", generator=self.gen)
+            self.mlp.train_step(self.Xd.index_select(0, sel), self.Yd.index_select(0, sel), self.lr, want_loss=False)
+
     def screen_and_eval(self, pop):
         """DE hook: pop (D, P) -> (indices evaluated, their exact values)."""
         import torch
@@ -135,10 +143,7 @@ class ScreenedSurrogate:
         if idx.size > 2:
             self.errors.append(spearman(pred[idx], vals))
         self._push(features_device(self.mlp.ctx, t[idx], self.lambdas), log_labels(prof).reshape(-1, N_BINS))
-        nb = min(1024, self.filled)
-        for _ in range(self.tune_steps):
-            sel = torch.randint(0, self.filled, (nb,), device="cuda", generator=self.gen)
-            self.mlp.train_step(self.Xd.index_select(0, sel), self.Yd.index_select(0, sel), self.lr, want_loss=False)
+        self.tune()
         t3 = time.perf_counter()
         self.seconds["predict"] += t1 - t0
         self.seconds["exact"] += t2 - t1
@@ -194,9 +199,11 @@ def train_surrogate(ctx, w, n_train=22528, epochs=20, precision="bf16x3", seed=2
 def compare_de(ctx, w, sur, seconds=None, eval_budget=None, plain_pop=64, screen_pop=512, precision="bf16x3",
                seed=20261207, log=print):
     """Plain DE, then screened DE (stopping at the plain run's best value when a time
-    budget is given).  Budgets: wall seconds or exact candidate evaluations."""
+    budget is given) starting from the trained model, which is restored afterwards.
+    Budgets: wall seconds or exact candidate evaluations."""
     import torch
     nhex = w.meta["n_hex"]
+    theta = sur.mlp.get_params()
     out = {}
     cfg = DEConfig(population_size=plain_pop, global_bounds=(0.0, 1e-6), init_bounds=(0.0, 1e-6),
                    max_iterations=10 ** 9, seed=seed + 1)
@@ -208,24 +215,26 @@ def compare_de(ctx, w, sur, seconds=None, eval_budget=None, plain_pop=64, screen
                     "evaluations": rp.evaluations, "generations": len(rp.history) - 1}
     log(f"plain DE: best {rp.best_value:.4e} after {rp.seconds:.1f} s, {rp.evaluations} candidates")
     exact_p.ch.close()
+    target = rp.best_value if seconds is not None else None
+    sur.mlp.set_params(theta)
     exact_s = ExactObjective(ctx, w, sur.keep, precision)
-    sur.exact = exact_s
+    s1 = ScreenedSurrogate(sur.mlp, sur.lambdas, sur.counts, exact_s, keep=sur.keep)
     cfg_s = DEConfig(population_size=screen_pop, global_bounds=(0.0, 1e-6), init_bounds=(0.0, 1e-6),
                      max_iterations=10 ** 9, seed=seed + 2)
-    de_s = DifferentialEvolution(exact_s, nhex, cfg_s, screen=sur.screen_and_eval)
-    rs = de_s.run(time_budget=seconds, target=rp.best_value if seconds is not None else None,
-                  eval_budget=eval_budget)
+    de_s = DifferentialEvolution(exact_s, nhex, cfg_s, screen=s1.screen_and_eval)
+    rs = de_s.run(time_budget=seconds, target=target, eval_budget=eval_budget)
     reach = next((h for h in rs.history if h[2] <= rp.best_value), None)
-    out["screened"] = {"population": screen_pop, "keep": sur.keep, "best": rs.best_value, "seconds": rs.seconds,
+    out["screened"] = {"population": screen_pop, "keep": s1.keep, "best": rs.best_value, "seconds": rs.seconds,
                        "evaluations": rs.evaluations, "generations": len(rs.history) - 1,
                        "seconds_to_plain_best": reach[1] if reach else None,
                        "evaluations_to_plain_best": reach[3] if reach else None,
-                       "survivor_spearman_median": float(np.median(sur.errors)) if sur.errors else None,
-                       "seconds_by_phase": {k: round(v, 3) for k, v in sur.seconds.items()}}
+                       "survivor_spearman_median": float(np.median(s1.errors)) if s1.errors else None,
+                       "seconds_by_phase": {k: round(v, 3) for k, v in s1.seconds.items()}}
     out["wall_ratio_vs_plain"] = (reach[1] / rp.seconds) if reach else None
     log(f"screened DE: best {rs.best_value:.4e} after {rs.seconds:.1f} s; reached plain best: "
         f"{(reach[1], reach[3]) if reach else None}")
     exact_s.ch.close()
+    sur.mlp.set_params(theta)
     return out
 
 
@@ -239,7 +248,7 @@ def run(n=512, n_train=90112, seconds=20.0, n_lambda=9, plain_pop=64, screen_pop
     out.update(tr)
     out.update(compare_de(ctx, w, sur, seconds=seconds, plain_pop=plain_pop, screen_pop=screen_pop,
                           precision=precision, seed=seed, log=log))
-    if out["wall_ratio_vs_plain"] is not None:
+    if out.get("wall_ratio_vs_plain") is not None:
         out["wall_ratio_incl_training"] = (out["screened"]["seconds_to_plain_best"] + out["data_seconds"] +
                                            out["train_seconds"]) / out["plain"]["seconds"]
     return out

@@ -68,7 +68,10 @@ def features(thickness, lambdas):
 def features_device(ctx, thickness, lambdas):
     """`features` computed on the GPU (csx_hex_phasor_features) -> float32 device tensor."""
     import torch
-    t = torch.as_tensor(np.ascontiguousarray(thickness, dtype=np.float64)).cuda()
+    if torch.is_tensor(thickness):
+        t = thickness.to(device="cuda", dtype=torch.float64).contiguous()
+    else:
+        t = torch.as_tensor(np.ascontiguousarray(thickness, dtype=np.float64)).cuda()
     lam = np.ascontiguousarray(lambdas, dtype=np.float64)
     nc, nh = t.shape
     out = torch.empty((nc * lam.size, 2 * nh + 1), dtype=torch.float32, device=t.device)
@@ -105,6 +108,17 @@ class MLP:
             self.close()
         except Exception:
             pass
+
+    def get_params(self):
+        n = lib().csx_mlp_param_count(self._p)
+        flat = np.zeros(n)
+        check(lib().csx_mlp_get_params(self._p, flat.ctypes.data_as(C.c_void_p)))
+        return flat
+
+    def set_params(self, flat):
+        """Replaces the parameters (the Adam state restarts)."""
+        flat = np.ascontiguousarray(flat, dtype=np.float64)
+        check(lib().csx_mlp_set_params(self._p, flat.ctypes.data_as(C.c_void_p)))
 
     def train_step(self, x_dev, t_dev, lr, want_loss=True):
         loss = C.c_double()

@@ -296,7 +296,8 @@ typedef struct csx_mlp csx_mlp;
 int csx_mlp_create(csx_ctx* ctx, int n_in, int n_h1, int n_h2, int n_out, uint64_t seed,
                    csx_mlp** out);
 void csx_mlp_destroy(csx_mlp* mlp);
-/* Host <-> device parameter copies, order W1 (n_in x n_h1, row-major), b1, W2, b2, W3, b3. */
+/* Host <-> device parameter copies, order W1 (n_in x n_h1, row-major), b1, W2, b2, W3, b3.
+ * Setting parameters also restarts the Adam state (moments and step count). */
 int csx_mlp_get_params(csx_mlp* mlp, double* flat);
 int csx_mlp_set_params(csx_mlp* mlp, const double* flat);
 size_t csx_mlp_param_count(csx_mlp* mlp);
