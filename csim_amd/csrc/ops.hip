@@ -156,7 +156,24 @@ __global__ void k_pointwise(double2* __restrict__ E, PwChain pc) {
 // pixel and wavelength however many mirrors), the source phasor once per source;
 // each field value is then source x factor(lambda), written at the output precision.
 // Differs from applying the elements in sequence only by FP64 rounding.
-template <int NL>
+// Phasor of an FP64 phase.  SPLIT (output in two BF16 planes, 16 significant bits): the
+// phase is reduced to [-pi, pi] in FP64 and the sine and cosine taken in FP32, whose
+// error (~1e-7) is far below the planes' rounding (2^-17); otherwise FP64 sincos.
+template <bool SPLIT>
+__device__ __forceinline__ void phasor(double ph, double* sn, double* cs) {
+  if (SPLIT) {
+    const double k = rint(ph * (0.5 / M_PI));
+    const float r = (float)(ph - k * (2.0 * M_PI));
+    float s, c;
+    sincosf(r, &s, &c);
+    *sn = s;
+    *cs = c;
+  } else {
+    sincos(ph, sn, cs);
+  }
+}
+
+template <int NL, bool SPLIT>
 __global__ __launch_bounds__(256) void k_pointwise_gen(double2* __restrict__ E, PwChain pc, int nouter) {
 #pragma clang fp contract(off)
   constexpr int nl = NL;
@@ -192,7 +209,7 @@ __global__ __launch_bounds__(256) void k_pointwise_gen(double2* __restrict__ E, 
     for (int x = 0; x < 2; x++) {
       if (any) {
         double sn, cs;
-        sincos(ph[x], &sn, &cs);
+        phasor<SPLIT>(ph[x], &sn, &cs);
         Fr[li][x] = cr[x] * cs - ci[x] * sn;
         Fi[li][x] = cr[x] * sn + ci[x] * cs;
       } else {
@@ -215,7 +232,7 @@ __global__ __launch_bounds__(256) void k_pointwise_gen(double2* __restrict__ E, 
         const double xx = -(Lx - ps) / 2.0 + c * ps, yy = -(Ly - ps) / 2.0 + r * ps;
         const double v = 2 * xx * tx / Lx + 2 * yy * ty / Ly;
         double sn, cs;
-        sincos(2.0 * M_PI * v, &sn, &cs);
+        phasor<SPLIT>(2.0 * M_PI * v, &sn, &cs);
         sr[x] = flux * cs;
         si[x] = flux * sn;
       }
@@ -271,14 +288,17 @@ int pointwise_chain(double2* E, int nfield, const PwChain& pc, hipStream_t s) {
       if (pc.op[k].kind != PW_MASK_1PI && pc.op[k].kind != PW_MIRROR)
         return fail(CSX_ERR_ARG, "pointwise: pixel-major pass takes masks and mirrors only");
     typedef void (*GenKernel)(double2*, PwChain, int);
-    static const GenKernel kGen[kMaxPwLam] = {
-        k_pointwise_gen<1>, k_pointwise_gen<2>, k_pointwise_gen<3>, k_pointwise_gen<4>,
-        k_pointwise_gen<5>, k_pointwise_gen<6>, k_pointwise_gen<7>, k_pointwise_gen<8>,
-        k_pointwise_gen<9>, k_pointwise_gen<10>, k_pointwise_gen<11>, k_pointwise_gen<12>,
-        k_pointwise_gen<13>, k_pointwise_gen<14>, k_pointwise_gen<15>, k_pointwise_gen<16>};
+#define CSX_GEN_ROW(S)                                                                                    \
+  {k_pointwise_gen<1, S>, k_pointwise_gen<2, S>, k_pointwise_gen<3, S>, k_pointwise_gen<4, S>,            \
+   k_pointwise_gen<5, S>, k_pointwise_gen<6, S>, k_pointwise_gen<7, S>, k_pointwise_gen<8, S>,            \
+   k_pointwise_gen<9, S>, k_pointwise_gen<10, S>, k_pointwise_gen<11, S>, k_pointwise_gen<12, S>,         \
+   k_pointwise_gen<13, S>, k_pointwise_gen<14, S>, k_pointwise_gen<15, S>, k_pointwise_gen<16, S>}
+    static const GenKernel kGen[2][kMaxPwLam] = {CSX_GEN_ROW(false), CSX_GEN_ROW(true)};
+#undef CSX_GEN_ROW
     static_assert(kMaxPwLam == 16, "one instance per wavelength count");
     const int nb = (int)((npix / 2 + 255) / 256);
-    hipLaunchKernelGGL(kGen[pc.nl - 1], dim3(nb), dim3(256), 0, s, E, pc, nfield / pc.nl);
+    const int split = pc.out_kind == PW_OUT_PLANES && pc.terms == 2 ? 1 : 0;   // 16-bit planes only
+    hipLaunchKernelGGL(kGen[split][pc.nl - 1], dim3(nb), dim3(256), 0, s, E, pc, nfield / pc.nl);
     CSX_HIP(hipGetLastError());
     return CSX_OK;
   }
