@@ -392,7 +392,10 @@ template <> struct MfmaBf16<32> {
   static __device__ __forceinline__ int dcol(int lane) { return lane & 31; }
 };
 
-template <int TERMS, int BM, int BN, int WM, int WN, bool DB, int WPE, int BK, int MF>
+// Zero chunk that direct-to-LDS loads of k beyond K read instead of the matrix.
+static __device__ __align__(16) unsigned int kZeroChunk[4];
+
+template <int TERMS, int BM, int BN, int WM, int WN, bool DB, int WPE, int BK, int MF, bool DMA = false>
 __global__ __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(WPE)))
 void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
   typedef MfmaBf16<MF> Mf;
@@ -493,6 +496,39 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
     }
   };
 
+  // Direct-to-LDS loads (global_load_lds_dwordx4): instruction i of wave w fills the
+  // 64 consecutive 16-byte LDS chunks (i NW + w) * 64 + lane of the stage, so each lane
+  // fetches the global chunk that belongs in its slot (the swizzle inverted: slot
+  // (row r, position p) holds chunk p ^ ((r >> 1) & 3)).  Rows beyond the matrix
+  // read a clamped row (those outputs are never stored); k beyond K reads zeros.
+  constexpr int NW = WM * WN, CA = NP * BM * CH, CB = NP * BN * CH, NI = DMA ? (CA + CB) / NT : 1;
+  static_assert(!DMA || (DB && BK == 32 && CA % (NW * 64) == 0 && CB % (NW * 64) == 0), "direct-load shape");
+  unsigned doff[NI];
+  int dk[NI];
+  if (DMA) {
+#pragma unroll
+    for (int i = 0; i < NI; i++) {
+      const int sl = (i * NW + wave) * 64 + lane;
+      const bool isA = sl < CA;
+      const int e = isA ? sl : sl - CA, rows = isA ? BM : BN;
+      const int q = e / (rows * CH), rem = e % (rows * CH), r = rem / CH, c = (rem % CH) ^ ((r >> 1) & 3);
+      dk[i] = c * 8;
+      doff[i] = isA ? (unsigned)(q * P.sA + (long)min(m0 + r, M - 1) * P.lda + c * 8)
+                    : (unsigned)(q * P.sB + (long)min(n0 + r, N - 1) * P.ldb + c * 8);
+    }
+  }
+  auto dload = [&](int k0, int buf) {
+#pragma unroll
+    for (int i = 0; i < NI; i++) {
+      const bool isA = (i * NW) * 64 < CA;              // uniform: whole instructions per operand
+      const GLOBAL unsigned short* src = (isA ? A : B) + doff[i] + k0;
+      const GLOBAL void* g = k0 + dk[i] < K ? (const GLOBAL void*)src : (const GLOBAL void*)kZeroChunk;
+      __attribute__((address_space(3))) void* l =
+          (__attribute__((address_space(3))) void*)(lds + buf * STAGE + (i * NW + wave) * 64 * 8);
+      __builtin_amdgcn_global_load_lds(g, l, 16, 0, 0);
+    }
+  };
+
   Acc accR[IM][JN], accI[IM][JN];
   double dR[TERMS == 3 ? IM : 1][TERMS == 3 ? JN : 1][TERMS == 3 ? NV : 1];
   double dI[TERMS == 3 ? IM : 1][TERMS == 3 ? JN : 1][TERMS == 3 ? NV : 1];
@@ -569,7 +605,18 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
   };
 
   const int nslab = (K + BK - 1) / BK;
-  if (DB) {
+  if (DMA) {
+    dload(0, 0);
+    __builtin_amdgcn_s_waitcnt(0x0F70);            // vmcnt(0): this wave's loads have landed
+    __syncthreads();
+    for (int s = 0; s < nslab; s++) {
+      const int cur = s & 1;
+      if (s + 1 < nslab) dload((s + 1) * BK, cur ^ 1);
+      compute(lds + cur * STAGE);
+      __builtin_amdgcn_s_waitcnt(0x0F70);
+      __syncthreads();
+    }
+  } else if (DB) {
     gload(0);
     sstore(0);
     __syncthreads();
@@ -664,6 +711,9 @@ static const PlanesVariant kVariants2[] = {
     {128, 128, 512, k_gemm_planes<2, 128, 128, 2, 4, true, 2, 32, 32>},
     {128, 64, 256, k_gemm_planes<2, 128, 64, 2, 2, false, 2, 32, 32>},
     {64, 128, 256, k_gemm_planes<2, 64, 128, 2, 2, false, 2, 32, 32>},
+    {64, 64, 256, k_gemm_planes<2, 64, 64, 2, 2, true, 2, 32, 16, true>},
+    {128, 128, 512, k_gemm_planes<2, 128, 128, 2, 4, true, 2, 32, 16, true>},
+    {128, 64, 256, k_gemm_planes<2, 128, 64, 2, 2, true, 2, 32, 16, true>},
 };
 static const PlanesVariant kVariants3[] = {
     {64, 64, 256, k_gemm_planes<3, 64, 64, 2, 2, false, 1, 32, 16>},
