@@ -154,6 +154,21 @@ class Chain:
         self._info = None
         return M, calib.value
 
+    def region_fields(self, radii=None, angles=(-180.0, 180.0), xlim=(-1e16, 1e16), reference_lambda=-1.0):
+        """Fields of every candidate at the region pixels after the last full
+        propagation: device complex128 tensor (n_slices, n_candidates, n_pix)."""
+        import torch
+        r = None
+        if radii is not None:
+            r = C.byref(Region(float(radii[0]), float(radii[1]), float(angles[0]), float(angles[1]),
+                               float(xlim[0]), float(xlim[1]), float(reference_lambda)))
+        npix, nc, nsl = C.c_int(), C.c_int(), C.c_int()
+        check(lib().csx_chain_region_fields(self._p, r, None, C.byref(npix), C.byref(nc), C.byref(nsl)))
+        F = torch.empty((nsl.value, nc.value, npix.value), dtype=torch.complex128, device="cuda")
+        check(lib().csx_chain_region_fields(self._p, r, C.c_void_p(F.data_ptr()), C.byref(npix), C.byref(nc),
+                                            C.byref(nsl)))
+        return F
+
     def set_graph(self, on: bool):
         """Replay every propagation after the first as one HIP graph."""
         check(lib().csx_chain_set_graph(self._p, int(bool(on))))

@@ -196,6 +196,14 @@ int csx_chain_hex_response(csx_chain* c, const char* element_name, const csx_reg
   CSX_GUARD_END
 }
 
+int csx_chain_region_fields(csx_chain* c, const csx_region* region, void* d_out, int* npix, int* ncand,
+                            int* nslices) {
+  CSX_GUARD_BEGIN
+  CSX_TRY(chk(c));
+  return c->impl.region_fields(region, static_cast<double2*>(d_out), npix, ncand, nslices);
+  CSX_GUARD_END
+}
+
 int csx_group_create(csx_chain* const* chains, int n, csx_group** out) {
   CSX_GUARD_BEGIN
   if (!chains || n <= 0 || !out) return fail(CSX_ERR_ARG, "csx_group_create: bad arguments");

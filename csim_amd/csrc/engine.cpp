@@ -1706,6 +1706,27 @@ int Chain::propagate(bool calibrating, bool sync) {
   return CSX_OK;
 }
 
+// The fields linearOptimizer::compute_jacobian / compute_objective_vector read
+// (src/optimization/linearOptimizer.cpp:420-505): every candidate's field at the
+// region pixels, slice-major like the response matrix.
+int Chain::region_fields(const csx_region* r, double2* d_out, int* npix, int* ncand_out, int* nslice) {
+  CSX_TRY(compile_all());
+  std::vector<int32_t> idx;
+  CSX_TRY(region_pixels(r, idx));
+  if (npix) *npix = (int)idx.size();
+  if (ncand_out) *ncand_out = ncand();
+  if (nslice) *nslice = nsrc() * nlam();
+  if (!d_out) return CSX_OK;
+  if (!have_full_) return fail(CSX_ERR_STATE, "region fields: no full propagation has run");
+  DevBuf& di = scratch("region_field_idx");
+  CSX_TRY(di.reserve(sizeof(int32_t) * idx.size()));
+  CSX_HIP(hipMemcpyAsync(di.p, idx.data(), sizeof(int32_t) * idx.size(), hipMemcpyHostToDevice, stream()));
+  CSX_TRY(gather_response(field_buf(out_buf_), out_geom_.npix(), di.as<int32_t>(), (int)idx.size(), ncand(),
+                          nsrc() * nlam(), d_out, stream()));
+  CSX_HIP(hipStreamSynchronize(stream()));
+  return CSX_OK;
+}
+
 int Chain::get_field(int cand, int src, int lam, double* out) {
   bool cal = have_cal_;
   if (!have_full_ && !have_cal_) return fail(CSX_ERR_STATE, "no propagation has run");

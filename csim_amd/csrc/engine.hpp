@@ -131,6 +131,8 @@ class Chain {
   int ensure_calibration();
   int reduce_contrast(const csx_region* r, double* out);
   int region_pixels(const csx_region* r, std::vector<int32_t>& idx);
+  // region-pixel fields of every candidate after the last full propagation
+  int region_fields(const csx_region* r, double2* d_out, int* npix, int* ncand_out, int* nslice);
   int hex_response(const std::string& elem, const csx_region* r, double2* d_M, int* npix, int* ncol, int* nslice,
                    double* calib);
   // ---- chain groups (one launch per contraction over several chains)

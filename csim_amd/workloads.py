@@ -195,6 +195,30 @@ def workload0(n=1024, r_ap=500, r_gauss=400, r_lyot=450):
                     meta={"seed": seed, "n": n, "fields": 1, "lambdas": [LAMBDA0]})
 
 
+def workload_zoomprop(n=256, pad=1.0, zoom=1.0):
+    """pupil -> zoomFftPropagator (dir 1) -> occulting mask on its grid ->
+    zoomFftPropagator (dir -1) -> Lyot stop -> 128^2-class image, one wavelength."""
+    seed = BASE_SEED + 11
+    pix = PIX * 1024 / n
+    r_ap = 500 * n // 1024
+    beam_r = r_ap * pix
+    nf = int(pad * n)
+    script = (_telescope(2 * beam_r) + _efield(n, pix, beam_r, [LAMBDA0], [(1.0, 0.0, 0.0)]) +
+              "start\n\tcoronagraph\n\tstart\n\t\tmask\n\t\tname:pupil\n\tend\n"
+              "\tstart\n\t\tmirror\n\t\tname:aberration\n\tend\n"
+              f"\tstart\n\t\tzoomFftPropagator\n\t\tname:to fpm\n\t\tdir:1\n\t\tpadFactor:{pad!r}\n"
+              f"\t\tzoomFactor:{zoom!r}\n\tend\n"
+              "\tstart\n\t\tmask\n\t\tname:occulter\n\tend\n"
+              f"\tstart\n\t\tzoomFftPropagator\n\t\tname:to lyot\n\t\tdir:-1\n\t\tpadFactor:{pad!r}\n\tend\n" +
+              _tail(LAMBDA0))
+    arrays = {("pupil", "filename"): pupil_amplitude(n, r_ap, 400 * n / 1024),
+              ("aberration", "filename"): aberration_sag(seed, n),
+              ("occulter", "filename"): 1.0 - circle(nf, 0.06 * nf),
+              ("lyot", "filename"): circle(n, 450 * n / 1024)}
+    return Workload(11, f"zoomFftPropagator pair around an occulter, {n}^2 pupil", script, arrays,
+                    meta={"seed": seed, "n": n, "fields": 1, "lambdas": [LAMBDA0]})
+
+
 def workload1(n=2048):
     """2048^2 field, PIAA mirrors around a 1 m Fresnel step, downsampled to 1024^2."""
     seed = BASE_SEED + 1

@@ -186,6 +186,15 @@ int csx_chain_set_graph(csx_chain* chain, int enable);
 int csx_chain_hex_response(csx_chain* chain, const char* element_name, const csx_region* region, void* d_M,
                            int* npix, int* ncol, int* nslices, double* calib_peak);
 
+/* Region-pixel fields of every candidate after the last full propagation: the
+ * fields linearOptimizer::compute_jacobian and compute_objective_vector read
+ * (src/optimization/linearOptimizer.cpp:420-505).  d_out (device, complex128):
+ * nslices = n_sources * n_lambdas slices (slice s * n_lambdas + l), each npix x
+ * ncand column-major (column c = candidate c), region pixels in
+ * regionContrast::get_region_pixels order.  d_out NULL: only the sizes. */
+int csx_chain_region_fields(csx_chain* chain, const csx_region* region, void* d_out, int* npix, int* ncand,
+                            int* nslices);
+
 /* Chain group (no reference counterpart; csim runs one configuration at a time):
  * several chains with identical element lists, e.g. the same script at 512^2 and
  * 1024^2 pupils, propagated together.  Every contraction stage of all chains is

@@ -17,6 +17,7 @@ product code:
   hex FPM         src/elements/complexHexMaskFPM.cpp
   fpmPupToLyot    src/elements/fpmPupToLyot.cpp:24-167, 240-496 (three mask kinds)
   fraunhoferFocal src/elements/fraunhoferFocal.cpp:23-72
+  zoomFftPropagator src/elements/zoomFftPropagator.cpp:23-91
   region contrast src/tools/regionContrast.cpp:89-205
   contrast curve  src/tools/contrastCurve.cpp:68-158
 The additive keys (fpmSamplesPerflD, fpmFOVflD, occulterRadiusFld, hexLayout)
@@ -286,6 +287,20 @@ class OracleChain:
                 E = zoomfft(Ef, (gf[1], gf[1], gf[2], gf[2]), (g[0], g[1], g[1], g[2], g[2]), -lf)
             elif t == "fpmPupToLyot":
                 E = self._pup_to_lyot(e, E, g, lam, calibrating, cand_sags)
+            elif t == "zoomFftPropagator":
+                # zoomFftPropagator::execute (src/elements/zoomFftPropagator.cpp:23-91):
+                # offset output grid (i - n/2) d; lambda*f from the first wavelength
+                d, pad = int(e.get("dir", 1)), float(e.get("padFactor", 1.0))
+                lam0 = m.lambdas[0]
+                if d == 1:
+                    n = int(pad * g[0])
+                    dxp = 2 * (m.R / (m.pix * g[0] * float(e.get("zoomFactor", 1.0)))) * lam0 * 80
+                    lf = lam0 * 3.52
+                else:
+                    n, dxp, lf = int(g[0] / pad), m.pix, -lam0 * 3.52
+                go = (n, (0 - n / 2.0) * dxp, dxp)
+                E = zoomfft(E, (g[1], g[1], g[2], g[2]), (go[0], go[1], go[1], go[2], go[2]), lf)
+                g = go
             elif t == "fraunhoferFocal":
                 ref = float(e["referenceLambda"])
                 spl, fov = float(e["samplesPerflD"]), float(e["FOVflD"])

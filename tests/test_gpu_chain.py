@@ -45,6 +45,23 @@ def test_workload0_fields_and_contrast(ctx):
     assert np.max(np.abs(img - img_ref)) <= 1e-12 * img_ref.max()
 
 
+@pytest.mark.parametrize("pad", [1.0, 2.0])
+def test_zoomfft_propagator_vs_oracle(ctx, pad):
+    """zoomFftPropagator pair (dir 1 onto csim's offset FPM grid, an occulting mask
+    there, dir -1 back) between pupil and Lyot stop: fields and contrast against the
+    oracle chain, FP64."""
+    w = W.workload_zoomprop(n=256, pad=pad)
+    ch = build(ctx, w)
+    oc = OracleChain(w.script, w.arrays)
+    ch.propagate(calibrating=False)
+    e = rel_rms(ch.field(), oc.field(0, 0))
+    c, _ = ch.contrast()
+    c_ref, _, _ = oc.contrast()
+    print(f"zoomFftPropagator pad {pad}: field {e:.3e}, contrast {c[0]:.10e} vs {c_ref:.10e}")
+    assert e < 1e-12
+    assert abs(c[0] - c_ref) <= 1e-12 * c_ref
+
+
 def test_workload2_reduced_fp64(ctx):
     w = W.workload2(n_lambda=3, offsets=(0.0, 2.0, 6.0))
     ch = build(ctx, w)
@@ -205,6 +222,40 @@ def test_graph_replay_matches_eager(ctx):
     for s in range(2):
         for l in range(3):
             assert np.array_equal(g.field(0, s, l), eager.field(0, s, l))
+
+
+def test_linear_optimizer_jacobian_and_step(ctx):
+    """linearOptimizer on the batched chain: the finite-difference Jacobian of the
+    region fields (one candidate batch) equals the derivative of the response-matrix
+    model E(s) = sum_k M_k exp(i 4 pi s_k / lambda) + M_out; the optimiser's steps
+    lower the dark-hole contrast from a random start."""
+    import torch
+    from csim_amd.linear import LinearConfig, LinearOptimizer
+    w = W.workload2(n=256, n_lambda=2, offsets=(0.0,), rings=3)
+    ch = build(ctx, w)
+    nhex = w.meta["n_hex"]
+    rng = np.random.default_rng(41)
+    s0 = rng.random(nhex) * 0.2e-6
+    ch.set_opt_data("fpm", "sags", s0)
+    M, _ = ch.hex_response("fpm")                                 # (nslice, nhex + 1, npix)
+    lams = np.asarray(w.meta["lambdas"])
+    opt = LinearOptimizer(ch, "fpm", LinearConfig(jacobian_dx=1e-12))
+    c0, J, e = opt.jacobian(s0)
+    t = torch.from_numpy(np.exp(1j * 4 * np.pi * s0[None, :] / lams[:, None])).cuda()       # (nl, nhex)
+    dE = M[:, :nhex, :] * (t * (1j * 4 * np.pi / torch.from_numpy(lams).cuda()[:, None]))[:, :, None]
+    Jref = torch.cat([dE.real, dE.imag], dim=2).permute(0, 2, 1).reshape(-1, nhex)
+    err = float(torch.linalg.norm(J - Jref) / torch.linalg.norm(Jref))
+    Eref = (M[:, :nhex, :] * t[:, :, None]).sum(dim=1) + M[:, nhex, :]
+    eref = -torch.cat([Eref.real, Eref.imag], dim=1).reshape(-1)
+    e_err = float(torch.linalg.norm(e - eref) / torch.linalg.norm(eref))
+    print(f"jacobian vs response derivative {err:.2e}, objective vector {e_err:.2e}")
+    assert err < 1e-4 and e_err < 1e-11
+    scale = float(torch.linalg.matrix_norm(J, ord=2))
+    cfg = LinearConfig(jacobian_dx=1e-12, mu_scan=(0.01 * scale, 0.3 * scale, 0.01 * scale),
+                       global_bounds=(0.0, 1e-6), max_iterations=3, contrast_rel_tolerance=0.0, stop_contrast=0.0)
+    res = LinearOptimizer(ch, "fpm", cfg).run(s0)
+    print([(h[0], f"{h[2]:.3e}") for h in res.history])
+    assert res.best_value < 0.5 * res.history[0][2]
 
 
 def test_chain_group_uneven_pupils_matches_chains(ctx):
