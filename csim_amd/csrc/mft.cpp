@@ -310,10 +310,10 @@ int MftPlan::run(hipStream_t s, KernelTimer* t) const {
   if (nf_ == 0) return CSX_OK;
   const bool timed = t && t->enabled();
   std::string base = std::string("gemm_") + prec_name(prec_);
-  if (timed) CSX_TRY(t->begin(base, flops1_, bytes1_, s));
+  // one event pair around the two contractions (they run back to back): the
+  // average launch duration is the pair's time over 2, gap between them included
+  if (timed) CSX_TRY(t->begin(base, flops1_ + flops2_, bytes1_ + bytes2_, s, 2));
   CSX_TRY(launch_gemm(probs1_.as<GemmProb>(), nt_, tiles1_, prec_, s, var1_));
-  if (timed) CSX_TRY(t->end(s));
-  if (timed) CSX_TRY(t->begin(base, flops2_, bytes2_, s));
   CSX_TRY(launch_gemm(probs2_.as<GemmProb>(), nf_, tiles2_, prec_, s, var2_));
   if (timed) CSX_TRY(t->end(s));
   return CSX_OK;
@@ -340,8 +340,8 @@ int KernelTimer::get_event(hipEvent_t* e) {
   free_.pop_back();
   return CSX_OK;
 }
-int KernelTimer::begin(const std::string& name, double flops, double bytes, hipStream_t s) {
-  Rec r{name, flops, bytes, nullptr, nullptr};
+int KernelTimer::begin(const std::string& name, double flops, double bytes, hipStream_t s, int launches) {
+  Rec r{name, flops, bytes, nullptr, nullptr, launches};
   CSX_TRY(get_event(&r.b));
   CSX_TRY(get_event(&r.e));
   open_.push_back(r);
@@ -358,7 +358,7 @@ int KernelTimer::collect() {
     float ms = 0;
     CSX_HIP(hipEventElapsedTime(&ms, r.b, r.e));
     Stat& st = stats_[r.name];
-    st.count++;
+    st.count += r.launches;
     st.ms += ms;
     st.flops += r.flops;
     st.bytes += r.bytes;

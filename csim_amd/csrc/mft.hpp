@@ -17,14 +17,15 @@ class KernelTimer {
   ~KernelTimer();
   void enable(bool on);
   bool enabled() const { return on_; }
-  int begin(const std::string& name, double flops, double bytes, hipStream_t s);
+  // `launches` kernels of `name` run back to back between begin and end
+  int begin(const std::string& name, double flops, double bytes, hipStream_t s, int launches = 1);
   int end(hipStream_t s);
   int collect();                        // after the stream is synchronised
   std::string json() const;
   struct Stat { long count = 0; double ms = 0, flops = 0, bytes = 0; };
   const std::map<std::string, Stat>& stats() const { return stats_; }
  private:
-  struct Rec { std::string name; double flops, bytes; hipEvent_t b, e; };   // a launch between two events
+  struct Rec { std::string name; double flops, bytes; hipEvent_t b, e; int launches; };   // launches between two events
   int get_event(hipEvent_t* e);
   bool on_ = false;
   std::vector<hipEvent_t> free_, all_;
