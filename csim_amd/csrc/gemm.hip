@@ -17,11 +17,14 @@ namespace csx {
 // ------------------------------------------------------------ problem plan --
 int plan_tiles(GemmProb* probs, int n, int tile_m, int tile_n) {
   int total = 0;
+  long elems = 0;
   for (int i = 0; i < n; i++) {
     probs[i].tiles_m = (probs[i].M + tile_m - 1) / tile_m;
     probs[i].tiles_n = (probs[i].N + tile_n - 1) / tile_n;
     probs[i].tile_start = total;
-    total += probs[i].tiles_m * probs[i].tiles_n;
+    probs[i].elem_start = elems;
+    total += probs[i].tiles_m * probs[i].tiles_n * (probs[i].ksplit > 1 ? probs[i].ksplit : 1);
+    elems += (long)probs[i].M * probs[i].N;
   }
   return total;
 }
@@ -414,12 +417,21 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
 
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   const GemmProb& P = probs[find_prob(probs, nprob, tile)];
+  const int tiles_mn = P.tiles_m * P.tiles_n;
   const int local = tile - P.tile_start;
-  const int tm = local % P.tiles_m, tn = local / P.tiles_m;
+  const int ks = local / tiles_mn, lmn = local - ks * tiles_mn;   // K split, output tile
+  const int tm = lmn % P.tiles_m, tn = lmn / P.tiles_m;
   const int m0 = tm * BM, n0 = tn * BN;
   const GLOBAL unsigned short* A = gin<unsigned short>(P.A);
   const GLOBAL unsigned short* B = gin<unsigned short>(P.B);
   const int M = P.M, N = P.N, K = P.K;
+  // K range of this block: whole slabs except at K itself
+  int kbeg = 0, kend = K;
+  if (TERMS == 2 && P.ksplit > 1) {
+    const int kc = ((K + P.ksplit - 1) / P.ksplit + BK - 1) / BK * BK;
+    kbeg = min(K, ks * kc);
+    kend = min(K, kbeg + kc);
+  }
 
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int wm = wave / WN, wn = wave % WN;
@@ -604,38 +616,55 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
    }
   };
 
-  const int nslab = (K + BK - 1) / BK;
+  const int nslab = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
+  if (nslab > 0) {
   if (DMA) {
-    dload(0, 0);
+    dload(kbeg, 0);
     __builtin_amdgcn_s_waitcnt(0x0F70);            // vmcnt(0): this wave's loads have landed
     __syncthreads();
     for (int s = 0; s < nslab; s++) {
       const int cur = s & 1;
-      if (s + 1 < nslab) dload((s + 1) * BK, cur ^ 1);
+      if (s + 1 < nslab) dload(kbeg + (s + 1) * BK, cur ^ 1);
       compute(lds + cur * STAGE);
       __builtin_amdgcn_s_waitcnt(0x0F70);
       __syncthreads();
     }
   } else if (DB) {
-    gload(0);
+    gload(kbeg);
     sstore(0);
     __syncthreads();
     for (int s = 0; s < nslab; s++) {
       const int cur = s & 1;
-      if (s + 1 < nslab) gload((s + 1) * BK);
+      if (s + 1 < nslab) gload(kbeg + (s + 1) * BK);
       compute(lds + cur * STAGE);
       if (s + 1 < nslab) sstore(cur ^ 1);
       __syncthreads();
     }
   } else {
-    gload(0);
+    gload(kbeg);
     for (int s = 0; s < nslab; s++) {
       __syncthreads();
       sstore(0);
       __syncthreads();
-      if (s + 1 < nslab) gload((s + 1) * BK);
+      if (s + 1 < nslab) gload(kbeg + (s + 1) * BK);
       compute(lds);
     }
+  }
+  }
+  if (TERMS == 2 && P.ksplit > 1) {
+    // this split's FP32 sums; gemm_splitk_finish adds the splits and applies the epilogue
+    float2* W = reinterpret_cast<float2*>(P.ws) + (long)ks * M * N;
+#pragma unroll
+    for (int i = 0; i < IM; i++)
+#pragma unroll
+      for (int j = 0; j < JN; j++)
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+          const int m = m0 + wm * (BM / WM) + i * MF + Mf::drow(lane, v);
+          const int n = n0 + wn * (BN / WN) + j * MF + Mf::dcol(lane);
+          if (m < M && n < N) W[(long)m * N + n] = make_float2(accR[i][j][v], accI[i][j][v]);
+        }
+    return;
   }
   // BF16-plane outputs are staged through LDS so that every lane stores 16
   // contiguous bytes of one plane (the MFMA layout gives a lane 4 rows of one
@@ -731,6 +760,34 @@ int planes_variant_tile(int terms, int v, int* bm, int* bn) {
   *bm = pv.bm; *bn = pv.bn;
   return CSX_OK;
 }
+__global__ void k_splitk_finish(const GemmProb* __restrict__ probs, int nprob, long total) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (probs[mid].elem_start <= e) lo = mid; else hi = mid - 1;
+    }
+    const GemmProb& P = probs[lo];
+    const long q = e - P.elem_start, mn = (long)P.M * P.N;
+    const int m = (int)(q / P.N), n = (int)(q - (long)m * P.N);
+    const float2* W = reinterpret_cast<const float2*>(P.ws);
+    double re = 0, im = 0;
+    for (int k = 0; k < P.ksplit; k++) {         // fixed order: deterministic
+      const float2 w = W[(long)k * mn + q];
+      re += w.x;
+      im += w.y;
+    }
+    epilogue_store(P, m, n, re, im);
+  }
+}
+int gemm_splitk_finish(const GemmProb* probs_dev, int n, long total_elems, hipStream_t s) {
+  if (total_elems <= 0) return CSX_OK;
+  const int blocks = (int)std::min<long>(8192, (total_elems + 255) / 256);
+  hipLaunchKernelGGL(k_splitk_finish, dim3(blocks), dim3(256), 0, s, probs_dev, n, total_elems);
+  CSX_HIP(hipGetLastError());
+  return CSX_OK;
+}
+
 int gemm_bf16planes(const GemmProb* probs_dev, int n, int total_tiles, int terms, int variant, hipStream_t s) {
   if (total_tiles <= 0) return CSX_OK;
   if (variant < 0 || variant >= planes_variant_count(terms)) return fail(CSX_ERR_ARG, "gemm_bf16planes: bad variant");

@@ -31,13 +31,23 @@ struct GemmProb {
   // q * sX bf16 elements after its base pointer
   long sA, sB, sC;
   int terms;
+  // split K (two-term BF16 planes only): ksplit > 1 blocks per output tile each take a
+  // K range and write their FP32 sums to ws ([split][m][n] complex float); a finishing
+  // launch (gemm_splitk_finish) adds the splits in order and applies the epilogue
+  int ksplit;
+  float* ws;
+  long elem_start;     // first output element of this problem among the launch's (finish kernel)
 };
 
 // Tile of the FP64 / FP32 MFMA kernels (64 x 64 outputs, 4 waves of 32 x 32).
 constexpr int kTileM = 64, kTileN = 64;
 
-// Fills tiles_m/tiles_n/tile_start; returns the total tile count.
+// Fills tiles_m/tiles_n/tile_start (and elem_start); returns the total tile count
+// (ksplit tiles per output tile when ksplit > 1).
 int plan_tiles(GemmProb* probs, int n, int tile_m, int tile_n);
+// Adds the K splits of every problem of a split-K launch and applies the epilogue;
+// total_elems = sum of M N over the problems.
+int gemm_splitk_finish(const GemmProb* probs_dev, int n, long total_elems, hipStream_t s);
 
 // probs_dev: device copy of the problem array (n entries), total_tiles from plan_tiles.
 int gemm_f64(const GemmProb* probs_dev, int n, int total_tiles, hipStream_t s, int variant = 0);

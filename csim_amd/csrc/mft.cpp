@@ -37,9 +37,9 @@ static int forced_variant(csx_precision p) {
   return v >= 0 && v < n_variants(p) ? v : -1;
 }
 
-// Autotuner cache: problem-list signature -> chosen variant (process lifetime).
-static std::map<std::string, int>& tune_cache() {
-  static std::map<std::string, int> c;
+// Autotuner cache: problem-list signature -> chosen (variant, K splits) (process lifetime).
+static std::map<std::string, std::pair<int, int>>& tune_cache() {
+  static std::map<std::string, std::pair<int, int>> c;
   return c;
 }
 
@@ -50,6 +50,33 @@ static std::string signature(const std::vector<GemmProb>& v, csx_precision p) {
     s += std::to_string(g.M) + 'x' + std::to_string(g.N) + 'x' + std::to_string(g.K);
   }
   return s;
+}
+
+// K splits worth trying for a launch: two-term planes whose 64 x 64 tiles do not fill
+// the chip twice over and whose K is long enough to share (1 = no split).
+static int split_candidate(const std::vector<GemmProb>& v, csx_precision p) {
+  if (p != CSX_BF16X3) return 1;
+  if (const char* e = std::getenv("CSX_GEMM_KSPLIT")) return std::max(1, std::atoi(e));
+  long tiles = 0;
+  int kmin = 1 << 30;
+  for (const auto& g : v) {
+    tiles += (long)((g.M + 63) / 64) * ((g.N + 63) / 64);
+    kmin = std::min(kmin, g.K);
+  }
+  if (tiles >= 512 || kmin < 512) return 1;
+  int k = (int)std::min<long>(8, (1024 + tiles - 1) / tiles);
+  while (k > 1 && kmin / k < 128) k--;
+  return std::max(k, 1);
+}
+
+// Sets each problem's K splits and its workspace slice (ws: splits x M x N complex float).
+static void set_split(std::vector<GemmProb>& v, int ksplit, float* ws) {
+  long off = 0;
+  for (auto& g : v) {
+    g.ksplit = ksplit;
+    g.ws = ksplit > 1 ? ws + off : nullptr;
+    off += ksplit > 1 ? 2L * ksplit * g.M * g.N : 0;
+  }
 }
 
 // Uploads the problem list planned for `variant` into `dev`; returns the tile count.
@@ -64,38 +91,65 @@ static int upload(std::vector<GemmProb>& v, csx_precision p, int variant, DevBuf
   return CSX_OK;
 }
 
-// Picks the fastest tile variant of one grouped launch by timing: two passes over
-// the variants, each timing 3 back-to-back launches between two HIP events on the
-// launch stream after one warm-up launch; a variant's time is its better pass, and
-// a later variant must beat the best earlier one by 2 % (choices within timing noise
-// stay put).  The launches write the stage's own output buffers only, which the
-// real run overwrites.
-static int choose_variant(std::vector<GemmProb>& v, csx_precision p, DevBuf& dev, hipStream_t s, int* variant,
-                          int* tiles) {
+static long total_elems(const std::vector<GemmProb>& v) {
+  long e = 0;
+  for (const auto& g : v) e += (long)g.M * g.N;
+  return e;
+}
+
+// One contraction launch (plus the split-K finishing launch when the problems are split).
+static int run_stage(const GemmProb* dev, const std::vector<GemmProb>& v, int tiles, csx_precision p, hipStream_t s,
+                     int variant, int ksplit, long elems) {
+  CSX_TRY(launch_gemm(dev, (int)v.size(), tiles, p, s, variant));
+  if (ksplit > 1) CSX_TRY(gemm_splitk_finish(dev, (int)v.size(), elems, s));
+  return CSX_OK;
+}
+
+// Picks the fastest (tile variant, K splits) of one grouped launch by timing: two
+// passes over the candidates, each timing 3 back-to-back launches between two HIP
+// events on the launch stream after one warm-up launch; a candidate's time is its
+// better pass, and a later one must beat the best earlier one by 2 % (choices within
+// timing noise stay put).  The launches write the stage's own output buffers (and
+// its workspace) only, which the real run overwrites.
+static int choose_variant(std::vector<GemmProb>& v, csx_precision p, DevBuf& dev, DevBuf& ws, hipStream_t s,
+                          int* variant, int* ksplit, int* tiles) {
   const int nv = n_variants(p);
+  const int kcand = split_candidate(v, p);
+  if (kcand > 1) CSX_TRY(ws.reserve(sizeof(float) * 2 * (size_t)kcand * total_elems(v)));
   int forced = forced_variant(p);
   if (nv == 1 || forced >= 0) {
     *variant = nv == 1 ? 0 : forced;
+    *ksplit = std::getenv("CSX_GEMM_KSPLIT") ? kcand : 1;
+    set_split(v, *ksplit, ws.as<float>());
     return upload(v, p, *variant, dev, s, tiles);
   }
   const std::string key = signature(v, p);
   auto it = tune_cache().find(key);
   if (it != tune_cache().end()) {
-    *variant = it->second;
+    *variant = it->second.first;
+    *ksplit = it->second.second;
+    set_split(v, *ksplit, ws.as<float>());
     return upload(v, p, *variant, dev, s, tiles);
   }
+  std::vector<std::pair<int, int>> cand;
+  for (int var = 0; var < nv; var++) cand.push_back({var, 1});
+  if (kcand > 1)
+    for (int var = 0; var < nv; var++) cand.push_back({var, kcand});
+  const long elems = total_elems(v);
   hipEvent_t e0, e1;
   CSX_HIP(hipEventCreate(&e0));
   CSX_HIP(hipEventCreate(&e1));
-  std::vector<float> tbest(nv, 1e30f);
+  std::vector<float> tbest(cand.size(), 1e30f);
   int rc = CSX_OK;
   for (int pass = 0; pass < 2 && rc == CSX_OK; pass++) {
-    for (int var = 0; var < nv && rc == CSX_OK; var++) {
+    for (size_t c = 0; c < cand.size() && rc == CSX_OK; c++) {
+      const int var = cand[c].first, ks = cand[c].second;
       int nt = 0;
+      set_split(v, ks, ws.as<float>());
       if ((rc = upload(v, p, var, dev, s, &nt)) != CSX_OK) break;
-      if ((rc = launch_gemm(dev.as<GemmProb>(), (int)v.size(), nt, p, s, var)) != CSX_OK) break;
+      if ((rc = run_stage(dev.as<GemmProb>(), v, nt, p, s, var, ks, elems)) != CSX_OK) break;
       if (hipEventRecord(e0, s) != hipSuccess) { rc = fail(CSX_ERR_HIP, "autotune: event record"); break; }
-      for (int r = 0; r < 3 && rc == CSX_OK; r++) rc = launch_gemm(dev.as<GemmProb>(), (int)v.size(), nt, p, s, var);
+      for (int r = 0; r < 3 && rc == CSX_OK; r++) rc = run_stage(dev.as<GemmProb>(), v, nt, p, s, var, ks, elems);
       if (rc != CSX_OK) break;
       if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
         rc = fail(CSX_ERR_HIP, "autotune: event wait");
@@ -103,18 +157,20 @@ static int choose_variant(std::vector<GemmProb>& v, csx_precision p, DevBuf& dev
       }
       float t = 0;
       (void)hipEventElapsedTime(&t, e0, e1);
-      tbest[var] = std::min(tbest[var], t / 3);
+      tbest[c] = std::min(tbest[c], t / 3);
     }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc != CSX_OK) return rc;
-  int best_v = 0;
-  for (int var = 1; var < nv; var++)
-    if (tbest[var] < tbest[best_v] * 0.98f) best_v = var;
-  tune_cache()[key] = best_v;
-  *variant = best_v;
-  return upload(v, p, best_v, dev, s, tiles);
+  size_t best = 0;
+  for (size_t c = 1; c < cand.size(); c++)
+    if (tbest[c] < tbest[best] * 0.98f) best = c;
+  tune_cache()[key] = cand[best];
+  *variant = cand[best].first;
+  *ksplit = cand[best].second;
+  set_split(v, *ksplit, ws.as<float>());
+  return upload(v, p, *variant, dev, s, tiles);
 }
 
 // Problems of different shapes (a chain group: 512^2 and 1024^2 pupils) have tiles of
@@ -291,8 +347,10 @@ int MftPlan::build(const std::vector<MftField>& fields, csx_precision p, hipStre
   spread_shapes(p1);
   spread_shapes(p2);
   // stage 2 reads stage 1's output, so stage 1 is tuned (and left written) first
-  CSX_TRY(choose_variant(p1, p, probs1_, s, &var1_, &tiles1_));
-  CSX_TRY(choose_variant(p2, p, probs2_, s, &var2_, &tiles2_));
+  CSX_TRY(choose_variant(p1, p, probs1_, ws1_, s, &var1_, &ks1_, &tiles1_));
+  CSX_TRY(choose_variant(p2, p, probs2_, ws2_, s, &var2_, &ks2_, &tiles2_));
+  elems1_ = total_elems(p1);
+  elems2_ = total_elems(p2);
   return CSX_OK;
 }
 
@@ -310,11 +368,14 @@ int MftPlan::run(hipStream_t s, KernelTimer* t) const {
   if (nf_ == 0) return CSX_OK;
   const bool timed = t && t->enabled();
   std::string base = std::string("gemm_") + prec_name(prec_);
-  // one event pair around the two contractions (they run back to back): the
-  // average launch duration is the pair's time over 2, gap between them included
+  // one event pair around the two contractions (they run back to back; a split-K
+  // finishing launch belongs to its contraction): the average launch duration is
+  // the pair's time over 2, gaps included
   if (timed) CSX_TRY(t->begin(base, flops1_ + flops2_, bytes1_ + bytes2_, s, 2));
   CSX_TRY(launch_gemm(probs1_.as<GemmProb>(), nt_, tiles1_, prec_, s, var1_));
+  if (ks1_ > 1) CSX_TRY(gemm_splitk_finish(probs1_.as<GemmProb>(), nt_, elems1_, s));
   CSX_TRY(launch_gemm(probs2_.as<GemmProb>(), nf_, tiles2_, prec_, s, var2_));
+  if (ks2_ > 1) CSX_TRY(gemm_splitk_finish(probs2_.as<GemmProb>(), nf_, elems2_, s));
   if (timed) CSX_TRY(t->end(s));
   return CSX_OK;
 }

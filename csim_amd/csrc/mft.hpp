@@ -71,9 +71,11 @@ class MftPlan {
 
  private:
   csx_precision prec_ = CSX_FP64;
-  DevBuf dft_, tbuf_, probs1_, probs2_, specs_;
+  DevBuf dft_, tbuf_, probs1_, probs2_, specs_, ws1_, ws2_;
   int nf_ = 0, nt_ = 0, tiles1_ = 0, tiles2_ = 0;   // fields, distinct first contractions
   int var1_ = 0, var2_ = 0;   // tile variant of each stage (index into the precision's list)
+  int ks1_ = 1, ks2_ = 1;     // K splits of each stage (split-K workspace ws1_/ws2_)
+  long elems1_ = 0, elems2_ = 0;
   double flops_ = 0, flops1_ = 0, flops2_ = 0, bytes1_ = 0, bytes2_ = 0;
 };
 

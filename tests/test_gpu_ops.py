@@ -113,6 +113,33 @@ def test_mft_planes_tile_variants(ctx, monkeypatch, prec, tol, nin, nout, batch)
             assert d < (tol / 10 if v in MFMA32[prec] else 1e-6)
 
 
+@pytest.mark.parametrize("ksplit", [2, 4])
+@pytest.mark.parametrize("nin,nout,batch", [(1024, 128, 2), (200, 72, 3)])
+def test_mft_planes_split_k(ctx, monkeypatch, ksplit, nin, nout, batch):
+    """Split-K contractions (FP32 sums per K range, added in order by the finishing
+    launch) stay at the split-BF16 error level against FP64 and agree with the
+    unsplit ones to a tenth of it (the FP32 sums round differently), for a 64x64, a
+    128x128 and a direct-to-LDS tile variant, on ragged K (200 = 3 slabs + 8)."""
+    rng = np.random.default_rng(nin + 11 * nout)
+    E = np.stack([rand_field(rng, nin, nin) for _ in range(batch)])
+    dx, lf = 1e-5, 650e-9
+    dxo = lf / (nin * dx) / 2
+    gi, go = sym(nin, dx), sym(nout, dxo)
+    ref = ops.to_host(ops.mft(ctx, ops.to_device(E), gi, gi, go, go, lf, "fp64"))
+    monkeypatch.setenv("CSX_GEMM_VARIANT", "0")
+    base = ops.to_host(ops.mft(ctx, ops.to_device(E), gi, gi, go, go, lf, "bf16x3"))
+    monkeypatch.setenv("CSX_GEMM_KSPLIT", str(ksplit))
+    for v in (0, 2, 11):
+        monkeypatch.setenv("CSX_GEMM_VARIANT", str(v))
+        out = ops.to_host(ops.mft(ctx, ops.to_device(E), gi, gi, go, go, lf, "bf16x3"))
+        for b in range(batch):
+            e, d = rel_rms(out[b], ref[b]), rel_rms(out[b], base[b])
+            print(f"split {ksplit} {nin}->{nout} variant {v} field {b}: vs fp64 {e:.3e}, vs unsplit {d:.3e}")
+            assert e < 5e-5 and d < 5e-6
+    monkeypatch.delenv("CSX_GEMM_KSPLIT")
+    monkeypatch.delenv("CSX_GEMM_VARIANT")
+
+
 @pytest.mark.parametrize("nin,nout,batch", [(1024, 256, 2), (200, 72, 3), (96, 136, 2)])
 def test_mft_fp64_tile_variants(ctx, monkeypatch, nin, nout, batch):
     """Every FP64 tile variant (single- or double-buffered, 64 to 128 wide) meets the
