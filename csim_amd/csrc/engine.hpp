@@ -209,6 +209,8 @@ class Chain {
   int out_buf_ = 0, out_buf_cal_ = 0;
   bool have_full_ = false, have_cal_ = false;
   DevBuf inten_, inten_cal_, peak_, region_idx_, contrast_;
+  double* contrast_host_ = nullptr;   // pinned landing buffer of the per-candidate contrasts
+  int contrast_host_n_ = 0;
   double calib_peak_ = 0;
   bool calib_valid_ = false;
   int n_region_ = 0;
