@@ -277,13 +277,17 @@ __global__ __launch_bounds__(256) void k_pointwise_gen(double2* __restrict__ E, 
   }
 }
 
-// The generation pass into two-term BF16 planes, 4 pixels per thread: past the FP64
-// phases everything is FP32 (its 24 bits are well above the planes' 16), each plane of
-// a field gets one 8-byte store per thread.
-template <int NL>
+// The generation pass into two-term BF16 planes, 8 pixels per thread: past the FP64
+// phases everything is FP32 (its 24 bits are well above the planes' 16).  The
+// wavelength loop is outermost: one mirror phasor per pixel and wavelength, the
+// source phasors (at most kBf2Outer source / candidate rows) kept in registers,
+// and each plane of a field written with one 16-byte store per thread.
+constexpr int kBf2Outer = 4;
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 __global__ __launch_bounds__(256) void k_pointwise_gen_bf2(PwChain pc, int nouter) {
 #pragma clang fp contract(off)
-  constexpr int PX = 4;
+  constexpr int PX = 8;
+  const int nl = pc.nl;
   const int npix = pc.rows * pc.cols;
   const int p0 = PX * (blockIdx.x * blockDim.x + threadIdx.x);
   if (p0 >= npix) return;
@@ -293,26 +297,55 @@ __global__ __launch_bounds__(256) void k_pointwise_gen_bf2(PwChain pc, int noute
   for (int k = 1; k < pc.nops; k++) {
     if (pc.op[k].kind != PW_MASK_1PI) continue;
     const GLOBAL f64x2* wp = reinterpret_cast<const GLOBAL f64x2*>(gin<double>(pc.op[k].p) + p0);
-    const f64x2 w01 = wp[0], w23 = wp[1];
-    const float w[PX] = {(float)w01[0], (float)w01[1], (float)w23[0], (float)w23[1]};
 #pragma unroll
-    for (int x = 0; x < PX; x++) {            // mask.cpp: E *= (1+i) m
-      const float tr = cr[x] * w[x] - ci[x] * w[x];
-      ci[x] = cr[x] * w[x] + ci[x] * w[x];
-      cr[x] = tr;
+    for (int x2 = 0; x2 < PX / 2; x2++) {      // mask.cpp: E *= (1+i) m
+      const f64x2 w2 = wp[x2];
+#pragma unroll
+      for (int y = 0; y < 2; y++) {
+        const int x = 2 * x2 + y;
+        const float w = (float)w2[y];
+        const float tr = cr[x] * w - ci[x] * w;
+        ci[x] = cr[x] * w + ci[x] * w;
+        cr[x] = tr;
+      }
     }
   }
-  float Fr[NL][PX], Fi[NL][PX];
+  // source phasors of every outer row (efield::init_point_sources, src/data/efield.cpp:179-206)
+  const double ps = pc.pixel_scale, Lx = ps * pc.cols, Ly = ps * pc.rows;
+  float sr[kBf2Outer][PX], si[kBf2Outer][PX];
 #pragma unroll
-  for (int li = 0; li < NL; li++) {
-    double ph[PX] = {0.0, 0.0, 0.0, 0.0};
+  for (int o = 0; o < kBf2Outer; o++) {
+    if (o >= nouter) break;
+    const double* prm = pc.src_params + 3 * o * nl;
+    const double flux = prm[0], tx = prm[1], ty = prm[2];
+#pragma unroll
+    for (int x = 0; x < PX; x++) {
+      const int p = p0 + x, c = p / pc.rows, r = p - c * pc.rows;
+      const double xx = -(Lx - ps) / 2.0 + c * ps, yy = -(Ly - ps) / 2.0 + r * ps;
+      const double v = 2 * xx * tx / Lx + 2 * yy * ty / Ly;
+      double sn, cs;
+      phasor<true>(2.0 * M_PI * v, &sn, &cs);
+      sr[o][x] = (float)(flux * cs);
+      si[o][x] = (float)(flux * sn);
+    }
+  }
+  const int stride = npix / PX;                // one plane in 16-byte units
+  for (int li = 0; li < nl; li++) {
+    float Fr[PX], Fi[PX];
+    double ph[PX];
+#pragma unroll
+    for (int x = 0; x < PX; x++) ph[x] = 0.0;
     bool any = false;
     for (int k = 1; k < pc.nops; k++) {
       if (pc.op[k].kind != PW_MIRROR) continue;
       const double a = -pc.op[k].sign * 2 * 2 * M_PI / pc.lambdas[li];   // mirror.cpp:66-72
       const GLOBAL f64x2* hp = reinterpret_cast<const GLOBAL f64x2*>(gin<double>(pc.op[k].p) + p0);
-      const f64x2 h01 = hp[0], h23 = hp[1];
-      ph[0] += a * h01[0]; ph[1] += a * h01[1]; ph[2] += a * h23[0]; ph[3] += a * h23[1];
+#pragma unroll
+      for (int x2 = 0; x2 < PX / 2; x2++) {
+        const f64x2 h2 = hp[x2];
+        ph[2 * x2] += a * h2[0];
+        ph[2 * x2 + 1] += a * h2[1];
+      }
       any = true;
     }
 #pragma unroll
@@ -320,52 +353,32 @@ __global__ __launch_bounds__(256) void k_pointwise_gen_bf2(PwChain pc, int noute
       if (any) {
         double sn, cs;
         phasor<true>(ph[x], &sn, &cs);
-        Fr[li][x] = cr[x] * (float)cs - ci[x] * (float)sn;
-        Fi[li][x] = cr[x] * (float)sn + ci[x] * (float)cs;
+        Fr[x] = cr[x] * (float)cs - ci[x] * (float)sn;
+        Fi[x] = cr[x] * (float)sn + ci[x] * (float)cs;
       } else {
-        Fr[li][x] = cr[x];
-        Fi[li][x] = ci[x];
+        Fr[x] = cr[x];
+        Fi[x] = ci[x];
       }
     }
-  }
-  const double ps = pc.pixel_scale, Lx = ps * pc.cols, Ly = ps * pc.rows;
-  float sr[PX], si[PX];
-  double lflux = 0, ltx = 0, lty = 0;
-  const int stride = npix / 4;                 // one plane in 8-byte units
-  for (int o = 0; o < nouter; o++) {
-    const double* prm = pc.src_params + 3 * o * NL;
-    const double flux = prm[0], tx = prm[1], ty = prm[2];
-    if (o == 0 || flux != lflux || tx != ltx || ty != lty) {
 #pragma unroll
-      for (int x = 0; x < PX; x++) {          // efield::init_point_sources (src/data/efield.cpp:179-206)
-        const int p = p0 + x, c = p / pc.rows, r = p - c * pc.rows;
-        const double xx = -(Lx - ps) / 2.0 + c * ps, yy = -(Ly - ps) / 2.0 + r * ps;
-        const double v = 2 * xx * tx / Lx + 2 * yy * ty / Ly;
-        double sn, cs;
-        phasor<true>(2.0 * M_PI * v, &sn, &cs);
-        sr[x] = (float)(flux * cs);
-        si[x] = (float)(flux * sn);
-      }
-      lflux = flux; ltx = tx; lty = ty;
-    }
-#pragma unroll
-    for (int li = 0; li < NL; li++) {
-      const long f = (long)o * NL + li;
+    for (int o = 0; o < kBf2Outer; o++) {
+      if (o >= nouter) break;
+      const long f = (long)o * nl + li;
       unsigned short h[4][PX];                 // planes: hi re, hi im, lo re, lo im
 #pragma unroll
       for (int x = 0; x < PX; x++) {
-        const float er = sr[x] * Fr[li][x] - si[x] * Fi[li][x];
-        const float ei = sr[x] * Fi[li][x] + si[x] * Fr[li][x];
+        const float er = sr[o][x] * Fr[x] - si[o][x] * Fi[x];
+        const float ei = sr[o][x] * Fi[x] + si[o][x] * Fr[x];
         h[0][x] = bf16_rne(er);
         h[1][x] = bf16_rne(ei);
         h[2][x] = bf16_rne(er - bf16_val(h[0][x]));
         h[3][x] = bf16_rne(ei - bf16_val(h[1][x]));
       }
-      typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
-      GLOBAL u32x2* d = reinterpret_cast<GLOBAL u32x2*>(gout<unsigned short>(pc.out) + f * 4 * npix + p0);
+      GLOBAL u32x4* d = reinterpret_cast<GLOBAL u32x4*>(gout<unsigned short>(pc.out) + f * 4 * npix + p0);
 #pragma unroll
       for (int q = 0; q < 4; q++)
-        d[q * stride] = u32x2{(unsigned)h[q][0] | ((unsigned)h[q][1] << 16), (unsigned)h[q][2] | ((unsigned)h[q][3] << 16)};
+        d[q * stride] = u32x4{(unsigned)h[q][0] | ((unsigned)h[q][1] << 16), (unsigned)h[q][2] | ((unsigned)h[q][3] << 16),
+                              (unsigned)h[q][4] | ((unsigned)h[q][5] << 16), (unsigned)h[q][6] | ((unsigned)h[q][7] << 16)};
     }
   }
 }
@@ -390,14 +403,9 @@ int pointwise_chain(double2* E, int nfield, const PwChain& pc, hipStream_t s) {
 #undef CSX_GEN_ROW
     static_assert(kMaxPwLam == 16, "one instance per wavelength count");
     const int split = pc.out_kind == PW_OUT_PLANES && pc.terms == 2 ? 1 : 0;   // 16-bit planes only
-    if (split && npix % 4 == 0) {
-      typedef void (*Bf2Kernel)(PwChain, int);
-      static const Bf2Kernel kBf2[kMaxPwLam] = {
-          k_pointwise_gen_bf2<1>, k_pointwise_gen_bf2<2>, k_pointwise_gen_bf2<3>, k_pointwise_gen_bf2<4>,
-          k_pointwise_gen_bf2<5>, k_pointwise_gen_bf2<6>, k_pointwise_gen_bf2<7>, k_pointwise_gen_bf2<8>,
-          k_pointwise_gen_bf2<9>, k_pointwise_gen_bf2<10>, k_pointwise_gen_bf2<11>, k_pointwise_gen_bf2<12>,
-          k_pointwise_gen_bf2<13>, k_pointwise_gen_bf2<14>, k_pointwise_gen_bf2<15>, k_pointwise_gen_bf2<16>};
-      hipLaunchKernelGGL(kBf2[pc.nl - 1], dim3((int)((npix / 4 + 255) / 256)), dim3(256), 0, s, pc, nfield / pc.nl);
+    if (split && npix % 8 == 0 && nfield / pc.nl <= kBf2Outer) {
+      hipLaunchKernelGGL(k_pointwise_gen_bf2, dim3((int)((npix / 8 + 255) / 256)), dim3(256), 0, s, pc,
+                         nfield / pc.nl);
       CSX_HIP(hipGetLastError());
       return CSX_OK;
     }
