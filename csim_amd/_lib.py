@@ -66,6 +66,8 @@ SIGNATURES = {
     "csx_chain_field_profiles": (_i, [_p, _p, _i, _p]),
     "csx_chain_contrast_curve": (_i, [_p, _i, _p, _p, _i]),
     "csx_chain_region_fields": (_i, [_p, _p, _p, _p, _p, _p]),
+    "csx_fits_read": (_i, [C.c_char_p, _p, _sz, _p, _p, _p]),
+    "csx_fits_write": (_i, [C.c_char_p, _p, _i, _i]),
     "csx_chain_set_profiling": (_i, [_p, _i]),
     "csx_chain_set_graph": (_i, [_p, _i]),
     "csx_chain_hex_response": (_i, [_p, C.c_char_p, C.POINTER(Region), _p, C.POINTER(_i), C.POINTER(_i),

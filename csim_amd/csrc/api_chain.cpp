@@ -196,6 +196,26 @@ int csx_chain_hex_response(csx_chain* c, const char* element_name, const csx_reg
   CSX_GUARD_END
 }
 
+int csx_fits_read(const char* path, double* out, size_t capacity, int* rows, int* cols, int* slices) {
+  CSX_GUARD_BEGIN
+  if (!path || !rows || !cols || !slices) return fail(CSX_ERR_ARG, "csx_fits_read: null argument");
+  std::vector<double> m;
+  CSX_TRY(fits_matrix(path, m, *rows, *cols, *slices));
+  if (out) {
+    if (capacity < m.size()) return fail(CSX_ERR_ARG, "csx_fits_read: capacity below rows x cols x slices");
+    std::memcpy(out, m.data(), sizeof(double) * m.size());
+  }
+  return CSX_OK;
+  CSX_GUARD_END
+}
+
+int csx_fits_write(const char* path, const double* data, int rows, int cols) {
+  CSX_GUARD_BEGIN
+  if (!path) return fail(CSX_ERR_ARG, "csx_fits_write: null path");
+  return write_fits_matrix(path, data, rows, cols);
+  CSX_GUARD_END
+}
+
 int csx_chain_region_fields(csx_chain* c, const csx_region* region, void* d_out, int* npix, int* ncand,
                             int* nslices) {
   CSX_GUARD_BEGIN

@@ -136,8 +136,52 @@ int read_fits(const std::string& path, std::vector<double>& data, std::vector<lo
   return CSX_OK;
 }
 
+// save_mat (src/lib/csim_fits.cpp:480-493): BITPIX -64, NAXIS1 = columns, NAXIS2 =
+// rows, the data row by row (the transposed column-major matrix), big-endian, header
+// and data each padded to 2880-byte blocks. As write_fits_array (:101-109) does, an
+// existing file is replaced and spaces in the name become underscores.
+int write_fits_matrix(const std::string& name, const double* colmajor, int rows, int cols) {
+  if (rows <= 0 || cols <= 0 || !colmajor) return fail(CSX_ERR_ARG, "write_fits: empty matrix");
+  std::string path = name;
+  if (!path.empty() && path[0] == '!') path.erase(0, 1);
+  std::replace(path.begin(), path.end(), ' ', '_');
+  std::string h;
+  auto card = [&](const std::string& key, const std::string& val) {
+    char c[81];
+    std::snprintf(c, sizeof c, "%-8s= %20s", key.c_str(), val.c_str());
+    std::string line(c);
+    line.resize(80, ' ');
+    h += line;
+  };
+  card("SIMPLE", "T");
+  card("BITPIX", "-64");
+  card("NAXIS", "2");
+  card("NAXIS1", std::to_string(cols));
+  card("NAXIS2", std::to_string(rows));
+  card("EXTEND", "T");
+  std::string end = "END";
+  end.resize(80, ' ');
+  h += end;
+  h.resize((h.size() + 2879) / 2880 * 2880, ' ');
+  std::vector<unsigned char> d((size_t)rows * cols * 8);
+  for (int r = 0; r < rows; r++)
+    for (int c = 0; c < cols; c++) {
+      double v = colmajor[(size_t)c * rows + r];
+      unsigned char b[8];
+      std::memcpy(b, &v, 8);
+      for (int k = 0; k < 8; k++) d[((size_t)r * cols + c) * 8 + k] = b[7 - k];
+    }
+  d.resize((d.size() + 2879) / 2880 * 2880, 0);
+  std::ofstream f(path, std::ios::binary);
+  if (!f) return fail(CSX_ERR_ARG, "cannot create FITS file " + path);
+  f.write(h.data(), (std::streamsize)h.size());
+  f.write(reinterpret_cast<const char*>(d.data()), (std::streamsize)d.size());
+  if (!f) return fail(CSX_ERR_ARG, "short write to FITS file " + path);
+  return CSX_OK;
+}
+
 // FITS matrix -> column-major rows x cols (rows = NAXIS2, cols = NAXIS1), csim_fits.cpp:195-233.
-static int fits_matrix(const std::string& path, std::vector<double>& m, int& rows, int& cols, int& slices) {
+int fits_matrix(const std::string& path, std::vector<double>& m, int& rows, int& cols, int& slices) {
   std::vector<double> d;
   std::vector<long> dims;
   CSX_TRY(read_fits(path, d, dims));

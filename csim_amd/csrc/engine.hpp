@@ -229,6 +229,10 @@ class Chain {
 // different pupil sizes, e.g. 512^2 and 1024^2 DE candidates) propagated together:
 // every contraction stage of all chains is one grouped GEMM launch over uneven
 // problem sizes; the other stages run chain by chain.
+// FITS matrices in csim's convention (column-major rows x cols [x slices]).
+int fits_matrix(const std::string& path, std::vector<double>& m, int& rows, int& cols, int& slices);
+int write_fits_matrix(const std::string& path, const double* colmajor, int rows, int cols);
+
 class ChainGroup {
  public:
   explicit ChainGroup(std::vector<Chain*> chains) : ch_(std::move(chains)) {}

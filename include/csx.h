@@ -186,6 +186,14 @@ int csx_chain_set_graph(csx_chain* chain, int enable);
 int csx_chain_hex_response(csx_chain* chain, const char* element_name, const csx_region* region, void* d_M,
                            int* npix, int* ncol, int* nslices, double* calib_peak);
 
+/* FITS matrices in csim's convention (src/lib/csim_fits.cpp): csx_fits_read
+ * reads BITPIX 8/16/32/-32/-64 images (NAXIS1 = columns, NAXIS2 = rows, NAXIS3 =
+ * slices) into a column-major rows x cols x slices array (out NULL: only the
+ * sizes); csx_fits_write writes a column-major rows x cols matrix as save_mat does
+ * (:480-493: BITPIX -64, the transposed matrix, big-endian). */
+int csx_fits_read(const char* path, double* out, size_t capacity, int* rows, int* cols, int* slices);
+int csx_fits_write(const char* path, const double* colmajor, int rows, int cols);
+
 /* Region-pixel fields of every candidate after the last full propagation: the
  * fields linearOptimizer::compute_jacobian and compute_objective_vector read
  * (src/optimization/linearOptimizer.cpp:420-505).  d_out (device, complex128):
