@@ -722,205 +722,10 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
       }
 }
 
-// Persistent two-term kernel: one block per CU walks its tiles (an XCD-contiguous
-// range, stride = blocks on that XCD) and streams their K slabs through an S-slot
-// ring of LDS stages with direct-to-LDS loads, S - 1 slabs ahead and across tile
-// boundaries, so the next tile's first slabs are in flight during an epilogue.
-// Only loads are outstanding at the partial waits: after an epilogue (stores) the
-// first wait is a full one.  The epilogue stages the output planes through the slot
-// just consumed (not refilled until the next slab's barrier).
-constexpr int vmcnt_imm(int n) { return (n & 15) | ((n >> 4) << 14) | 0x0F70; }
-
-template <int BM, int BN, int WM, int WN, int S>
-__global__ __launch_bounds__(64 * WM * WN) void k_gemm_planes_ps(const GemmProb* __restrict__ probs, int nprob,
-                                                                 int total_tiles) {
-  typedef MfmaBf16<16> Mf;
-  typedef Mf::acc_t Acc;
-  constexpr int NV = Mf::NV, MF = 16, BK = 32, NP = 4, CH = BK / 8;
-  constexpr int NT = 64 * WM * WN, NW = WM * WN;
-  constexpr int IM = BM / WM / MF, JN = BN / WN / MF;
-  constexpr int PA = BM * BK, PB = BN * BK, STAGE = NP * (PA + PB);
-  constexpr int CA = NP * BM * CH, CB = NP * BN * CH, NI = (CA + CB) / NT;
-  static_assert(CA % (NW * 64) == 0 && CB % (NW * 64) == 0 && S >= 2 && S <= 4, "persistent shape");
-  static_assert(4 * BM * BN <= STAGE, "epilogue staging fits one slot");
-  __shared__ __align__(16) unsigned short lds[S * STAGE];
-
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int wm = wave / WN, wn = wave % WN;
-  // tiles of this block: XCD b % 8 owns a contiguous range, its blocks stride through it
-  // (with fewer than 8 blocks, one range per block)
-  const int G = gridDim.x, NX = G < 8 ? G : 8;
-  const int xcd = blockIdx.x % NX, jb = blockIdx.x / NX, nbx = (G - xcd + NX - 1) / NX;
-  const int tbeg = (int)((long)total_tiles * xcd / NX), tend = (int)((long)total_tiles * (xcd + 1) / NX);
-  auto tile_of = [&](int i) { const int tt = tbeg + jb + i * nbx; return tt < tend ? tt : -1; };
-
-  // producer: the tile and slab whose loads go out next
-  int p_i = 0, p_s = 0, p_nslab = 0, p_K = 0;
-  const GLOBAL unsigned short* pA = nullptr;
-  const GLOBAL unsigned short* pB = nullptr;
-  unsigned doff[NI];
-  int dk[NI];
-  bool p_valid = false;
-  auto p_setup = [&]() {
-    const int tt = tile_of(p_i);
-    p_valid = tt >= 0;
-    if (!p_valid) return;
-    const GemmProb& P = probs[find_prob(probs, nprob, tt)];
-    const int local = tt - P.tile_start, tm = local % P.tiles_m, tn = local / P.tiles_m;
-    const int m0 = tm * BM, n0 = tn * BN;
-    pA = gin<unsigned short>(P.A);
-    pB = gin<unsigned short>(P.B);
-    p_K = P.K;
-    p_nslab = (P.K + BK - 1) / BK;
-    p_s = 0;
-#pragma unroll
-    for (int i = 0; i < NI; i++) {
-      const int sl = (i * NW + wave) * 64 + lane;
-      const bool isA = sl < CA;
-      const int e = isA ? sl : sl - CA, rows = isA ? BM : BN;
-      const int q = e / (rows * CH), rem = e % (rows * CH), r = rem / CH, c = (rem % CH) ^ ((r >> 1) & 3);
-      dk[i] = c * 8;
-      doff[i] = isA ? (unsigned)(q * P.sA + (long)min(m0 + r, P.M - 1) * P.lda + c * 8)
-                    : (unsigned)(q * P.sB + (long)min(n0 + r, P.N - 1) * P.ldb + c * 8);
-    }
-  };
-  auto p_issue = [&](int slot) {
-    const int k0 = p_s * BK;
-#pragma unroll
-    for (int i = 0; i < NI; i++) {
-      const bool isA = (i * NW) * 64 < CA;
-      const GLOBAL unsigned short* src = (isA ? pA : pB) + doff[i] + k0;
-      const GLOBAL void* g = k0 + dk[i] < p_K ? (const GLOBAL void*)src : (const GLOBAL void*)kZeroChunk;
-      __attribute__((address_space(3))) void* l =
-          (__attribute__((address_space(3))) void*)(lds + slot * STAGE + (i * NW + wave) * 64 * 8);
-      __builtin_amdgcn_global_load_lds(g, l, 16, 0, 0);
-    }
-    if (++p_s == p_nslab) { p_i++; p_setup(); }
-  };
-
-  Acc accR[IM][JN], accI[IM][JN];
-  const int fr = Mf::frow(lane);
-  const u32x4 sgn = u32x4{0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
-  auto compute = [&](const unsigned short* base) {
-    const int fk = Mf::fchunk(lane, 0);
-    u32x4 b[NP][JN];
-#pragma unroll
-    for (int q = 0; q < NP; q++)
-#pragma unroll
-      for (int j = 0; j < JN; j++)
-        b[q][j] = *reinterpret_cast<const u32x4*>(base + NP * PA + q * PB + swz<BK>(wn * (BN / WN) + j * MF + fr, fk));
-#pragma unroll
-    for (int i = 0; i < IM; i++) {
-      u32x4 a[NP];
-#pragma unroll
-      for (int q = 0; q < NP; q++) a[q] = *reinterpret_cast<const u32x4*>(base + q * PA + swz<BK>(wm * (BM / WM) + i * MF + fr, fk));
-#pragma unroll
-      for (int ta = 0; ta < 2; ta++) {
-        const bf16x8 ar = __builtin_bit_cast(bf16x8, a[2 * ta]);
-        const bf16x8 ai = __builtin_bit_cast(bf16x8, a[2 * ta + 1]);
-        const bf16x8 ain = __builtin_bit_cast(bf16x8, a[2 * ta + 1] ^ sgn);
-#pragma unroll
-        for (int tb = 0; tb < 2 - ta; tb++) {
-#pragma unroll
-          for (int j = 0; j < JN; j++) {
-            accR[i][j] = Mf::mma(ar, __builtin_bit_cast(bf16x8, b[2 * tb][j]), accR[i][j]);
-            accI[i][j] = Mf::mma(ar, __builtin_bit_cast(bf16x8, b[2 * tb + 1][j]), accI[i][j]);
-          }
-#pragma unroll
-          for (int j = 0; j < JN; j++) {
-            accR[i][j] = Mf::mma(ain, __builtin_bit_cast(bf16x8, b[2 * tb + 1][j]), accR[i][j]);
-            accI[i][j] = Mf::mma(ai, __builtin_bit_cast(bf16x8, b[2 * tb][j]), accI[i][j]);
-          }
-        }
-      }
-    }
-  };
-
-  p_setup();
-  int issued = 0;
-  for (int i = 0; i < S - 1 && p_valid; i++) { p_issue(issued % S); issued++; }
-  int g = 0;                                     // slabs consumed
-  bool after_epilogue = false;
-  for (int c_i = 0;; c_i++) {
-    const int tt = tile_of(c_i);
-    if (tt < 0) break;
-    const GemmProb& P = probs[find_prob(probs, nprob, tt)];
-    const int local = tt - P.tile_start, tm = local % P.tiles_m, tn = local / P.tiles_m;
-    const int m0 = tm * BM, n0 = tn * BN, M = P.M, N = P.N;
-    const int nslab = (P.K + BK - 1) / BK;
-#pragma unroll
-    for (int i = 0; i < IM; i++)
-#pragma unroll
-      for (int j = 0; j < JN; j++)
-#pragma unroll
-        for (int v = 0; v < NV; v++) { accR[i][j][v] = 0.f; accI[i][j][v] = 0.f; }
-    for (int s = 0; s < nslab; s++, g++) {
-      // slab g has landed once at most (issued - g - 1) later slabs' loads are pending
-      const int ahead = issued - g - 1;
-      if (after_epilogue || ahead <= 0) __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
-      else if (ahead == 1) __builtin_amdgcn_s_waitcnt(vmcnt_imm(NI));
-      else __builtin_amdgcn_s_waitcnt(vmcnt_imm(2 * NI));
-      after_epilogue = false;
-      __syncthreads();                           // every wave's loads of slab g are in; slot (g-1) % S is free
-      if (p_valid) { p_issue(issued % S); issued++; }
-      compute(lds + (g % S) * STAGE);
-    }
-    // epilogue through the slot of the last slab
-    unsigned short* stg = lds + ((g - 1 + S) % S) * STAGE;
-    if (P.out_kind == OUT_PLANES && !P.transC && P.terms == 2) {
-      __syncthreads();                           // every wave is done reading that slot
-#pragma unroll
-      for (int i = 0; i < IM; i++)
-#pragma unroll
-        for (int j = 0; j < JN; j++)
-#pragma unroll
-          for (int v = 0; v < NV; v++) {
-            const int ml = wm * (BM / WM) + i * MF + Mf::drow(lane, v);
-            const int nl = wn * (BN / WN) + j * MF + Mf::dcol(lane);
-            double vr, vi;
-            epilogue_value(P, m0 + ml, n0 + nl, accR[i][j][v], accI[i][j][v], vr, vi);
-            const int pos = ml * BN + ((((nl >> 3) ^ (((ml >> 2) & 3) << 1))) << 3) + (nl & 7);
-            float xr = (float)vr, xi = (float)vi;
-#pragma unroll
-            for (int tt2 = 0; tt2 < 2; tt2++) {
-              const unsigned short hr = bf16_rne(xr), hi = bf16_rne(xi);
-              stg[(2 * tt2) * BM * BN + pos] = hr;
-              stg[(2 * tt2 + 1) * BM * BN + pos] = hi;
-              xr -= bf16_val(hr);
-              xi -= bf16_val(hi);
-            }
-          }
-      __syncthreads();
-      constexpr int CPR = BN / 8;
-      GLOBAL unsigned short* C = gout<unsigned short>(P.C);
-      for (int e = t; e < 4 * BM * CPR; e += NT) {
-        const int q = e / (BM * CPR), rem = e % (BM * CPR), ml = rem / CPR, cc = rem % CPR;
-        const int m = m0 + ml, n = n0 + cc * 8;
-        if (m < M && n < N)
-          *reinterpret_cast<GLOBAL u32x4*>(C + q * P.sC + (long)m * P.ldc + n) =
-              *reinterpret_cast<const u32x4*>(stg + q * BM * BN + ml * BN + ((cc ^ (((ml >> 2) & 3) << 1)) << 3));
-      }
-    } else {
-#pragma unroll
-      for (int i = 0; i < IM; i++)
-#pragma unroll
-        for (int j = 0; j < JN; j++)
-#pragma unroll
-          for (int v = 0; v < NV; v++) {
-            const int m = m0 + wm * (BM / WM) + i * MF + Mf::drow(lane, v);
-            const int n = n0 + wn * (BN / WN) + j * MF + Mf::dcol(lane);
-            epilogue_store(P, m, n, accR[i][j][v], accI[i][j][v]);
-          }
-    }
-    after_epilogue = true;
-  }
-}
-
 // Tile variants of each term count, in the order the tuner tries them.
 struct PlanesVariant {
   int bm, bn, threads;
   void (*kernel)(const GemmProb*, int);
-  void (*persistent)(const GemmProb*, int, int);   // one block per CU walking its tiles (no split K)
 };
 // (the last template argument is the minimum waves per SIMD the registers must allow)
 static const PlanesVariant kVariants2[] = {
@@ -938,8 +743,6 @@ static const PlanesVariant kVariants2[] = {
     {64, 64, 256, k_gemm_planes<2, 64, 64, 2, 2, true, 2, 32, 16, true>},
     {128, 128, 512, k_gemm_planes<2, 128, 128, 2, 4, true, 2, 32, 16, true>},
     {128, 64, 256, k_gemm_planes<2, 128, 64, 2, 2, true, 2, 32, 16, true>},
-    {64, 64, 256, nullptr, k_gemm_planes_ps<64, 64, 2, 2, 4>},
-    {64, 64, 512, nullptr, k_gemm_planes_ps<64, 64, 2, 4, 4>},
 };
 static const PlanesVariant kVariants3[] = {
     {64, 64, 256, k_gemm_planes<3, 64, 64, 2, 2, false, 1, 32, 16>},
@@ -956,21 +759,6 @@ int planes_variant_tile(int terms, int v, int* bm, int* bn) {
   const PlanesVariant& pv = terms == 2 ? kVariants2[v] : kVariants3[v];
   *bm = pv.bm; *bn = pv.bn;
   return CSX_OK;
-}
-bool planes_variant_splits_k(int terms, int v) {
-  if (v < 0 || v >= planes_variant_count(terms)) return false;
-  return (terms == 2 ? kVariants2[v] : kVariants3[v]).persistent == nullptr;
-}
-static int cu_count() {
-  static int n[64] = {0};
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
-  if (n[dev] == 0) {
-    int c = 0;
-    if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c <= 0) c = 256;
-    n[dev] = c;
-  }
-  return n[dev];
 }
 __global__ void k_splitk_finish(const GemmProb* __restrict__ probs, int nprob, long total) {
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
@@ -1004,12 +792,7 @@ int gemm_bf16planes(const GemmProb* probs_dev, int n, int total_tiles, int terms
   if (total_tiles <= 0) return CSX_OK;
   if (variant < 0 || variant >= planes_variant_count(terms)) return fail(CSX_ERR_ARG, "gemm_bf16planes: bad variant");
   const PlanesVariant& pv = terms == 2 ? kVariants2[variant] : kVariants3[variant];
-  if (pv.persistent) {
-    const int grid = std::min(total_tiles, cu_count());
-    hipLaunchKernelGGL(pv.persistent, dim3(grid), dim3(pv.threads), 0, s, probs_dev, n, total_tiles);
-  } else {
-    hipLaunchKernelGGL(pv.kernel, dim3(total_tiles), dim3(pv.threads), 0, s, probs_dev, n);
-  }
+  hipLaunchKernelGGL(pv.kernel, dim3(total_tiles), dim3(pv.threads), 0, s, probs_dev, n);
   CSX_HIP(hipGetLastError());
   return CSX_OK;
 }

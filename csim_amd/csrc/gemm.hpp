@@ -61,7 +61,6 @@ int gemm_f32(const GemmProb* probs_dev, int n, int total_tiles, hipStream_t s);
 // tile the problem list must be planned with).
 int planes_variant_count(int terms);
 int planes_variant_tile(int terms, int variant, int* bm, int* bn);
-bool planes_variant_splits_k(int terms, int variant);   // false: persistent variants (no split K)
 int gemm_bf16planes(const GemmProb* probs_dev, int n, int total_tiles, int terms, int variant, hipStream_t s);
 // complex128 -> `terms` BF16 planes per complex element (hi, mid, lo of re and im),
 // field f's planes at out + f * 2 * terms * npix, plane q at + q * npix.

@@ -119,7 +119,7 @@ static int choose_variant(std::vector<GemmProb>& v, csx_precision p, DevBuf& dev
   int forced = forced_variant(p);
   if (nv == 1 || forced >= 0) {
     *variant = nv == 1 ? 0 : forced;
-    *ksplit = std::getenv("CSX_GEMM_KSPLIT") && planes_variant_splits_k(terms_of(p), *variant) ? kcand : 1;
+    *ksplit = std::getenv("CSX_GEMM_KSPLIT") ? kcand : 1;
     set_split(v, *ksplit, ws.as<float>());
     return upload(v, p, *variant, dev, s, tiles);
   }
@@ -134,8 +134,7 @@ static int choose_variant(std::vector<GemmProb>& v, csx_precision p, DevBuf& dev
   std::vector<std::pair<int, int>> cand;
   for (int var = 0; var < nv; var++) cand.push_back({var, 1});
   if (kcand > 1)
-    for (int var = 0; var < nv; var++)
-      if (planes_variant_splits_k(terms_of(p), var)) cand.push_back({var, kcand});
+    for (int var = 0; var < nv; var++) cand.push_back({var, kcand});
   const long elems = total_elems(v);
   hipEvent_t e0, e1;
   CSX_HIP(hipEventCreate(&e0));

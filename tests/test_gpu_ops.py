@@ -83,7 +83,7 @@ def test_mft_mixed_error_level(ctx, prec, tol):
     assert e < tol
 
 
-VARIANTS = {"bf16x3": 16, "bf16x6": 3}
+VARIANTS = {"bf16x3": 14, "bf16x6": 3}
 MFMA32 = {"bf16x3": {7, 8, 9, 10}, "bf16x6": set()}   # variants on the 32x32x16 MFMA (gemm.hip)
 
 
