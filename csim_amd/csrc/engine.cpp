@@ -8,6 +8,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <sstream>
@@ -317,6 +318,11 @@ struct RealMap {
   std::vector<double> host;
   int rows = 0, cols = 0;
   std::map<long, DevBuf> dev;   // keyed by rows*65536+cols of the grid it is resampled to
+  std::map<long, std::shared_ptr<const Support>> sup;   // its zero pattern on that grid
+  std::shared_ptr<const Support> support_for(const Geom& cur) const {
+    auto it = sup.find((long)cur.ny * 65536 + cur.nx);
+    return it == sup.end() ? nullptr : it->second;
+  }
   int device_for(Chain& ch, const Geom& cur, const double** out) {
     long key = (long)cur.ny * 65536 + cur.nx;
     auto it = dev.find(key);
@@ -356,6 +362,7 @@ struct RealMap {
       DevBuf b;
       CSX_TRY(b.reserve(m.size() * sizeof(double)));
       CSX_HIP(hipMemcpy(b.p, m.data(), m.size() * sizeof(double), hipMemcpyHostToDevice));
+      sup[key] = Support::of_mask(m.data(), cur.ny, cur.nx);
       it = dev.emplace(key, std::move(b)).first;
     }
     *out = it->second.as<double>();
@@ -366,6 +373,7 @@ struct RealMap {
     host.assign(d, d + (size_t)r * c);
     rows = r; cols = c;
     dev.clear();
+    sup.clear();
     return CSX_OK;
   }
   int load(const std::string& path) {
@@ -389,11 +397,14 @@ struct MaskElem : Element {
     ch.invalidate();
     return map.set(d, r, c);
   }
+  bool keeps_support() const override { return true; }
   int compile(Chain& ch, Compile& c, int) override {
     if (map.host.empty()) return fail(CSX_ERR_STATE, "mask '" + name + "' has no data");
     const double* m = nullptr;
     CSX_TRY(map.device_for(ch, c.geom, &m));
-    return push_pointwise(ch, c, PwOp{PW_MASK_1PI, m, 0.0, 1});
+    CSX_TRY(push_pointwise(ch, c, PwOp{PW_MASK_1PI, m, 0.0, 1}));
+    c.support = Support::intersect(c.support, map.support_for(c.geom));   // E (1+i) m is zero where m is
+    return CSX_OK;
   }
 };
 
@@ -417,6 +428,7 @@ struct MirrorElem : Element {
     ch.invalidate();
     return map.set(d, r, c);
   }
+  bool keeps_support() const override { return true; }   // a phase factor
   int compile(Chain& ch, Compile& c, int) override {
     if (map.host.empty()) return fail(CSX_ERR_STATE, "mirror '" + name + "' has no data");
     const double* h = nullptr;
@@ -430,6 +442,7 @@ struct ComplexMaskElem : Element {
   std::vector<double2> cube;   // rows x cols x slices (amp e^{-i ph})
   int rows = 0, cols = 0, slices = 0, maskIndex = 0;
   DevBuf dev;
+  bool keeps_support() const override { return true; }   // a pointwise factor
   int set(Chain&, const std::string& k, const std::string& a) override {
     if (set_common(k, a) || k == "complexMask") return CSX_OK;
     if (k == "maskIndex") { maskIndex = std::atoi(a.c_str()); return CSX_OK; }
@@ -584,7 +597,8 @@ struct DownsampleElem : Element {
 static int compile_mft(Chain& ch, Compile& c, const std::vector<Geom>& gout_per_lam,
                        const std::vector<double>& lf_per_lam, const std::string& label,
                        const std::vector<MftField>* masks = nullptr, bool out_conv = false,
-                       const double* epi_mask_1pi = nullptr, bool expand = false) {
+                       const double* epi_mask_1pi = nullptr, bool expand = false,
+                       std::shared_ptr<const Support> epi_sup = nullptr) {
   CSX_TRY(flush_pointwise(ch, c, /*to_conv=*/true));
   // expand: the fields are one candidate's copy; the MFT writes every candidate's
   // output (its own mask each) from that copy, whose first contraction the plan
@@ -602,6 +616,9 @@ static int compile_mft(Chain& ch, Compile& c, const std::vector<Geom>& gout_per_
   const size_t ein = T ? (size_t)4 * T : sizeof(float2);
   const int wkind = T ? FK_PLANES : (c64 ? FK_C64 : FK_C128);
   const bool from_conv = c.in_conv;
+  // CSX_ZERO_SKIP=0 turns the zero-pattern tables off (the dense contractions)
+  const char* zs = std::getenv("CSX_ZERO_SKIP");
+  const bool zero_skip = !(zs && zs[0] == '0');
   std::vector<MftField> F(nf);
   const Geom& g0 = gout_per_lam[0];
   for (int f = 0; f < nf; f++) {
@@ -621,6 +638,7 @@ static int compile_mft(Chain& ch, Compile& c, const std::vector<Geom>& gout_per_
     }
     m.in_x = gx(gin); m.in_y = gy(gin); m.out_x = gx(go); m.out_y = gy(go);
     m.lambda_f = lf_per_lam[l];
+    if (zero_skip) m.in_sup = c.support;
     if (masks) {
       const MftField& mk = (*masks)[f];
       m.mask_kind = mk.mask_kind; m.mask = mk.mask;
@@ -628,6 +646,7 @@ static int compile_mft(Chain& ch, Compile& c, const std::vector<Geom>& gout_per_
     } else if (epi_mask_1pi) {
       m.mask_kind = MASK_REAL_1PI; m.mask = epi_mask_1pi;
       m.mrows = go.ny; m.mcols = go.nx; m.row0 = 0; m.col0 = 0;
+      if (zero_skip) m.out_sup = epi_sup;
     }
   }
   auto plan = std::make_unique<MftPlan>();
@@ -664,6 +683,7 @@ static int compile_mft(Chain& ch, Compile& c, const std::vector<Geom>& gout_per_
     c.buf = 1 - c.buf;
   }
   c.geom = g0;
+  c.support = epi_mask_1pi ? epi_sup : nullptr;
   return CSX_OK;
 }
 
@@ -684,6 +704,7 @@ struct FraunhoferFocalElem : Element {
     return geom_symmetric(N, N, scale / samplesPerflD, scale / samplesPerflD);
   }
   int geometry(Chain& ch, const Geom&, Geom& out) override { out = out_geom(ch); return CSX_OK; }
+  bool keeps_support() const override { return true; }
   int compile(Chain& ch, Compile& c, int) override {
     if (samplesPerflD <= 0 || FOVflD <= 0) return fail(CSX_ERR_ARG, "fraunhoferFocal: samplesPerflD/FOVflD unset");
     const Efield& ef = ch.efield();
@@ -724,6 +745,7 @@ struct ZoomFftPropElem : Element {
     double lf;
     return out_geom(ch, in, out, lf);
   }
+  bool keeps_support() const override { return true; }
   int compile(Chain& ch, Compile& c, int) override {
     if (skipped(c)) return CSX_OK;
     Geom out;
@@ -904,6 +926,7 @@ struct DoubleFraunhoferElem : Element {
     if (has_hex) { hex.zoomFactor = zoomFactor; CSX_TRY(hex.init(ch.efield())); }
     return CSX_OK;
   }
+  bool keeps_support() const override { return true; }
   int compile(Chain& ch, Compile& c, int index) override {
     if (skipped(c)) return CSX_OK;
     // the element holding the candidate sags expands the shared fields to every candidate
@@ -999,18 +1022,20 @@ struct DoubleFraunhoferElem : Element {
     CSX_TRY(compile_mft(ch, c, gf, lf, type + " in " + name, &mk, /*out_conv=*/true, nullptr, expand));
     // a mask element right after folds into the epilogue of the return MFT
     const double* lyot = nullptr;
+    std::shared_ptr<const Support> lyot_sup;
     int claim = 0;
     if (Element* nx = ch.elem_at(index + 1)) {
       if (auto* me = dynamic_cast<MaskElem*>(nx)) {
         if (!me->skipped(c) && !me->map.host.empty()) {
           CSX_TRY(me->map.device_for(ch, E, &lyot));
+          lyot_sup = me->map.support_for(E);
           claim = 1;
         }
       }
     }
     bool next_mft = is_mft_element(ch.elem_at(index + 1 + claim), c);
     std::vector<Geom> back(nl, E);
-    CSX_TRY(compile_mft(ch, c, back, lfo, type + " out " + name, nullptr, next_mft, lyot));
+    CSX_TRY(compile_mft(ch, c, back, lfo, type + " out " + name, nullptr, next_mft, lyot, false, lyot_sup));
     c.claimed = claim;
     return CSX_OK;
   }
@@ -1626,10 +1651,12 @@ int Chain::compile_one(bool calibrating, Compile& c) {
   }
   c.in_conv = false;
   c.claimed = 0;
+  c.support.reset();
   for (int i = 0; i < (int)elems_.size(); i++) {
     Element* e = elems_[i].get();
     if (c.claimed > 0) { c.claimed--; continue; }
     if (e->skipped(c) && !e->calibration_aware()) continue;
+    if (!e->keeps_support()) c.support.reset();
     CSX_TRY(e->compile(*this, c, i));
   }
   CSX_TRY(flush_pointwise(*this, c));

@@ -81,6 +81,9 @@ struct Compile {
   // candidate sags every candidate's fields are identical, so they are carried once
   // (live = 1) and that element expands them to every candidate.
   int live = 0;
+  // zero pattern of the current fields (null: dense): masks narrow it, an MFT reads
+  // it for its input and leaves its epilogue mask's; every other element clears it
+  std::shared_ptr<const Support> support;
 };
 
 // ---------------------------------------------------------------- elements --
@@ -105,6 +108,9 @@ struct Element {
   // true: compile() is called for the calibration run even when skipped(c) (the
   // element then applies its own calibration behaviour)
   virtual bool calibration_aware() const { return false; }
+  // true: compile() keeps Compile::support valid for the fields it leaves (a
+  // pointwise factor, or an MFT that reads and resets it); false: it is cleared first
+  virtual bool keeps_support() const { return false; }
 };
 
 class Chain {

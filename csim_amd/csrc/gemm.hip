@@ -47,6 +47,26 @@ __device__ __forceinline__ int find_prob(const GemmProb* __restrict__ probs, int
   return lo;
 }
 
+// K range of a BM x BN tile from the problem's block table (GemmProb::kr): whole
+// slabs of BK from a multiple of BK, so the slab grid, and with it the order of
+// every sum, stays that of the full range; only products that are exact zeros drop.
+template <int BM, int BN, int BK>
+__device__ __forceinline__ void tile_krange(const GemmProb& P, int m0, int n0, int& kbeg, int& kend) {
+  if (!P.kr) return;
+  const GLOBAL int* kr = gin<int>(P.kr);
+  int lo = P.K, hi = 0;
+  const int mb1 = min((m0 + BM + kKrBlock - 1) / kKrBlock, (P.M + kKrBlock - 1) / kKrBlock);
+  const int nb1 = min((n0 + BN + kKrBlock - 1) / kKrBlock, P.kr_ld);
+  for (int mb = m0 / kKrBlock; mb < mb1; mb++)
+    for (int nb = n0 / kKrBlock; nb < nb1; nb++) {
+      const long b = 2 * ((long)mb * P.kr_ld + nb);
+      lo = min(lo, kr[b]);
+      hi = max(hi, kr[b + 1]);
+    }
+  kbeg = max(kbeg, lo / BK * BK);
+  kend = max(kbeg, min(kend, hi));
+}
+
 // Epilogue: alpha, mask (epilogue_value), store (shared by every kernel).
 __device__ __forceinline__ void epilogue_value(const GemmProb& P, int m, int n, double re, double im, double& out_r,
                                                double& out_i) {
@@ -140,6 +160,8 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const GemmProb* __restrict__ pr
 #pragma unroll
     for (int j = 0; j < 2; j++) { accR[i][j] = V4{0, 0, 0, 0}; accI[i][j] = V4{0, 0, 0, 0}; }
 
+  int kbeg = 0, kend = K;
+  tile_krange<kTileM, kTileN, BK>(P, m0, n0, kbeg, kend);
   C2 ra[4], rb[4];
   // unconditional loads from clamped (in-bounds) addresses, zeroed by select
   const int am = m0 + lrow, bn = n0 + lrow;
@@ -154,8 +176,8 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const GemmProb* __restrict__ pr
       rb[q] = B[brow + kc];
     }
   };
-  gload(0);
-  for (int k0 = 0; k0 < K; k0 += BK) {
+  gload(kbeg);
+  for (int k0 = kbeg; k0 < kend; k0 += BK) {
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < 4; q++) {
@@ -164,7 +186,7 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const GemmProb* __restrict__ pr
       Bs[lrow][lk + q] = (bn < N && kin) ? rb[q] : C2{0, 0};
     }
     __syncthreads();
-    if (k0 + BK < K) gload(k0 + BK);
+    if (k0 + BK < kend) gload(k0 + BK);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       C2 a[2], b[2];
@@ -265,13 +287,15 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_f64db(const GemmProb* __r
       for (int q = 0; q < 4; q++) Bs[buf][r][kq + q] = (rin && k0 + kq + q < K) ? rb[c][q] : z;
     }
   };
-  const int nslab = (K + BK - 1) / BK;
-  gload(0);
-  sstore(0, 0);
+  int kbeg = 0, kend = K;
+  tile_krange<BM, BN, BK>(P, m0, n0, kbeg, kend);
+  const int nslab = (kend - kbeg + BK - 1) / BK;
+  gload(kbeg);
+  sstore(0, kbeg);
   __syncthreads();
   for (int sl = 0; sl < nslab; sl++) {
     const int cur = sl & 1;
-    if (sl + 1 < nslab) gload((sl + 1) * BK);
+    if (sl + 1 < nslab) gload(kbeg + (sl + 1) * BK);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       f64x2 a[IM], b[JN];
@@ -294,7 +318,7 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_f64db(const GemmProb* __r
           accI[i][j] = Mma<double>::mma(a[i].y, b[j].x, accI[i][j]);
         }
     }
-    if (sl + 1 < nslab) sstore(cur ^ 1, (sl + 1) * BK);
+    if (sl + 1 < nslab) sstore(cur ^ 1, kbeg + (sl + 1) * BK);
     __syncthreads();
   }
 #pragma unroll
@@ -432,6 +456,7 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
     kbeg = min(K, ks * kc);
     kend = min(K, kbeg + kc);
   }
+  tile_krange<BM, BN, BK>(P, m0, n0, kbeg, kend);
 
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int wm = wave / WN, wn = wave % WN;

@@ -37,7 +37,15 @@ struct GemmProb {
   int ksplit;
   float* ws;
   long elem_start;     // first output element of this problem among the launch's (finish kernel)
+  // optional K range per 64 x 64 block of outputs: int pair [lo, hi) at
+  // kr[2 * (mb * kr_ld + nb)] bounds the nonzero products of every output of block
+  // (mb, nb); a tile runs the whole K slabs of the union over its blocks (the
+  // products it skips are exact zeros).  lo >= hi: the block's outputs are zero.
+  const int* kr;
+  int kr_ld;
+  unsigned kr_tag;     // digest of the table (tile tuning keys on it)
 };
+constexpr int kKrBlock = 64;
 
 // Tile of the FP64 / FP32 MFMA kernels (64 x 64 outputs, 4 waves of 32 x 32).
 constexpr int kTileM = 64, kTileN = 64;

@@ -13,6 +13,83 @@ namespace csx {
 
 int terms_of(csx_precision p) { return p == CSX_BF16X3 ? 2 : (p == CSX_BF16X6 ? 3 : 0); }
 
+// ------------------------------------------------------------- zero patterns --
+std::shared_ptr<const Support> Support::of_mask(const double* m, int ny, int nx) {
+  auto s = std::make_shared<Support>();
+  s->ny = ny; s->nx = nx;
+  s->lo.assign(nx, ny);
+  s->hi.assign(nx, -1);
+  for (int x = 0; x < nx; x++) {
+    const double* c = m + (size_t)x * ny;
+    int lo = 0, hi = ny - 1;
+    while (lo < ny && c[lo] == 0.0) lo++;
+    while (hi >= lo && c[hi] == 0.0) hi--;
+    if (lo <= hi) { s->lo[x] = lo; s->hi[x] = hi; }
+  }
+  return s;
+}
+bool Support::dense() const {
+  for (int x = 0; x < nx; x++)
+    if (lo[x] != 0 || hi[x] != ny - 1) return false;
+  return true;
+}
+std::shared_ptr<const Support> Support::intersect(const std::shared_ptr<const Support>& a,
+                                                  const std::shared_ptr<const Support>& b) {
+  if (!a) return b;
+  if (!b) return a;
+  if (a->ny != b->ny || a->nx != b->nx) return nullptr;   // unrelated grids: assume nothing
+  auto s = std::make_shared<Support>(*a);
+  for (int x = 0; x < s->nx; x++) {
+    s->lo[x] = std::max(a->lo[x], b->lo[x]);
+    s->hi[x] = std::min(a->hi[x], b->hi[x]);
+    if (s->lo[x] > s->hi[x]) { s->lo[x] = s->ny; s->hi[x] = -1; }
+  }
+  return s;
+}
+
+// K range tables (GemmProb::kr).  Stage 1 (M = out_y, N = in_x, K = in_y): the
+// input rows that may be nonzero in the columns of each N block.  Stage 2 (M =
+// out_x, N = out_y, K = in_x): the input columns holding any nonzero; a block whose
+// outputs the epilogue mask zeroes takes no K at all.  Empty when nothing is skipped.
+static std::vector<int> stage1_ranges(const MftField& F) {
+  const Support* s = F.in_sup.get();
+  if (!s || s->ny != F.in_y.n || s->nx != F.in_x.n || s->dense()) return {};
+  const int M = F.out_y.n, N = F.in_x.n, K = F.in_y.n;
+  const int mb = (M + kKrBlock - 1) / kKrBlock, nb = (N + kKrBlock - 1) / kKrBlock;
+  std::vector<int> r((size_t)2 * mb * nb);
+  for (int j = 0; j < nb; j++) {
+    int lo = K, hi = 0;
+    for (int x = j * kKrBlock; x < std::min(N, (j + 1) * kKrBlock); x++)
+      if (s->lo[x] <= s->hi[x]) { lo = std::min(lo, s->lo[x]); hi = std::max(hi, s->hi[x] + 1); }
+    for (int i = 0; i < mb; i++) { r[2 * ((size_t)i * nb + j)] = lo; r[2 * ((size_t)i * nb + j) + 1] = hi; }
+  }
+  return r;
+}
+static std::vector<int> stage2_ranges(const MftField& F) {
+  const int M = F.out_x.n, N = F.out_y.n, K = F.in_x.n;
+  int klo = 0, khi = K;
+  const Support* si = F.in_sup.get();
+  if (si && si->ny == F.in_y.n && si->nx == K) {
+    klo = K; khi = 0;
+    for (int x = 0; x < K; x++)
+      if (si->lo[x] <= si->hi[x]) { klo = std::min(klo, x); khi = x + 1; }
+  }
+  const Support* so = F.out_sup.get();
+  const bool use_out = so && so->ny == N && so->nx == M;
+  if (!use_out && klo == 0 && khi == K) return {};
+  const int mb = (M + kKrBlock - 1) / kKrBlock, nb = (N + kKrBlock - 1) / kKrBlock;
+  std::vector<int> r((size_t)2 * mb * nb);
+  for (int i = 0; i < mb; i++)
+    for (int j = 0; j < nb; j++) {
+      bool live = !use_out;
+      for (int x = i * kKrBlock; !live && x < std::min(M, (i + 1) * kKrBlock); x++)
+        live = so->lo[x] <= so->hi[x] && so->lo[x] < (j + 1) * kKrBlock && so->hi[x] >= j * kKrBlock;
+      r[2 * ((size_t)i * nb + j)] = live ? klo : K;
+      r[2 * ((size_t)i * nb + j) + 1] = live ? khi : 0;
+    }
+  return r;
+}
+
 int launch_gemm(const GemmProb* probs_dev, int n, int tiles, csx_precision p, hipStream_t s, int variant) {
   switch (p) {
     case CSX_FP64: return gemm_f64(probs_dev, n, tiles, s, variant);
@@ -48,6 +125,7 @@ static std::string signature(const std::vector<GemmProb>& v, csx_precision p) {
   for (const auto& g : v) {
     s += ':';
     s += std::to_string(g.M) + 'x' + std::to_string(g.N) + 'x' + std::to_string(g.K);
+    if (g.kr) s += 'r' + std::to_string(g.kr_tag);
   }
   return s;
 }
@@ -329,6 +407,60 @@ int MftPlan::build(const std::vector<MftField>& fields, csx_precision p, hipStre
     bytes2_ += (double)esz * F.out_y.n * F.in_x.n;      // each T read once
   }
   flops_ = flops1_ + flops2_;
+  // K range tables, one copy of each distinct table
+  {
+    std::map<std::vector<int>, size_t> tindex;
+    std::map<size_t, unsigned> tag;
+    std::vector<int> all;
+    std::vector<long> off1(nt_, -1), off2(nf_, -1);
+    auto place = [&](std::vector<int>&& t) -> long {
+      if (t.empty()) return -1;
+      auto it = tindex.find(t);
+      if (it != tindex.end()) return (long)it->second;
+      size_t o = all.size();
+      unsigned h = 2166136261u;                    // FNV-1a over the entries
+      for (int v : t) h = (h ^ (unsigned)v) * 16777619u;
+      tag[o] = h;
+      all.insert(all.end(), t.begin(), t.end());
+      tindex.emplace(std::move(t), o);
+      return (long)o;
+    };
+    // products each output needs (its block's K range): the flop count of a stage
+    auto needed = [](const std::vector<int>& t, int M, int N) {
+      const int nb = (N + kKrBlock - 1) / kKrBlock;
+      double w = 0;
+      for (size_t b = 0; 2 * b < t.size(); b++) {
+        const int i = (int)(b / nb), j = (int)(b % nb);
+        const int rows = std::min(kKrBlock, M - i * kKrBlock), cols = std::min(kKrBlock, N - j * kKrBlock);
+        w += (double)rows * cols * std::max(0, t[2 * b + 1] - t[2 * b]);
+      }
+      return 8.0 * w;
+    };
+    for (int u = 0; u < nt_; u++) {
+      std::vector<int> t = stage1_ranges(fields[first[u]]);
+      if (!t.empty()) flops1_ += needed(t, p1[u].M, p1[u].N) - 8.0 * p1[u].M * p1[u].N * p1[u].K;
+      off1[u] = place(std::move(t));
+    }
+    for (int f = 0; f < nf_; f++) {
+      std::vector<int> t = stage2_ranges(fields[f]);
+      if (!t.empty()) flops2_ += needed(t, p2[f].M, p2[f].N) - 8.0 * p2[f].M * p2[f].N * p2[f].K;
+      off2[f] = place(std::move(t));
+    }
+    flops_ = flops1_ + flops2_;
+    if (!all.empty()) {
+      CSX_TRY(kr_.reserve(all.size() * sizeof(int)));
+      CSX_HIP(hipMemcpyAsync(kr_.p, all.data(), all.size() * sizeof(int), hipMemcpyHostToDevice, s));
+      CSX_HIP(hipStreamSynchronize(s));   // `all` is a host temporary
+      auto bind = [&](GemmProb& g, long off) {
+        if (off < 0) return;
+        g.kr = kr_.as<int>() + off;
+        g.kr_ld = (g.N + kKrBlock - 1) / kKrBlock;
+        g.kr_tag = tag[(size_t)off];
+      };
+      for (int u = 0; u < nt_; u++) bind(p1[u], off1[u]);
+      for (int f = 0; f < nf_; f++) bind(p2[f], off2[f]);
+    }
+  }
   // DFT kernels are read once per launch as well (each distinct one)
   {
     std::set<size_t> ay(offAy.begin(), offAy.end()), bx(offBx.begin(), offBx.end());

@@ -2,6 +2,7 @@
 // two grouped complex GEMMs over a whole batch of fields.
 #pragma once
 #include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -42,6 +43,19 @@ class KernelTimer {
 enum FieldKind : int { FK_C128 = 0, FK_C64 = 1, FK_PLANES = 2 };
 int terms_of(csx_precision p);   // BF16 terms per value (0 for FP64 / FP32)
 
+// Zero pattern of a column-major ny x nx array: per column x, the rows [lo, hi]
+// outside which it holds exact zeros (lo > hi: the column is zero).  Masks give it
+// to the fields they multiply; the contractions skip the K slabs it leaves zero.
+struct Support {
+  int ny = 0, nx = 0;
+  std::vector<int> lo, hi;
+  static std::shared_ptr<const Support> of_mask(const double* colmajor, int ny, int nx);
+  // rows of b within those of a (the fields a masked by b); null is dense
+  static std::shared_ptr<const Support> intersect(const std::shared_ptr<const Support>& a,
+                                                  const std::shared_ptr<const Support>& b);
+  bool dense() const;
+};
+
 struct MftField {
   const void* in = nullptr;    // FieldKind in_kind: complex128, complex64 or BF16 planes
   void* out = nullptr;         // FieldKind out_kind
@@ -52,6 +66,9 @@ struct MftField {
   int mask_kind = MASK_NONE;
   const void* mask = nullptr;  // complex128 / double, column-major mrows x mcols
   int mrows = 0, mcols = 0, row0 = 0, col0 = 0;
+  // optional zero patterns (sizes in_y x in_x and out_y x out_x; null: dense): the
+  // input field's, and the output's when the epilogue mask zeroes it there
+  std::shared_ptr<const Support> in_sup, out_sup;
 };
 
 class MftPlan {
@@ -71,7 +88,7 @@ class MftPlan {
 
  private:
   csx_precision prec_ = CSX_FP64;
-  DevBuf dft_, tbuf_, probs1_, probs2_, specs_, ws1_, ws2_;
+  DevBuf dft_, tbuf_, probs1_, probs2_, specs_, ws1_, ws2_, kr_;
   int nf_ = 0, nt_ = 0, tiles1_ = 0, tiles2_ = 0;   // fields, distinct first contractions
   int var1_ = 0, var2_ = 0;   // tile variant of each stage (index into the precision's list)
   int ks1_ = 1, ks2_ = 1;     // K splits of each stage (split-K workspace ws1_/ws2_)

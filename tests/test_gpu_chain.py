@@ -316,3 +316,30 @@ def test_hex_response_matrix_reproduces_propagation(ctx, kind):
     lin = d_o.cpu().numpy()
     print(kind, "propagated", c, "linear", lin)
     assert np.max(np.abs(lin - c) / c) < 1e-11
+
+
+@pytest.mark.parametrize("precision", ["fp64", "bf16x3"])
+def test_zero_pattern_skipping_is_exact(ctx, precision, monkeypatch):
+    """The pupil and Lyot masks zero whole rows and blocks of the fields the MFTs
+    contract; the contractions skip those K slabs and the Lyot-zeroed output blocks.
+    On one tile variant the fields equal the dense contractions' exactly, and the
+    counted (needed) flops drop."""
+    monkeypatch.setenv("CSX_GEMM_VARIANT", "0")
+    w = W.workload2(n_lambda=3, offsets=(0.0, 2.0))
+    runs = {}
+    for skip in ("1", "0"):
+        monkeypatch.setenv("CSX_ZERO_SKIP", skip)
+        ch = build(ctx, w, precision)
+        ch.set_profiling(True)
+        ch.propagate(False)
+        prof = ch.profile()
+        flops = sum(v["flops_total"] for k, v in prof.items() if k.startswith("gemm"))
+        fields = [ch.field(0, s, li) for s in range(2) for li in range(3)]
+        runs[skip] = (flops, fields)
+        ch.close()
+    (f_skip, e_skip), (f_dense, e_dense) = runs["1"], runs["0"]
+    print(f"{precision}: gemm flops with zero skipping {f_skip:.4e}, dense {f_dense:.4e} "
+          f"({f_skip / f_dense:.3f})")
+    assert 0.5 < f_skip / f_dense < 0.97
+    for a, b in zip(e_skip, e_dense):
+        assert np.array_equal(a, b)
