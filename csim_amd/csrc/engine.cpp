@@ -603,8 +603,10 @@ static int compile_mft(Chain& ch, Compile& c, const std::vector<Geom>& gout_per_
   // expand: the fields are one candidate's copy; the MFT writes every candidate's
   // output (its own mask each) from that copy, whose first contraction the plan
   // then computes once
-  const int nf_in = ch.nfield_for(c), nl = ch.nlam();
-  const int nf = expand ? ch.nfield_all(c) : nf_in;
+  const int nf_in = ch.nfield_for(c), nl = ch.nlam(), ns = ch.nsrc();
+  const int nf = ch.nfield_out(c, expand);
+  const int ns_in = c.live_src ? 1 : ns;
+  const Efield& ef = ch.efield();
   const Geom gin = c.geom;
   const int T = terms_of(ch.precision());
   const bool c64 = ch.precision() == CSX_FP32;
@@ -626,7 +628,17 @@ static int compile_mft(Chain& ch, Compile& c, const std::vector<Geom>& gout_per_
     const Geom& go = gout_per_lam[l];
     if (go.nx != g0.nx || go.ny != g0.ny) return fail(CSX_ERR_UNSUPPORTED, label + ": output size varies with lambda");
     MftField& m = F[f];
-    const size_t fi = (size_t)(f % nf_in);
+    // input field of output f = (candidate, source, lambda): the one shared copy of
+    // the candidates and of the sources where the buffers carry one
+    const int s = (f / nl) % ns, cand = f / (nl * ns);
+    const size_t fi = (size_t)(((c.live > 0 ? 0 : cand) * ns_in + (c.live_src ? 0 : s)) * nl + l);
+    if (fi >= (size_t)nf_in) return fail(CSX_ERR_STATE, label + ": input field out of range");
+    if (c.live_src) {   // efield::init_point_sources (src/data/efield.cpp:179-206) folded in
+      const double Lx = ef.pixelScale * gin.nx, Ly = ef.pixelScale * gin.ny;
+      m.in_scale = ef.sources[s].flux;
+      m.tilt_x = 2 * ef.sources[s].tipX / Lx;
+      m.tilt_y = 2 * ef.sources[s].tiltY / Ly;
+    }
     m.in = mixed ? (const void*)(conv + fi * gin.npix() * ein) : (const void*)(in + fi * gin.npix());
     m.in_kind = wkind;
     if (out_conv) {
@@ -656,6 +668,7 @@ static int compile_mft(Chain& ch, Compile& c, const std::vector<Geom>& gout_per_
   size_t nconv = (size_t)nf_in * gin.npix();
   long npix_in = gin.npix();
   if (expand) c.live = 0;
+  c.live_src = 0;
   Chain* chp = &ch;
   if (!from_conv && (c64 || T)) {   // operands to the working precision (no fused producer)
     c.stages.push_back([=](hipStream_t s) {
@@ -933,7 +946,7 @@ struct DoubleFraunhoferElem : Element {
     const double* cs_dev = nullptr;
     int cs_nhex = 0, cs_ncand = 0;
     const bool expand = c.live > 0 && ch.candidate_sags(name, &cs_dev, &cs_nhex, &cs_ncand) == CSX_OK;
-    const int nl = ch.nlam(), nf = expand ? ch.nfield_all(c) : ch.nfield_for(c), ncand = ch.ncand_for(c);
+    const int nl = ch.nlam(), nf = ch.nfield_out(c, expand), ncand = ch.ncand_for(c);
     const Geom E = c.geom;
     std::vector<Geom> gf(nl);
     std::vector<double> lf(nl), lfo(nl);
@@ -1615,6 +1628,17 @@ int Chain::compile_one(bool calibrating, Compile& c) {
   if (!calibrating && ncand() > 1)
     for (auto& e : elems_)
       if (e->name == cand_elem_ && !e->skipped(c)) c.live = 1;
+  // several point sources and nothing but masks and mirrors before the first MFT:
+  // the buffers carry one unit source up to it and it applies every source
+  // (CSX_SOURCE_FOLD=0 generates every source's field instead)
+  c.live_src = 0;
+  const char* sf = std::getenv("CSX_SOURCE_FOLD");
+  if (!calibrating && ef_.fields.empty() && nsrc() > 1 && !(sf && sf[0] == '0'))
+    for (auto& e : elems_) {
+      if (dynamic_cast<MaskElem*>(e.get()) || dynamic_cast<MirrorElem*>(e.get())) continue;
+      if (is_mft_element(e.get(), c)) c.live_src = 1;
+      break;
+    }
   // initial fields (efield::init_point_sources / init / filename slices)
   const int nf = nfield_for(c), nl = nlam(), ns = nsrc();
   const long npix = c.geom.npix();
@@ -1624,7 +1648,8 @@ int Chain::compile_one(bool calibrating, Compile& c) {
   std::vector<double> p((size_t)3 * nf);
   for (int f = 0; f < nf; f++) {
     int s = (f / nl) % ns;
-    if (!ef_.sources.empty()) { p[3 * f] = ef_.sources[s].flux; p[3 * f + 1] = ef_.sources[s].tipX; p[3 * f + 2] = ef_.sources[s].tiltY; }
+    if (c.live_src) { p[3 * f] = 1.0; p[3 * f + 1] = 0; p[3 * f + 2] = 0; }
+    else if (!ef_.sources.empty()) { p[3 * f] = ef_.sources[s].flux; p[3 * f + 1] = ef_.sources[s].tipX; p[3 * f + 2] = ef_.sources[s].tiltY; }
     else { p[3 * f] = ef_.constant ? ef_.constantValue : 1.0; p[3 * f + 1] = 0; p[3 * f + 2] = 0; }
   }
   CSX_TRY(prm.reserve(sizeof(double) * p.size()));
@@ -1661,6 +1686,7 @@ int Chain::compile_one(bool calibrating, Compile& c) {
   }
   CSX_TRY(flush_pointwise(*this, c));
   if (c.in_conv) return fail(CSX_ERR_STATE, "chain ends on converted fields");
+  if (c.live_src) return fail(CSX_ERR_STATE, "point sources were not applied");
   CSX_TRY(expand_candidates(*this, c));
   return CSX_OK;
 }

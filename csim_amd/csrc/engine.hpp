@@ -81,6 +81,9 @@ struct Compile {
   // candidate sags every candidate's fields are identical, so they are carried once
   // (live = 1) and that element expands them to every candidate.
   int live = 0;
+  // 1: the buffers hold one unit point source per (candidate copy, wavelength); the
+  // next MFT applies every source (flux and tilt folded into its DFT kernels)
+  int live_src = 0;
   // zero pattern of the current fields (null: dense): masks narrow it, an MFT reads
   // it for its input and leaves its epilogue mask's; every other element clears it
   std::shared_ptr<const Support> support;
@@ -170,7 +173,11 @@ class Chain {
   int ncand_for(const Compile& c) const { return c.calibrating ? 1 : ncand(); }
   // fields in the buffers at this point of the compile / with every candidate
   int nfield_for(const Compile& c) const {
-    return (c.live > 0 ? c.live : ncand_for(c)) * ef_.nsources() * (int)ef_.lambdas.size();
+    return (c.live > 0 ? c.live : ncand_for(c)) * (c.live_src ? 1 : ef_.nsources()) * (int)ef_.lambdas.size();
+  }
+  // fields an MFT writes from the current ones (every source; every candidate if it expands them)
+  int nfield_out(const Compile& c, bool expand) const {
+    return (expand || c.live <= 0 ? ncand_for(c) : c.live) * ef_.nsources() * (int)ef_.lambdas.size();
   }
   int nfield_all(const Compile& c) const { return ncand_for(c) * ef_.nsources() * (int)ef_.lambdas.size(); }
   int nlam() const { return (int)ef_.lambdas.size(); }

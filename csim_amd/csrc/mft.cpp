@@ -314,10 +314,18 @@ int MftPlan::build(const std::vector<MftField>& fields, csx_precision p, hipStre
     elems += (size_t)o.n * i.n;
     return off;
   };
+  // output grids of the DFT kernels: a source tilt shifts them (MftField::tilt_x/y)
+  std::vector<csx_grid> ky(nf_), kx(nf_);
+  for (int f = 0; f < nf_; f++) {
+    ky[f] = fields[f].out_y;
+    kx[f] = fields[f].out_x;
+    ky[f].x0 -= fields[f].tilt_y * fields[f].lambda_f;
+    kx[f].x0 -= fields[f].tilt_x * fields[f].lambda_f;
+  }
   std::vector<size_t> offAy(nf_), offBx(nf_);
   for (int f = 0; f < nf_; f++) {
-    offAy[f] = add(fields[f].out_y, fields[f].in_y, fields[f].lambda_f);
-    offBx[f] = add(fields[f].out_x, fields[f].in_x, fields[f].lambda_f);
+    offAy[f] = add(ky[f], fields[f].in_y, fields[f].lambda_f);
+    offBx[f] = add(kx[f], fields[f].in_x, fields[f].lambda_f);
   }
   CSX_TRY(dft_.reserve(elems * esz));
   std::vector<DftSpec> specs;
@@ -348,7 +356,7 @@ int MftPlan::build(const std::vector<MftField>& fields, csx_precision p, hipStre
   for (int f = 0; f < nf_; f++) {
     const MftField& F = fields[f];
     auto key = std::make_tuple(F.in, F.in_kind, F.in_x.n, F.in_x.x0, F.in_x.dx, F.in_y.n, F.in_y.x0, F.in_y.dx,
-                               F.out_y.n, F.out_y.x0, F.out_y.dx, F.lambda_f);
+                               ky[f].n, ky[f].x0, ky[f].dx, F.lambda_f);
     auto it = t_index.find(key);
     if (it != t_index.end()) { tof[f] = it->second; continue; }
     tof[f] = (int)first.size();
@@ -392,7 +400,7 @@ int MftPlan::build(const std::vector<MftField>& fields, csx_precision p, hipStre
     b.out_kind = F.out_kind == FK_PLANES ? OUT_PLANES : (F.out_kind == FK_C64 ? OUT_C64 : OUT_C128);
     b.terms = T;
     // s = dx dy / (i lf) = -i dx dy / lf
-    b.alpha_re = 0.0; b.alpha_im = -(F.in_x.dx * F.in_y.dx) / F.lambda_f;
+    b.alpha_re = 0.0; b.alpha_im = -F.in_scale * (F.in_x.dx * F.in_y.dx) / F.lambda_f;
     b.mask_kind = F.mask_kind;
     if (F.mask_kind != MASK_NONE) {
       b.mask = F.mask; b.ldm = F.mrows;

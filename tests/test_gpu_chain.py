@@ -343,3 +343,46 @@ def test_zero_pattern_skipping_is_exact(ctx, precision, monkeypatch):
     assert 0.5 < f_skip / f_dense < 0.97
     for a, b in zip(e_skip, e_dense):
         assert np.array_equal(a, b)
+
+
+def test_point_sources_folded_into_the_first_mft(ctx, monkeypatch):
+    """Several point sources before the first MFT: the generation pass writes one unit
+    source per wavelength and the MFT applies each source's flux and tilt through its
+    DFT kernels (shifted output grids).  Sources with flux != 1 and a y tilt (their
+    first contraction no longer shared) against the oracle chain, against the path
+    that generates every source, and the mixed path against FP64."""
+    import re
+    w = W.workload2(n_lambda=3, offsets=(0.0, 2.0))
+    script = re.sub(r"(\tpointSource:[^\n]*\n)+",
+                    "\tpointSource:1.0, 0.0, 0.0\n\tpointSource:0.7, 3.5, 1.25\n"
+                    "\tpointSource:1.3, -2.0, 0.0\n", w.script)
+    assert script.count("pointSource:") == 3
+
+    def run(precision, fold):
+        monkeypatch.setenv("CSX_SOURCE_FOLD", fold)
+        ch = Chain(ctx, script)
+        W.apply(ch, w)
+        ch.set_precision(precision)
+        ch.set_profiling(True)
+        ch.propagate(False)
+        flops = sum(v["flops_total"] for k, v in ch.profile().items() if k.startswith("gemm"))
+        fields = [ch.field(0, s, li) for s in range(3) for li in range(3)]
+        c, _ = ch.contrast()
+        ch.close()
+        return flops, fields, c[0]
+
+    f_fold, e_fold, c_fold = run("fp64", "1")
+    f_gen, e_gen, c_gen = run("fp64", "0")
+    oc = OracleChain(script, w.arrays, sags=w.meta["sags"])
+    worst_gen = max(rel_rms(a, b) for a, b in zip(e_fold, e_gen))
+    worst_orc = max(rel_rms(e_fold[3 * s + li], oc.field(s, li)) for s in range(3) for li in range(3))
+    c_ref, _, _ = oc.contrast()
+    print(f"folded sources: vs generated {worst_gen:.3e}, vs oracle {worst_orc:.3e}, "
+          f"contrast {c_fold:.10e} / {c_gen:.10e} / {c_ref:.10e}, gemm flops {f_fold:.4e} vs {f_gen:.4e}")
+    assert worst_gen < 1e-13 and worst_orc < 1e-12
+    assert abs(c_fold - c_ref) <= 1e-12 * c_ref
+    assert f_fold < 0.9 * f_gen            # the two untilted-in-y sources share stage 1
+    _, e_mix, _ = run("bf16x3", "1")
+    worst_mix = max(rel_rms(a, b) for a, b in zip(e_mix, e_fold))
+    print(f"bf16x3 folded vs fp64 {worst_mix:.3e}")
+    assert worst_mix < 3e-5
