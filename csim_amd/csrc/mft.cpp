@@ -215,8 +215,9 @@ static int choose_variant(std::vector<GemmProb>& v, csx_precision p, DevBuf& dev
     for (int var = 0; var < nv; var++) cand.push_back({var, kcand});
   const long elems = total_elems(v);
   hipEvent_t e0, e1;
-  CSX_HIP(hipEventCreate(&e0));
-  CSX_HIP(hipEventCreate(&e1));
+  // timing-only events: no system-scope fence (no cache writeback between the timed launches)
+  CSX_HIP(hipEventCreateWithFlags(&e0, hipEventDisableSystemFence));
+  CSX_HIP(hipEventCreateWithFlags(&e1, hipEventDisableSystemFence));
   std::vector<float> tbest(cand.size(), 1e30f);
   int rc = CSX_OK;
   for (int pass = 0; pass < 2 && rc == CSX_OK; pass++) {
@@ -229,7 +230,7 @@ static int choose_variant(std::vector<GemmProb>& v, csx_precision p, DevBuf& dev
       if (hipEventRecord(e0, s) != hipSuccess) { rc = fail(CSX_ERR_HIP, "autotune: event record"); break; }
       for (int r = 0; r < 3 && rc == CSX_OK; r++) rc = run_stage(dev.as<GemmProb>(), v, nt, p, s, var, ks, elems);
       if (rc != CSX_OK) break;
-      if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+      if (hipEventRecord(e1, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
         rc = fail(CSX_ERR_HIP, "autotune: event wait");
         break;
       }
@@ -533,7 +534,9 @@ void KernelTimer::enable(bool on) {
 int KernelTimer::get_event(hipEvent_t* e) {
   if (free_.empty()) {
     hipEvent_t n;
-    CSX_HIP(hipEventCreate(&n));
+    // read only after the stream is synchronised: no system-scope fence, whose cache
+    // writeback would stall the launches between the timed ones
+    CSX_HIP(hipEventCreateWithFlags(&n, hipEventDisableSystemFence));
     all_.push_back(n);
     free_.push_back(n);
   }
