@@ -319,9 +319,14 @@ struct RealMap {
   int rows = 0, cols = 0;
   std::map<long, DevBuf> dev;   // keyed by rows*65536+cols of the grid it is resampled to
   std::map<long, std::shared_ptr<const Support>> sup;   // its zero pattern on that grid
+  std::map<long, DevBuf> dev32;   // a float copy when every value is exact in float (a GEMM epilogue reads half)
   std::shared_ptr<const Support> support_for(const Geom& cur) const {
     auto it = sup.find((long)cur.ny * 65536 + cur.nx);
     return it == sup.end() ? nullptr : it->second;
+  }
+  const float* f32_for(const Geom& cur) const {
+    auto it = dev32.find((long)cur.ny * 65536 + cur.nx);
+    return it == dev32.end() ? nullptr : it->second.as<float>();
   }
   int device_for(Chain& ch, const Geom& cur, const double** out) {
     long key = (long)cur.ny * 65536 + cur.nx;
@@ -363,6 +368,18 @@ struct RealMap {
       CSX_TRY(b.reserve(m.size() * sizeof(double)));
       CSX_HIP(hipMemcpy(b.p, m.data(), m.size() * sizeof(double), hipMemcpyHostToDevice));
       sup[key] = Support::of_mask(m.data(), cur.ny, cur.nx);
+      std::vector<float> m32(m.size());
+      bool exact = true;
+      for (size_t i = 0; i < m.size() && exact; i++) {
+        m32[i] = (float)m[i];
+        exact = (double)m32[i] == m[i];
+      }
+      if (exact) {
+        DevBuf b32;
+        CSX_TRY(b32.reserve(m32.size() * sizeof(float)));
+        CSX_HIP(hipMemcpy(b32.p, m32.data(), m32.size() * sizeof(float), hipMemcpyHostToDevice));
+        dev32[key] = std::move(b32);
+      }
       it = dev.emplace(key, std::move(b)).first;
     }
     *out = it->second.as<double>();
@@ -374,6 +391,7 @@ struct RealMap {
     rows = r; cols = c;
     dev.clear();
     sup.clear();
+    dev32.clear();
     return CSX_OK;
   }
   int load(const std::string& path) {
@@ -598,7 +616,7 @@ static int compile_mft(Chain& ch, Compile& c, const std::vector<Geom>& gout_per_
                        const std::vector<double>& lf_per_lam, const std::string& label,
                        const std::vector<MftField>* masks = nullptr, bool out_conv = false,
                        const double* epi_mask_1pi = nullptr, bool expand = false,
-                       std::shared_ptr<const Support> epi_sup = nullptr) {
+                       std::shared_ptr<const Support> epi_sup = nullptr, const float* epi_mask_f32 = nullptr) {
   CSX_TRY(flush_pointwise(ch, c, /*to_conv=*/true));
   // expand: the fields are one candidate's copy; the MFT writes every candidate's
   // output (its own mask each) from that copy, whose first contraction the plan
@@ -656,7 +674,8 @@ static int compile_mft(Chain& ch, Compile& c, const std::vector<Geom>& gout_per_
       m.mask_kind = mk.mask_kind; m.mask = mk.mask;
       m.mrows = mk.mrows; m.mcols = mk.mcols; m.row0 = mk.row0; m.col0 = mk.col0;
     } else if (epi_mask_1pi) {
-      m.mask_kind = MASK_REAL_1PI; m.mask = epi_mask_1pi;
+      m.mask_kind = epi_mask_f32 ? MASK_REAL_1PI_F32 : MASK_REAL_1PI;
+      m.mask = epi_mask_f32 ? (const void*)epi_mask_f32 : (const void*)epi_mask_1pi;
       m.mrows = go.ny; m.mcols = go.nx; m.row0 = 0; m.col0 = 0;
       if (zero_skip) m.out_sup = epi_sup;
     }
@@ -1036,19 +1055,21 @@ struct DoubleFraunhoferElem : Element {
     // a mask element right after folds into the epilogue of the return MFT
     const double* lyot = nullptr;
     std::shared_ptr<const Support> lyot_sup;
+    const float* lyot32 = nullptr;
     int claim = 0;
     if (Element* nx = ch.elem_at(index + 1)) {
       if (auto* me = dynamic_cast<MaskElem*>(nx)) {
         if (!me->skipped(c) && !me->map.host.empty()) {
           CSX_TRY(me->map.device_for(ch, E, &lyot));
           lyot_sup = me->map.support_for(E);
+          lyot32 = me->map.f32_for(E);
           claim = 1;
         }
       }
     }
     bool next_mft = is_mft_element(ch.elem_at(index + 1 + claim), c);
     std::vector<Geom> back(nl, E);
-    CSX_TRY(compile_mft(ch, c, back, lfo, type + " out " + name, nullptr, next_mft, lyot, false, lyot_sup));
+    CSX_TRY(compile_mft(ch, c, back, lfo, type + " out " + name, nullptr, next_mft, lyot, false, lyot_sup, lyot32));
     c.claimed = claim;
     return CSX_OK;
   }

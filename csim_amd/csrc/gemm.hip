@@ -82,8 +82,8 @@ __device__ __forceinline__ void epilogue_value(const GemmProb& P, int m, int n, 
         vi = vr * w.y + vi * w.x;
         vr = tr;
       } else {
-        double w = gin<double>(P.mask)[off];
-        if (P.mask_kind == MASK_REAL_1PI) {   // csim mask: (1+i) w
+        const double w = P.mask_kind == MASK_REAL_1PI_F32 ? (double)gin<float>(P.mask)[off] : gin<double>(P.mask)[off];
+        if (P.mask_kind == MASK_REAL_1PI || P.mask_kind == MASK_REAL_1PI_F32) {   // csim mask: (1+i) w
           double tr = (vr - vi) * w;
           vi = (vr + vi) * w;
           vr = tr;
@@ -98,10 +98,13 @@ __device__ __forceinline__ void epilogue_value(const GemmProb& P, int m, int n, 
   out_r = vr;
   out_i = vi;
 }
-__device__ __forceinline__ void epilogue_store(const GemmProb& P, int m, int n, double re, double im) {
+// zero: the tile took no K (its sums are exact zeros), so the output is zero whatever
+// the mask and the mask is not read
+__device__ __forceinline__ void epilogue_store(const GemmProb& P, int m, int n, double re, double im,
+                                               bool zero = false) {
   if (m >= P.M || n >= P.N) return;
-  double vr, vi;
-  epilogue_value(P, m, n, re, im, vr, vi);
+  double vr = 0.0, vi = 0.0;
+  if (!zero) epilogue_value(P, m, n, re, im, vr, vi);
   long off = P.transC ? (long)n * P.ldc + m : (long)m * P.ldc + n;
   if (P.out_kind == OUT_C128) gout<f64x2>(P.C)[off] = f64x2{vr, vi};
   else if (P.out_kind == OUT_C64) gout<f32x2>(P.C)[off] = f32x2{(float)vr, (float)vi};
@@ -213,7 +216,7 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const GemmProb* __restrict__ pr
       for (int v = 0; v < 4; v++) {
         int m = m0 + wm * 32 + i * 16 + Mma<T>::row(lane, v);
         int n = n0 + wn * 32 + j * 16 + (lane & 15);
-        epilogue_store(P, m, n, (double)accR[i][j][v], (double)accI[i][j][v]);
+        epilogue_store(P, m, n, (double)accR[i][j][v], (double)accI[i][j][v], kend <= kbeg);
       }
 }
 
@@ -329,7 +332,7 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_f64db(const GemmProb* __r
       for (int v = 0; v < 4; v++) {
         const int m = m0 + wm * (BM / WM) + i * 16 + Mma<double>::row(lane, v);
         const int n = n0 + wn * (BN / WN) + j * 16 + (lane & 15);
-        epilogue_store(P, m, n, accR[i][j][v], accI[i][j][v]);
+        epilogue_store(P, m, n, accR[i][j][v], accI[i][j][v], nslab == 0);
       }
 }
 
@@ -708,8 +711,8 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
           const int nl = wn * (BN / WN) + j * MF + Mf::dcol(lane);
           double re = accR[i][j][v], im = accI[i][j][v];
           if (TERMS == 3) { re += dR[TERMS == 3 ? i : 0][TERMS == 3 ? j : 0][v]; im += dI[TERMS == 3 ? i : 0][TERMS == 3 ? j : 0][v]; }
-          double vr, vi;
-          epilogue_value(P, m0 + ml, n0 + nl, re, im, vr, vi);
+          double vr = 0.0, vi = 0.0;
+          if (nslab > 0) epilogue_value(P, m0 + ml, n0 + nl, re, im, vr, vi);
           const int pos = ml * BN + ((((nl >> 3) ^ (((ml >> 2) & 3) << 1))) << 3) + (nl & 7);
           float xr = (float)vr, xi = (float)vi;
 #pragma unroll
@@ -743,7 +746,7 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
         const int n = n0 + wn * (BN / WN) + j * MF + Mf::dcol(lane);
         double re = accR[i][j][v], im = accI[i][j][v];
         if (TERMS == 3) { re += dR[TERMS == 3 ? i : 0][TERMS == 3 ? j : 0][v]; im += dI[TERMS == 3 ? i : 0][TERMS == 3 ? j : 0][v]; }
-        epilogue_store(P, m, n, re, im);
+        epilogue_store(P, m, n, re, im, nslab == 0);
       }
 }
 

@@ -7,7 +7,10 @@
 namespace csx {
 
 // MASK_COMPLEX_CLIP: complex factor inside the mask rectangle, zero outside it.
-enum MaskKind : int { MASK_NONE = 0, MASK_COMPLEX = 1, MASK_REAL_1PI = 2, MASK_REAL = 3, MASK_COMPLEX_CLIP = 4 };
+// MASK_REAL_1PI_F32: MASK_REAL_1PI with the mask stored in float (values exact in float).
+enum MaskKind : int {
+  MASK_NONE = 0, MASK_COMPLEX = 1, MASK_REAL_1PI = 2, MASK_REAL = 3, MASK_COMPLEX_CLIP = 4, MASK_REAL_1PI_F32 = 5
+};
 enum OutKind : int { OUT_C128 = 0, OUT_C64 = 1, OUT_PLANES = 2 };
 
 // One problem of a grouped launch:  C(m, n) = alpha * sum_k A(m, k) B(n, k),

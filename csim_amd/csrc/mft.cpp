@@ -130,15 +130,15 @@ static std::string signature(const std::vector<GemmProb>& v, csx_precision p) {
   return s;
 }
 
-// K splits worth trying for a launch: two-term planes whose 64 x 64 tiles do not fill
-// the chip twice over and whose K is long enough to share (1 = no split).
-static int split_candidate(const std::vector<GemmProb>& v, csx_precision p) {
+// K splits worth trying for a launch on a bm x bn tile: two-term planes whose tiles do
+// not fill the chip twice over and whose K is long enough to share (1 = no split).
+static int split_candidate(const std::vector<GemmProb>& v, csx_precision p, int bm, int bn) {
   if (p != CSX_BF16X3) return 1;
   if (const char* e = std::getenv("CSX_GEMM_KSPLIT")) return std::max(1, std::atoi(e));
   long tiles = 0;
   int kmin = 1 << 30;
   for (const auto& g : v) {
-    tiles += (long)((g.M + 63) / 64) * ((g.N + 63) / 64);
+    tiles += (long)((g.M + bm - 1) / bm) * ((g.N + bn - 1) / bn);
     kmin = std::min(kmin, g.K);
   }
   if (tiles >= 512 || kmin < 512) return 1;
@@ -192,12 +192,19 @@ static int run_stage(const GemmProb* dev, const std::vector<GemmProb>& v, int ti
 static int choose_variant(std::vector<GemmProb>& v, csx_precision p, DevBuf& dev, DevBuf& ws, hipStream_t s,
                           int* variant, int* ksplit, int* tiles) {
   const int nv = n_variants(p);
-  const int kcand = split_candidate(v, p);
-  if (kcand > 1) CSX_TRY(ws.reserve(sizeof(float) * 2 * (size_t)kcand * total_elems(v)));
+  std::vector<int> kc(nv, 1);   // K splits worth trying with each variant's tile
+  int kmax = 1;
+  for (int var = 0; var < nv && p == CSX_BF16X3; var++) {
+    int bm = 64, bn = 64;
+    CSX_TRY(planes_variant_tile(terms_of(p), var, &bm, &bn));
+    kc[var] = split_candidate(v, p, bm, bn);
+    kmax = std::max(kmax, kc[var]);
+  }
+  if (kmax > 1) CSX_TRY(ws.reserve(sizeof(float) * 2 * (size_t)kmax * total_elems(v)));
   int forced = forced_variant(p);
   if (nv == 1 || forced >= 0) {
     *variant = nv == 1 ? 0 : forced;
-    *ksplit = std::getenv("CSX_GEMM_KSPLIT") ? kcand : 1;
+    *ksplit = std::getenv("CSX_GEMM_KSPLIT") ? kc[*variant] : 1;
     set_split(v, *ksplit, ws.as<float>());
     return upload(v, p, *variant, dev, s, tiles);
   }
@@ -211,8 +218,8 @@ static int choose_variant(std::vector<GemmProb>& v, csx_precision p, DevBuf& dev
   }
   std::vector<std::pair<int, int>> cand;
   for (int var = 0; var < nv; var++) cand.push_back({var, 1});
-  if (kcand > 1)
-    for (int var = 0; var < nv; var++) cand.push_back({var, kcand});
+  for (int var = 0; var < nv; var++)
+    if (kc[var] > 1) cand.push_back({var, kc[var]});
   const long elems = total_elems(v);
   hipEvent_t e0, e1;
   // timing-only events: no system-scope fence (no cache writeback between the timed launches)
