@@ -792,6 +792,13 @@ struct ZoomFftPropElem : Element {
 
 static bool is_mft_element(Element* e, const Compile& c);
 
+// The S x S window of the FPM grid outside which every candidate's hex mask is 1.
+struct BabinetWindow {
+  bool ok = false;
+  int S = 0, row0 = 0, col0 = 0, ncm = 1;
+  const double2* masks = nullptr;   // per (candidate, lambda), S x S column-major
+};
+
 // ---- complexHexMaskFPM sub-element (src/elements/complexHexMaskFPM.cpp)
 struct HexFpm {
   int maskNRings = -1, fpmArraySize = -1, nSubPix = -1, fpmMaskSize = -1, useOnlyThisHex = -1;
@@ -974,6 +981,7 @@ struct DoubleFraunhoferElem : Element {
     std::vector<MftField> mk(nf);
     Chain* chp = &ch;
     std::function<int(hipStream_t)> mask_stage;
+    BabinetWindow bw;
     if (has_hex) {
       const int S = hex.raster.S;
       size_t plane = (size_t)S * S;
@@ -1018,6 +1026,11 @@ struct DoubleFraunhoferElem : Element {
         m.mrows = S; m.mcols = S;
         m.row0 = gf[l].ny / 2 - S / 2; m.col0 = gf[l].nx / 2 - S / 2;
       }
+      // every candidate's mask is 1 outside the same window of every FPM grid
+      bw.ok = use != kUseOnlyPerCandidate && S % 8 == 0;
+      for (int l = 0; l < nl && bw.ok; l++)
+        bw.ok = gf[l].ny == gf[0].ny && gf[l].nx == gf[0].nx && gf[0].ny >= S && gf[0].nx >= S;
+      bw.S = S; bw.row0 = gf[0].ny / 2 - S / 2; bw.col0 = gf[0].nx / 2 - S / 2; bw.ncm = ncm; bw.masks = dm;
     } else if (!cube.empty() || occulterRadiusFld >= 0) {
       // one mask for every slice (maskIndex latches to the first slice, :86-87)
       std::vector<double2> m;
@@ -1050,8 +1063,14 @@ struct DoubleFraunhoferElem : Element {
       }
     }
     if (mask_stage) { c.stages.push_back(mask_stage); c.stage_names.push_back("hex masks " + name); }
+    // A candidate batch whose hex masks are transparent outside one window splits by
+    // Babinet: the unmasked FPM field and its return MFT once per source and
+    // wavelength, per candidate only the window's (M - 1) E (CSX_BABINET=0: off)
+    const char* bz = std::getenv("CSX_BABINET");
+    const bool babinet = expand && has_hex && bw.ok && ncand > 1 && !(bz && bz[0] == '0');
     // pupil -> FPM (mask in the epilogue) -> pupil grid of the input
-    CSX_TRY(compile_mft(ch, c, gf, lf, type + " in " + name, &mk, /*out_conv=*/true, nullptr, expand));
+    CSX_TRY(compile_mft(ch, c, gf, lf, type + " in " + name, babinet ? nullptr : &mk, /*out_conv=*/true, nullptr,
+                        babinet ? false : expand));
     // a mask element right after folds into the epilogue of the return MFT
     const double* lyot = nullptr;
     std::shared_ptr<const Support> lyot_sup;
@@ -1069,8 +1088,100 @@ struct DoubleFraunhoferElem : Element {
     }
     bool next_mft = is_mft_element(ch.elem_at(index + 1 + claim), c);
     std::vector<Geom> back(nl, E);
-    CSX_TRY(compile_mft(ch, c, back, lfo, type + " out " + name, nullptr, next_mft, lyot, false, lyot_sup, lyot32));
+    if (babinet) {
+      CSX_TRY(compile_babinet(ch, c, E, gf, lfo, bw, next_mft, lyot, lyot32, lyot_sup));
+    } else {
+      CSX_TRY(compile_mft(ch, c, back, lfo, type + " out " + name, nullptr, next_mft, lyot, false, lyot_sup, lyot32));
+    }
     c.claimed = claim;
+    return CSX_OK;
+  }
+
+  // FPM -> Lyot of a candidate batch by Babinet.  The fields are the one shared copy's
+  // FPM-plane fields (unmasked).  E_lyot[cand] = MFT(E) + MFT_window((M_cand - 1) E):
+  // the first term once per (source, wavelength) into a scratch buffer stored like the
+  // output, which the per-candidate window MFT adds in its epilogue before the Lyot mask.
+  int compile_babinet(Chain& ch, Compile& c, const Geom& E, const std::vector<Geom>& gf,
+                      const std::vector<double>& lfo, const BabinetWindow& bw, bool next_mft, const double* lyot,
+                      const float* lyot32, const std::shared_ptr<const Support>& lyot_sup) {
+    const int T = terms_of(ch.precision());
+    const bool c64 = ch.precision() == CSX_FP32, mixed = T || c64;
+    const bool out_conv = next_mft && mixed;
+    const int nl = ch.nlam(), ns = ch.nsrc();
+    const int nf_in = ch.nfield_for(c), nf = ch.nfield_out(c, true);
+    if (nf_in != ns * nl || c.live_src) return fail(CSX_ERR_STATE, "babinet: unexpected field layout");
+    const size_t ew = T ? (size_t)4 * T : (c64 ? sizeof(float2) : sizeof(double2));   // working precision
+    const int wkind = T ? FK_PLANES : (c64 ? FK_C64 : FK_C128);
+    const size_t eo = out_conv ? ew : sizeof(double2);
+    const int okind = out_conv ? wkind : FK_C128;
+    const Geom& gF = c.geom;
+    const char* Ein = mixed ? static_cast<const char*>(ch.c64_buf()) : reinterpret_cast<const char*>(ch.field_buf(c.buf));
+    if (mixed != c.in_conv) return fail(CSX_ERR_STATE, "babinet: FPM fields not at the working precision");
+    char* out = out_conv ? static_cast<char*>(ch.c64_buf()) : reinterpret_cast<char*>(ch.field_buf(1 - c.buf));
+    DevBuf& base = ch.scratch("babinet base " + name + (c.calibrating ? " cal" : ""));
+    DevBuf& win = ch.scratch("babinet window " + name + (c.calibrating ? " cal" : ""));
+    const long wn = (long)bw.S * bw.S;
+    CSX_TRY(base.reserve(eo * (size_t)nf_in * E.npix()));
+    CSX_TRY(win.reserve(ew * (size_t)nf * wn));
+    Chain* chp = &ch;
+    // the unmasked return MFT, once per (source, lambda)
+    std::vector<MftField> FB(nf_in);
+    for (int f = 0; f < nf_in; f++) {
+      const int l = f % nl;
+      MftField& m = FB[f];
+      m.in = Ein + (size_t)f * gF.npix() * ew; m.in_kind = wkind;
+      m.out = static_cast<char*>(base.p) + (size_t)f * E.npix() * eo; m.out_kind = okind;
+      m.in_x = gx(gf[l]); m.in_y = gy(gf[l]); m.out_x = gx(E); m.out_y = gy(E);
+      m.lambda_f = lfo[l];
+    }
+    auto pb = std::make_unique<MftPlan>();
+    CSX_TRY(pb->build(FB, ch.precision(), ch.stream()));
+    const MftPlan* ppb = pb.get();
+    c.plans.push_back(std::move(pb));
+    c.mft_at[c.stages.size()] = FB;
+    c.stages.push_back([=](hipStream_t s) { return ppb->run(s, chp->timer()); });
+    c.stage_names.push_back(type + " out (shared) " + name);
+    // per candidate: (M - 1) E on the window
+    const long npF = gF.npix();
+    const int rowsF = gF.ny;
+    void* W = win.p;
+    const BabinetWindow b = bw;
+    c.stages.push_back([=](hipStream_t s) {
+      return babinet_window(Ein, wkind, T, npF, rowsF, b.masks, b.S, b.row0, b.col0, nf, nl, ns, b.ncm, W, s);
+    });
+    c.stage_names.push_back("babinet window " + name);
+    // window -> Lyot, plus the shared term, Lyot mask
+    std::vector<MftField> FD(nf);
+    for (int f = 0; f < nf; f++) {
+      const int l = f % nl, fi = f % nf_in;
+      MftField& m = FD[f];
+      m.in = static_cast<const char*>(win.p) + (size_t)f * wn * ew; m.in_kind = wkind;
+      m.out = out + (size_t)f * E.npix() * eo; m.out_kind = okind;
+      m.in_x = csx_grid{bw.S, gf[l].x0 + bw.col0 * gf[l].dx, gf[l].dx};
+      m.in_y = csx_grid{bw.S, gf[l].y0 + bw.row0 * gf[l].dy, gf[l].dy};
+      m.out_x = gx(E); m.out_y = gy(E);
+      m.lambda_f = lfo[l];
+      m.add = static_cast<const char*>(base.p) + (size_t)fi * E.npix() * eo;
+      if (lyot) {
+        m.mask_kind = lyot32 ? MASK_REAL_1PI_F32 : MASK_REAL_1PI;
+        m.mask = lyot32 ? (const void*)lyot32 : (const void*)lyot;
+        m.mrows = E.ny; m.mcols = E.nx; m.row0 = 0; m.col0 = 0;
+        const char* zs = std::getenv("CSX_ZERO_SKIP");
+        if (!(zs && zs[0] == '0')) m.out_sup = lyot_sup;
+      }
+    }
+    auto pd = std::make_unique<MftPlan>();
+    CSX_TRY(pd->build(FD, ch.precision(), ch.stream()));
+    const MftPlan* ppd = pd.get();
+    c.plans.push_back(std::move(pd));
+    c.mft_at[c.stages.size()] = FD;
+    c.stages.push_back([=](hipStream_t s) { return ppd->run(s, chp->timer()); });
+    c.stage_names.push_back(type + " out (window) " + name);
+    c.live = 0;
+    c.in_conv = out_conv;
+    if (!out_conv) c.buf = 1 - c.buf;
+    c.geom = E;
+    c.support = lyot ? lyot_sup : nullptr;
     return CSX_OK;
   }
 };

@@ -72,6 +72,24 @@ __device__ __forceinline__ void epilogue_value(const GemmProb& P, int m, int n, 
                                                double& out_i) {
   double vr = P.alpha_re * re - P.alpha_im * im;
   double vi = P.alpha_re * im + P.alpha_im * re;
+  if (P.addend) {
+    const long off = P.transC ? (long)n * P.ldc + m : (long)m * P.ldc + n;
+    if (P.out_kind == OUT_C128) {
+      const f64x2 a = gin<f64x2>(P.addend)[off];
+      vr += a.x; vi += a.y;
+    } else if (P.out_kind == OUT_C64) {
+      const f32x2 a = gin<f32x2>(P.addend)[off];
+      vr += a.x; vi += a.y;
+    } else {
+      const GLOBAL unsigned short* a = gin<unsigned short>(P.addend);
+      double ar = 0.0, ai = 0.0;
+      for (int t = 0; t < P.terms; t++) {
+        ar += (double)bf16_val(a[(2 * t) * P.sC + off]);
+        ai += (double)bf16_val(a[(2 * t + 1) * P.sC + off]);
+      }
+      vr += ar; vi += ai;
+    }
+  }
   if (P.mask_kind != MASK_NONE) {
     int mr = m - P.mm0, nc = n - P.mn0;
     if (mr >= 0 && mr < P.mM && nc >= 0 && nc < P.mN) {
@@ -98,8 +116,9 @@ __device__ __forceinline__ void epilogue_value(const GemmProb& P, int m, int n, 
   out_r = vr;
   out_i = vi;
 }
-// zero: the tile took no K (its sums are exact zeros), so the output is zero whatever
-// the mask and the mask is not read
+// zero: the tile took no K, which the K range tables give only where the products are
+// exact zeros and no addend is bound, or where the epilogue mask is zero: the output is
+// zero and the mask is not read
 __device__ __forceinline__ void epilogue_store(const GemmProb& P, int m, int n, double re, double im,
                                                bool zero = false) {
   if (m >= P.M || n >= P.N) return;

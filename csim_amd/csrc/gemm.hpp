@@ -47,6 +47,8 @@ struct GemmProb {
   const int* kr;
   int kr_ld;
   unsigned kr_tag;     // digest of the table (tile tuning keys on it)
+  // optional addend stored like C (out_kind, ldc, sC, terms): C = mask (alpha sum + addend)
+  const void* addend;
 };
 constexpr int kKrBlock = 64;
 

@@ -68,7 +68,7 @@ static std::vector<int> stage1_ranges(const MftField& F) {
 static std::vector<int> stage2_ranges(const MftField& F) {
   const int M = F.out_x.n, N = F.out_y.n, K = F.in_x.n;
   int klo = 0, khi = K;
-  const Support* si = F.in_sup.get();
+  const Support* si = F.add ? nullptr : F.in_sup.get();   // with an addend only the mask zeroes a block
   if (si && si->ny == F.in_y.n && si->nx == K) {
     klo = K; khi = 0;
     for (int x = 0; x < K; x++)
@@ -409,6 +409,7 @@ int MftPlan::build(const std::vector<MftField>& fields, csx_precision p, hipStre
     b.terms = T;
     // s = dx dy / (i lf) = -i dx dy / lf
     b.alpha_re = 0.0; b.alpha_im = -F.in_scale * (F.in_x.dx * F.in_y.dx) / F.lambda_f;
+    b.addend = F.add;
     b.mask_kind = F.mask_kind;
     if (F.mask_kind != MASK_NONE) {
       b.mask = F.mask; b.ldm = F.mrows;

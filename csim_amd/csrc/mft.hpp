@@ -74,6 +74,9 @@ struct MftField {
   // kernel's exp(-2 pi i xo x / lf) the phase ramp is the kernel of an output grid
   // shifted by tilt lf, so the sources of one input share its first contraction.
   double in_scale = 1.0, tilt_x = 0.0, tilt_y = 0.0;
+  // optional addend stored like the output (out_kind, out_y x out_x per plane):
+  // E_out = mask (s A_y E_in B_x^T + add)
+  const void* add = nullptr;
 };
 
 class MftPlan {

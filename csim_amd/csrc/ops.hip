@@ -997,3 +997,61 @@ int gather_response(const double2* E, long npix_field, const int32_t* idx, int n
 }
 
 }  // namespace csx
+
+namespace csx {
+
+// ------------------------------------------------------------ Babinet window --
+// (M - 1) E on the hex window: the part of a candidate's FPM-plane field that
+// differs from the shared, unmasked one (the mask is 1 outside the window).
+__global__ void k_babinet_window(const void* __restrict__ E, int kind, int terms, long e_npix, int rows,
+                                 const double2* __restrict__ masks, int S, int row0, int col0, int nl, int ns,
+                                 int ncm, void* __restrict__ W) {
+  const int f = blockIdx.y;
+  const int l = f % nl, s = (f / nl) % ns, cand = f / (nl * ns);
+  const long fin = (long)s * nl + l;
+  const long wn = (long)S * S;
+  const GLOBAL f64x2* M = gin<f64x2>(masks) + ((long)min(cand, ncm - 1) * nl + l) * wn;
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < (int)wn; p += gridDim.x * blockDim.x) {
+    const int c = p / S, r = p - c * S;
+    const long src = (long)(col0 + c) * rows + (row0 + r);
+    double er, ei;
+    if (kind == 0) {
+      const f64x2 v = gin<f64x2>(E)[fin * e_npix + src];
+      er = v.x; ei = v.y;
+    } else if (kind == 1) {
+      const f32x2 v = gin<f32x2>(E)[fin * e_npix + src];
+      er = v.x; ei = v.y;
+    } else {
+      const GLOBAL unsigned short* e = gin<unsigned short>(E) + fin * 2 * terms * e_npix;
+      er = 0.0; ei = 0.0;
+      for (int t = 0; t < terms; t++) {
+        er += (double)bf16_val(e[(2 * t) * e_npix + src]);
+        ei += (double)bf16_val(e[(2 * t + 1) * e_npix + src]);
+      }
+    }
+    const f64x2 m = M[p];
+    const double mr = m.x - 1.0, mi = m.y;
+    const double wr = mr * er - mi * ei, wi = mr * ei + mi * er;
+    if (kind == 0) {
+      gout<f64x2>(W)[f * wn + p] = f64x2{wr, wi};
+    } else if (kind == 1) {
+      gout<f32x2>(W)[f * wn + p] = f32x2{(float)wr, (float)wi};
+    } else {
+      store_planes(gout<unsigned short>(W) + (long)f * 2 * terms * wn, p, wn, terms, wr, wi);
+    }
+  }
+}
+
+int babinet_window(const void* E, int kind, int terms, long e_npix, int rows, const double2* masks, int S,
+                   int row0, int col0, int nfield, int nl, int ns, int ncm, void* W, hipStream_t s) {
+  if (nfield <= 0) return CSX_OK;
+  if (S <= 0 || row0 < 0 || col0 < 0 || nl <= 0 || ns <= 0 || ncm <= 0 || (kind == 2 && terms <= 0))
+    return fail(CSX_ERR_ARG, "babinet_window: bad shape");
+  const int blocks = std::min(64, (S * S + 255) / 256);
+  hipLaunchKernelGGL(k_babinet_window, dim3(blocks, nfield), dim3(256), 0, s, E, kind, terms, e_npix, rows,
+                     reinterpret_cast<const double2*>(masks), S, row0, col0, nl, ns, ncm, W);
+  CSX_HIP(hipGetLastError());
+  return CSX_OK;
+}
+
+}  // namespace csx

@@ -92,6 +92,13 @@ int spectrum_mul(double2* P, const double2* Pin, int nin, int n2, int nfield, co
 int crop_add(double2* E, const double2* Ein, int nin, int n, const double2* P, int n2, int nfield, int nl,
              const double2* scale_dev, int babinet_from, hipStream_t s);
 // M[s * nl + l][c][j] = E[(c * nslice + s * nl + l)][idx[j]], nslice = nsrc * nl
+// Babinet window of a candidate batch (doubleFraunhofer(Interp)FPM with the hex mask
+// transparent outside its S x S window): W[f] = (M[cand, lambda] - 1) * E[f_in] on the
+// window at (row0, col0) of the FPM grid (rows x ...), f = (cand * ns + s) * nl + l,
+// f_in = s * nl + l, masks of candidate min(cand, ncm - 1).  kind: 0 complex128,
+// 1 complex64, 2 `terms` BF16 planes (input field stride e_npix, output S * S).
+int babinet_window(const void* E, int kind, int terms, long e_npix, int rows, const double2* masks, int S,
+                   int row0, int col0, int nfield, int nl, int ns, int ncm, void* W, hipStream_t s);
 int gather_response(const double2* E, long npix_field, const int32_t* idx, int nidx, int ncol, int nslice,
                     double2* M, hipStream_t s);
 

@@ -279,7 +279,9 @@ def test_chain_group_uneven_pupils_matches_chains(ctx, monkeypatch):
     g = ChainGroup(chains)
     got = g.contrast()
     print("grouped MFT stages", g.merged_launches(), "contrasts", got)
-    assert g.merged_launches() == 3         # pupil->FPM, FPM->Lyot, Lyot->image: two GEMM launches each
+    # pupil->FPM, FPM->Lyot (the shared unmasked field, then each candidate's hex window,
+    # by Babinet), Lyot->image: two GEMM launches each
+    assert g.merged_launches() == 4
     ref = np.concatenate(alone)
     assert np.max(np.abs(got - ref) / ref) < 1e-12
 
@@ -386,3 +388,35 @@ def test_point_sources_folded_into_the_first_mft(ctx, monkeypatch):
     worst_mix = max(rel_rms(a, b) for a, b in zip(e_mix, e_fold))
     print(f"bf16x3 folded vs fp64 {worst_mix:.3e}")
     assert worst_mix < 3e-5
+
+
+@pytest.mark.parametrize("precision", ["fp64", "bf16x3"])
+def test_candidate_batch_babinet_split(ctx, precision, monkeypatch):
+    """A candidate batch through doubleFraunhoferInterpFPM: the hex mask is 1 outside its
+    window, so the return MFT of the shared unmasked FPM field runs once per (source,
+    wavelength) and each candidate adds the window's (M - 1) E.  Fields and contrasts
+    against the direct per-candidate path (CSX_BABINET=0), and the flops it saves."""
+    w = W.workload2(n_lambda=3, offsets=(0.0, 2.0))
+    ncand = 5
+    sags = W.thickness_to_sag(W.hex_thickness(w.meta["seed"], 11, w.meta["n_hex"], n_cand=ncand))
+    res = {}
+    for bab in ("1", "0"):
+        monkeypatch.setenv("CSX_BABINET", bab)
+        ch = build(ctx, w, precision)
+        ch.set_candidates("fpm", sags)
+        ch.set_profiling(True)
+        c, _ = ch.contrast()
+        flops = sum(v["flops_total"] for k, v in ch.profile().items() if k.startswith("gemm"))
+        fields = [ch.field(k, s, li) for k in range(ncand) for s in range(2) for li in range(3)]
+        res[bab] = (np.asarray(c), flops, fields)
+        ch.close()
+    (c1, f1, e1), (c0, f0, e0) = res["1"], res["0"]
+    worst = max(rel_rms(a, b) for a, b in zip(e1, e0))
+    dc = float(np.max(np.abs(c1 - c0) / c0))
+    print(f"{precision} babinet vs direct: fields {worst:.3e}, contrast rel {dc:.3e}, "
+          f"gemm flops {f1:.4e} vs {f0:.4e} ({f1 / f0:.3f})")
+    if precision == "fp64":
+        assert worst < 1e-12 and dc < 1e-12
+    else:
+        assert worst < 3e-5 and dc < 1e-5
+    assert f1 < 0.75 * f0
