@@ -390,7 +390,7 @@ def test_point_sources_folded_into_the_first_mft(ctx, monkeypatch):
     assert worst_mix < 3e-5
 
 
-@pytest.mark.parametrize("precision", ["fp64", "bf16x3"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16x3", "bf16x6"])
 def test_candidate_batch_babinet_split(ctx, precision, monkeypatch):
     """A candidate batch through doubleFraunhoferInterpFPM: the hex mask is 1 outside its
     window, so the return MFT of the shared unmasked FPM field runs once per (source,
