@@ -678,6 +678,7 @@ static int compile_mft(Chain& ch, Compile& c, const std::vector<Geom>& gout_per_
       m.mask = epi_mask_f32 ? (const void*)epi_mask_f32 : (const void*)epi_mask_1pi;
       m.mrows = go.ny; m.mcols = go.nx; m.row0 = 0; m.col0 = 0;
       if (zero_skip) m.out_sup = epi_sup;
+      m.out_read_sup = zero_skip && out_conv;   // the next MFT reads it through epi_sup
     }
   }
   auto plan = std::make_unique<MftPlan>();
@@ -1167,7 +1168,10 @@ struct DoubleFraunhoferElem : Element {
         m.mask = lyot32 ? (const void*)lyot32 : (const void*)lyot;
         m.mrows = E.ny; m.mcols = E.nx; m.row0 = 0; m.col0 = 0;
         const char* zs = std::getenv("CSX_ZERO_SKIP");
-        if (!(zs && zs[0] == '0')) m.out_sup = lyot_sup;
+        if (!(zs && zs[0] == '0')) {
+          m.out_sup = lyot_sup;
+          m.out_read_sup = out_conv;   // the next MFT reads it through lyot_sup
+        }
       }
     }
     auto pd = std::make_unique<MftPlan>();

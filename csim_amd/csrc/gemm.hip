@@ -50,21 +50,26 @@ __device__ __forceinline__ int find_prob(const GemmProb* __restrict__ probs, int
 // K range of a BM x BN tile from the problem's block table (GemmProb::kr): whole
 // slabs of BK from a multiple of BK, so the slab grid, and with it the order of
 // every sum, stays that of the full range; only products that are exact zeros drop.
+// Returns true when every block of the tile is zero and never read (lo = -1): the
+// tile writes nothing.
 template <int BM, int BN, int BK>
-__device__ __forceinline__ void tile_krange(const GemmProb& P, int m0, int n0, int& kbeg, int& kend) {
-  if (!P.kr) return;
+__device__ __forceinline__ bool tile_krange(const GemmProb& P, int m0, int n0, int& kbeg, int& kend) {
+  if (!P.kr) return false;
   const GLOBAL int* kr = gin<int>(P.kr);
   int lo = P.K, hi = 0;
+  bool skip = true;
   const int mb1 = min((m0 + BM + kKrBlock - 1) / kKrBlock, (P.M + kKrBlock - 1) / kKrBlock);
   const int nb1 = min((n0 + BN + kKrBlock - 1) / kKrBlock, P.kr_ld);
   for (int mb = m0 / kKrBlock; mb < mb1; mb++)
     for (int nb = n0 / kKrBlock; nb < nb1; nb++) {
       const long b = 2 * ((long)mb * P.kr_ld + nb);
-      lo = min(lo, kr[b]);
+      skip = skip && kr[b] < 0;
+      lo = min(lo, kr[b] < 0 ? P.K : kr[b]);
       hi = max(hi, kr[b + 1]);
     }
   kbeg = max(kbeg, lo / BK * BK);
   kend = max(kbeg, min(kend, hi));
+  return skip;
 }
 
 // Epilogue: alpha, mask (epilogue_value), store (shared by every kernel).
@@ -478,7 +483,7 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
     kbeg = min(K, ks * kc);
     kend = min(K, kbeg + kc);
   }
-  tile_krange<BM, BN, BK>(P, m0, n0, kbeg, kend);
+  if (tile_krange<BM, BN, BK>(P, m0, n0, kbeg, kend)) return;   // uniform over the block
 
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int wm = wave / WN, wn = wave % WN;

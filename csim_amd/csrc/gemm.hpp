@@ -43,7 +43,8 @@ struct GemmProb {
   // optional K range per 64 x 64 block of outputs: int pair [lo, hi) at
   // kr[2 * (mb * kr_ld + nb)] bounds the nonzero products of every output of block
   // (mb, nb); a tile runs the whole K slabs of the union over its blocks (the
-  // products it skips are exact zeros).  lo >= hi: the block's outputs are zero.
+  // products it skips are exact zeros).  lo >= hi: the block's outputs are zero;
+  // lo = -1 (hi = 0): zero and never read, a tile of such blocks writes nothing.
   const int* kr;
   int kr_ld;
   unsigned kr_tag;     // digest of the table (tile tuning keys on it)

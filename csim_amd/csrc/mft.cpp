@@ -78,13 +78,29 @@ static std::vector<int> stage2_ranges(const MftField& F) {
   const bool use_out = so && so->ny == N && so->nx == M;
   if (!use_out && klo == 0 && khi == K) return {};
   const int mb = (M + kKrBlock - 1) / kKrBlock, nb = (N + kKrBlock - 1) / kKrBlock;
+  // rows the next MFT's first contraction reads in each column band (the union of
+  // the band's column supports, and of its pair's: a 128-wide tile covers two bands),
+  // widened to whole blocks as its slabs are
+  std::vector<int> rlo(mb, N), rhi(mb, 0);
+  if (use_out && F.out_read_sup) {
+    std::vector<int> blo(mb, N), bhi(mb, 0);
+    for (int i = 0; i < mb; i++)
+      for (int x = i * kKrBlock; x < std::min(M, (i + 1) * kKrBlock); x++)
+        if (so->lo[x] <= so->hi[x]) { blo[i] = std::min(blo[i], so->lo[x]); bhi[i] = std::max(bhi[i], so->hi[x] + 1); }
+    for (int i = 0; i < mb; i++) {
+      const int p = std::min(i ^ 1, mb - 1);
+      rlo[i] = std::min(blo[i], blo[p]) / kKrBlock * kKrBlock;
+      rhi[i] = (std::max(bhi[i], bhi[p]) + kKrBlock - 1) / kKrBlock * kKrBlock;
+    }
+  }
   std::vector<int> r((size_t)2 * mb * nb);
   for (int i = 0; i < mb; i++)
     for (int j = 0; j < nb; j++) {
       bool live = !use_out;
       for (int x = i * kKrBlock; !live && x < std::min(M, (i + 1) * kKrBlock); x++)
         live = so->lo[x] <= so->hi[x] && so->lo[x] < (j + 1) * kKrBlock && so->hi[x] >= j * kKrBlock;
-      r[2 * ((size_t)i * nb + j)] = live ? klo : K;
+      const bool read = !(use_out && F.out_read_sup) || (j * kKrBlock < rhi[i] && (j + 1) * kKrBlock > rlo[i]);
+      r[2 * ((size_t)i * nb + j)] = live ? klo : (read ? K : -1);
       r[2 * ((size_t)i * nb + j) + 1] = live ? khi : 0;
     }
   return r;
@@ -449,7 +465,7 @@ int MftPlan::build(const std::vector<MftField>& fields, csx_precision p, hipStre
       for (size_t b = 0; 2 * b < t.size(); b++) {
         const int i = (int)(b / nb), j = (int)(b % nb);
         const int rows = std::min(kKrBlock, M - i * kKrBlock), cols = std::min(kKrBlock, N - j * kKrBlock);
-        w += (double)rows * cols * std::max(0, t[2 * b + 1] - t[2 * b]);
+        w += (double)rows * cols * std::max(0, t[2 * b + 1] - std::max(0, t[2 * b]));
       }
       return 8.0 * w;
     };

@@ -69,6 +69,9 @@ struct MftField {
   // optional zero patterns (sizes in_y x in_x and out_y x out_x; null: dense): the
   // input field's, and the output's when the epilogue mask zeroes it there
   std::shared_ptr<const Support> in_sup, out_sup;
+  // the output feeds an MFT that reads it only within out_sup (its first contraction's
+  // K ranges): zero blocks that range never reaches are not written at all
+  bool out_read_sup = false;
   // point source applied by this MFT to its input: E_in * in_scale *
   // exp(2 pi i (tilt_x x + tilt_y y)) (cycles per unit length).  With the DFT
   // kernel's exp(-2 pi i xo x / lf) the phase ramp is the kernel of an output grid
