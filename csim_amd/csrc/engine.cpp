@@ -324,9 +324,14 @@ struct RealMap {
     auto it = sup.find((long)cur.ny * 65536 + cur.nx);
     return it == sup.end() ? nullptr : it->second;
   }
+  std::map<long, DevBuf> dev8;    // a byte copy when every value is 0 or 1
   const float* f32_for(const Geom& cur) const {
     auto it = dev32.find((long)cur.ny * 65536 + cur.nx);
     return it == dev32.end() ? nullptr : it->second.as<float>();
+  }
+  const unsigned char* u8_for(const Geom& cur) const {
+    auto it = dev8.find((long)cur.ny * 65536 + cur.nx);
+    return it == dev8.end() ? nullptr : it->second.as<unsigned char>();
   }
   int device_for(Chain& ch, const Geom& cur, const double** out) {
     long key = (long)cur.ny * 65536 + cur.nx;
@@ -374,6 +379,16 @@ struct RealMap {
         m32[i] = (float)m[i];
         exact = (double)m32[i] == m[i];
       }
+      bool binary = true;
+      for (size_t i = 0; i < m.size() && binary; i++) binary = m[i] == 0.0 || m[i] == 1.0;
+      if (binary) {
+        std::vector<unsigned char> m8(m.size());
+        for (size_t i = 0; i < m.size(); i++) m8[i] = m[i] == 1.0 ? 1 : 0;
+        DevBuf b8;
+        CSX_TRY(b8.reserve(m8.size()));
+        CSX_HIP(hipMemcpy(b8.p, m8.data(), m8.size(), hipMemcpyHostToDevice));
+        dev8[key] = std::move(b8);
+      }
       if (exact) {
         DevBuf b32;
         CSX_TRY(b32.reserve(m32.size() * sizeof(float)));
@@ -392,6 +407,7 @@ struct RealMap {
     dev.clear();
     sup.clear();
     dev32.clear();
+    dev8.clear();
     return CSX_OK;
   }
   int load(const std::string& path) {
@@ -616,7 +632,8 @@ static int compile_mft(Chain& ch, Compile& c, const std::vector<Geom>& gout_per_
                        const std::vector<double>& lf_per_lam, const std::string& label,
                        const std::vector<MftField>* masks = nullptr, bool out_conv = false,
                        const double* epi_mask_1pi = nullptr, bool expand = false,
-                       std::shared_ptr<const Support> epi_sup = nullptr, const float* epi_mask_f32 = nullptr) {
+                       std::shared_ptr<const Support> epi_sup = nullptr, const float* epi_mask_f32 = nullptr,
+                       const unsigned char* epi_mask_u8 = nullptr) {
   CSX_TRY(flush_pointwise(ch, c, /*to_conv=*/true));
   // expand: the fields are one candidate's copy; the MFT writes every candidate's
   // output (its own mask each) from that copy, whose first contraction the plan
@@ -674,8 +691,10 @@ static int compile_mft(Chain& ch, Compile& c, const std::vector<Geom>& gout_per_
       m.mask_kind = mk.mask_kind; m.mask = mk.mask;
       m.mrows = mk.mrows; m.mcols = mk.mcols; m.row0 = mk.row0; m.col0 = mk.col0;
     } else if (epi_mask_1pi) {
-      m.mask_kind = epi_mask_f32 ? MASK_REAL_1PI_F32 : MASK_REAL_1PI;
-      m.mask = epi_mask_f32 ? (const void*)epi_mask_f32 : (const void*)epi_mask_1pi;
+      // the narrowest exact copy of the mask (bytes for a binary stop, then float)
+      m.mask_kind = epi_mask_u8 ? MASK_REAL_1PI_U8 : (epi_mask_f32 ? MASK_REAL_1PI_F32 : MASK_REAL_1PI);
+      m.mask = epi_mask_u8 ? (const void*)epi_mask_u8
+                           : (epi_mask_f32 ? (const void*)epi_mask_f32 : (const void*)epi_mask_1pi);
       m.mrows = go.ny; m.mcols = go.nx; m.row0 = 0; m.col0 = 0;
       if (zero_skip) m.out_sup = epi_sup;
       m.out_read_sup = zero_skip && out_conv;   // the next MFT reads it through epi_sup
@@ -1076,6 +1095,7 @@ struct DoubleFraunhoferElem : Element {
     const double* lyot = nullptr;
     std::shared_ptr<const Support> lyot_sup;
     const float* lyot32 = nullptr;
+    const unsigned char* lyot8 = nullptr;
     int claim = 0;
     if (Element* nx = ch.elem_at(index + 1)) {
       if (auto* me = dynamic_cast<MaskElem*>(nx)) {
@@ -1083,6 +1103,7 @@ struct DoubleFraunhoferElem : Element {
           CSX_TRY(me->map.device_for(ch, E, &lyot));
           lyot_sup = me->map.support_for(E);
           lyot32 = me->map.f32_for(E);
+          lyot8 = me->map.u8_for(E);
           claim = 1;
         }
       }
@@ -1090,9 +1111,10 @@ struct DoubleFraunhoferElem : Element {
     bool next_mft = is_mft_element(ch.elem_at(index + 1 + claim), c);
     std::vector<Geom> back(nl, E);
     if (babinet) {
-      CSX_TRY(compile_babinet(ch, c, E, gf, lfo, bw, next_mft, lyot, lyot32, lyot_sup));
+      CSX_TRY(compile_babinet(ch, c, E, gf, lfo, bw, next_mft, lyot, lyot32, lyot8, lyot_sup));
     } else {
-      CSX_TRY(compile_mft(ch, c, back, lfo, type + " out " + name, nullptr, next_mft, lyot, false, lyot_sup, lyot32));
+      CSX_TRY(compile_mft(ch, c, back, lfo, type + " out " + name, nullptr, next_mft, lyot, false, lyot_sup, lyot32,
+                          lyot8));
     }
     c.claimed = claim;
     return CSX_OK;
@@ -1104,7 +1126,8 @@ struct DoubleFraunhoferElem : Element {
   // output, which the per-candidate window MFT adds in its epilogue before the Lyot mask.
   int compile_babinet(Chain& ch, Compile& c, const Geom& E, const std::vector<Geom>& gf,
                       const std::vector<double>& lfo, const BabinetWindow& bw, bool next_mft, const double* lyot,
-                      const float* lyot32, const std::shared_ptr<const Support>& lyot_sup) {
+                      const float* lyot32, const unsigned char* lyot8,
+                      const std::shared_ptr<const Support>& lyot_sup) {
     const int T = terms_of(ch.precision());
     const bool c64 = ch.precision() == CSX_FP32, mixed = T || c64;
     const bool out_conv = next_mft && mixed;
@@ -1164,8 +1187,8 @@ struct DoubleFraunhoferElem : Element {
       m.lambda_f = lfo[l];
       m.add = static_cast<const char*>(base.p) + (size_t)fi * E.npix() * eo;
       if (lyot) {
-        m.mask_kind = lyot32 ? MASK_REAL_1PI_F32 : MASK_REAL_1PI;
-        m.mask = lyot32 ? (const void*)lyot32 : (const void*)lyot;
+        m.mask_kind = lyot8 ? MASK_REAL_1PI_U8 : (lyot32 ? MASK_REAL_1PI_F32 : MASK_REAL_1PI);
+        m.mask = lyot8 ? (const void*)lyot8 : (lyot32 ? (const void*)lyot32 : (const void*)lyot);
         m.mrows = E.ny; m.mcols = E.nx; m.row0 = 0; m.col0 = 0;
         const char* zs = std::getenv("CSX_ZERO_SKIP");
         if (!(zs && zs[0] == '0')) {

@@ -105,8 +105,10 @@ __device__ __forceinline__ void epilogue_value(const GemmProb& P, int m, int n, 
         vi = vr * w.y + vi * w.x;
         vr = tr;
       } else {
-        const double w = P.mask_kind == MASK_REAL_1PI_F32 ? (double)gin<float>(P.mask)[off] : gin<double>(P.mask)[off];
-        if (P.mask_kind == MASK_REAL_1PI || P.mask_kind == MASK_REAL_1PI_F32) {   // csim mask: (1+i) w
+        const double w = P.mask_kind == MASK_REAL_1PI_U8    ? (double)gin<unsigned char>(P.mask)[off]
+                         : P.mask_kind == MASK_REAL_1PI_F32 ? (double)gin<float>(P.mask)[off]
+                                                            : gin<double>(P.mask)[off];
+        if (P.mask_kind != MASK_REAL) {   // csim mask: (1+i) w
           double tr = (vr - vi) * w;
           vi = (vr + vi) * w;
           vr = tr;

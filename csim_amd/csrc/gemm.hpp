@@ -7,9 +7,11 @@
 namespace csx {
 
 // MASK_COMPLEX_CLIP: complex factor inside the mask rectangle, zero outside it.
-// MASK_REAL_1PI_F32: MASK_REAL_1PI with the mask stored in float (values exact in float).
+// MASK_REAL_1PI_F32 / _U8: MASK_REAL_1PI with the mask stored in float (values exact in
+// float) or in bytes (values 0 and 1 only).
 enum MaskKind : int {
-  MASK_NONE = 0, MASK_COMPLEX = 1, MASK_REAL_1PI = 2, MASK_REAL = 3, MASK_COMPLEX_CLIP = 4, MASK_REAL_1PI_F32 = 5
+  MASK_NONE = 0, MASK_COMPLEX = 1, MASK_REAL_1PI = 2, MASK_REAL = 3, MASK_COMPLEX_CLIP = 4, MASK_REAL_1PI_F32 = 5,
+  MASK_REAL_1PI_U8 = 6
 };
 enum OutKind : int { OUT_C128 = 0, OUT_C64 = 1, OUT_PLANES = 2 };
 
