@@ -400,6 +400,7 @@ int gemm_f32(const GemmProb* probs_dev, int n, int total_tiles, hipStream_t s) {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 
 // ------------------------------------------------- BF16-plane MFMA GEMM --
 // Both operands arrive as BF16 planes (split once: DFT kernels at plan time,
@@ -725,6 +726,100 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
   // column).  Chunk cc (8 columns) of tile row ml sits at cc ^ (2 ((ml >> 2) & 3)):
   // the 4 rows one store instruction covers land on distinct banks.
   constexpr bool kStaged = 2 * TERMS * BM * BN <= (DB ? 2 : 1) * STAGE;
+  if (TERMS == 2 && BN % 64 == 0 && kStaged && P.out_kind == OUT_PLANES && !P.transC && P.terms == TERMS) {
+    // The FP32 sums go through LDS (row ml, column nl ^ 16 (ml & 3)); then each thread
+    // finishes 8 consecutive outputs of one row with 16-byte reads of the addend planes
+    // and of a whole-grid real mask, the epilogue arithmetic of epilogue_value, the BF16
+    // split, and one 16-byte store per plane.
+    float2* st = reinterpret_cast<float2*>(lds);
+    __syncthreads();                               // every wave is done reading operand tiles
+#pragma unroll
+    for (int i = 0; i < IM; i++)
+#pragma unroll
+      for (int j = 0; j < JN; j++)
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+          const int ml = wm * (BM / WM) + i * MF + Mf::drow(lane, v);
+          const int nl = wn * (BN / WN) + j * MF + Mf::dcol(lane);
+          st[ml * BN + (nl ^ ((ml & 3) << 4))] = make_float2(accR[i][j][v], accI[i][j][v]);
+        }
+    __syncthreads();
+    const bool empty = nslab == 0;
+    const int mk = P.mask_kind;
+    const bool vmask = mk == MASK_NONE || ((mk == MASK_REAL_1PI || mk == MASK_REAL_1PI_F32 || mk == MASK_REAL_1PI_U8 ||
+                                            mk == MASK_REAL) && P.mm0 == 0 && P.mn0 == 0 && P.mM >= M && P.mN >= N);
+    constexpr int CPR = BN / 8;
+    GLOBAL unsigned short* C = gout<unsigned short>(P.C);
+    for (int e = t; e < BM * CPR; e += NT) {
+      const int ml = e / CPR, cc = e % CPR;
+      const int m = m0 + ml, n = n0 + cc * 8;      // N is a multiple of 8: chunks are whole
+      if (m >= M || n >= N) continue;
+      const long off = (long)m * P.ldc + n;
+      float av[4][8];                              // addend planes
+      if (P.addend && !empty) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const u32x4 w = *reinterpret_cast<const GLOBAL u32x4*>(gin<unsigned short>(P.addend) + q * P.sC + off);
+#pragma unroll
+          for (int k = 0; k < 4; k++) {
+            av[q][2 * k] = bf16_val((unsigned short)(w[k] & 0xFFFFu));
+            av[q][2 * k + 1] = bf16_val((unsigned short)(w[k] >> 16));
+          }
+        }
+      }
+      double wm8[8];                               // mask values (whole-grid real masks)
+      if (vmask && mk != MASK_NONE && !empty) {
+        const long mo = (long)m * P.ldm + n;
+        if (mk == MASK_REAL_1PI_U8) {
+          const u32x2 w = *reinterpret_cast<const GLOBAL u32x2*>(gin<unsigned char>(P.mask) + mo);
+#pragma unroll
+          for (int k = 0; k < 8; k++) wm8[k] = (double)((w[k >> 2] >> (8 * (k & 3))) & 0xFFu);
+        } else if (mk == MASK_REAL_1PI_F32) {
+#pragma unroll
+          for (int k = 0; k < 8; k++) wm8[k] = (double)gin<float>(P.mask)[mo + k];
+        } else {
+#pragma unroll
+          for (int k = 0; k < 8; k++) wm8[k] = gin<double>(P.mask)[mo + k];
+        }
+      }
+      unsigned short h[4][8];
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const float2 a = st[ml * BN + ((cc * 8 + k) ^ ((ml & 3) << 4))];
+        double vr = 0.0, vi = 0.0;
+        if (!empty) {
+          if (vmask) {
+            vr = P.alpha_re * (double)a.x - P.alpha_im * (double)a.y;
+            vi = P.alpha_re * (double)a.y + P.alpha_im * (double)a.x;
+            if (P.addend) {
+              vr += (double)av[0][k] + (double)av[2][k];
+              vi += (double)av[1][k] + (double)av[3][k];
+            }
+            if (mk == MASK_REAL) {
+              vr *= wm8[k]; vi *= wm8[k];
+            } else if (mk != MASK_NONE) {          // csim mask: (1+i) w
+              const double tr = (vr - vi) * wm8[k];
+              vi = (vr + vi) * wm8[k];
+              vr = tr;
+            }
+          } else {
+            epilogue_value(P, m, n + k, (double)a.x, (double)a.y, vr, vi);
+          }
+        }
+        float xr = (float)vr, xi = (float)vi;
+        h[0][k] = bf16_rne(xr);
+        h[1][k] = bf16_rne(xi);
+        h[2][k] = bf16_rne(xr - bf16_val(h[0][k]));
+        h[3][k] = bf16_rne(xi - bf16_val(h[1][k]));
+      }
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        *reinterpret_cast<GLOBAL u32x4*>(C + q * P.sC + off) =
+            u32x4{(unsigned)h[q][0] | ((unsigned)h[q][1] << 16), (unsigned)h[q][2] | ((unsigned)h[q][3] << 16),
+                  (unsigned)h[q][4] | ((unsigned)h[q][5] << 16), (unsigned)h[q][6] | ((unsigned)h[q][7] << 16)};
+    }
+    return;
+  }
   if (kStaged && P.out_kind == OUT_PLANES && !P.transC && P.terms == TERMS) {
     __syncthreads();                               // every wave is done reading operand tiles
 #pragma unroll
