@@ -147,7 +147,7 @@ static std::string signature(const std::vector<GemmProb>& v, csx_precision p) {
 }
 
 // K splits worth trying for a launch on a bm x bn tile: two-term planes whose tiles do
-// not fill the chip twice over and whose K is long enough to share (1 = no split).
+// not fill the chip four times over and whose K is long enough to share (1 = no split).
 static int split_candidate(const std::vector<GemmProb>& v, csx_precision p, int bm, int bn) {
   if (p != CSX_BF16X3) return 1;
   if (const char* e = std::getenv("CSX_GEMM_KSPLIT")) return std::max(1, std::atoi(e));
@@ -157,8 +157,8 @@ static int split_candidate(const std::vector<GemmProb>& v, csx_precision p, int 
     tiles += (long)((g.M + bm - 1) / bm) * ((g.N + bn - 1) / bn);
     kmin = std::min(kmin, g.K);
   }
-  if (tiles >= 512 || kmin < 512) return 1;
-  int k = (int)std::min<long>(8, (1024 + tiles - 1) / tiles);
+  if (tiles >= 1024 || kmin < 512) return 1;
+  int k = (int)std::min<long>(8, (2048 + tiles - 1) / tiles);
   while (k > 1 && kmin / k < 128) k--;
   return std::max(k, 1);
 }
