@@ -596,7 +596,10 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
     }
   };
 
-  Acc accR[IM][JN], accI[IM][JN];
+  // Three terms: the products of the two leading (fixed-point) parts go to their own
+  // accumulators hR/hI, whose sums over one K step are exact (planes.hpp); both sets
+  // are added into the FP64 sums dR/dI after every K step.
+  Acc accR[IM][JN], accI[IM][JN], hR[IM][JN], hI[IM][JN];
   double dR[TERMS == 3 ? IM : 1][TERMS == 3 ? JN : 1][TERMS == 3 ? NV : 1];
   double dI[TERMS == 3 ? IM : 1][TERMS == 3 ? JN : 1][TERMS == 3 ? NV : 1];
 #pragma unroll
@@ -604,7 +607,7 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
 #pragma unroll
     for (int j = 0; j < JN; j++)
 #pragma unroll
-      for (int v = 0; v < NV; v++) { accR[i][j][v] = 0.f; accI[i][j][v] = 0.f; }
+      for (int v = 0; v < NV; v++) { accR[i][j][v] = 0.f; accI[i][j][v] = 0.f; hR[i][j][v] = 0.f; hI[i][j][v] = 0.f; }
   if (TERMS == 3) {
 #pragma unroll
     for (int i = 0; i < (TERMS == 3 ? IM : 1); i++)
@@ -638,20 +641,22 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
         const bf16x8 ain = __builtin_bit_cast(bf16x8, a[2 * ta + 1] ^ sgn);
 #pragma unroll
         for (int tb = 0; tb < TERMS - ta; tb++) {
+          Acc (&xR)[IM][JN] = (TERMS == 3 && ta == 0 && tb == 0) ? hR : accR;
+          Acc (&xI)[IM][JN] = (TERMS == 3 && ta == 0 && tb == 0) ? hI : accI;
           // two sweeps over j: updates of one accumulator are 2 JN MFMAs apart
 #pragma unroll
           for (int j = 0; j < JN; j++) {
             const bf16x8 br = __builtin_bit_cast(bf16x8, b[2 * tb][j]);
             const bf16x8 bi = __builtin_bit_cast(bf16x8, b[2 * tb + 1][j]);
-            accR[i][j] = Mf::mma(ar, br, accR[i][j]);
-            accI[i][j] = Mf::mma(ar, bi, accI[i][j]);
+            xR[i][j] = Mf::mma(ar, br, xR[i][j]);
+            xI[i][j] = Mf::mma(ar, bi, xI[i][j]);
           }
 #pragma unroll
           for (int j = 0; j < JN; j++) {
             const bf16x8 br = __builtin_bit_cast(bf16x8, b[2 * tb][j]);
             const bf16x8 bi = __builtin_bit_cast(bf16x8, b[2 * tb + 1][j]);
-            accR[i][j] = Mf::mma(ain, bi, accR[i][j]);
-            accI[i][j] = Mf::mma(ai, br, accI[i][j]);
+            xR[i][j] = Mf::mma(ain, bi, xR[i][j]);
+            xI[i][j] = Mf::mma(ai, br, xI[i][j]);
           }
         }
       }
@@ -663,8 +668,11 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
         for (int j = 0; j < (TERMS == 3 ? JN : 1); j++) {
 #pragma unroll
           for (int v = 0; v < (TERMS == 3 ? NV : 1); v++) {
-            dR[i][j][v] += accR[i][j][v]; dI[i][j][v] += accI[i][j][v];
-            accR[i][j][v] = 0.f; accI[i][j][v] = 0.f;
+            dR[i][j][v] += (double)hR[i][j][v];
+            dR[i][j][v] += (double)accR[i][j][v];
+            dI[i][j][v] += (double)hI[i][j][v];
+            dI[i][j][v] += (double)accI[i][j][v];
+            accR[i][j][v] = 0.f; accI[i][j][v] = 0.f; hR[i][j][v] = 0.f; hI[i][j][v] = 0.f;
           }
         }
     }
@@ -820,7 +828,87 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
     }
     return;
   }
-  if (kStaged && P.out_kind == OUT_PLANES && !P.transC && P.terms == TERMS) {
+  if (TERMS == 3 && P.out_kind == OUT_PLANES) {
+    // Three-term planes (the next contraction's operand, K along n): every value of a
+    // window of kPlaneWindow consecutive columns of a row gets its leading part on the
+    // window's fixed-point grid (planes.hpp).  A row's window lies in one wave: the
+    // kPlaneWindow / MF subtiles j of a window, MF lanes each (Mf::dcol).
+    constexpr int JW = kPlaneWindow / MF;
+    static_assert(TERMS != 3 || (JN % JW == 0 && (BN / WN) % kPlaneWindow == 0), "window within a wave");
+    double vr[IM][JN][NV], vi[IM][JN][NV];
+#pragma unroll
+    for (int i = 0; i < IM; i++)
+#pragma unroll
+      for (int j = 0; j < JN; j++)
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+          const int m = m0 + wm * (BM / WM) + i * MF + Mf::drow(lane, v);
+          const int n = n0 + wn * (BN / WN) + j * MF + Mf::dcol(lane);
+          vr[i][j][v] = 0.0; vi[i][j][v] = 0.0;
+          if (nslab > 0 && m < M && n < N)
+            epilogue_value(P, m, n, dR[TERMS == 3 ? i : 0][TERMS == 3 ? j : 0][v],
+                           dI[TERMS == 3 ? i : 0][TERMS == 3 ? j : 0][v], vr[i][j][v], vi[i][j][v]);
+        }
+    int ew[IM][JN / JW][NV];
+#pragma unroll
+    for (int i = 0; i < IM; i++)
+#pragma unroll
+      for (int w = 0; w < JN / JW; w++)
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+          float mx = 0.0f;
+#pragma unroll
+          for (int j = w * JW; j < (w + 1) * JW; j++)
+            mx = fmaxf(mx, (float)fmax(fabs(vr[i][j][v]), fabs(vi[i][j][v])));
+          ew[i][w][v] = window_exponent(window_max<MF>(mx));
+        }
+    GLOBAL unsigned short* C = gout<unsigned short>(P.C);
+    if (kStaged && !P.transC && P.terms == TERMS) {
+      __syncthreads();                             // every wave is done reading operand tiles
+#pragma unroll
+      for (int i = 0; i < IM; i++)
+#pragma unroll
+        for (int j = 0; j < JN; j++)
+#pragma unroll
+          for (int v = 0; v < NV; v++) {
+            const int ml = wm * (BM / WM) + i * MF + Mf::drow(lane, v);
+            const int nl = wn * (BN / WN) + j * MF + Mf::dcol(lane);
+            const int pos = ml * BN + ((((nl >> 3) ^ (((ml >> 2) & 3) << 1))) << 3) + (nl & 7);
+            unsigned short hr[3], hi[3];
+            split3(vr[i][j][v], ew[i][j / JW][v], hr);
+            split3(vi[i][j][v], ew[i][j / JW][v], hi);
+#pragma unroll
+            for (int tt = 0; tt < 3; tt++) {
+              lds[(2 * tt) * BM * BN + pos] = hr[tt];
+              lds[(2 * tt + 1) * BM * BN + pos] = hi[tt];
+            }
+          }
+      __syncthreads();
+      constexpr int CPR = BN / 8;
+      for (int e = t; e < 2 * TERMS * BM * CPR; e += NT) {
+        const int q = e / (BM * CPR), rem = e % (BM * CPR), ml = rem / CPR, cc = rem % CPR;
+        const int m = m0 + ml, n = n0 + cc * 8;    // N is a multiple of 8: chunks are whole
+        if (m < M && n < N)
+          *reinterpret_cast<GLOBAL u32x4*>(C + q * P.sC + (long)m * P.ldc + n) =
+              *reinterpret_cast<const u32x4*>(lds + q * BM * BN + ml * BN + ((cc ^ (((ml >> 2) & 3) << 1)) << 3));
+      }
+      return;
+    }
+#pragma unroll
+    for (int i = 0; i < IM; i++)
+#pragma unroll
+      for (int j = 0; j < JN; j++)
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+          const int m = m0 + wm * (BM / WM) + i * MF + Mf::drow(lane, v);
+          const int n = n0 + wn * (BN / WN) + j * MF + Mf::dcol(lane);
+          if (m < M && n < N)
+            store_planes3(C, P.transC ? (long)n * P.ldc + m : (long)m * P.ldc + n, P.sC, vr[i][j][v], vi[i][j][v],
+                          ew[i][j / JW][v]);
+        }
+    return;
+  }
+  if (TERMS == 2 && kStaged && P.out_kind == OUT_PLANES && !P.transC && P.terms == TERMS) {
     __syncthreads();                               // every wave is done reading operand tiles
 #pragma unroll
     for (int i = 0; i < IM; i++)
@@ -946,17 +1034,22 @@ int gemm_bf16planes(const GemmProb* probs_dev, int n, int total_tiles, int terms
   return CSX_OK;
 }
 
+// Three-term planes: npix is a multiple of kPlaneWindow (whole windows per column of a
+// field whose row count is), and the windows are 32 consecutive lanes of the loop.
 __global__ void k_split_planes(const double2* __restrict__ in, unsigned short* __restrict__ out, long npix,
                                long total, int terms) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
     long f = i / npix, p = i - f * npix;
     double2 v = in[i];
-    store_planes(out + f * 2 * terms * npix, p, npix, terms, v.x, v.y);
+    int e = 0;
+    if (terms == 3) e = window_exponent(window_max<kPlaneWindow>((float)fmax(fabs(v.x), fabs(v.y))));
+    store_planes_w(out + f * 2 * terms * npix, p, npix, terms, v.x, v.y, e);
   }
 }
 int split_c128_to_planes(const void* in, void* out, long npix, int nfield, int terms, hipStream_t s) {
   long total = npix * nfield;
   if (total <= 0) return CSX_OK;
+  if (terms == 3 && npix % kPlaneWindow) return fail(CSX_ERR_ARG, "split_c128_to_planes: pixels per field not whole windows");
   int blocks = (int)std::min<long>(8192, (total + 255) / 256);
   hipLaunchKernelGGL(k_split_planes, dim3(blocks), dim3(256), 0, s, (const double2*)in, (unsigned short*)out, npix,
                      total, terms);
@@ -977,7 +1070,8 @@ __global__ void k_dft_matrices(const DftSpec* __restrict__ specs, int nspec) {
     double f = t - rint(t);
     double sn, cs;
     sincospi(-2.0 * f, &sn, &cs);
-    if (S.terms > 0) store_planes(static_cast<unsigned short*>(S.out), e, total, S.terms, cs, sn);
+    // |cos|, |sin| <= 1: exponent 0 (unit 2^-8) bounds every window of three-term planes
+    if (S.terms > 0) store_planes_w(static_cast<unsigned short*>(S.out), e, total, S.terms, cs, sn, 0);
     else if (S.out_c64) static_cast<float2*>(S.out)[e] = make_float2((float)cs, (float)sn);
     else static_cast<double2*>(S.out)[e] = make_double2(cs, sn);
   }

@@ -323,6 +323,9 @@ int MftPlan::build(const std::vector<MftField>& fields, csx_precision p, hipStre
     if (f.in_kind != want_in) return fail(CSX_ERR_ARG, "mft: input storage does not match the precision");
     if (T && (f.in_x.n % 8 || f.in_y.n % 8 || f.out_y.n % 8))
       return fail(CSX_ERR_UNSUPPORTED, "mft: split-BF16 precisions need input sizes and output rows divisible by 8");
+    // three-term planes are read in whole windows of 32 along every K (planes.hpp)
+    if (T == 3 && (f.in_x.n % 32 || f.in_y.n % 32 || f.out_y.n % 32))
+      return fail(CSX_ERR_UNSUPPORTED, "mft: bf16x6 needs input sizes and output rows divisible by 32");
   }
   // distinct DFT kernels
   std::map<DftKey, size_t> index;     // key -> element offset

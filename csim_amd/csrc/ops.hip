@@ -148,7 +148,13 @@ __global__ void k_pointwise(double2* __restrict__ E, PwChain pc) {
     }
     if (pc.out_kind == PW_OUT_C128) Ef[p] = f64x2{er, ei};
     else if (pc.out_kind == PW_OUT_C64) gout<f32x2>(pc.out)[(long)f * npix + p] = f32x2{(float)er, (float)ei};
-    else store_planes(gout<unsigned short>(pc.out) + (long)f * 2 * pc.terms * npix, p, npix, pc.terms, er, ei);
+    else {
+      // three-term planes: windows of 32 consecutive pixels of a column (rows and the
+      // grid stride are multiples of 32, so a window's lanes run the loop together)
+      int e = 0;
+      if (pc.terms == 3) e = window_exponent(window_max<kPlaneWindow>((float)fmax(fabs(er), fabs(ei))));
+      store_planes_w(gout<unsigned short>(pc.out) + (long)f * 2 * pc.terms * npix, p, npix, pc.terms, er, ei, e);
+    }
   }
 }
 // Pixel-major generation pass: a thread owns 2 consecutive pixels of every field.
@@ -260,6 +266,20 @@ __global__ __launch_bounds__(256) void k_pointwise_gen(double2* __restrict__ E, 
         // BF16 planes of field f: plane q at (2 terms f + q) npix; two pixels = one 32-bit store
         GLOBAL unsigned* d = reinterpret_cast<GLOBAL unsigned*>(gout<unsigned short>(pc.out) + f * 2 * pc.terms * npix + p0);
         const int stride = npix / 2;
+        if (pc.terms == 3) {
+          // windows of 32 pixels = 16 consecutive lanes (rows are a multiple of 32)
+          const float mx = (float)fmax(fmax(fabs(er[0]), fabs(ei[0])), fmax(fabs(er[1]), fabs(ei[1])));
+          const int e = window_exponent(window_max<kPlaneWindow / 2>(mx));
+          unsigned short h[2][2][3];
+#pragma unroll
+          for (int x = 0; x < 2; x++) { split3(er[x], e, h[x][0]); split3(ei[x], e, h[x][1]); }
+#pragma unroll
+          for (int t = 0; t < 3; t++) {
+            d[(2 * t) * stride] = (unsigned)h[0][0][t] | ((unsigned)h[1][0][t] << 16);
+            d[(2 * t + 1) * stride] = (unsigned)h[0][1][t] | ((unsigned)h[1][1][t] << 16);
+          }
+          continue;
+        }
         float xr[2] = {(float)er[0], (float)er[1]}, xi[2] = {(float)ei[0], (float)ei[1]};
         for (int t = 0; t < pc.terms; t++) {
           unsigned short hr[2], hi[2];
@@ -396,6 +416,8 @@ int pointwise_chain(double2* E, int nfield, const PwChain& pc, hipStream_t s) {
   if (pc.nops <= 0 || nfield <= 0) return CSX_OK;
   if (pc.out_kind != PW_OUT_C128 && !pc.out) return fail(CSX_ERR_ARG, "pointwise: no output buffer");
   long npix = (long)pc.rows * pc.cols;
+  if (pc.out_kind == PW_OUT_PLANES && pc.terms == 3 && pc.rows % kPlaneWindow)
+    return fail(CSX_ERR_UNSUPPORTED, "pointwise: three-term planes need a row count divisible by 32");
   if (pc.nl > 0) {
     if (pc.op[0].kind != PW_SOURCE || pc.nl > kMaxPwLam || npix % 2 || nfield % pc.nl)
       return fail(CSX_ERR_ARG, "pointwise: pixel-major pass preconditions");
@@ -1052,7 +1074,10 @@ __global__ void k_babinet_window(const void* __restrict__ E, int kind, int terms
     } else if (kind == 1) {
       gout<f32x2>(W)[f * wn + p] = f32x2{(float)wr, (float)wi};
     } else {
-      store_planes(gout<unsigned short>(W) + (long)f * 2 * terms * wn, p, wn, terms, wr, wi);
+      // three-term planes: windows of 32 pixels of a column (S is a multiple of 32)
+      int e = 0;
+      if (terms == 3) e = window_exponent(window_max<kPlaneWindow>((float)fmax(fabs(wr), fabs(wi))));
+      store_planes_w(gout<unsigned short>(W) + (long)f * 2 * terms * wn, p, wn, terms, wr, wi, e);
     }
   }
 }
@@ -1062,6 +1087,8 @@ int babinet_window(const void* E, int kind, int terms, long e_npix, int rows, co
   if (nfield <= 0) return CSX_OK;
   if (S <= 0 || row0 < 0 || col0 < 0 || nl <= 0 || ns <= 0 || ncm <= 0 || (kind == 2 && terms <= 0))
     return fail(CSX_ERR_ARG, "babinet_window: bad shape");
+  if (kind == 2 && terms == 3 && S % kPlaneWindow)
+    return fail(CSX_ERR_UNSUPPORTED, "babinet_window: three-term planes need a window size divisible by 32");
   const int blocks = std::min(64, (S * S + 255) / 256);
   hipLaunchKernelGGL(k_babinet_window, dim3(blocks, nfield), dim3(256), 0, s, E, kind, terms, e_npix, rows,
                      reinterpret_cast<const double2*>(masks), S, row0, col0, nl, ns, ncm, W);

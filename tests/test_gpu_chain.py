@@ -110,22 +110,29 @@ def _dark_hole_errors(ctx, w, precision):
 @pytest.mark.parametrize("precision", ["fp32", "bf16x3", "bf16x6"])
 def test_mixed_contrast_gate_on_dark_hole(ctx, precision):
     """1e-11 absolute contrast gate (3-10 lambda/D dark-hole mean) on the 9-wavelength
-    occulter chain; the per-pixel maximum is reported."""
+    occulter chain (mean C ~ 3e-6), every mixed precision; the per-pixel maximum is
+    reported."""
     w = W.workload2_dark()
     c, dmean, dmax = _dark_hole_errors(ctx, w, precision)
     print(f"{precision}: dark hole mean C {c:.4e}; |delta mean| {dmean:.3e}; max |delta C| per pixel {dmax:.3e}")
-    if precision in ("fp32", "bf16x6"):
-        assert dmean <= 1e-11
+    assert dmean <= 1e-11
 
 
-@pytest.mark.parametrize("precision", ["fp32", "bf16x6"])
-def test_workload2_mixed_relative_error(ctx, precision):
-    """The random 217-hex FPM leaves the region bright (C ~ 1e-2): the mixed path's
-    relative error of the region mean is what is bounded there."""
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "bf16x6"])
+def test_mixed_contrast_gate_on_bench_workload(ctx, precision):
+    """The bench's own workload: 9 wavelengths x 3 sources (0, 2, 6 lambda/D), the
+    random 217-hex FPM, 27 fields.  The 6 lambda/D source passes outside the FPM and
+    its core lies in the 3-10 lambda/D region (mean C ~ 7e-3), so the 1e-11 absolute
+    gate asks for a relative error of ~1.4e-9.  bf16x6 (leading parts on a
+    fixed-point grid, their products summed exactly) meets it; fp32 and bf16x3 round
+    every sum or operand at 2^-24 / 2^-17 and stay at a relative error below 1e-6."""
     w = W.workload2()
     c, dmean, dmax = _dark_hole_errors(ctx, w, precision)
-    print(f"{precision}: region mean C {c:.4e}; relative error {dmean / c:.3e}; max |delta C| {dmax:.3e}")
+    print(f"{precision}: region mean C {c:.4e}; |delta mean| {dmean:.3e} (relative {dmean / c:.3e}); "
+          f"max |delta C| per pixel {dmax:.3e}")
     assert dmean / c < 1e-6
+    if precision == "bf16x6":
+        assert dmean <= 1e-11
 
 
 @pytest.mark.parametrize("kind", ["doubleFraunhoferInterpFPM", "fpmPupToLyot"])
