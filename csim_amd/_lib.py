@@ -48,6 +48,7 @@ SIGNATURES = {
     "csx_ctx_synchronize": (_i, [_p]),
     "csx_last_error": (C.c_char_p, []),
     "csx_version": (C.c_char_p, []),
+    "csx_tune_report": (_i, [C.c_char_p, _sz]),
     "csx_chain_from_script": (_i, [_p, C.c_char_p, C.POINTER(_p)]),
     "csx_chain_destroy": (None, [_p]),
     "csx_chain_set_array": (_i, [_p, C.c_char_p, C.c_char_p, _p, _i, _i]),
@@ -141,6 +142,15 @@ def check(rc):
         msg = lib().csx_last_error()
         raise CsxError(rc, msg.decode() if msg else "")
     return rc
+
+
+def tune_report() -> dict:
+    """The contraction tuner's choices in this process (csx_tune_report)."""
+    import json
+    n = check(lib().csx_tune_report(None, 0))
+    buf = C.create_string_buffer(n + 1)
+    check(lib().csx_tune_report(buf, n + 1))
+    return json.loads(buf.value.decode())
 
 
 def last_error() -> str:

@@ -40,6 +40,18 @@ extern "C" {
 const char* csx_last_error(void) { return g_err.c_str(); }
 const char* csx_version(void) { return "csx 0.1 (gfx950)"; }
 
+int csx_tune_report(char* buf, size_t cap) {
+  CSX_GUARD_BEGIN
+  const std::string j = tune_report();
+  if (buf && cap) {
+    const size_t n = std::min(cap - 1, j.size());
+    std::memcpy(buf, j.data(), n);
+    buf[n] = 0;
+  }
+  return (int)j.size();
+  CSX_GUARD_END
+}
+
 int csx_ctx_create(int device, unsigned flags, csx_ctx** out) {
   if (!out) return fail(CSX_ERR_ARG, "null out");
   if (flags & ~CSX_CTX_DEFAULT_STREAM) return fail(CSX_ERR_ARG, "csx_ctx_create: unknown flags");

@@ -365,13 +365,14 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_f64db(const GemmProb* __r
 struct F64Variant {
   int bm, bn, threads;
   void (*kernel)(const GemmProb*, int);
+  const char* name;
 };
 static const F64Variant kVariantsF64[] = {
-    {64, 64, 256, k_gemm_nt<double>},
-    {64, 64, 256, k_gemm_f64db<64, 64, 2, 2>},
-    {64, 128, 256, k_gemm_f64db<64, 128, 2, 2>},
-    {128, 64, 256, k_gemm_f64db<128, 64, 2, 2>},
-    {128, 128, 512, k_gemm_f64db<128, 128, 2, 4>},
+    {64, 64, 256, k_gemm_nt<double>, "nt"},
+    {64, 64, 256, k_gemm_f64db<64, 64, 2, 2>, "64,64,2,2"},
+    {64, 128, 256, k_gemm_f64db<64, 128, 2, 2>, "64,128,2,2"},
+    {128, 64, 256, k_gemm_f64db<128, 64, 2, 2>, "128,64,2,2"},
+    {128, 128, 512, k_gemm_f64db<128, 128, 2, 4>, "128,128,2,4"},
 };
 int f64_variant_count() { return (int)(sizeof(kVariantsF64) / sizeof(kVariantsF64[0])); }
 int f64_variant_tile(int v, int* bm, int* bn) {
@@ -452,7 +453,17 @@ template <> struct MfmaBf16<32> {
 // Zero chunk that direct-to-LDS loads of k beyond K read instead of the matrix.
 static __device__ __align__(16) unsigned int kZeroChunk[4];
 
-template <int TERMS, int BM, int BN, int WM, int WN, bool DB, int WPE, int BK, int MF, bool DMA = false>
+// s_waitcnt with vmcnt(N) (expcnt and lgkmcnt not waited for): at most N vector
+// memory instructions of this wave still in flight.
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 15) | (((N >> 4) & 3) << 14) | 0x0F70);
+}
+
+// DMA: 0 loads through registers; 1 direct-to-LDS loads into a two-slot ring (the next
+// slab in flight during a slab's MFMAs); R >= 3 an R-slot ring, R - 1 slabs in flight.
+template <int TERMS, int BM, int BN, int WM, int WN, bool DB, int WPE, int BK, int MF, int DMA = 0>
 __global__ __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(WPE)))
 void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
   typedef MfmaBf16<MF> Mf;
@@ -466,8 +477,11 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
   constexpr int ITA = NP * BM * CH / NT, ITB = NP * BN * CH / NT;
   static_assert(BK == 32 || BK == 64, "slab depth");
   static_assert((NP * BM * CH) % NT == 0 && (NP * BN * CH) % NT == 0 && NT % CH == 0, "loader shape");
-  static_assert(TERMS == 2 || IM * JN <= 4, "FP64 flush registers");
-  __shared__ __align__(16) unsigned short lds[(DB ? 2 : 1) * STAGE];
+  static_assert(TERMS == 2 || IM * JN * Mf::NV <= 32, "FP64 flush registers");
+  constexpr int RING = DMA == 0 ? 0 : (DMA < 2 ? 2 : DMA);
+  static_assert(RING <= 4, "ring depth");
+  constexpr int NBUF = DMA ? RING : (DB ? 2 : 1);   // LDS stages
+  __shared__ __align__(16) unsigned short lds[NBUF * STAGE];
 
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   const GemmProb& P = probs[find_prob(probs, nprob, tile)];
@@ -661,7 +675,8 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
         }
       }
     }
-    if (TERMS == 3) {
+    // the leading products' sums are exact within one K window only: flushed after each
+    if (TERMS == 3 && ((kk + 1) * Mf::KSTEP) % kPlaneWindow == 0) {
 #pragma unroll
       for (int i = 0; i < (TERMS == 3 ? IM : 1); i++)
 #pragma unroll
@@ -669,27 +684,46 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
 #pragma unroll
           for (int v = 0; v < (TERMS == 3 ? NV : 1); v++) {
             dR[i][j][v] += (double)hR[i][j][v];
-            dR[i][j][v] += (double)accR[i][j][v];
             dI[i][j][v] += (double)hI[i][j][v];
-            dI[i][j][v] += (double)accI[i][j][v];
-            accR[i][j][v] = 0.f; accI[i][j][v] = 0.f; hR[i][j][v] = 0.f; hI[i][j][v] = 0.f;
+            hR[i][j][v] = 0.f; hI[i][j][v] = 0.f;
           }
         }
     }
    }
+    // the other products are 2^-8 of the leading ones: their FP32 sums are flushed per slab
+    if (TERMS == 3) {
+#pragma unroll
+      for (int i = 0; i < (TERMS == 3 ? IM : 1); i++)
+#pragma unroll
+        for (int j = 0; j < (TERMS == 3 ? JN : 1); j++) {
+#pragma unroll
+          for (int v = 0; v < (TERMS == 3 ? NV : 1); v++) {
+            dR[i][j][v] += (double)accR[i][j][v];
+            dI[i][j][v] += (double)accI[i][j][v];
+            accR[i][j][v] = 0.f; accI[i][j][v] = 0.f;
+          }
+        }
+    }
   };
 
   const int nslab = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
   if (nslab > 0) {
   if (DMA) {
-    dload(kbeg, 0);
-    __builtin_amdgcn_s_waitcnt(0x0F70);            // vmcnt(0): this wave's loads have landed
+    // slab s lives in slot s % RING; before slab s is read every wave waits until at
+    // most the loads of the slabs issued after it are in flight (NI instructions each)
+    auto wait_after = [&](int later) {
+      if (RING >= 4 && later >= 2) wait_vmcnt<(RING >= 4 ? 2 * NI : 0)>();
+      else if (RING >= 3 && later >= 1) wait_vmcnt<(RING >= 3 ? NI : 0)>();
+      else wait_vmcnt<0>();
+    };
+    for (int q = 0; q < RING - 1 && q < nslab; q++) dload(kbeg + q * BK, q);
+    wait_after(min(RING - 1, nslab) - 1);
     __syncthreads();
     for (int s = 0; s < nslab; s++) {
-      const int cur = s & 1;
-      if (s + 1 < nslab) dload(kbeg + (s + 1) * BK, cur ^ 1);
-      compute(lds + cur * STAGE);
-      __builtin_amdgcn_s_waitcnt(0x0F70);
+      const int nx = s + RING - 1;                 // slot (s - 1) % RING: read by every wave before the last barrier
+      if (nx < nslab) dload(kbeg + nx * BK, nx % RING);
+      compute(lds + (s % RING) * STAGE);
+      if (s + 1 < nslab) wait_after(min(nx, nslab - 1) - (s + 1));
       __syncthreads();
     }
   } else if (DB) {
@@ -733,7 +767,7 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
   // contiguous bytes of one plane (the MFMA layout gives a lane 4 rows of one
   // column).  Chunk cc (8 columns) of tile row ml sits at cc ^ (2 ((ml >> 2) & 3)):
   // the 4 rows one store instruction covers land on distinct banks.
-  constexpr bool kStaged = 2 * TERMS * BM * BN <= (DB ? 2 : 1) * STAGE;
+  constexpr bool kStaged = 2 * TERMS * BM * BN <= NBUF * STAGE;
   if (TERMS == 2 && BN % 64 == 0 && kStaged && P.out_kind == OUT_PLANES && !P.transC && P.terms == TERMS) {
     // The FP32 sums go through LDS (row ml, column nl ^ 16 (ml & 3)); then each thread
     // finishes 8 consecutive outputs of one row with 16-byte reads of the addend planes
@@ -963,29 +997,51 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
 struct PlanesVariant {
   int bm, bn, threads;
   void (*kernel)(const GemmProb*, int);
+  const char* name;    // template arguments: the tuner cache keys its choices on them
 };
+#define CSX_PLANES(BM, BN, TH, ...) {BM, BN, TH, k_gemm_planes<__VA_ARGS__>, #__VA_ARGS__}
 // (the last template argument is the minimum waves per SIMD the registers must allow)
 static const PlanesVariant kVariants2[] = {
-    {64, 64, 256, k_gemm_planes<2, 64, 64, 2, 2, false, 3, 32, 16>},
-    {128, 128, 256, k_gemm_planes<2, 128, 128, 2, 2, true, 1, 32, 16>},
-    {128, 128, 512, k_gemm_planes<2, 128, 128, 2, 4, true, 2, 32, 16>},
-    {128, 64, 256, k_gemm_planes<2, 128, 64, 2, 2, false, 2, 32, 16>},
-    {64, 64, 256, k_gemm_planes<2, 64, 64, 2, 2, true, 2, 32, 16>},
-    {64, 64, 256, k_gemm_planes<2, 64, 64, 2, 2, false, 2, 64, 16>},
-    {64, 64, 256, k_gemm_planes<2, 64, 64, 2, 2, true, 2, 64, 16>},
-    {128, 128, 256, k_gemm_planes<2, 128, 128, 2, 2, false, 1, 32, 32>},
-    {128, 128, 512, k_gemm_planes<2, 128, 128, 2, 4, true, 2, 32, 32>},
-    {128, 64, 256, k_gemm_planes<2, 128, 64, 2, 2, false, 2, 32, 32>},
-    {64, 128, 256, k_gemm_planes<2, 64, 128, 2, 2, false, 2, 32, 32>},
-    {64, 64, 256, k_gemm_planes<2, 64, 64, 2, 2, true, 2, 32, 16, true>},
-    {128, 128, 512, k_gemm_planes<2, 128, 128, 2, 4, true, 2, 32, 16, true>},
-    {128, 64, 256, k_gemm_planes<2, 128, 64, 2, 2, true, 2, 32, 16, true>},
+    CSX_PLANES(64, 64, 256, 2, 64, 64, 2, 2, false, 3, 32, 16),
+    CSX_PLANES(128, 128, 256, 2, 128, 128, 2, 2, true, 1, 32, 16),
+    CSX_PLANES(128, 128, 512, 2, 128, 128, 2, 4, true, 2, 32, 16),
+    CSX_PLANES(128, 64, 256, 2, 128, 64, 2, 2, false, 2, 32, 16),
+    CSX_PLANES(64, 64, 256, 2, 64, 64, 2, 2, true, 2, 32, 16),
+    CSX_PLANES(64, 64, 256, 2, 64, 64, 2, 2, false, 2, 64, 16),
+    CSX_PLANES(64, 64, 256, 2, 64, 64, 2, 2, true, 2, 64, 16),
+    CSX_PLANES(128, 128, 256, 2, 128, 128, 2, 2, false, 1, 32, 32),
+    CSX_PLANES(128, 128, 512, 2, 128, 128, 2, 4, true, 2, 32, 32),
+    CSX_PLANES(128, 64, 256, 2, 128, 64, 2, 2, false, 2, 32, 32),
+    CSX_PLANES(64, 128, 256, 2, 64, 128, 2, 2, false, 2, 32, 32),
+    CSX_PLANES(64, 64, 256, 2, 64, 64, 2, 2, true, 2, 32, 16, true),
+    CSX_PLANES(128, 128, 512, 2, 128, 128, 2, 4, true, 2, 32, 16, true),
+    CSX_PLANES(128, 64, 256, 2, 128, 64, 2, 2, true, 2, 32, 16, true),
+    CSX_PLANES(64, 64, 256, 2, 64, 64, 2, 2, true, 2, 32, 16, 3),
+    CSX_PLANES(64, 64, 256, 2, 64, 64, 2, 2, true, 2, 32, 16, 4),
+    CSX_PLANES(128, 64, 256, 2, 128, 64, 2, 2, true, 2, 32, 16, 3),
 };
 static const PlanesVariant kVariants3[] = {
-    {64, 64, 256, k_gemm_planes<3, 64, 64, 2, 2, false, 1, 32, 16>},
-    {128, 64, 512, k_gemm_planes<3, 128, 64, 4, 2, false, 2, 32, 16>},
-    {64, 64, 256, k_gemm_planes<3, 64, 64, 2, 2, true, 1, 32, 16>},
+    CSX_PLANES(64, 64, 256, 3, 64, 64, 2, 2, false, 1, 32, 16),
+    CSX_PLANES(128, 64, 512, 3, 128, 64, 4, 2, false, 2, 32, 16),
+    CSX_PLANES(64, 64, 256, 3, 64, 64, 2, 2, true, 1, 32, 16),
+    CSX_PLANES(64, 64, 256, 3, 64, 64, 2, 2, true, 1, 32, 16, true),
+    CSX_PLANES(64, 64, 256, 3, 64, 64, 2, 2, false, 1, 32, 32),
+    CSX_PLANES(128, 64, 256, 3, 128, 64, 2, 2, false, 1, 32, 32),
+    CSX_PLANES(64, 128, 256, 3, 64, 128, 2, 2, false, 1, 32, 32),
+    CSX_PLANES(128, 64, 512, 3, 128, 64, 4, 2, true, 2, 32, 16, true),
+    CSX_PLANES(64, 64, 256, 3, 64, 64, 2, 2, true, 1, 32, 16, 3),
+    CSX_PLANES(64, 64, 256, 3, 64, 64, 2, 2, true, 1, 32, 32, 3),
 };
+
+std::string gemm_variant_names() {
+  std::string s = "f64:";
+  for (const auto& v : kVariantsF64) { s += v.name; s += ';'; }
+  s += "p2:";
+  for (const auto& v : kVariants2) { s += v.name; s += ';'; }
+  s += "p3:";
+  for (const auto& v : kVariants3) { s += v.name; s += ';'; }
+  return s;
+}
 
 int planes_variant_count(int terms) {
   return terms == 2 ? (int)(sizeof(kVariants2) / sizeof(kVariants2[0]))

@@ -2,6 +2,8 @@
 // Fourier transform (csim zoomFft, src/lib/csim_fft.cpp:670-826, restated as
 // E_out = s * A_y * E_in * B_x^T).
 #pragma once
+#include <string>
+
 #include "common.hpp"
 
 namespace csx {
@@ -76,6 +78,8 @@ int gemm_f32(const GemmProb* probs_dev, int n, int total_tiles, hipStream_t s);
 // (OUT_PLANES).  `variant` selects a tile shape (planes_variant_tile gives the
 // tile the problem list must be planned with).
 int planes_variant_count(int terms);
+// Every tile variant of every precision, in table order (the tuner cache checks it).
+std::string gemm_variant_names();
 int planes_variant_tile(int terms, int variant, int* bm, int* bn);
 int gemm_bf16planes(const GemmProb* probs_dev, int n, int total_tiles, int terms, int variant, hipStream_t s);
 // complex128 -> `terms` BF16 planes per complex element (hi, mid, lo of re and im),

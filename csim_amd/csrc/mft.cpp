@@ -9,6 +9,10 @@
 #include <set>
 #include <tuple>
 
+#include <dlfcn.h>
+#include <mutex>
+#include <unistd.h>
+
 namespace csx {
 
 int terms_of(csx_precision p) { return p == CSX_BF16X3 ? 2 : (p == CSX_BF16X6 ? 3 : 0); }
@@ -130,19 +134,106 @@ static int forced_variant(csx_precision p) {
   return v >= 0 && v < n_variants(p) ? v : -1;
 }
 
-// Autotuner cache: problem-list signature -> chosen (variant, K splits) (process lifetime).
-static std::map<std::string, std::pair<int, int>>& tune_cache() {
-  static std::map<std::string, std::pair<int, int>> c;
-  return c;
+// Autotuner cache: problem-list signature -> chosen (variant, K splits).  The choices
+// persist in a JSON file so that every process (and every run on a fresh device of the
+// same kind) launches the same kernels: CSX_TUNE_CACHE names it, by default
+// tune_cache.json beside libcsx.so.  The file is read on first use; a choice timed in
+// this process is added and the file rewritten (through a temporary and a rename).
+// Entries of another kernel set (gemm_variant_names digest) are ignored.  Layout:
+//   {"format": 1, "kernels": "<digest>",
+//    "choices": {
+//     "<signature>": [variant, ksplit],
+//     ...}}
+struct TuneState {
+  std::mutex mu;
+  bool loaded = false;
+  std::string path, digest;
+  std::map<std::string, std::pair<int, int>> choices;
+  std::map<std::string, int> source;   // signature -> 0 file, 1 timed here (this process's lookups)
+};
+static TuneState& tune_state() {
+  static TuneState t;
+  return t;
+}
+static std::string kernel_digest() {
+  unsigned long long h = 1469598103934665603ull;   // FNV-1a, 64 bit
+  for (unsigned char c : gemm_variant_names()) h = (h ^ c) * 1099511628211ull;
+  char b[32];
+  std::snprintf(b, sizeof b, "%016llx", h);
+  return b;
+}
+static std::string default_cache_path() {
+  if (const char* e = std::getenv("CSX_TUNE_CACHE")) return e;
+  Dl_info info;
+  if (dladdr(reinterpret_cast<void*>(&kernel_digest), &info) && info.dli_fname) {
+    std::string f = info.dli_fname;
+    const size_t k = f.rfind('/');
+    return (k == std::string::npos ? std::string(".") : f.substr(0, k)) + "/tune_cache.json";
+  }
+  return "tune_cache.json";
+}
+static void load_tune_cache(TuneState& t) {
+  if (t.loaded) return;
+  t.loaded = true;
+  t.path = default_cache_path();
+  t.digest = kernel_digest();
+  std::FILE* f = std::fopen(t.path.c_str(), "r");
+  if (!f) return;
+  std::string text;
+  char buf[4096];
+  size_t n;
+  while ((n = std::fread(buf, 1, sizeof buf, f)) > 0) text.append(buf, n);
+  std::fclose(f);
+  const std::string kk = "\"kernels\": \"";
+  const size_t kp = text.find(kk);
+  if (kp == std::string::npos || text.compare(kp + kk.size(), t.digest.size(), t.digest) != 0) return;
+  size_t pos = text.find("\"choices\"");
+  if (pos == std::string::npos) return;
+  pos += 9;
+  while (true) {                                      // "<signature>": [v, k] entries
+    const size_t q0 = text.find('"', pos);
+    if (q0 == std::string::npos) break;
+    const size_t q1 = text.find('"', q0 + 1);
+    if (q1 == std::string::npos) break;
+    int v = -1, k = 0;
+    if (std::sscanf(text.c_str() + q1 + 1, ": [%d, %d]", &v, &k) == 2 && v >= 0 && k >= 1)
+      t.choices[text.substr(q0 + 1, q1 - q0 - 1)] = {v, k};
+    pos = q1 + 1;
+  }
+}
+static void save_tune_cache(const TuneState& t) {
+  const std::string tmp = t.path + ".tmp" + std::to_string((long)getpid());
+  std::FILE* f = std::fopen(tmp.c_str(), "w");
+  if (!f) return;                                     // read-only location: in memory only
+  std::fprintf(f, "{\"format\": 1, \"kernels\": \"%s\",\n \"choices\": {", t.digest.c_str());
+  bool first = true;
+  for (const auto& kv : t.choices) {
+    std::fprintf(f, "%s\n  \"%s\": [%d, %d]", first ? "" : ",", kv.first.c_str(), kv.second.first, kv.second.second);
+    first = false;
+  }
+  std::fprintf(f, "}}\n");
+  const bool ok = std::fclose(f) == 0;
+  if (!ok || std::rename(tmp.c_str(), t.path.c_str()) != 0) std::remove(tmp.c_str());
 }
 
+// precision, then the problems' shapes (and K-range table digests) in launch order,
+// runs of equal shapes written once with their count: "3:27*1024x256x256"
 static std::string signature(const std::vector<GemmProb>& v, csx_precision p) {
   std::string s = std::to_string((int)p);
+  std::string last;
+  int run = 0;
+  auto flush = [&]() {
+    if (run) s += ':' + std::to_string(run) + '*' + last;
+  };
   for (const auto& g : v) {
-    s += ':';
-    s += std::to_string(g.M) + 'x' + std::to_string(g.N) + 'x' + std::to_string(g.K);
-    if (g.kr) s += 'r' + std::to_string(g.kr_tag);
+    std::string one = std::to_string(g.M) + 'x' + std::to_string(g.N) + 'x' + std::to_string(g.K);
+    if (g.kr) one += 'r' + std::to_string(g.kr_tag);
+    if (one == last) { run++; continue; }
+    flush();
+    last = one;
+    run = 1;
   }
+  flush();
   return s;
 }
 
@@ -225,10 +316,21 @@ static int choose_variant(std::vector<GemmProb>& v, csx_precision p, DevBuf& dev
     return upload(v, p, *variant, dev, s, tiles);
   }
   const std::string key = signature(v, p);
-  auto it = tune_cache().find(key);
-  if (it != tune_cache().end()) {
-    *variant = it->second.first;
-    *ksplit = it->second.second;
+  TuneState& ts = tune_state();
+  {
+    std::lock_guard<std::mutex> lock(ts.mu);
+    load_tune_cache(ts);
+    auto it = ts.choices.find(key);
+    if (it != ts.choices.end() && it->second.first < nv && it->second.second <= std::max(1, kc[it->second.first])) {
+      *variant = it->second.first;
+      *ksplit = it->second.second;
+      if (!ts.source.count(key)) ts.source[key] = 0;
+    } else {
+      *variant = -1;
+    }
+  }
+  if (*variant >= 0) {
+    if (*ksplit > 1) CSX_TRY(ws.reserve(sizeof(float) * 2 * (size_t)*ksplit * total_elems(v)));
     set_split(v, *ksplit, ws.as<float>());
     return upload(v, p, *variant, dev, s, tiles);
   }
@@ -268,7 +370,18 @@ static int choose_variant(std::vector<GemmProb>& v, csx_precision p, DevBuf& dev
   size_t best = 0;
   for (size_t c = 1; c < cand.size(); c++)
     if (tbest[c] < tbest[best] * 0.98f) best = c;
-  tune_cache()[key] = cand[best];
+  if (std::getenv("CSX_TUNE_LOG")) {
+    std::fprintf(stderr, "csx tune %s:", key.c_str());
+    for (size_t c = 0; c < cand.size(); c++)
+      std::fprintf(stderr, " v%d/k%d %.1fus%s", cand[c].first, cand[c].second, tbest[c] * 1e3, c == best ? "*" : "");
+    std::fprintf(stderr, "\n");
+  }
+  {
+    std::lock_guard<std::mutex> lock(ts.mu);
+    ts.choices[key] = cand[best];
+    ts.source[key] = 1;
+    save_tune_cache(ts);
+  }
   *variant = cand[best].first;
   *ksplit = cand[best].second;
   set_split(v, *ksplit, ws.as<float>());
@@ -520,6 +633,28 @@ int MftPlan::build(const std::vector<MftField>& fields, csx_precision p, hipStre
   elems1_ = total_elems(p1);
   elems2_ = total_elems(p2);
   return CSX_OK;
+}
+
+// Choices looked up or timed in this process: {"file": path, "from_file": n, "timed": n,
+// "choices": {"signature": [variant, ksplit, source], ...}} (source 0 file, 1 timed).
+std::string tune_report() {
+  TuneState& t = tune_state();
+  std::lock_guard<std::mutex> lock(t.mu);
+  load_tune_cache(t);
+  int nf = 0, nt = 0;
+  std::string c;
+  for (const auto& kv : t.source) {
+    const auto& ch = t.choices[kv.first];
+    (kv.second ? nt : nf)++;
+    char b[64];
+    std::snprintf(b, sizeof b, "[%d, %d, %d]", ch.first, ch.second, kv.second);
+    c += (c.empty() ? "\"" : ", \"") + kv.first + "\": " + b;
+  }
+  std::string path;
+  for (char ch : t.path) { if (ch == '"' || ch == '\\') path += '\\'; path += ch; }
+  return "{\"file\": \"" + path + "\", \"kernels\": \"" + t.digest + "\", \"entries\": " +
+         std::to_string(t.choices.size()) + ", \"from_file\": " + std::to_string(nf) +
+         ", \"timed\": " + std::to_string(nt) + ", \"choices\": {" + c + "}}";
 }
 
 static const char* prec_name(csx_precision p) {

@@ -108,5 +108,7 @@ class MftPlan {
 };
 
 int launch_gemm(const GemmProb* probs_dev, int n, int tiles, csx_precision p, hipStream_t s, int variant = 0);
+// JSON summary of the tuner's choices in this process (file, counts, per signature).
+std::string tune_report();
 
 }  // namespace csx

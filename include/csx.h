@@ -64,6 +64,14 @@ int csx_ctx_set_stream(csx_ctx* ctx, void* hip_stream);
 int csx_ctx_synchronize(csx_ctx* ctx);
 const char* csx_last_error(void);
 const char* csx_version(void);
+/* Contraction tile tuner (no reference counterpart; csim's nearest relative is FFTW's
+ * wisdom file, src/lib/csim_fft.cpp:20-36): the (tile variant, K split) of every
+ * grouped launch is timed once per problem signature and kept in a JSON file
+ * (CSX_TUNE_CACHE, default tune_cache.json beside the library) that later processes
+ * read, so every run launches the same kernels.  JSON summary of this process's
+ * lookups {"file", "kernels", "from_file", "timed", "choices": {signature: [variant,
+ * ksplit, source]}}; returns its length (buf NULL: only the length). */
+int csx_tune_report(char* buf, size_t cap);
 
 /* ------------------------------------------------------ element / script API */
 /* Parse a csim input script (src/lib/csim_parser.cpp:60-190 grammar; blocks

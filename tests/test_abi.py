@@ -71,3 +71,35 @@ def test_hex_rasterization_matches_oracle_exactly():
         on, oh = oracle.hex_interp_data(design, 40, 8, 1.0 / 30, 0.5, 0.9)
         assert nhex == hx.size
         assert np.array_equal(nsub, on) and np.array_equal(hexnum, oh)
+
+
+def _tune_report_with(path):
+    import json
+    import subprocess
+    import sys
+    code = "import json; from csim_amd import _lib; print(json.dumps(_lib.tune_report()))"
+    env = dict(os.environ, CSX_TUNE_CACHE=str(path))
+    r = subprocess.run([sys.executable, "-c", code], cwd=ROOT, env=env, capture_output=True, text=True, check=True)
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def test_tuner_cache_file_is_read_for_its_kernel_set(tmp_path):
+    """The tuner's persisted choices are read back when the file names the library's
+    kernel set, and ignored when it names another (a changed variant table)."""
+    digest = _tune_report_with(tmp_path / "none.json")["kernels"]
+    assert re.fullmatch(r"[0-9a-f]{16}", digest)
+    good = tmp_path / "good.json"
+    good.write_text('{"format": 1, "kernels": "%s",\n "choices": {\n  "2:27*1024x256x256": [3, 1],\n'
+                    '  "3:9*128x128x1024r12": [7, 1]}}\n' % digest)
+    assert _tune_report_with(good)["entries"] == 2
+    other = tmp_path / "other.json"
+    other.write_text(good.read_text().replace(digest, "0123456789abcdef"))
+    assert _tune_report_with(other)["entries"] == 0
+
+
+def test_committed_tuner_cache_matches_the_kernel_set():
+    """csim_amd/tune_cache.json (the choices timed on an MI355X) names this build's
+    kernel set, so GPU runs launch the kernels it lists instead of re-timing."""
+    path = os.path.join(ROOT, "csim_amd", "tune_cache.json")
+    assert os.path.exists(path)
+    assert _tune_report_with(path)["entries"] > 0

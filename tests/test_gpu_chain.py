@@ -265,32 +265,78 @@ def test_linear_optimizer_jacobian_and_step(ctx):
     assert res.best_value < 0.5 * res.history[0][2]
 
 
-def test_chain_group_uneven_pupils_matches_chains(ctx, monkeypatch):
-    """A 512^2 and a 1024^2 DE batch (same script, radii halved) in one group: every
-    contraction stage is one grouped launch over both sizes, and the contrasts equal
-    those of the chains run alone.  One tile variant for every launch: the tuner
-    could otherwise pick kernels that sum K in another order (or split it) for the
-    grouped and the single launches, which agree only to FP32 rounding."""
-    from csim_amd.chain import ChainGroup
-    monkeypatch.setenv("CSX_GEMM_VARIANT", "0")
-    rng = np.random.default_rng(31)
-    chains, alone = [], []
+def _de_batch_chains(ctx, precision, n_cand=32, n_lambda=9, seed=31):
+    """The DE-generation batch with uneven pupils: n_cand candidates x n_lambda
+    wavelengths on a 512^2 and on a 1024^2 pupil chain (radii halved on the small one)."""
+    rng = np.random.default_rng(seed)
+    chains, sags = [], []
     for n in (512, 1024):
-        w = W.workload3(n=n, n_lambda=3)
-        t = rng.random((4, w.meta["n_hex"])) * 1e-6
-        ch = build(ctx, w, "bf16x3")
-        ch.set_candidates("fpm", W.thickness_to_sag(t))
-        c, _ = ch.contrast()
-        alone.append(c)
+        w = W.workload3(n=n, n_lambda=n_lambda)
+        s = W.thickness_to_sag(rng.random((n_cand, w.meta["n_hex"])) * 1e-6)
+        ch = build(ctx, w, precision)
+        ch.set_candidates("fpm", s)
         chains.append(ch)
+        sags.append((w, s))
+    return chains, sags
+
+
+@pytest.mark.parametrize("precision,tol", [("bf16x3", 2e-5), ("bf16x6", 1e-8), ("fp64", 1e-12)])
+def test_chain_group_uneven_pupils_matches_chains(ctx, precision, tol):
+    """The full DE generation of uneven pupils: 32 candidates x 9 wavelengths on a 512^2
+    and 32 x 9 on a 1024^2 pupil, 576 fields, in one group whose every contraction
+    stage is one grouped launch over both sizes.  Its contrasts equal those of the two
+    chains run alone, each launch on the tile variant the tuner chose for it (grouped
+    and single launches may sum K in another order: the tolerance is the precision's)."""
+    from csim_amd.chain import ChainGroup
+    chains, _ = _de_batch_chains(ctx, precision)
+    alone = [ch.contrast()[0] for ch in chains]
     g = ChainGroup(chains)
     got = g.contrast()
-    print("grouped MFT stages", g.merged_launches(), "contrasts", got)
+    ref = np.concatenate(alone)
+    err = float(np.max(np.abs(got - ref) / ref))
+    print(f"{precision}: grouped MFT stages {g.merged_launches()}, {got.size} candidates, "
+          f"contrast {ref.min():.3e}..{ref.max():.3e}, max rel diff grouped vs alone {err:.3e}")
     # pupil->FPM, FPM->Lyot (the shared unmasked field, then each candidate's hex window,
     # by Babinet), Lyot->image: two GEMM launches each
     assert g.merged_launches() == 4
-    ref = np.concatenate(alone)
-    assert np.max(np.abs(got - ref) / ref) < 1e-12
+    assert err < tol
+    g.close()
+    for ch in chains:
+        ch.close()
+
+
+def test_de_batch_mixed_gate_against_fp64(ctx):
+    """The same 576-field generation in bf16x6 against FP64: every candidate's
+    3-10 lambda/D region contrast within the 1e-11 absolute gate."""
+    from csim_amd.chain import ChainGroup
+    res = {}
+    for precision in ("fp64", "bf16x6"):
+        chains, _ = _de_batch_chains(ctx, precision)
+        g = ChainGroup(chains)
+        res[precision] = g.contrast()
+        g.close()
+        for ch in chains:
+            ch.close()
+    d = np.abs(res["bf16x6"] - res["fp64"])
+    print(f"bf16x6 vs fp64 over {d.size} candidates: contrast {res['fp64'].min():.3e}..{res['fp64'].max():.3e}, "
+          f"max |delta C| {d.max():.3e}")
+    assert d.max() <= 1e-11
+
+
+def test_de_batch_candidates_vs_oracle(ctx):
+    """One candidate of each pupil size of the DE batch (9 wavelengths) against the
+    oracle chain in FP64: fields to 1e-12 relative RMS, contrasts to 1e-12."""
+    chains, sags = _de_batch_chains(ctx, "fp64", n_cand=4)
+    for ch, (w, s) in zip(chains, sags):
+        c, _ = ch.contrast()
+        oc = OracleChain(w.script, w.arrays, sags=w.meta["sags"])
+        k = 2
+        worst = max(rel_rms(ch.field(k, 0, li), oc.field(0, li, cand_sags=s[k])) for li in range(9))
+        c_ref, _, _ = oc.contrast(cand_sags=s[k])
+        print(f"{w.meta['n']}^2 candidate {k}: worst field rel rms {worst:.3e}, contrast {c[k]:.10e} vs {c_ref:.10e}")
+        assert worst < 1e-12
+        assert abs(c[k] - c_ref) <= 1e-12 * c_ref
+        ch.close()
 
 
 @pytest.mark.parametrize("kind", ["doubleFraunhoferInterpFPM", "fpmPupToLyot"])

@@ -83,8 +83,8 @@ def test_mft_mixed_error_level(ctx, prec, tol):
     assert e < tol
 
 
-VARIANTS = {"bf16x3": 14, "bf16x6": 3}
-MFMA32 = {"bf16x3": {7, 8, 9, 10}, "bf16x6": set()}   # variants on the 32x32x16 MFMA (gemm.hip)
+VARIANTS = {"bf16x3": 17, "bf16x6": 10}
+MFMA32 = {"bf16x3": {7, 8, 9, 10}, "bf16x6": {4, 5, 6, 9}}   # variants on the 32x32x16 MFMA (gemm.hip)
 
 
 PLANES_SIZES = {"bf16x3": [(1024, 256, 2), (200, 72, 3), (96, 136, 2)],
