@@ -243,6 +243,90 @@ def test_hex_raster_invariants():
     assert np.allclose(m2[full], m1[full] ** 2, atol=1e-13)
 
 
+def _interior_pixel_hexes(hx, hy, lr, A, scale, gap, S, nsp, ps, fpm_radius):
+    """For every mask pixel whose sub-samples all land on design cells inside one hex's
+    inscribed circle, that hex (else -1).  Geometry of complexHexMaskFPM.cpp, stated
+    independently of the rasteriser: hex h is centred at design column 0.5 A + hx s
+    and row 0.5 A + hy s with s = 0.5 A / lr * fpmScaleFactor (make_hex_array,
+    :317-366), its inner radius is R = s (1 - hexGap) sqrt(3) / 2, and a mask pixel
+    (row i, column j) samples design row trunc((0.5 + x xc1) A + 0.5) at
+    x = (i - S/2 - 0.5 + (0.5 + a) / nSubPix) pixelScale and design column likewise
+    from j (make_interpolation_data, :372-469).  A hexagon contains its inscribed
+    circle, so these pixels are wholly one hex whatever the hexagon test."""
+    s = 0.5 * A / lr * scale
+    R = s * (1 - gap) * math.sqrt(3) / 2
+    Xc, Yc = 0.5 * A + hx * s, 0.5 * A + hy * s
+    xc1 = 0.5 / fpm_radius * scale
+    sub = (np.arange(S)[:, None] - S / 2 - 0.5 + (0.5 + np.arange(nsp)[None, :]) / nsp) * ps
+    idx = np.trunc((0.5 + sub * xc1) * A + 0.5)
+    lo, hi = idx.min(axis=1), idx.max(axis=1)
+    out = np.full((S, S), -1)
+    for i in range(S):
+        for j in range(S):
+            if lo[i] < 0 or hi[i] >= A or lo[j] < 0 or hi[j] >= A:
+                continue
+            yc, xc = 0.5 * (lo[i] + hi[i]), 0.5 * (lo[j] + hi[j])
+            h = int(np.argmin((Xc - xc) ** 2 + (Yc - yc) ** 2))
+            far = max(math.hypot(c - Xc[h], r - Yc[h]) for r in (lo[i], hi[i]) for c in (lo[j], hi[j]))
+            if far < R - 1e-9:
+                out[i, j] = h
+    return out, R, 1.0 / (xc1 * A)
+
+
+@pytest.mark.parametrize("layout,rings", [("csim", 5), ("standard", 3)])
+def test_hex_raster_geometry_pinned(layout, rings):
+    """The oracle's hex rasterisation against the geometry of complexHexMaskFPM.cpp.
+    (1) Pixels wholly inside one hex carry that hex's number with all nSubPix^2
+    samples: the hex lattice is not symmetric under x <-> y, so a transposed raster
+    fails (planted below).  (2) Each hex of the design array holds the area of a
+    hexagon of inner radius R = s (1 - hexGap) sqrt(3) / 2 in design cells, which a
+    hexStepPix or hexGap off by 10 % fails.  (3) On a design without gaps, each
+    interior hex (six neighbours; csim's slot 0 also takes the off-design samples of
+    a pixel, which only edge hexes meet) holds that area in samples of the mask."""
+    hx, hy, _ = oracle.hex_centers(rings, layout)
+    lr = rings if layout == "csim" else rings * math.sqrt(3) + 1
+    A, scale, gap, S, nsp, ps, rad = 600, 0.85, 0.03, 56, 6, 1.0 / 24, 1.0
+    design = oracle.hex_design_array(hx, hy, lr, A, scale, gap)
+    nsub, hexnum = oracle.hex_interp_data(design, S, nsp, ps, rad, scale)
+    want, R, cell = _interior_pixel_hexes(hx, hy, lr, A, scale, gap, S, nsp, ps, rad)
+    inside = want >= 0
+    assert inside.sum() > S * S // 4 and len(np.unique(want[inside])) >= 0.5 * hx.size
+    assert np.array_equal(hexnum[..., 0][inside], want[inside])
+    assert np.all(nsub[..., 0][inside] == nsp * nsp) and np.all(hexnum[..., 1][inside] == -1)
+    assert np.mean(hexnum.transpose(1, 0, 2)[..., 0][inside] != want[inside]) > 0.3
+    # (2) design cells per hex (hexes within the design array)
+    s = 0.5 * A / lr * scale
+    Xc, Yc = 0.5 * A + hx * s, 0.5 * A + hy * s
+    ext = 2 * R / math.sqrt(3) + 2
+    on_design = (np.abs(Xc - 0.5 * A) + ext < 0.5 * A) & (np.abs(Yc - 0.5 * A) + ext < 0.5 * A)
+    cells = np.bincount(design[design >= 0].ravel(), minlength=hx.size)[on_design]
+    area = 2 * math.sqrt(3) * R ** 2
+    assert on_design.sum() >= 7
+    assert np.max(np.abs(cells / area - 1)) < 1.0 / R
+    for wrong in (scale * 1.1, scale / 1.1):
+        bad = oracle.hex_design_array(hx, hy, lr, A, wrong, gap)
+        cb = np.bincount(bad[bad >= 0].ravel(), minlength=hx.size)[on_design]
+        assert np.max(np.abs(cb / area - 1)) > 0.1
+    # (3) samples per interior hex on a design without gaps
+    d0 = oracle.hex_design_array(hx, hy, lr, A, scale, 0.0)
+    n0, h0 = oracle.hex_interp_data(d0, S, nsp, ps, rad, scale)
+    counts = np.zeros(hx.size)
+    for k in range(3):
+        ok = h0[..., k] >= 0
+        np.add.at(counts, h0[..., k][ok], n0[..., k][ok])
+    R0 = s * math.sqrt(3) / 2
+    dist = np.hypot(hx[:, None] - hx[None, :], hy[:, None] - hy[None, :])
+    interior = (np.sum(np.abs(dist - math.sqrt(3)) < 1e-9, axis=1) == 6)
+    half = min(S / 2 * ps / cell, 0.5 * A)
+    interior &= (np.abs(Xc - 0.5 * A) + ext < half) & (np.abs(Yc - 0.5 * A) + ext < half)
+    assert interior.sum() >= 1
+    expect = 2 * math.sqrt(3) * R0 ** 2 * (cell / (ps / nsp)) ** 2
+    rel = np.abs(counts[interior] / expect - 1)
+    print(f"{layout}: {inside.sum()} interior pixels; design area error {np.max(np.abs(cells / area - 1)):.4f} "
+          f"over {on_design.sum()} hexes; sample area error {rel.max():.4f} over {interior.sum()} interior hexes")
+    assert rel.max() < 2.0 / R0
+
+
 # --------------------------------------------------------- known answer ---
 def test_linear_objective_known_answer():
     """optHexRespObjective on csim's shipped response matrix and DE optimum reproduces
