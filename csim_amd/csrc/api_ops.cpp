@@ -35,6 +35,7 @@ static int use_device(csx_ctx* ctx) {
   return CSX_OK;
 }
 
+
 extern "C" {
 
 const char* csx_last_error(void) { return g_err.c_str(); }
@@ -115,42 +116,61 @@ static int mft_fields(csx_grid in_x, csx_grid in_y, csx_grid out_x, csx_grid out
   return CSX_OK;
 }
 
+// Bytes of the working-precision copy of the input (none for FP64).
+static size_t mft_conv_bytes(csx_grid in_x, csx_grid in_y, int batch, csx_precision p) {
+  if (p == CSX_FP64 || batch <= 0) return 0;
+  const size_t e = p == CSX_FP32 ? sizeof(float2) : (size_t)4 * terms_of(p);
+  return Arena::round((size_t)batch * in_x.n * in_y.n * e);
+}
+
 size_t csx_mft_workspace_size(csx_grid in_x, csx_grid in_y, csx_grid out_x, csx_grid out_y, int batch,
                               csx_precision p) {
-  size_t e = p == CSX_FP64 ? sizeof(double2) : (p == CSX_FP32 ? sizeof(float2) : (size_t)4 * terms_of(p));
-  size_t conv = p == CSX_FP64 ? 0 : (size_t)batch * in_x.n * in_y.n * e;
-  return conv + (size_t)batch * out_y.n * in_x.n * e +
-         (size_t)batch * ((size_t)out_y.n * in_y.n + (size_t)out_x.n * in_x.n) * e;
+  return mft_conv_bytes(in_x, in_y, batch, p) + MftPlan::arena_bytes(in_x.n, in_y.n, out_x.n, out_y.n, batch, p);
+}
+
+// csx_mft over device memory from `arena` (work buffers and plan, nothing allocated)
+// or, with arena NULL, from the context's scratch and the plan's own buffers.
+static int mft_run(csx_ctx* ctx, csx_grid in_x, csx_grid in_y, csx_grid out_x, csx_grid out_y,
+                   const double* lambda_f, int batch, const void* d_in, void* d_out, csx_precision p, Arena* arena) {
+  if (batch <= 0 || !lambda_f || !d_in || !d_out) return fail(CSX_ERR_ARG, "csx_mft: empty batch or null pointer");
+  const int T = terms_of(p);
+  const void* src = d_in;
+  int kind = FK_C128;
+  size_t ein = sizeof(double2);
+  size_t n = (size_t)batch * in_x.n * in_y.n;
+  DevBuf conv;
+  conv.arena = arena;
+  DevBuf& work = arena ? conv : ctx->work;
+  if (p == CSX_FP32) {
+    CSX_TRY(work.reserve(n * sizeof(float2)));
+    CSX_TRY(convert_c128_to_c64(d_in, work.p, n, ctx->stream));
+    src = work.p; kind = FK_C64; ein = sizeof(float2);
+  } else if (T) {
+    CSX_TRY(work.reserve(n * 4 * T));
+    CSX_TRY(split_c128_to_planes(d_in, work.p, (long)in_x.n * in_y.n, batch, T, ctx->stream));
+    src = work.p; kind = FK_PLANES; ein = 4 * T;
+  }
+  std::vector<MftField> F;
+  CSX_TRY(mft_fields(in_x, in_y, out_x, out_y, lambda_f, batch, src, d_out, kind, ein, F));
+  MftPlan plan;
+  CSX_TRY(plan.build(F, p, ctx->stream, arena));
+  CSX_TRY(plan.run(ctx->stream));
+  CSX_HIP(hipStreamSynchronize(ctx->stream));   // plan buffers are released (or free for reuse) on return
+  return CSX_OK;
 }
 
 int csx_mft(csx_ctx* ctx, csx_grid in_x, csx_grid in_y, csx_grid out_x, csx_grid out_y, const double* lambda_f,
             int batch, const void* d_in, void* d_out, csx_precision p, void* d_work) {
   CSX_GUARD_BEGIN
   CSX_TRY(use_device(ctx));
-  (void)d_work;
-  const int T = terms_of(p);
-  const void* src = d_in;
-  int kind = FK_C128;
-  size_t ein = sizeof(double2);
-  size_t n = (size_t)batch * in_x.n * in_y.n;
-  if (p == CSX_FP32) {
-    CSX_TRY(ctx->work.reserve(n * sizeof(float2)));
-    CSX_TRY(convert_c128_to_c64(d_in, ctx->work.p, n, ctx->stream));
-    src = ctx->work.p; kind = FK_C64; ein = sizeof(float2);
-  } else if (T) {
-    CSX_TRY(ctx->work.reserve(n * 4 * T));
-    CSX_TRY(split_c128_to_planes(d_in, ctx->work.p, (long)in_x.n * in_y.n, batch, T, ctx->stream));
-    src = ctx->work.p; kind = FK_PLANES; ein = 4 * T;
-  }
-  std::vector<MftField> F;
-  CSX_TRY(mft_fields(in_x, in_y, out_x, out_y, lambda_f, batch, src, d_out, kind, ein, F));
-  MftPlan plan;
-  CSX_TRY(plan.build(F, p, ctx->stream));
-  CSX_TRY(plan.run(ctx->stream));
-  CSX_HIP(hipStreamSynchronize(ctx->stream));   // plan buffers are released on return
-  return CSX_OK;
+  if (!d_work) return mft_run(ctx, in_x, in_y, out_x, out_y, lambda_f, batch, d_in, d_out, p, nullptr);
+  Arena a;
+  a.base = static_cast<char*>(d_work);
+  a.cap = csx_mft_workspace_size(in_x, in_y, out_x, out_y, batch, p);
+  return mft_run(ctx, in_x, in_y, out_x, out_y, lambda_f, batch, d_in, d_out, p, &a);
   CSX_GUARD_END
 }
+
 
 // ------------------------------------------------------- FFT / Fresnel ---
 static int upload_doubles(csx_ctx* ctx, DevBuf& buf, const double* h, int n) {

@@ -26,23 +26,46 @@ int fail(int code, const std::string& msg);
     if (rc_ != CSX_OK) return rc_;      \
   } while (0)
 
-// Device buffer owned by the library (RAII, no copies).
+// Device memory a caller hands over (a workspace): consecutive 256-byte aligned slices.
+struct Arena {
+  char* base = nullptr;
+  size_t cap = 0, off = 0;
+  static size_t round(size_t n) { return (n + 255) / 256 * 256; }
+};
+
+// Device buffer owned by the library (RAII, no copies), or a slice of an Arena.
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  Arena* arena = nullptr;   // set: reserve() takes slices of it and nothing is freed
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes), arena(o.arena) { o.p = nullptr; o.bytes = 0; o.arena = nullptr; }
   DevBuf& operator=(DevBuf&& o) noexcept {
-    if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+    if (this != &o) {
+      release();
+      p = o.p; bytes = o.bytes; arena = o.arena;
+      o.p = nullptr; o.bytes = 0; o.arena = nullptr;
+    }
     return *this;
   }
   ~DevBuf() { release(); }
-  void release() { if (p) (void)hipFree(p); p = nullptr; bytes = 0; }
-  // Grows (never shrinks) to at least n bytes; contents are not preserved.
+  void release() { if (p && !arena) (void)hipFree(p); p = nullptr; bytes = 0; }
+  // Grows (never shrinks) to at least n bytes; contents are not preserved.  With an
+  // arena the buffer becomes the arena's next slice (no allocation).
   int reserve(size_t n) {
     if (n <= bytes) return CSX_OK;
+    if (arena) {
+      const size_t need = Arena::round(n);
+      if (arena->off + need > arena->cap)
+        return fail(CSX_ERR_NOMEM, "workspace too small: " + std::to_string(arena->off + need) + " bytes needed, " +
+                                       std::to_string(arena->cap) + " given");
+      p = arena->base + arena->off;
+      arena->off += need;
+      bytes = n;
+      return CSX_OK;
+    }
     release();
     if (n == 0) return CSX_OK;
     hipError_t e = hipMalloc(&p, n);

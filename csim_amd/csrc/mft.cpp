@@ -296,12 +296,14 @@ static int run_stage(const GemmProb* dev, const std::vector<GemmProb>& v, int ti
 // better pass, and a later one must beat the best earlier one by 2 % (choices within
 // timing noise stay put).  The launches write the stage's own output buffers (and
 // its workspace) only, which the real run overwrites.
+// tune = false (a plan built in caller memory): no timing launches, no split-K
+// workspace; the persisted choice without its K split, else variant 0.
 static int choose_variant(std::vector<GemmProb>& v, csx_precision p, DevBuf& dev, DevBuf& ws, hipStream_t s,
-                          int* variant, int* ksplit, int* tiles) {
+                          int* variant, int* ksplit, int* tiles, bool tune = true) {
   const int nv = n_variants(p);
   std::vector<int> kc(nv, 1);   // K splits worth trying with each variant's tile
   int kmax = 1;
-  for (int var = 0; var < nv && p == CSX_BF16X3; var++) {
+  for (int var = 0; var < nv && p == CSX_BF16X3 && tune; var++) {
     int bm = 64, bn = 64;
     CSX_TRY(planes_variant_tile(terms_of(p), var, &bm, &bn));
     kc[var] = split_candidate(v, p, bm, bn);
@@ -311,7 +313,7 @@ static int choose_variant(std::vector<GemmProb>& v, csx_precision p, DevBuf& dev
   int forced = forced_variant(p);
   if (nv == 1 || forced >= 0) {
     *variant = nv == 1 ? 0 : forced;
-    *ksplit = std::getenv("CSX_GEMM_KSPLIT") ? kc[*variant] : 1;
+    *ksplit = std::getenv("CSX_GEMM_KSPLIT") && tune ? kc[*variant] : 1;
     set_split(v, *ksplit, ws.as<float>());
     return upload(v, p, *variant, dev, s, tiles);
   }
@@ -321,12 +323,14 @@ static int choose_variant(std::vector<GemmProb>& v, csx_precision p, DevBuf& dev
     std::lock_guard<std::mutex> lock(ts.mu);
     load_tune_cache(ts);
     auto it = ts.choices.find(key);
-    if (it != ts.choices.end() && it->second.first < nv && it->second.second <= std::max(1, kc[it->second.first])) {
+    if (it != ts.choices.end() && it->second.first < nv &&
+        (!tune || it->second.second <= std::max(1, kc[it->second.first]))) {
       *variant = it->second.first;
-      *ksplit = it->second.second;
+      *ksplit = tune ? it->second.second : 1;
       if (!ts.source.count(key)) ts.source[key] = 0;
     } else {
-      *variant = -1;
+      *variant = tune ? -1 : 0;
+      *ksplit = 1;
     }
   }
   if (*variant >= 0) {
@@ -418,8 +422,21 @@ struct DftKey {
 };
 }  // namespace
 
-int MftPlan::build(const std::vector<MftField>& fields, csx_precision p, hipStream_t s) {
+size_t MftPlan::arena_bytes(int in_x, int in_y, int out_x, int out_y, int batch, csx_precision p) {
+  const int T = terms_of(p);
+  const size_t esz = T ? (size_t)4 * T : (p == CSX_FP32 ? sizeof(float2) : sizeof(double2));
+  const size_t b = (size_t)std::max(batch, 0);
+  return Arena::round(b * ((size_t)out_y * in_y + (size_t)out_x * in_x) * esz) +   // DFT kernels
+         Arena::round(2 * b * sizeof(DftSpec)) +                                    // their specs
+         Arena::round(b * (size_t)out_y * in_x * esz) +                             // stage-1 outputs
+         2 * Arena::round(b * sizeof(GemmProb));                                    // problem arrays
+}
+
+int MftPlan::build(const std::vector<MftField>& fields, csx_precision p, hipStream_t s, Arena* arena) {
   prec_ = p;
+  for (DevBuf* d : {&dft_, &tbuf_, &probs1_, &probs2_, &specs_, &ws1_, &ws2_, &kr_}) {
+    if (arena && d->arena != arena) { d->release(); d->arena = arena; }
+  }
   nf_ = (int)fields.size();
   flops_ = flops1_ = flops2_ = bytes1_ = bytes2_ = 0;
   if (nf_ == 0) return CSX_OK;
@@ -628,8 +645,8 @@ int MftPlan::build(const std::vector<MftField>& fields, csx_precision p, hipStre
   spread_shapes(p1);
   spread_shapes(p2);
   // stage 2 reads stage 1's output, so stage 1 is tuned (and left written) first
-  CSX_TRY(choose_variant(p1, p, probs1_, ws1_, s, &var1_, &ks1_, &tiles1_));
-  CSX_TRY(choose_variant(p2, p, probs2_, ws2_, s, &var2_, &ks2_, &tiles2_));
+  CSX_TRY(choose_variant(p1, p, probs1_, ws1_, s, &var1_, &ks1_, &tiles1_, arena == nullptr));
+  CSX_TRY(choose_variant(p2, p, probs2_, ws2_, s, &var2_, &ks2_, &tiles2_, arena == nullptr));
   elems1_ = total_elems(p1);
   elems2_ = total_elems(p2);
   return CSX_OK;

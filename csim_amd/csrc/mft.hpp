@@ -86,7 +86,13 @@ class MftPlan {
  public:
   // Builds DFT kernel matrices (FP64 phases, stored at the working precision),
   // the stage-1 workspace and both problem arrays.  Field pointers are baked in.
-  int build(const std::vector<MftField>& fields, csx_precision p, hipStream_t s);
+  // With an arena every device buffer is a slice of it (no allocation) and no tile
+  // variant is timed: each launch takes the tuner's persisted choice, else variant 0,
+  // never split K.
+  int build(const std::vector<MftField>& fields, csx_precision p, hipStream_t s, Arena* arena = nullptr);
+  // Bytes an arena must hold for build() over `batch` fields of these grids (an upper
+  // bound: every field with its own DFT kernels and first contraction).
+  static size_t arena_bytes(int in_x, int in_y, int out_x, int out_y, int batch, csx_precision p);
   // Enqueues both GEMM stages (no allocation, no host sync: graph-capturable).
   int run(hipStream_t s, KernelTimer* timer = nullptr) const;
   size_t workspace_bytes() const { return dft_.bytes + tbuf_.bytes; }

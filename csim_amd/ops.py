@@ -113,17 +113,27 @@ def _grid(n, x0, dx):
     return Grid(int(n), float(x0), float(dx))
 
 
-def mft(ctx: Context, E, in_x, in_y, out_x, out_y, lambda_f, precision="fp64"):
-    """csim zoomFft in closed form on a batch.  Grids are (n, x0, dx) tuples."""
+def mft_workspace_size(in_x, in_y, out_x, out_y, batch, precision="fp64"):
+    return int(lib().csx_mft_workspace_size(_grid(*in_x), _grid(*in_y), _grid(*out_x), _grid(*out_y),
+                                            int(batch), PRECISIONS[precision]))
+
+
+def mft(ctx: Context, E, in_x, in_y, out_x, out_y, lambda_f, precision="fp64", work=None):
+    """csim zoomFft in closed form on a batch.  Grids are (n, x0, dx) tuples.  work:
+    None (the library allocates) or a device tensor of at least mft_workspace_size
+    bytes that holds every work buffer of the call."""
     torch = _torch()
     assert E.dtype == torch.complex128 and E.is_contiguous()
     batch = E.shape[0]
     assert E.shape[1] == in_x[0] and E.shape[2] == in_y[0]
     lf = np.ascontiguousarray(np.broadcast_to(np.asarray(lambda_f, dtype=np.float64), (batch,)))
     out = torch.empty((batch, out_x[0], out_y[0]), dtype=torch.complex128, device=E.device)
+    if work is not None:
+        assert work.is_cuda and work.numel() * work.element_size() >= mft_workspace_size(in_x, in_y, out_x, out_y,
+                                                                                          batch, precision)
     check(lib().csx_mft(ctx.handle, _grid(*in_x), _grid(*in_y), _grid(*out_x), _grid(*out_y),
                         lf.ctypes.data_as(C.c_void_p), batch, _ptr(E), _ptr(out),
-                        PRECISIONS[precision], None))
+                        PRECISIONS[precision], None if work is None else _ptr(work)))
     return out
 
 

@@ -235,8 +235,11 @@ typedef struct { int n; double x0; double dx; } csx_grid;
  * A_y[r][m] = exp(-2 pi i y_out[r] y_in[m] / lf),  B_x[c][n] likewise in x,
  * for a batch of fields.  lambda_f[b] is the signed lambda*focal length of field
  * b.  d_in: batch x in_y.n x in_x.n complex128; d_out: batch x out_y.n x out_x.n.
- * Fields are column-major (rows = y).  d_work: NULL (library scratch) or a device
- * buffer of at least csx_mft_workspace_size bytes. */
+ * Fields are column-major (rows = y).  d_work: NULL (the library allocates its
+ * buffers and times tile variants on first use of a shape) or a device buffer of at
+ * least csx_mft_workspace_size bytes, 256-byte aligned, which then holds every work
+ * buffer of the call (nothing allocated; no tile timing: the persisted tuner choice
+ * or the 64 x 64 tile).  A d_work buffer may be reused once the call has returned. */
 size_t csx_mft_workspace_size(csx_grid in_x, csx_grid in_y, csx_grid out_x, csx_grid out_y,
                               int batch, csx_precision p);
 int csx_mft(csx_ctx* ctx, csx_grid in_x, csx_grid in_y, csx_grid out_x, csx_grid out_y,

@@ -120,6 +120,28 @@ def test_mft_planes_tile_variants(ctx, monkeypatch, prec, tol, case):
             assert d < (tol / 10 if v in MFMA32[prec] else 1e-6)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3", "bf16x6"])
+def test_mft_caller_workspace(ctx, monkeypatch, prec):
+    """csx_mft with a caller workspace (d_work of csx_mft_workspace_size bytes, every
+    buffer carved from it) gives the fields of the call that allocates, bit for bit on
+    the same tile variant."""
+    import torch
+    rng = np.random.default_rng(17)
+    nin, nout, batch = 256, 64, 3
+    E = np.stack([rand_field(rng, nin, nin) for _ in range(batch)])
+    dx, lf = 1e-5, 650e-9
+    gi, go = sym(nin, dx), sym(nout, lf / (nin * dx) / 2)
+    lfs = [lf, lf * 1.05, lf * 0.95]
+    size = ops.mft_workspace_size(gi, gi, go, go, batch, prec)
+    work = torch.empty(size, dtype=torch.uint8, device="cuda")
+    monkeypatch.setenv("CSX_GEMM_VARIANT", "0")
+    ref = ops.to_host(ops.mft(ctx, ops.to_device(E), gi, gi, go, go, lfs, prec))
+    got = ops.to_host(ops.mft(ctx, ops.to_device(E), gi, gi, go, go, lfs, prec, work=work))
+    monkeypatch.delenv("CSX_GEMM_VARIANT")
+    print(f"{prec}: workspace {size} bytes, max |diff| {np.max(np.abs(got - ref)):.3e}")
+    assert np.array_equal(got, ref)
+
+
 @pytest.mark.parametrize("ksplit", [2, 4])
 @pytest.mark.parametrize("nin,nout,batch", [(1024, 128, 2), (200, 72, 3)])
 def test_mft_planes_split_k(ctx, monkeypatch, ksplit, nin, nout, batch):
