@@ -81,6 +81,8 @@ SIGNATURES = {
     "csx_chain_profile_json": (_i, [_p, C.c_char_p, _sz]),
     "csx_mft_workspace_size": (_sz, [Grid, Grid, Grid, Grid, _i, _i]),
     "csx_mft": (_i, [_p, Grid, Grid, Grid, Grid, _p, _i, _p, _p, _i, _p]),
+    "csx_mft_noalloc": (_i, [_p, Grid, Grid, Grid, Grid, _p, _i, _p, _p, _i]),
+    "csx_mft_noalloc_capacity": (_sz, []),
     "csx_fresnel_as": (_i, [_p, _p, _i, _i, _p, _d, _d]),
     "csx_fft2": (_i, [_p, _p, _i, _i, _i]),
     "csx_downsample": (_i, [_p, _p, _i, _i, _i, _i, _d, _p]),

@@ -35,6 +35,33 @@ static int use_device(csx_ctx* ctx) {
   return CSX_OK;
 }
 
+// Scratch of the allocation-free matrix Fourier transform (csx_mft_noalloc): every GPU
+// the library runs on holds one copy for the life of the process.  Host code uses the
+// per-device address hipGetSymbolAddress gives (scratch_ptr), never the symbol itself.
+// Nothing in it has to start at zero: each buffer carved from it is written before it
+// is read within the call.
+constexpr size_t kNoAllocScratchBytes = (size_t)128 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+static std::mutex g_scratch_mu;
+static std::vector<void*> g_scratch_ptr;   // by device index
+
+// Resolves this device's copy of `scratch` (the device is current).
+static int resolve_scratch(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (device < 0) return fail(CSX_ERR_ARG, "bad device");
+  if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize(device + 1, nullptr);
+  if (g_scratch_ptr[device]) return CSX_OK;
+  void* p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(CSX_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_ptr[device] = p;
+  return CSX_OK;
+}
+static void* scratch_ptr(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return device >= 0 && device < (int)g_scratch_ptr.size() ? g_scratch_ptr[device] : nullptr;
+}
 
 extern "C" {
 
@@ -64,6 +91,7 @@ int csx_ctx_create(int device, unsigned flags, csx_ctx** out) {
   csx_ctx* c = new (std::nothrow) csx_ctx();
   if (!c) return fail(CSX_ERR_NOMEM, "context allocation failed");
   c->device = device;
+  if (int rc = resolve_scratch(device); rc != CSX_OK) { delete c; return rc; }
   if (flags & CSX_CTX_DEFAULT_STREAM) {
     c->stream = nullptr;   // the null stream
     c->own_stream = false;
@@ -171,6 +199,24 @@ int csx_mft(csx_ctx* ctx, csx_grid in_x, csx_grid in_y, csx_grid out_x, csx_grid
   CSX_GUARD_END
 }
 
+int csx_mft_noalloc(csx_ctx* ctx, csx_grid in_x, csx_grid in_y, csx_grid out_x, csx_grid out_y,
+                    const double* lambda_f, int batch, const void* d_in, void* d_out, csx_precision p) {
+  CSX_GUARD_BEGIN
+  CSX_TRY(use_device(ctx));
+  const size_t need = csx_mft_workspace_size(in_x, in_y, out_x, out_y, batch, p);
+  if (need > kNoAllocScratchBytes)
+    return fail(CSX_ERR_ARG, "csx_mft_noalloc: the call needs " + std::to_string(need) +
+                                 " bytes of scratch, more than its " + std::to_string(kNoAllocScratchBytes) +
+                                 " (use csx_mft with a workspace)");
+  CSX_TRY(resolve_scratch(ctx->device));
+  Arena a;
+  a.base = static_cast<char*>(scratch_ptr(ctx->device));
+  a.cap = kNoAllocScratchBytes;
+  return mft_run(ctx, in_x, in_y, out_x, out_y, lambda_f, batch, d_in, d_out, p, &a);
+  CSX_GUARD_END
+}
+
+size_t csx_mft_noalloc_capacity(void) { return kNoAllocScratchBytes; }
 
 // ------------------------------------------------------- FFT / Fresnel ---
 static int upload_doubles(csx_ctx* ctx, DevBuf& buf, const double* h, int n) {

@@ -246,6 +246,19 @@ int csx_mft(csx_ctx* ctx, csx_grid in_x, csx_grid in_y, csx_grid out_x, csx_grid
             const double* lambda_f, int batch, const void* d_in, void* d_out,
             csx_precision p, void* d_work);
 
+/* Allocation-free csx_mft (no counterpart in the reference): for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, hosts
+ * that forbid hipMalloc).  The same contraction as csx_mft, with every work buffer in
+ * one 128 MiB scratch array that each GPU holds for the life of the process; calls
+ * whose csx_mft_workspace_size exceeds it fail with CSX_ERR_ARG.  No tile timing: the
+ * persisted tuner choice, else the 64 x 64 tile.  Every call on one GPU shares that
+ * GPU's scratch: calls that overlap on one GPU (two threads or streams) race; calls in
+ * sequence do not (each returns after its work is done). */
+int csx_mft_noalloc(csx_ctx* ctx, csx_grid in_x, csx_grid in_y, csx_grid out_x, csx_grid out_y,
+                    const double* lambda_f, int batch, const void* d_in, void* d_out, csx_precision p);
+/* Bytes of the scratch array csx_mft_noalloc works in. */
+size_t csx_mft_noalloc_capacity(void);
+
 /* Angular-spectrum Fresnel step, fresnelPropagateAS::execute
  * (src/lib/csim_fft.cpp:371-392): per field b, E <- IFFT2(FFT2(E) . H_b),
  * H = exp(i 2pi/lambda z sqrt(1 - (lambda/apRad)^2/4 (p^2+q^2))).  n x n, n a
