@@ -628,12 +628,15 @@ struct DownsampleElem : Element {
 // are applied first.  Optional epilogue: per-field masks (FPM plane) or one real
 // csim mask (1+i) m over the whole output grid.  out_conv writes the result at
 // the working precision into the conversion buffer for a following MFT.
+// gin_per_lam: the input grid of each wavelength's fields when it varies with lambda
+// (csim's doubleFraunhoferFPM FPM plane, whose pixel is proportional to lambda); the
+// current geometry c.geom otherwise.
 static int compile_mft(Chain& ch, Compile& c, const std::vector<Geom>& gout_per_lam,
                        const std::vector<double>& lf_per_lam, const std::string& label,
                        const std::vector<MftField>* masks = nullptr, bool out_conv = false,
                        const double* epi_mask_1pi = nullptr, bool expand = false,
                        std::shared_ptr<const Support> epi_sup = nullptr, const float* epi_mask_f32 = nullptr,
-                       const unsigned char* epi_mask_u8 = nullptr) {
+                       const unsigned char* epi_mask_u8 = nullptr, const std::vector<Geom>* gin_per_lam = nullptr) {
   CSX_TRY(flush_pointwise(ch, c, /*to_conv=*/true));
   // expand: the fields are one candidate's copy; the MFT writes every candidate's
   // output (its own mask each) from that copy, whose first contraction the plan
@@ -683,7 +686,9 @@ static int compile_mft(Chain& ch, Compile& c, const std::vector<Geom>& gout_per_
       m.out = out + (size_t)f * go.npix();
       m.out_kind = FK_C128;
     }
-    m.in_x = gx(gin); m.in_y = gy(gin); m.out_x = gx(go); m.out_y = gy(go);
+    const Geom& gi = gin_per_lam ? (*gin_per_lam)[l] : gin;
+    if (gi.nx != gin.nx || gi.ny != gin.ny) return fail(CSX_ERR_STATE, label + ": input size varies with lambda");
+    m.in_x = gx(gi); m.in_y = gy(gi); m.out_x = gx(go); m.out_y = gy(go);
     m.lambda_f = lf_per_lam[l];
     if (zero_skip) m.in_sup = c.support;
     if (masks) {
@@ -1113,8 +1118,9 @@ struct DoubleFraunhoferElem : Element {
     if (babinet) {
       CSX_TRY(compile_babinet(ch, c, E, gf, lfo, bw, next_mft, lyot, lyot32, lyot8, lyot_sup));
     } else {
+      // the fields are on the FPM grid of their own wavelength (gf[l])
       CSX_TRY(compile_mft(ch, c, back, lfo, type + " out " + name, nullptr, next_mft, lyot, false, lyot_sup, lyot32,
-                          lyot8));
+                          lyot8, &gf));
     }
     c.claimed = claim;
     return CSX_OK;

@@ -1063,6 +1063,9 @@ __global__ void k_splitk_finish(const GemmProb* __restrict__ probs, int nprob, l
     const GemmProb& P = probs[lo];
     const long q = e - P.elem_start, mn = (long)P.M * P.N;
     const int m = (int)(q / P.N), n = (int)(q - (long)m * P.N);
+    // a block that is zero and never read (K range lo = -1): its tiles wrote no sums
+    // and nothing is stored there
+    if (P.kr && P.kr[2 * ((long)(m / kKrBlock) * P.kr_ld + n / kKrBlock)] < 0) continue;
     const float2* W = reinterpret_cast<const float2*>(P.ws);
     double re = 0, im = 0;
     for (int k = 0; k < P.ksplit; k++) {         // fixed order: deterministic

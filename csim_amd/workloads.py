@@ -219,8 +219,9 @@ def workload_zoomprop(n=256, pad=1.0, zoom=1.0):
                     meta={"seed": seed, "n": n, "fields": 1, "lambdas": [LAMBDA0]})
 
 
-def workload1(n=2048):
-    """2048^2 field, PIAA mirrors around a 1 m Fresnel step, downsampled to 1024^2."""
+def workload1(n=2048, fresnel_pad=1):
+    """2048^2 field, PIAA mirrors around a 1 m Fresnel step, downsampled to 1024^2.
+    fresnel_pad 2: the step on csim's 2n zero pad (fresnelPropagator padFactor)."""
     seed = BASE_SEED + 1
     pix = PIX
     r_ap = 1000 * n // 2048                  # 1000 px at 2048
@@ -232,7 +233,8 @@ def workload1(n=2048):
               "\tstart\n\t\tmask\n\t\tname:pupil\n\tend\n"
               "\tstart\n\t\tmirror\n\t\tname:aberration\n\tend\n"
               "\tstart\n\t\tmirror\n\t\tname:piaa1\n\t\tposition:0\n\tend\n"
-              "\tstart\n\t\tfresnelPropagator\n\t\tname:piaa gap\n\tend\n"
+              "\tstart\n\t\tfresnelPropagator\n\t\tname:piaa gap\n" +
+              (f"\t\tpadFactor:{fresnel_pad}\n" if fresnel_pad != 1 else "") + "\tend\n"
               "\tstart\n\t\tmirror\n\t\tname:piaa2\n\t\tposition:1.0\n\t\tsign:-1\n\tend\n"
               "\tstart\n\t\tdownsample\n\t\tname:ds\n\t\tsampleFactor:2\n\tend\n" +
               _fpm_block("doubleFraunhoferFPM", LAMBDA0, occulter=3.0) + "\n" + _tail(LAMBDA0))
@@ -302,6 +304,62 @@ def workload3(n=1024, n_sub_pix=16, n_lambda=9):
     w.wid = 3
     w.name = f"DE batch member: {n_lambda} lambda, {w.meta['n_hex']}-hex FPM, {n}^2 pupil"
     return w
+
+
+def workload_native(kind="cube", n=256, pad=0.25, n_lambda=3, offsets=(0.0, 2.0), mask_index=None):
+    """csim's own FPM geometry, none of the additive keys.
+    cube: doubleFraunhoferFPM (doubleFraunhoferFPM.cpp:45-97): lambda*f = 3.52 lambda, FPM
+      pixel dxprime = 2 (R / (dx N zoomFactor)) lambda 80 on the offset grid (i - pad N / 2)
+      dxprime, a two-slice complex mask cube (maskFilename amp,ph; bound from memory as
+      "mask") placed at the centre, slice maskIndex (unset: the latch to slice 0).
+    hex: doubleFraunhoferInterpFPM (doubleFraunhoferInterpFPM.cpp:86-125): the FPM grid
+      pixel is the hex mask's, which complexHexMaskFPM derives from lambdaRef and
+      fpmFRatio (complexHexMaskFPM.cpp:197-203); lambda*f = 3.52 lambda."""
+    seed = BASE_SEED + 12
+    pix = PIX * 1024 / n
+    r_ap = 500 * n // 1024
+    beam_r = r_ap * pix
+    Lx_over_D = n / (2.0 * r_ap)
+    sources = [(1.0, t * Lx_over_D / 2.0, 0.0) for t in offsets]
+    lams = poly_lambdas(n_lambda)
+    lam_ref = 630e-9
+    lines = ["\tstart", "\t\tdoubleFraunhoferFPM" if kind == "cube" else "\t\tdoubleFraunhoferInterpFPM",
+             "\t\tname:fpm", "\t\tdisableForCalibration:on", f"\t\tpadFactor:{pad!r}", "\t\tzoomFactor:1"]
+    complex_arrays, meta = {}, {"seed": seed, "n": n, "fields": n_lambda * len(offsets), "lambdas": lams}
+    if kind == "cube":
+        if mask_index is not None:
+            lines.append(f"\t\tmaskIndex:{mask_index}")
+        S = max(8, int(n * pad) // 4)
+        x = np.arange(S) - (S - 1) / 2.0
+        spot = np.sqrt(x[None, :] ** 2 + x[:, None] ** 2) < S / 5.0
+        for sl in range(2):
+            amp = np.where(spot, 0.05 * sl, 0.6 + 0.4 * philox(seed, 2 + 2 * sl, S * S).reshape(S, S))
+            ph = 2 * math.pi * philox(seed, 3 + 2 * sl, S * S).reshape(S, S)
+            complex_arrays[("fpm", "mask", sl)] = amp * np.exp(-1j * ph)   # initMask: amp exp(-i ph)
+    elif kind == "hex":
+        S, rings = 32, 3
+        lamf_over_D = 3.52 * lam_ref / (2 * beam_r)
+        fratio = 0.25 * lamf_over_D / (2 * (r_ap / S) * lam_ref)      # hex pixel ~ lambda f / (4 D)
+        pixel = 2 * (r_ap / S) * lam_ref * fratio
+        lines += ["\t\tstart", "\t\t\tcomplexHexMaskFPM", "\t\t\tname:hexmask", "\t\t\thexLayout:standard",
+                  f"\t\t\tmaskNRings:{rings}", "\t\t\thexStep:1", "\t\t\tfpmArraySize:512",
+                  "\t\t\tfpmScaleFactor:1.0", "\t\t\thexGap:0.02", "\t\t\tnSubPix:4", f"\t\t\tfpmMaskSize:{S}",
+                  f"\t\t\tlambdaRef:{lam_ref!r}", f"\t\t\tfpmFRatio:{fratio!r}",
+                  f"\t\t\tfpmRadius:{S / 2 * pixel!r}", "\t\t\tphaseSign:1", "\t\tend"]
+        n_hex = 1 + 3 * rings * (rings + 1)
+        t = hex_thickness(seed, 1, n_hex)[0]
+        meta.update(n_hex=n_hex, thickness=t, sags=thickness_to_sag(t))
+    else:
+        raise ValueError(kind)
+    lines.append("\tend")
+    script = (_telescope(2 * beam_r) + _efield(n, pix, beam_r, lams, sources) +
+              "start\n\tcoronagraph\n\tstart\n\t\tmask\n\t\tname:pupil\n\tend\n"
+              "\tstart\n\t\tmirror\n\t\tname:aberration\n\tend\n" +
+              "\n".join(lines) + "\n" + _tail(lam_ref))
+    arrays = {("pupil", "filename"): pupil_amplitude(n, r_ap, 400 * n / 1024),
+              ("aberration", "filename"): aberration_sag(seed, n),
+              ("lyot", "filename"): circle(n, 450 * n / 1024)}
+    return Workload(12, f"native {kind} FPM geometry, pad {pad}, {n}^2 pupil", script, arrays, complex_arrays, meta)
 
 
 def apply(chain, w: Workload):

@@ -197,7 +197,7 @@ def resample_to(m, in_pix, n_in, g_out):
 
 
 class HexGeometry:
-    def __init__(self, h):
+    def __init__(self, h, model=None, zoom=1.0):
         self.layout = h.get("hexLayout", "csim")
         self.rings = int(h["maskNRings"])
         self.step = float(h.get("hexStep", 1))
@@ -206,10 +206,16 @@ class HexGeometry:
         self.gap = float(h["hexGap"])
         self.nsp = int(h["nSubPix"])
         self.S = int(h["fpmMaskSize"])
+        # init_mask (complexHexMaskFPM.cpp:193-234): pixelScale, else the physical mask
+        # size over the mask size, else 2 (R / (dx S zoom)) lambdaRef fpmFRatio with the
+        # parent element's zoomFactor
         if "pixelScale" in h:
             self.pixel = float(h["pixelScale"])
-        else:   # init_mask (complexHexMaskFPM.cpp:193-234) with the physical mask size
+        elif "fpmMaskPhysicalSize" in h:
             self.pixel = float(h["fpmMaskPhysicalSize"]) / self.S
+        else:
+            self.pixel = (2 * (model.R / (model.pix * self.S * zoom)) * float(h["lambdaRef"]) *
+                          float(h["fpmFRatio"]))
         self.radius = float(h["fpmRadius"])
         self.phase_sign = float(h.get("phaseSign", 1))
         hx, hy, _ = hex_centers(self.rings, self.layout, self.step)
@@ -229,7 +235,7 @@ class OracleChain:
         self.hex = {}
         for e in self.m.elements:
             if e["hex"] is not None:
-                self.hex[e["name"]] = HexGeometry(e["hex"])
+                self.hex[e["name"]] = HexGeometry(e["hex"], self.m, float(e.get("zoomFactor", 1.0)))
 
     def _fpm_geom(self, e, lam, g):
         m = self.m
@@ -267,7 +273,17 @@ class OracleChain:
                 E = mirror_apply(E, h, lam, float(e.get("sign", 1)))
             elif t == "fresnelPropagator":
                 z = float(e.get("propagationSign", 1)) * abs(els[i + 1]["position"] - els[i - 1]["position"])
-                E = fresnel_as(E, lam, m.ap_rad, z)
+                pad = int(float(e.get("padFactor", 1)))
+                if pad == 1:
+                    E = fresnel_as(E, lam, m.ap_rad, z)
+                else:
+                    # fresnelPropagator.cpp:45-54: any other padFactor embeds the field in a
+                    # 2N x 2N zero pad (rows N/2 .. 3N/2 - 1) with apRad * padFactor
+                    r, c = E.shape
+                    P = np.zeros((2 * r, 2 * c), dtype=np.complex128)
+                    P[r // 2:r // 2 + r, c // 2:c // 2 + c] = E
+                    P = fresnel_as(P, lam, pad * m.ap_rad, z)
+                    E = P[r // 2:r // 2 + r, c // 2:c // 2 + c].copy()
             elif t == "downsample":
                 f = int(e["sampleFactor"])
                 E = downsample(E, f, g[2])
@@ -284,6 +300,13 @@ class OracleChain:
                     x = gf[1] + np.arange(gf[0]) * gf[2]
                     rr = np.sqrt(x[None, :] ** 2 + x[:, None] ** 2) / m.loD(ref)
                     Ef = Ef * np.where(rr < float(e["occulterRadiusFld"]), 0.0, 1.0)
+                elif (e["name"], "mask") in m.arrays:
+                    # doubleFraunhoferFPM.cpp:84-97: the complex mask cube, centred; an unset
+                    # maskIndex latches to the first slice executed (slice 0) for good
+                    K = np.asarray(m.arrays[(e["name"], "mask")], dtype=np.complex128)
+                    if K.ndim == 3:
+                        K = K[..., max(int(e.get("maskIndex", -1)), 0)]
+                    Ef = center_mask_multiply(Ef, K)
                 E = zoomfft(Ef, (gf[1], gf[1], gf[2], gf[2]), (g[0], g[1], g[1], g[2], g[2]), -lf)
             elif t == "fpmPupToLyot":
                 E = self._pup_to_lyot(e, E, g, lam, calibrating, cand_sags)

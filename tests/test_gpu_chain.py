@@ -473,3 +473,91 @@ def test_candidate_batch_babinet_split(ctx, precision, monkeypatch):
     else:
         assert worst < 3e-5 and dc < 1e-5
     assert f1 < 0.75 * f0
+
+
+def test_split_k_on_every_stage(ctx, monkeypatch):
+    """Split K forced on every contraction of a bf16x3 chain, the FPM -> Lyot stage
+    included (its Lyot-zeroed blocks that the next contraction never reads get no
+    sums and no stores): fields and contrast equal the unsplit ones to FP32 rounding."""
+    w = W.workload2(n_lambda=3, offsets=(0.0, 2.0))
+    monkeypatch.setenv("CSX_GEMM_VARIANT", "0")
+    res = []
+    for split in (None, "2"):
+        if split:
+            monkeypatch.setenv("CSX_GEMM_KSPLIT", split)
+        ch = build(ctx, w, "bf16x3")
+        c, _ = ch.contrast()
+        res.append((c[0], [ch.field(0, s, li) for s in range(2) for li in range(3)]))
+        ch.close()
+    monkeypatch.delenv("CSX_GEMM_KSPLIT")
+    (c0, f0), (c1, f1) = res
+    worst = max(rel_rms(a, b) for a, b in zip(f1, f0))
+    print(f"split K on every stage: fields {worst:.3e}, contrast {c1:.10e} vs {c0:.10e}")
+    assert worst < 5e-6 and abs(c1 - c0) <= 1e-5 * c0
+
+
+def _with_cube(w):
+    """The workload's arrays with the FPM mask cube stacked (rows x cols x slices) for the oracle."""
+    arrays = dict(w.arrays)
+    sl = sorted(k[2] for k in w.complex_arrays if k[:2] == ("fpm", "mask"))
+    if sl:
+        arrays[("fpm", "mask")] = np.stack([w.complex_arrays[("fpm", "mask", k)] for k in sl], axis=2)
+    return arrays
+
+
+def _native_vs_oracle(ctx, w, sags=None):
+    ch = build(ctx, w)
+    oc = OracleChain(w.script, _with_cube(w), sags=sags)
+    ch.propagate(False)
+    ns, nl = ch.info()["n_sources"], len(w.meta["lambdas"])
+    worst = max(rel_rms(ch.field(0, s, li), oc.field(s, li)) for s in range(ns) for li in range(nl))
+    c, _ = ch.contrast()
+    c_ref, _, _ = oc.contrast()
+    ch.close()
+    return worst, c[0], c_ref
+
+
+@pytest.mark.parametrize("n,pad,mask_index", [(256, 0.25, None), (256, 1.0, 1), (1024, 0.25, None)])
+def test_native_double_fraunhofer_fpm_vs_oracle(ctx, n, pad, mask_index):
+    """doubleFraunhoferFPM on csim's own geometry (no additive keys): lambda*f = 3.52
+    lambda, FPM pixel dxprime = 2 (R / (dx N zoom)) lambda 80 on the offset grid of pad N
+    pixels, the complex mask cube bound as maskFilename and centred, slice maskIndex or,
+    unset, the latch to slice 0 for every wavelength (doubleFraunhoferFPM.cpp:45-97).
+    Fields and contrast against the oracle chain, FP64."""
+    w = W.workload_native("cube", n=n, pad=pad, n_lambda=3 if n == 256 else 2,
+                          offsets=(0.0, 2.0) if n == 256 else (0.0,), mask_index=mask_index)
+    worst, c, c_ref = _native_vs_oracle(ctx, w)
+    print(f"doubleFraunhoferFPM {n}^2 pad {pad} maskIndex {mask_index}: field {worst:.3e}, "
+          f"contrast {c:.10e} vs {c_ref:.10e}")
+    assert worst < 1e-12 and abs(c - c_ref) <= 1e-12 * c_ref
+
+
+@pytest.mark.parametrize("n,pad", [(256, 0.25), (256, 1.0), (1024, 0.25)])
+def test_native_interp_fpm_vs_oracle(ctx, n, pad):
+    """doubleFraunhoferInterpFPM on the hex mask's own pixel grid: no pixelScale key, so
+    complexHexMaskFPM derives it from lambdaRef and fpmFRatio (complexHexMaskFPM.cpp:
+    197-203), and the FPM grid is pad N pixels of it with lambda*f = 3.52 lambda
+    (doubleFraunhoferInterpFPM.cpp:86-125).  FP64 against the oracle chain."""
+    w = W.workload_native("hex", n=n, pad=pad, n_lambda=2, offsets=(0.0, 2.0) if n == 256 else (0.0,))
+    worst, c, c_ref = _native_vs_oracle(ctx, w, sags=w.meta["sags"])
+    print(f"doubleFraunhoferInterpFPM {n}^2 pad {pad}: field {worst:.3e}, contrast {c:.10e} vs {c_ref:.10e}")
+    assert worst < 1e-12 and abs(c - c_ref) <= 1e-12 * c_ref
+
+
+def test_fresnel_pad_factor_chain_vs_oracle(ctx):
+    """fresnelPropagator padFactor:2 (fresnelPropagator.cpp:45-54): the field embedded in
+    a 2N zero pad, apRad * padFactor, cropped back; the PIAA chain at 512^2 (1024^2 FFTs)
+    against the oracle, FP64."""
+    w = W.workload1(n=512, fresnel_pad=2)
+    ch = build(ctx, w)
+    oc = OracleChain(w.script, w.arrays)
+    ch.propagate(False)
+    e = rel_rms(ch.field(), oc.field(0, 0))
+    w1 = W.workload1(n=512)
+    ch1 = build(ctx, w1)
+    ch1.propagate(False)
+    d = rel_rms(ch1.field(), ch.field())
+    print(f"fresnel padFactor 2 chain: field rel rms vs oracle {e:.3e} (differs from unpadded by {d:.3e})")
+    assert e < 1e-12 and d > 1e-6
+    ch.close()
+    ch1.close()
