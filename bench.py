@@ -112,28 +112,90 @@ def region_mask(ch):
     return (r >= 3) & (r <= 10)
 
 
-def cpu_baseline(w, budget_s=12.0):
-    """The oracle (FP64 CPU restatement of csim, Bluestein zoomFft, OpenMP) on a
-    bounded sample of the same workload's fields."""
+def host_cpu():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(w, budget_s=12.0, cand_sags=None, what="pupil->FPM->Lyot->image"):
+    """The oracle (FP64 CPU restatement of csim, Bluestein zoomFft, OpenMP over columns,
+    the oracle's own FFT: FFTW is not on this host) on a bounded sample of the same
+    workload's fields."""
     import oracle
     from oracle.chain import OracleChain
     oc = OracleChain(w.script, w.arrays, sags=w.meta.get("sags"))
     nl = len(oc.m.lambdas)
+    nc = 1 if cand_sags is None else len(cand_sags)
     n = 0
     t0 = time.perf_counter()
     while True:
-        s, li = divmod(n % (oc.m.nsrc * nl), nl)
-        oc.field(s, li)
+        k = n % (oc.m.nsrc * nl * nc)
+        cand, r = divmod(k, oc.m.nsrc * nl)
+        s, li = divmod(r, nl)
+        oc.field(s, li, cand_sags=None if cand_sags is None else cand_sags[cand])
         n += 1
         if time.perf_counter() - t0 > budget_s:
             break
     dt = time.perf_counter() - t0
     return {"value": round(n / dt, 4), "unit": "propagations/s", "cores": oracle.num_threads(), "kind": "port",
-            "sample": f"{n} propagations (pupil->FPM->Lyot->image) cycling over the workload's "
-                      f"{oc.m.nsrc * nl} fields, Bluestein chirp-z zoomFft, {dt:.1f} s"}
+            "host_cpu": host_cpu(), "fft": "the oracle's radix-2 FFT (FFTW absent)",
+            "sample": f"{n} propagations ({what}) cycling over the workload's {oc.m.nsrc * nl * nc} fields, "
+                      f"Bluestein chirp-z zoomFft, {dt:.1f} s"}
 
 
-def de_batch_uneven(ctx, precision, steps=5):
+def leg_single(ctx, w, precision, steps, what):
+    """One-field configurations (C0, C1): FP64 plain and the mixed precision beside it,
+    propagations/s, the contraction roofline, the per-stage timer, and the mixed
+    path's contrast error against FP64 on the workload's own region."""
+    out = {}
+    res = {}
+    for p in ("fp64", precision):
+        ch = build_chain(ctx, w, p)
+        dt, c, prof = time_chain(ch, steps, 3)
+        res[p] = (c, ch.normalized_image(), ch)
+        out[p] = {"value": round(w.meta["fields"] * steps / dt, 3), "unit": "propagations/s",
+                  "ms_per_propagation": round(dt / steps * 1e3, 4), "roofline": gemm_roofline(prof, p),
+                  "stages": {k: {"launches": v["launches"], "ms_per_launch": round(v["ms_total"] / max(1, v["launches"]), 4),
+                                 "bytes_per_launch": v["bytes_total"] / max(1, v["launches"])}
+                             for k, v in prof.items()}}
+    c64, img64, ch64 = res["fp64"]
+    cm, imgm, chm = res[precision]
+    sel = region_mask(ch64)
+    out["speedup_vs_fp64"] = round(out[precision]["value"] / out["fp64"]["value"], 3)
+    out["contrast_error"] = {"region_mean_fp64": float(c64[0]), "region_mean_abs": float(abs(cm[0] - c64[0])),
+                             "max_pixel_abs": float(np.max(np.abs(imgm - img64)[sel])),
+                             "pass_on_timed_workload": float(abs(cm[0] - c64[0])) <= 1e-11}
+    out["workload"] = what
+    for _, _, ch in res.values():
+        ch.close()
+    return out
+
+
+def leg_c1_fresnel(leg):
+    """The angular-spectrum step of the 2048^2 chain against HBM: the field read and
+    written once per step (134,217,728 B, FP64 complex), and, with the generation pass
+    that applies the first PIAA sag (and the pupil mask and aberration maps), the
+    mirror + AS step against its compulsory 167,772,160 B (field in and out and one sag)."""
+    st = leg["fp64"]["stages"]
+    if "fresnel_as" not in st:
+        return None
+    t_as = st["fresnel_as"]["ms_per_launch"] * 1e-3
+    t_pw = st.get("pointwise", {}).get("ms_per_launch", 0.0) * 1e-3
+    return {"bound": "hbm", "peak": 8000.0, "unit": "GB/s",
+            "as_step_ms": round(t_as * 1e3, 4), "as_step_bytes": 134217728,
+            "as_step_achieved": round(134217728 / t_as / 1e9, 1),
+            "as_step_frac": round(134217728 / t_as / 8e12, 4),
+            "mirror_as_ms": round((t_as + t_pw) * 1e3, 4), "mirror_as_compulsory_bytes": 167772160,
+            "mirror_as_achieved": round(167772160 / (t_as + t_pw) / 1e9, 1),
+            "mirror_as_frac": round(167772160 / (t_as + t_pw) / 8e12, 4)}
+
+
+def de_batch_uneven(ctx, precision, steps=5, cpu=False):
     """DE generation with uneven pupils: 64 candidates x 9 wavelengths, 32 on 512^2
     and 32 on 1024^2 pupils (radii halved), one grouped launch per contraction
     stage; beside it, the two chains run one after the other."""
@@ -141,7 +203,7 @@ def de_batch_uneven(ctx, precision, steps=5):
     from csim_amd import workloads as W
     from csim_amd.chain import ChainGroup
     rng = np.random.Generator(np.random.Philox(20261203 + 3))
-    chains, flops = [], 0.0
+    chains, flops, members = [], 0.0, []
     for n in (512, 1024):
         w = W.workload3(n=n, n_lambda=9)
         t = rng.random((32, w.meta["n_hex"])) * 1e-6
@@ -149,6 +211,7 @@ def de_batch_uneven(ctx, precision, steps=5):
         ch.set_candidates("fpm", W.thickness_to_sag(t))
         ch.contrast()                        # compile, tune, calibrate
         chains.append(ch)
+        members.append((w, W.thickness_to_sag(t)))
     g = ChainGroup(chains)
     g.contrast()
     torch.cuda.synchronize()
@@ -174,6 +237,14 @@ def de_batch_uneven(ctx, precision, steps=5):
     g.close()
     for ch in chains:
         ch.close()
+    if cpu:
+        # the oracle on candidates of each pupil size; a generation is 288 fields of each
+        cb = [cpu_baseline(wm, 3.0, cand_sags=sg[:4]) for wm, sg in members]
+        t_gen = 288 / cb[0]["value"] + 288 / cb[1]["value"]
+        res["cpu_baseline"] = {"value": round(fields / t_gen, 4), "unit": "propagations/s", "cores": cb[0]["cores"],
+                               "kind": "port", "host_cpu": cb[0]["host_cpu"], "fft": cb[0]["fft"],
+                               "sample": f"512^2: {cb[0]['sample']}; 1024^2: {cb[1]['sample']}; rate of a "
+                                         "generation (288 fields of each size) from the two"}
     return res
 
 
@@ -238,6 +309,7 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-uneven", action="store_true")
+    ap.add_argument("--no-legs", action="store_true", help="skip the C0 and C1 single-field legs")
     ap.add_argument("--pmc-probe", action="store_true", help="short run for rocprofv3 --pmc counter passes")
     a = ap.parse_args(argv)
     if a.gpus != 1:
@@ -302,7 +374,18 @@ def main(argv=None):
             out["other_precisions"] = others
         plain.close()
     if not a.no_uneven:
-        out["de_batch_uneven_512_1024"] = de_batch_uneven(ctx, a.precision)
+        out["de_batch_uneven_512_1024"] = de_batch_uneven(ctx, a.precision, cpu=not a.no_cpu)
+    if not a.no_legs:
+        out["C0_mono_1024"] = leg_single(ctx, W.workload0(), a.precision, max(10, a.steps), "C0: 1024^2 pupil, "
+                                         "650 nm, occulter 3 l/D, Lyot 0.9D, 128^2 image; one field per step")
+        l1 = leg_single(ctx, W.workload1(), a.precision, max(5, a.steps // 2), "C1: 2048^2 field, two PIAA "
+                        "sags around a 1 m angular-spectrum step at 650 nm, downsample x2, then the C0 chain")
+        l1["fresnel_roofline"] = leg_c1_fresnel(l1)
+        out["C1_piaa_fresnel_2048"] = l1
+        if not a.no_cpu:
+            out["C0_mono_1024"]["cpu_baseline"] = cpu_baseline(W.workload0(), 4.0)
+            out["C1_piaa_fresnel_2048"]["cpu_baseline"] = cpu_baseline(
+                W.workload1(), 6.0, what="PIAA sags, Fresnel AS, downsample, pupil->FPM->Lyot->image")
     out["tuner"] = {k: v for k, v in _lib.tune_report().items() if k != "choices"}
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w)

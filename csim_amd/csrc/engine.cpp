@@ -217,6 +217,20 @@ Geom geom_offset(int nx, int ny, double dx, double dy) {
 static csx_grid gx(const Geom& g) { return csx_grid{g.nx, g.x0, g.dx}; }
 static csx_grid gy(const Geom& g) { return csx_grid{g.ny, g.y0, g.dy}; }
 
+// A stage bracketed by HIP events of the chain's kernel timer while profiling is on
+// (flops and bytes: the algorithmic work of one run of the stage).
+static std::function<int(hipStream_t)> timed_stage(Chain& ch, const std::string& name, double flops, double bytes,
+                                                   std::function<int(hipStream_t)> f) {
+  Chain* chp = &ch;
+  return [=](hipStream_t s) {
+    KernelTimer* t = chp->timer();
+    if (t) CSX_TRY(t->begin(name, flops, bytes, s));
+    CSX_TRY(f(s));
+    if (t) CSX_TRY(t->end(s));
+    return CSX_OK;
+  };
+}
+
 // ======================================================= fused pointwise ==
 // Consecutive pointwise elements (and the point-source initialisation) are
 // applied in one pass over the fields; any other element flushes them first.
@@ -251,7 +265,13 @@ static int flush_pointwise(Chain& ch, Compile& c, bool to_conv = false) {
   for (int i = 1; i < pc.nops; i++) pm = pm && (pc.op[i].kind == PW_MASK_1PI || pc.op[i].kind == PW_MIRROR);
   pc.nl = pm ? ch.nlam() : 0;
   std::string name = "pointwise x" + std::to_string(pc.nops) + (pc.out_kind != PW_OUT_C128 ? " -> working precision" : "");
-  c.stages.push_back([=](hipStream_t s) { return pointwise_chain(E, nf, pc, s); });
+  // bytes: every field written (and read unless generated), each real map read once
+  const double npix = (double)c.geom.npix();
+  const double osz = pc.out_kind == PW_OUT_C128 ? 16.0 : (pc.out_kind == PW_OUT_C64 ? 8.0 : 4.0 * T);
+  int maps = 0;
+  for (int i = 0; i < pc.nops; i++) maps += pc.op[i].kind == PW_MASK_1PI || pc.op[i].kind == PW_MIRROR;
+  const double bytes = nf * npix * (osz + (pc.op[0].kind == PW_SOURCE ? 0.0 : 16.0)) + maps * npix * 8.0;
+  c.stages.push_back(timed_stage(ch, "pointwise", 0.0, bytes, [=](hipStream_t s) { return pointwise_chain(E, nf, pc, s); }));
   c.stage_names.push_back(name);
   c.pending.clear();
   return CSX_OK;
@@ -555,21 +575,25 @@ struct FresnelElem : Element {
     int nf = ch.nfield_for(c);
     Chain* chp = &ch;
     bool cal = c.calibrating;
+    // algorithmic work: two 2-D FFTs of the (padded) grid, each field read and written once
+    const int nn = padFactor == 1 ? n : 2 * n;
+    const double fl = nf * 2.0 * 5.0 * (double)nn * nn * std::log2((double)nn * nn);
+    const double by = nf * 32.0 * (double)n * n;
     if (padFactor == 1) {
-      c.stages.push_back([=](hipStream_t s) {
+      c.stages.push_back(timed_stage(ch, "fresnel_as", fl, by, [=](hipStream_t s) {
         const double* l = cal ? chp->scratch("lam_cal").as<double>() : chp->lambdas_dev();
         return fresnel_batched(E, n, nf, l, apRad, z, s);
-      });
+      }));
     } else {
       // embed in a 2n zero pad with apRad * padFactor (fresnelPropagator.cpp:45-54)
       double2* P = ch.field_buf(1 - c.buf);
       double ap = padFactor * apRad;
-      c.stages.push_back([=](hipStream_t s) {
+      c.stages.push_back(timed_stage(ch, "fresnel_as", fl, by, [=](hipStream_t s) {
         const double* l = cal ? chp->scratch("lam_cal").as<double>() : chp->lambdas_dev();
         CSX_TRY(embed_center(E, n, n, P, 2 * n, 2 * n, nf, 0, s));
         CSX_TRY(fresnel_batched(P, 2 * n, nf, l, ap, z, s));
         return embed_center(P, 2 * n, 2 * n, E, n, n, nf, 1, s);
-      });
+      }));
     }
     c.stage_names.push_back("fresnelPropagator " + name);
     return CSX_OK;
@@ -612,9 +636,10 @@ struct DownsampleElem : Element {
     double2* o = ch.field_buf(1 - c.buf);
     int nf = ch.nfield_for(c);
     Geom g = c.geom;
-    c.stages.push_back([=](hipStream_t s) {
-      return resample_batched(in, g.ny, g.nx, nf, o, out.ny, out.nx, dty, dtx, s);
-    });
+    c.stages.push_back(timed_stage(ch, "downsample", 0.0, nf * 16.0 * ((double)g.npix() + (double)out.npix()),
+                                   [=](hipStream_t s) {
+                                     return resample_batched(in, g.ny, g.nx, nf, o, out.ny, out.nx, dty, dtx, s);
+                                   }));
     c.stage_names.push_back("downsample " + name);
     c.buf = 1 - c.buf;
     c.geom = out;

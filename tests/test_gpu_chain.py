@@ -561,3 +561,40 @@ def test_fresnel_pad_factor_chain_vs_oracle(ctx):
     assert e < 1e-12 and d > 1e-6
     ch.close()
     ch1.close()
+
+
+def test_c4_1261_hex_fp64_vs_oracle(ctx):
+    """The 20-ring standard layout (1261 hexes) of the surrogate's scale-up through the
+    chain at a 256^2 pupil, 2 wavelengths: FP64 fields and contrast against the oracle."""
+    w = W.workload2(n=256, rings=20, n_lambda=2, offsets=(0.0,))
+    assert w.meta["n_hex"] == 1261
+    ch = build(ctx, w)
+    oc = OracleChain(w.script, w.arrays, sags=w.meta["sags"])
+    ch.propagate(False)
+    worst = max(rel_rms(ch.field(0, 0, li), oc.field(0, li)) for li in range(2))
+    c, _ = ch.contrast()
+    c_ref, _, _ = oc.contrast()
+    print(f"1261 hexes, 256^2: field {worst:.3e}, contrast {c[0]:.10e} vs {c_ref:.10e}")
+    assert worst < 1e-12 and abs(c[0] - c_ref) <= 1e-12 * c_ref
+    ch.close()
+
+
+@pytest.mark.parametrize("precision", ["bf16x3", "bf16x6"])
+def test_c4_1261_hex_mixed_vs_fp64(ctx, precision):
+    """The 1261-hex layout at a 1024^2 pupil, 9 wavelengths, a batch of 8 candidates:
+    mixed precision against FP64 (3-10 lambda/D region contrasts; bf16x6 within the
+    1e-11 absolute gate, bf16x3 to 1e-6 relative)."""
+    w = W.workload2(n=1024, rings=20, n_lambda=9, offsets=(0.0,))
+    sags = W.thickness_to_sag(W.hex_thickness(w.meta["seed"], 21, w.meta["n_hex"], n_cand=8))
+    res = {}
+    for p in ("fp64", precision):
+        ch = build(ctx, w, p)
+        ch.set_candidates("fpm", sags)
+        res[p], _ = ch.contrast()
+        ch.close()
+    d = np.abs(res[precision] - res["fp64"])
+    print(f"1261 hexes, 1024^2, {precision}: contrast {res['fp64'].min():.3e}..{res['fp64'].max():.3e}, "
+          f"max |delta C| {d.max():.3e}, max rel {np.max(d / res['fp64']):.3e}")
+    assert np.max(d / res["fp64"]) < 1e-6
+    if precision == "bf16x6":
+        assert d.max() <= 1e-11
