@@ -452,11 +452,17 @@ struct MaskElem : Element {
     return map.set(d, r, c);
   }
   bool keeps_support() const override { return true; }
-  int compile(Chain& ch, Compile& c, int) override {
+  int pointwise_op(Chain& ch, const Geom& g, PwOp* op) override {
     if (map.host.empty()) return fail(CSX_ERR_STATE, "mask '" + name + "' has no data");
     const double* m = nullptr;
-    CSX_TRY(map.device_for(ch, c.geom, &m));
-    CSX_TRY(push_pointwise(ch, c, PwOp{PW_MASK_1PI, m, 0.0, 1}));
+    CSX_TRY(map.device_for(ch, g, &m));
+    *op = PwOp{PW_MASK_1PI, m, 0.0, 1};
+    return 1;
+  }
+  int compile(Chain& ch, Compile& c, int) override {
+    PwOp op;
+    if (int rc = pointwise_op(ch, c.geom, &op); rc < 0) return rc;
+    CSX_TRY(push_pointwise(ch, c, op));
     c.support = Support::intersect(c.support, map.support_for(c.geom));   // E (1+i) m is zero where m is
     return CSX_OK;
   }
@@ -483,11 +489,17 @@ struct MirrorElem : Element {
     return map.set(d, r, c);
   }
   bool keeps_support() const override { return true; }   // a phase factor
-  int compile(Chain& ch, Compile& c, int) override {
+  int pointwise_op(Chain& ch, const Geom& g, PwOp* op) override {
     if (map.host.empty()) return fail(CSX_ERR_STATE, "mirror '" + name + "' has no data");
     const double* h = nullptr;
-    CSX_TRY(map.device_for(ch, c.geom, &h));
-    return push_pointwise(ch, c, PwOp{PW_MIRROR, h, sign, 1});
+    CSX_TRY(map.device_for(ch, g, &h));
+    *op = PwOp{PW_MIRROR, h, sign, 1};
+    return 1;
+  }
+  int compile(Chain& ch, Compile& c, int) override {
+    PwOp op;
+    if (int rc = pointwise_op(ch, c.geom, &op); rc < 0) return rc;
+    return push_pointwise(ch, c, op);
   }
 };
 
@@ -563,7 +575,7 @@ struct FresnelElem : Element {
     return unknown_key(type, k);
   }
   int compile(Chain& ch, Compile& c, int index) override {
-    CSX_TRY(flush_pointwise(ch, c));
+    if (c.in_conv) return fail(CSX_ERR_STATE, "fresnelPropagator: fields at a working precision");
     Element* prev = ch.elem_at(index - 1);
     Element* next = ch.elem_at(index + 1);
     if (!prev || !next) return fail(CSX_ERR_STATE, "fresnelPropagator needs a previous and a next element");
@@ -580,11 +592,46 @@ struct FresnelElem : Element {
     const double fl = nf * 2.0 * 5.0 * (double)nn * nn * std::log2((double)nn * nn);
     const double by = nf * 32.0 * (double)n * n;
     if (padFactor == 1) {
-      c.stages.push_back(timed_stage(ch, "fresnel_as", fl, by, [=](hipStream_t s) {
+      // the pointwise elements before the step (pending: point-source generation, masks,
+      // mirrors) run in its first pass as it reads the fields, the masks and mirrors
+      // right after it in its last pass as it writes them: no separate pointwise pass
+      PwChain pre{}, post{};
+      const bool has_pre = !c.pending.empty();
+      if (has_pre) {
+        pre.nops = (int)c.pending.size();
+        for (int i = 0; i < pre.nops; i++) pre.op[i] = c.pending[i];
+        c.pending.clear();
+      }
+      int claim = 0;
+      for (int j = index + 1; j < ch.n_elems() && claim < kMaxPwOps; j++) {
+        Element* e = ch.elem_at(j);
+        if (e->skipped(c)) break;
+        PwOp op;
+        const int rc = e->pointwise_op(ch, c.geom, &op);
+        if (rc < 0) return rc;
+        if (rc == 0) break;
+        post.op[claim++] = op;
+      }
+      post.nops = claim;
+      for (PwChain* pc : {&pre, &post}) {
+        pc->out_kind = PW_OUT_C128;
+        pc->rows = n; pc->cols = n;
+        pc->pixel_scale = ch.efield().pixelScale;
+        pc->src_params = ch.source_params(cal);
+        pc->lambdas = cal ? ch.scratch("lam_cal").as<double>() : ch.lambdas_dev();
+      }
+      const bool has_post = claim > 0;
+      c.claimed = claim;
+      const double by_f = by + (has_pre && pre.op[0].kind == PW_SOURCE ? -16.0 * nf * (double)n * n : 0.0);
+      c.stages.push_back(timed_stage(ch, "fresnel_as", fl, by_f, [=](hipStream_t s) {
         const double* l = cal ? chp->scratch("lam_cal").as<double>() : chp->lambdas_dev();
-        return fresnel_batched(E, n, nf, l, apRad, z, s);
+        return fresnel_batched(E, n, nf, l, apRad, z, s, has_pre ? &pre : nullptr, has_post ? &post : nullptr);
       }));
+      c.stage_names.push_back("fresnelPropagator " + name + (has_pre ? " (+ pointwise before)" : "") +
+                              (has_post ? " (+ pointwise after)" : ""));
+      return CSX_OK;
     } else {
+      CSX_TRY(flush_pointwise(ch, c));
       // embed in a 2n zero pad with apRad * padFactor (fresnelPropagator.cpp:45-54)
       double2* P = ch.field_buf(1 - c.buf);
       double ap = padFactor * apRad;

@@ -114,6 +114,9 @@ struct Element {
   // true: compile() keeps Compile::support valid for the fields it leaves (a
   // pointwise factor, or an MFT that reads and resets it); false: it is cleared first
   virtual bool keeps_support() const { return false; }
+  // A pointwise element (mask, mirror) on fields of geometry g as one factor of a fused
+  // pass: 1 (op filled), 0 (not a pointwise element), or a negative error code.
+  virtual int pointwise_op(Chain& ch, const Geom& g, PwOp* op) { (void)ch; (void)g; (void)op; return 0; }
 };
 
 class Chain {

@@ -4,6 +4,8 @@
 #include "planes.hpp"
 
 #include <algorithm>
+#include <mutex>
+#include <map>
 #include <cstdlib>
 
 namespace csx {
@@ -93,6 +95,64 @@ int k_point_sources(double2* E, int rows, int cols, int nfield, const double* pa
   return CSX_OK;
 }
 
+// Per-field constants of a pointwise chain (point source, mirror phase factors).
+struct PwField {
+  bool gen;
+  int first;
+  double flux, tx, ty, ps, Lx, Ly;
+  double mir[kMaxPwOps];            // mirror phase factor per op (mirror.cpp:66-72)
+};
+__device__ __forceinline__ PwField pw_field(const PwChain& pc, int f) {
+  PwField F;
+  F.gen = pc.nops > 0 && pc.op[0].kind == PW_SOURCE;
+  F.first = F.gen ? 1 : 0;
+  F.flux = F.tx = F.ty = 0;
+  if (F.gen) { const double* prm = pc.src_params + 3 * f; F.flux = prm[0]; F.tx = prm[1]; F.ty = prm[2]; }
+  F.ps = pc.pixel_scale; F.Lx = F.ps * pc.cols; F.Ly = F.ps * pc.rows;
+#pragma unroll
+  for (int k = 0; k < kMaxPwOps; k++)
+    F.mir[k] = (k < pc.nops && pc.op[k].kind == PW_MIRROR) ? -pc.op[k].sign * 2 * 2 * M_PI / pc.lambdas[f] : 0.0;
+  return F;
+}
+// Value of pixel p (column-major) of field f after the chain: generated from the point
+// source, or the value (er, ei) passed in, times every element factor.
+__device__ __forceinline__ void pw_point(const PwChain& pc, const PwField& F, int f, int p, double& er, double& ei) {
+#pragma clang fp contract(off)
+  if (F.gen) {
+    // efield::init_point_sources (src/data/efield.cpp:179-206)
+    int c = p / pc.rows, r = p - c * pc.rows;
+    double x = -(F.Lx - F.ps) / 2.0 + c * F.ps, y = -(F.Ly - F.ps) / 2.0 + r * F.ps;
+    double v = 2 * x * F.tx / F.Lx + 2 * y * F.ty / F.Ly;
+    double sn, cs;
+    sincos(2.0 * M_PI * v, &sn, &cs);
+    er = F.flux * cs; ei = F.flux * sn;
+  }
+  // unrolled over the largest op count: every index constant (no private arrays in memory)
+#pragma unroll
+  for (int k = 0; k < kMaxPwOps; k++) {
+    if (k < F.first || k >= pc.nops) continue;
+    const PwOp& o = pc.op[k];
+    if (o.kind == PW_MASK_1PI) {           // mask.cpp: E *= (1+i) m
+      double w = gin<double>(o.p)[p];
+      double tr = er * w - ei * w;
+      ei = er * w + ei * w;
+      er = tr;
+    } else if (o.kind == PW_MIRROR) {
+      double ph = F.mir[k] * gin<double>(o.p)[p];
+      double sn, cs;
+      sincos(ph, &sn, &cs);
+      double tr = er * cs - ei * sn;
+      ei = er * sn + ei * cs;
+      er = tr;
+    } else if (o.kind == PW_CFACTOR) {
+      f64x2 w = gin<f64x2>(o.p)[(long)(f / o.per) * ((long)pc.rows * pc.cols) + p];
+      double tr = er * w.x - ei * w.y;
+      ei = er * w.y + ei * w.x;
+      er = tr;
+    }
+  }
+}
+
 // One pass: point-source generation and/or element factors, one field per
 // blockIdx.y (per-field constants hoisted), output in place (complex128) or at
 // the working precision of a following MFT (complex64 or BF16 planes).
@@ -100,52 +160,16 @@ __global__ void k_pointwise(double2* __restrict__ E, PwChain pc) {
 #pragma clang fp contract(off)
   const int f = blockIdx.y;
   const long npix = (long)pc.rows * pc.cols;
-  const bool gen = pc.nops > 0 && pc.op[0].kind == PW_SOURCE;
-  const int first = gen ? 1 : 0;
-  double flux = 0, tx = 0, ty = 0;
-  if (gen) { const double* prm = pc.src_params + 3 * f; flux = prm[0]; tx = prm[1]; ty = prm[2]; }
-  const double ps = pc.pixel_scale, Lx = ps * pc.cols, Ly = ps * pc.rows;
-  double mir[kMaxPwOps];                    // mirror phase factor per op (mirror.cpp:66-72)
-#pragma unroll
-  for (int k = 0; k < kMaxPwOps; k++)
-    mir[k] = (k < pc.nops && pc.op[k].kind == PW_MIRROR) ? -pc.op[k].sign * 2 * 2 * M_PI / pc.lambdas[f] : 0.0;
+  const PwField F = pw_field(pc, f);
   GLOBAL f64x2* Ef = gout<f64x2>(E) + (long)f * npix;
   const int np32 = (int)npix;                // fields are far below 2^31 pixels
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np32; p += gridDim.x * blockDim.x) {
-    double er, ei;
-    if (gen) {
-      // efield::init_point_sources (src/data/efield.cpp:179-206)
-      int c = p / pc.rows, r = p - c * pc.rows;
-      double x = -(Lx - ps) / 2.0 + c * ps, y = -(Ly - ps) / 2.0 + r * ps;
-      double v = 2 * x * tx / Lx + 2 * y * ty / Ly;
-      double sn, cs;
-      sincos(2.0 * M_PI * v, &sn, &cs);
-      er = flux * cs; ei = flux * sn;
-    } else {
+    double er = 0.0, ei = 0.0;
+    if (!F.gen) {
       f64x2 e = Ef[p];
       er = e.x; ei = e.y;
     }
-    for (int k = first; k < pc.nops; k++) {
-      const PwOp& o = pc.op[k];
-      if (o.kind == PW_MASK_1PI) {           // mask.cpp: E *= (1+i) m
-        double w = gin<double>(o.p)[p];
-        double tr = er * w - ei * w;
-        ei = er * w + ei * w;
-        er = tr;
-      } else if (o.kind == PW_MIRROR) {
-        double ph = mir[k] * gin<double>(o.p)[p];
-        double sn, cs;
-        sincos(ph, &sn, &cs);
-        double tr = er * cs - ei * sn;
-        ei = er * sn + ei * cs;
-        er = tr;
-      } else if (o.kind == PW_CFACTOR) {
-        f64x2 w = gin<f64x2>(o.p)[(long)(f / o.per) * npix + p];
-        double tr = er * w.x - ei * w.y;
-        ei = er * w.y + ei * w.x;
-        er = tr;
-      }
-    }
+    pw_point(pc, F, f, p, er, ei);
     if (pc.out_kind == PW_OUT_C128) Ef[p] = f64x2{er, ei};
     else if (pc.out_kind == PW_OUT_C64) gout<f32x2>(pc.out)[(long)f * npix + p] = f32x2{(float)er, (float)ei};
     else {
@@ -458,14 +482,124 @@ int pointwise_chain(double2* E, int nfield, const PwChain& pc, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------- FFT --
-// Radix-2 Stockham autosort transforms staged in LDS.  Stage m = 2^s:
-//   y[k + 2jm] = x[k + jm] + w x[k + jm + n/2],  y[k + 2jm + m] = x[k + jm] - w x[k + jm + n/2],
-//   w = exp(sign i pi k / m) = tw[k n/(2m)],  tw[q] = exp(i 2 pi q / n) (conjugated for sign -1).
+// Stockham autosort transforms staged in LDS: radix 8 while the length allows, then
+// one radix-4 or radix-2 stage, the butterflies in registers.  Stage of radix R on
+// sub-transforms of length L (L = 1, R, R R', ...):
+//   butterfly b < n/R, k = b mod L, j = b div L:
+//   x_s = in[b + s n/R] w^(s k), w = exp(sign i 2 pi / (L R)), s < R;  y = DFT_R(x);
+//   out[j L R + k + t L] = y_t.
 // Each stage reads its operands into registers, synchronises, writes in place.
-constexpr int kFftThreads = 256;
+// tw[q] = exp(i 2 pi q / n), q < n (conjugated for sign -1).
+constexpr int kFftThreads = 256;         // threads of a contiguous (one column) transform
+constexpr int kFftPanelThreads = 512;    // threads of a strided (panel of rows) transform
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+// a * w (SIGN > 0) or a * conj(w)
+template <int SIGN>
+__device__ __forceinline__ double2 cmul_tw(double2 a, double2 w) {
+  const double wy = SIGN > 0 ? w.y : -w.y;
+  return make_double2(a.x * w.x - a.y * wy, a.x * wy + a.y * w.x);
+}
+// a * (SIGN i)
+template <int SIGN>
+__device__ __forceinline__ double2 mul_si(double2 a) {
+  return SIGN > 0 ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
+}
+
+// y_t = sum_s x_s exp(SIGN i 2 pi s t / R), in place, natural order in and out
+template <int R, int SIGN> struct Dft;
+template <int SIGN> struct Dft<2, SIGN> {
+  static __device__ __forceinline__ void run(double2 (&v)[2]) {
+    const double2 a = v[0];
+    v[0] = cadd(a, v[1]);
+    v[1] = csub(a, v[1]);
+  }
+};
+template <int SIGN> struct Dft<4, SIGN> {
+  static __device__ __forceinline__ void run(double2 (&v)[4]) {
+    const double2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+    const double2 b0 = cadd(v[1], v[3]), b1 = mul_si<SIGN>(csub(v[1], v[3]));
+    v[0] = cadd(a0, b0); v[2] = csub(a0, b0);
+    v[1] = cadd(a1, b1); v[3] = csub(a1, b1);
+  }
+};
+template <int SIGN> struct Dft<8, SIGN> {
+  static __device__ __forceinline__ void run(double2 (&v)[8]) {
+    double2 e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+    Dft<4, SIGN>::run(e);
+    Dft<4, SIGN>::run(o);
+    const double h = 0.70710678118654752440;   // 1 / sqrt(2)
+    // o_t exp(SIGN i 2 pi t / 8): t = 1: (1 + SIGN i) / sqrt 2, t = 2: SIGN i, t = 3: (-1 + SIGN i) / sqrt 2
+    const double2 o1 = SIGN > 0 ? make_double2((o[1].x - o[1].y) * h, (o[1].y + o[1].x) * h)
+                                : make_double2((o[1].x + o[1].y) * h, (o[1].y - o[1].x) * h);
+    const double2 o2 = mul_si<SIGN>(o[2]);
+    const double2 o3 = SIGN > 0 ? make_double2((-o[3].x - o[3].y) * h, (-o[3].y + o[3].x) * h)
+                                : make_double2((-o[3].x + o[3].y) * h, (-o[3].y - o[3].x) * h);
+    v[0] = cadd(e[0], o[0]); v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], o1);   v[5] = csub(e[1], o1);
+    v[2] = cadd(e[2], o2);   v[6] = csub(e[2], o2);
+    v[3] = cadd(e[3], o3);   v[7] = csub(e[3], o3);
+  }
+};
+
+// One radix-R stage of a transform of length n at buf by nth threads (tid < nth), Q
+// butterflies per thread at most.  Every thread of the block calls it (it synchronises).
+template <int R, int Q, int SIGN>
+__device__ __forceinline__ void fft_stage(double2* buf, int n, int L, const double2* __restrict__ tw, int tid,
+                                          int nth) {
+  const int nb = n / R, span = n / (L * R);
+  double2 v[Q][R];
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const int b = tid + q * nth;
+    if (b < nb) {
+      const int k = b & (L - 1);
+#pragma unroll
+      for (int s = 0; s < R; s++) v[q][s] = buf[b + s * nb];
+      if (k) {
+#pragma unroll
+        for (int s = 1; s < R; s++) v[q][s] = cmul_tw<SIGN>(v[q][s], tw[(s * k * span) & (n - 1)]);
+      }
+      Dft<R, SIGN>::run(v[q]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const int b = tid + q * nth;
+    if (b < nb) {
+      const int k = b & (L - 1), j = b / L;
+      const int base = j * L * R + k;
+#pragma unroll
+      for (int t = 0; t < R; t++) buf[base + t * L] = v[q][t];
+    }
+  }
+  __syncthreads();
+}
+template <int R, int SIGN>
+__device__ __forceinline__ void fft_stage_q(double2* buf, int n, int L, const double2* __restrict__ tw, int tid,
+                                            int nth) {
+  // the launch keeps n / R <= (16 / R) nth: at most 16 values per thread in registers
+  const int q = (n / R + nth - 1) / nth;
+  if (q <= 1) fft_stage<R, 1, SIGN>(buf, n, L, tw, tid, nth);
+  else if (q <= 2 || R == 8) fft_stage<R, 2, SIGN>(buf, n, L, tw, tid, nth);
+  else if (q <= 4 || R == 4) fft_stage<R, (R == 8 ? 2 : 4), SIGN>(buf, n, L, tw, tid, nth);
+  else fft_stage<R, (R == 2 ? 8 : (R == 4 ? 4 : 2)), SIGN>(buf, n, L, tw, tid, nth);
+}
+// Whole transform of length n (power of two), in place in LDS.
+template <int SIGN>
+__device__ __forceinline__ void fft_lds(double2* buf, int n, const double2* __restrict__ tw, int tid, int nth) {
+  int L = 1;
+  while (L < n) {
+    const int rem = n / L;
+    if (rem % 8 == 0) { fft_stage_q<8, SIGN>(buf, n, L, tw, tid, nth); L *= 8; }
+    else if (rem % 4 == 0) { fft_stage_q<4, SIGN>(buf, n, L, tw, tid, nth); L *= 4; }
+    else { fft_stage_q<2, SIGN>(buf, n, L, tw, tid, nth); L *= 2; }
+  }
 }
 
 // sin and cos of a large FP64 argument (|x| up to ~1e9 rad): x is reduced by
@@ -482,124 +616,114 @@ __device__ __forceinline__ void sincos_reduced(double x, double* s, double* c) {
   sincos(r, s, c);
 }
 
-// count independent transforms of length n stored at buf + t*n; Q >= butterflies per thread.
-template <int Q>
-__device__ void lds_fft(double2* buf, int n, int count, int sign, const double2* __restrict__ tw) {
-  const int half = n >> 1;
-  const int nb = half * count;
-  for (int m = 1; m < n; m <<= 1) {
-    const int l = n / (2 * m);
-    double2 o0[Q], o1[Q];
-#pragma unroll
-    for (int q = 0; q < Q; q++) {
-      int b = threadIdx.x + q * kFftThreads;
-      if (b < nb) {
-        int t = b / half, bb = b - t * half;
-        int j = bb / m, k = bb - j * m;
-        const double2* x = buf + t * n;
-        double2 c0 = x[k + j * m], c1 = x[k + j * m + half];
-        double2 w = tw[k * l];
-        if (sign < 0) w.y = -w.y;
-        double2 wc1 = cmul(w, c1);
-        o0[q] = make_double2(c0.x + wc1.x, c0.y + wc1.y);
-        o1[q] = make_double2(c0.x - wc1.x, c0.y - wc1.y);
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < Q; q++) {
-      int b = threadIdx.x + q * kFftThreads;
-      if (b < nb) {
-        int t = b / half, bb = b - t * half;
-        int j = bb / m, k = bb - j * m;
-        double2* x = buf + t * n;
-        x[k + 2 * j * m] = o0[q];
-        x[k + 2 * j * m + m] = o1[q];
-      }
-    }
-    __syncthreads();
-  }
-}
-
 __global__ void k_twiddles(double2* tw, int n) {
-  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n / 2; q += gridDim.x * blockDim.x) {
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
     double sn, cs;
     sincospi(2.0 * (double)q / (double)n, &sn, &cs);
     tw[q] = make_double2(cs, sn);
   }
 }
 
-// Along the contiguous (row) index: one column of one field per block.
-template <int Q>
-__global__ __launch_bounds__(kFftThreads) void k_fft_cols(double2* __restrict__ E, int n, int sign, double scale,
-                                                          const double2* __restrict__ tw) {
+// Along the contiguous (row) index: one column (n values) of one field per block.  The
+// field's pointwise chain `pre` (point-source generation, masks, mirrors: the elements
+// before the transform) is applied as the column is read, `post` (the masks and
+// mirrors after it) and `scale` as it is written.
+template <int SIGN>
+__global__ __launch_bounds__(kFftThreads) void k_fft_contig(double2* __restrict__ E, int n, double scale,
+                                                            const double2* __restrict__ tw, PwChain pre, int has_pre,
+                                                            PwChain post, int has_post) {
   extern __shared__ double2 sbuf[];
-  long base = (long)blockIdx.x * n;         // field*n*n + col*n
-  for (int i = threadIdx.x; i < n; i += blockDim.x) sbuf[i] = E[base + i];
+  const int f = blockIdx.y, p0 = blockIdx.x * n;
+  GLOBAL f64x2* Ef = gout<f64x2>(E) + (long)f * n * n;
+  if (has_pre) {
+    const PwField F = pw_field(pre, f);
+    for (int i = threadIdx.x; i < n; i += kFftThreads) {
+      double er = 0.0, ei = 0.0;
+      if (!F.gen) { const f64x2 e = Ef[p0 + i]; er = e.x; ei = e.y; }
+      pw_point(pre, F, f, p0 + i, er, ei);
+      sbuf[i] = make_double2(er, ei);
+    }
+  } else {
+    for (int i = threadIdx.x; i < n; i += kFftThreads) {
+      const f64x2 e = Ef[p0 + i];
+      sbuf[i] = make_double2(e.x, e.y);
+    }
+  }
   __syncthreads();
-  lds_fft<Q>(sbuf, n, 1, sign, tw);
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    double2 v = sbuf[i];
-    E[base + i] = make_double2(v.x * scale, v.y * scale);
+  fft_lds<SIGN>(sbuf, n, tw, threadIdx.x, kFftThreads);
+  if (has_post) {
+    const PwField G = pw_field(post, f);
+    for (int i = threadIdx.x; i < n; i += kFftThreads) {
+      double er = sbuf[i].x * scale, ei = sbuf[i].y * scale;
+      pw_point(post, G, f, p0 + i, er, ei);
+      Ef[p0 + i] = f64x2{er, ei};
+    }
+  } else {
+    for (int i = threadIdx.x; i < n; i += kFftThreads) Ef[p0 + i] = f64x2{sbuf[i].x * scale, sbuf[i].y * scale};
   }
 }
 
-// Along the strided (column) index: R consecutive rows of one field per block.
-// mode 0: plain transform.  mode 1: forward, multiply by the angular-spectrum
-// transfer function, inverse (the Fresnel step), all in LDS.
-template <int Q>
-__global__ __launch_bounds__(kFftThreads) void k_fft_rows(double2* __restrict__ E, int n, int R, int sign,
-                                                          int mode, const double* __restrict__ lambdas,
-                                                          double ap_rad, double z, const double2* __restrict__ tw) {
-  extern __shared__ double2 sbuf[];          // [R][n]
-  int blocks_per_field = n / R;
-  int field = blockIdx.x / blocks_per_field;
-  int r0 = (blockIdx.x - field * blocks_per_field) * R;
-  double2* F = E + (long)field * n * n;
-  for (int e = threadIdx.x; e < R * n; e += blockDim.x) {
-    int c = e / R, rr = e - c * R;           // consecutive threads: consecutive rows of one column
-    sbuf[rr * n + c] = F[(long)c * n + r0 + rr];
+// Along the strided (column) index: a panel of RP consecutive rows of one field per
+// block (RP n-value transforms, kFftPanelThreads / RP threads each); the panel is read
+// and written as RP-value runs of every column (16 RP contiguous bytes).  Rows sit
+// n + 1 values apart in LDS (the runs' RP values fall on distinct banks).
+// MODE 0: the transform of sign SIGN.  MODE 1: forward, times the angular-spectrum
+// transfer function, inverse: the Fresnel step's middle pass.
+template <int MODE, int SIGN>
+__global__ __launch_bounds__(kFftPanelThreads) void k_fft_panel(double2* __restrict__ E, int n, int RP,
+                                                                const double* __restrict__ lambdas, double ap_rad,
+                                                                double z, const double2* __restrict__ tw) {
+  extern __shared__ double2 sbuf[];
+  const int f = blockIdx.y, r0 = blockIdx.x * RP, ld = n + 1;
+  const int nth = kFftPanelThreads / RP, g = threadIdx.x / nth, tid = threadIdx.x - g * nth;
+  GLOBAL f64x2* F = gout<f64x2>(E) + (long)f * n * n;
+  for (int e = threadIdx.x; e < RP * n; e += kFftPanelThreads) {
+    const int c = e / RP, rr = e - c * RP;
+    const f64x2 v = F[(long)c * n + r0 + rr];
+    sbuf[rr * ld + c] = make_double2(v.x, v.y);
   }
   __syncthreads();
-  if (mode == 0) {
-    lds_fft<Q>(sbuf, n, R, sign, tw);
+  double2* row = sbuf + g * ld;
+  if (MODE == 0) {
+    fft_lds<SIGN>(row, n, tw, tid, nth);
   } else {
-    lds_fft<Q>(sbuf, n, R, -1, tw);
+    fft_lds<-1>(row, n, tw, tid, nth);
     {
 #pragma clang fp contract(off)
       // csim: exp(i*k*z * sqrt(1. - 1./4.*pow(lambda/apRad, 2.)*H)), Goodman 4-20
-      double lam = lambdas[field];
-      double k = 2. * M_PI / lam;
-      double q = lam / ap_rad;
-      double c4 = 1. / 4. * (q * q);
-      for (int e = threadIdx.x; e < R * n; e += blockDim.x) {
-        int rr = e / n, c = e - rr * n;
-        int r = r0 + rr;
-        int pr = r < n / 2 ? r : r - n;        // fft_shift of regspace(-n/2, n/2-1)
-        int pc = c < n / 2 ? c : c - n;
-        double H = (double)pc * (double)pc + (double)pr * (double)pr;
-        double ph = k * z * sqrt(1. - c4 * H);
+      const double lam = lambdas[f];
+      const double k = 2. * M_PI / lam;
+      const double q = lam / ap_rad;
+      const double c4 = 1. / 4. * (q * q);
+      const int r = r0 + g;
+      const int pr = r < n / 2 ? r : r - n;      // fft_shift of regspace(-n/2, n/2-1)
+      for (int c = tid; c < n; c += nth) {
+        const int pc = c < n / 2 ? c : c - n;
+        const double H = (double)pc * (double)pc + (double)pr * (double)pr;
+        const double ph = k * z * sqrt(1. - c4 * H);
         double sn, cs;
         sincos_reduced(ph, &sn, &cs);
-        sbuf[e] = cmul(sbuf[e], make_double2(cs, sn));
+        row[c] = cmul(row[c], make_double2(cs, sn));
       }
     }
     __syncthreads();
-    lds_fft<Q>(sbuf, n, R, +1, tw);
+    fft_lds<+1>(row, n, tw, tid, nth);
   }
-  for (int e = threadIdx.x; e < R * n; e += blockDim.x) {
-    int c = e / R, rr = e - c * R;
-    F[(long)c * n + r0 + rr] = sbuf[rr * n + c];
+  for (int e = threadIdx.x; e < RP * n; e += kFftPanelThreads) {
+    const int c = e / RP, rr = e - c * RP;
+    const double2 v = sbuf[rr * ld + c];
+    F[(long)c * n + r0 + rr] = f64x2{v.x, v.y};
   }
 }
 
-static int rows_per_block(int n) {
-  int R = 4096 / n;                          // 64 KiB of LDS
-  if (R > 8) R = 8;
-  if (R < 1) R = 1;
-  while (R > 1 && n % R) R >>= 1;
-  while (R > 1 && R * (n / 2) > 16 * kFftThreads) R >>= 1;
-  return R;
+// Rows per panel: 128 KiB of LDS at most (with the padding), at least n / 16 threads
+// per transform (16 values per thread per stage), at most n.
+static int panel_rows(int n) {
+  int rp = 8;
+  while (rp > 1 && ((size_t)rp * (n + 1) * sizeof(double2) > (size_t)132 << 10 || kFftPanelThreads / rp < n / 16))
+    rp >>= 1;
+  while (rp > 1 && rp > n) rp >>= 1;
+  return rp;
 }
 
 static int check_fft_n(int n) {
@@ -607,77 +731,82 @@ static int check_fft_n(int n) {
   return CSX_OK;
 }
 
-// Twiddle tables, one per transform length, kept for the process lifetime.
+// Twiddle tables exp(i 2 pi q / n), q < n, one per transform length, kept for the
+// process lifetime (one per device).
 static const double2* twiddles(int n, hipStream_t s) {
-  static double2* tables[13] = {nullptr};
-  int lg = 0;
-  while ((1 << lg) < n) lg++;
-  if (!tables[lg]) {
-    double2* p = nullptr;
-    if (hipMalloc(&p, sizeof(double2) * std::max(1, n / 2)) != hipSuccess) return nullptr;
-    hipLaunchKernelGGL(k_twiddles, dim3(std::max(1, (n / 2 + 255) / 256)), dim3(256), 0, s, p, n);
-    tables[lg] = p;
-  }
-  return tables[lg];
+  static std::mutex mu;
+  static std::map<std::pair<int, int>, double2*> tables;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  std::lock_guard<std::mutex> lock(mu);
+  auto key = std::make_pair(dev, n);
+  auto it = tables.find(key);
+  if (it != tables.end()) return it->second;
+  double2* p = nullptr;
+  if (hipMalloc(&p, sizeof(double2) * n) != hipSuccess) return nullptr;
+  hipLaunchKernelGGL(k_twiddles, dim3(std::max(1, (n + 255) / 256)), dim3(256), 0, s, p, n);
+  tables[key] = p;
+  return p;
 }
 
-static int qfor(long nb) {
-  long q = (nb + kFftThreads - 1) / kFftThreads;
-  int Q = 1;
-  while (Q < q) Q <<= 1;
-  return Q;
-}
-
-#define CSX_FFT_DISPATCH(Q, KERNEL, ...)                                              \
-  switch (Q) {                                                                        \
-    case 1: hipLaunchKernelGGL(KERNEL<1>, __VA_ARGS__); break;                       \
-    case 2: hipLaunchKernelGGL(KERNEL<2>, __VA_ARGS__); break;                       \
-    case 4: hipLaunchKernelGGL(KERNEL<4>, __VA_ARGS__); break;                       \
-    case 8: hipLaunchKernelGGL(KERNEL<8>, __VA_ARGS__); break;                       \
-    case 16: hipLaunchKernelGGL(KERNEL<16>, __VA_ARGS__); break;                     \
-    default: return fail(CSX_ERR_ARG, "fft: unsupported size");                      \
-  }
-
-static int launch_cols(double2* E, int n, int nfield, int sign, double scale, const double2* tw, hipStream_t s) {
-  int Q = qfor(n / 2);
-  size_t lds = sizeof(double2) * n;
-  CSX_FFT_DISPATCH(Q, k_fft_cols, dim3(n * nfield), dim3(kFftThreads), lds, s, E, n, sign, scale, tw);
+static int launch_contig(double2* E, int n, int nfield, int sign, double scale, const double2* tw, hipStream_t s,
+                         const PwChain* pre = nullptr, const PwChain* post = nullptr) {
+  const size_t lds = sizeof(double2) * n;
+  PwChain none{};
+  const PwChain& a = pre ? *pre : none;
+  const PwChain& b = post ? *post : none;
+  if (sign < 0)
+    hipLaunchKernelGGL(k_fft_contig<-1>, dim3(n, nfield), dim3(kFftThreads), lds, s, E, n, scale, tw, a, pre ? 1 : 0,
+                       b, post ? 1 : 0);
+  else
+    hipLaunchKernelGGL(k_fft_contig<+1>, dim3(n, nfield), dim3(kFftThreads), lds, s, E, n, scale, tw, a, pre ? 1 : 0,
+                       b, post ? 1 : 0);
   CSX_HIP(hipGetLastError());
   return CSX_OK;
 }
-static int launch_rows(double2* E, int n, int nfield, int sign, int mode, const double* lambdas, double ap_rad,
-                       double z, const double2* tw, hipStream_t s) {
-  int R = rows_per_block(n);
-  int Q = qfor((long)R * (n / 2));
-  size_t lds = sizeof(double2) * n * R;
-  CSX_FFT_DISPATCH(Q, k_fft_rows, dim3((n / R) * nfield), dim3(kFftThreads), lds, s, E, n, R, sign, mode, lambdas,
-                   ap_rad, z, tw);
+template <int MODE, int SIGN>
+static int launch_panel_t(double2* E, int n, int nfield, const double* lambdas, double ap_rad, double z,
+                          const double2* tw, hipStream_t s) {
+  const int rp = panel_rows(n);
+  const size_t lds = sizeof(double2) * (size_t)rp * (n + 1);
+  static bool attr = false;
+  if (!attr) {   // dynamic LDS beyond 64 KiB
+    CSX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_panel<MODE, SIGN>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)(136 << 10)));
+    attr = true;
+  }
+  hipLaunchKernelGGL((k_fft_panel<MODE, SIGN>), dim3(n / rp, nfield), dim3(kFftPanelThreads), lds, s, E, n, rp,
+                     lambdas, ap_rad, z, tw);
   CSX_HIP(hipGetLastError());
   return CSX_OK;
 }
 
 int fft2_batched(double2* E, int n, int nfield, int dir, hipStream_t s) {
   CSX_TRY(check_fft_n(n));
+  if (nfield <= 0) return CSX_OK;
   const double2* tw = twiddles(n, s);
   if (!tw) return fail(CSX_ERR_NOMEM, "fft: twiddle table allocation failed");
   if (dir < 0) {
-    CSX_TRY(launch_cols(E, n, nfield, -1, 1.0, tw, s));
-    CSX_TRY(launch_rows(E, n, nfield, -1, 0, nullptr, 0, 0, tw, s));
+    CSX_TRY(launch_contig(E, n, nfield, -1, 1.0, tw, s));
+    CSX_TRY((launch_panel_t<0, -1>(E, n, nfield, nullptr, 0, 0, tw, s)));
   } else {
-    CSX_TRY(launch_rows(E, n, nfield, +1, 0, nullptr, 0, 0, tw, s));
-    CSX_TRY(launch_cols(E, n, nfield, +1, 1.0 / ((double)n * (double)n), tw, s));
+    CSX_TRY((launch_panel_t<0, +1>(E, n, nfield, nullptr, 0, 0, tw, s)));
+    CSX_TRY(launch_contig(E, n, nfield, +1, 1.0 / ((double)n * (double)n), tw, s));
   }
   return CSX_OK;
 }
 
-int fresnel_batched(double2* E, int n, int nfield, const double* lambdas_dev, double ap_rad,
-                    double z, hipStream_t s) {
+int fresnel_batched(double2* E, int n, int nfield, const double* lambdas_dev, double ap_rad, double z, hipStream_t s,
+                    const PwChain* pre, const PwChain* post) {
   CSX_TRY(check_fft_n(n));
+  if (nfield <= 0) return CSX_OK;
+  if ((pre && (pre->rows != n || pre->cols != n)) || (post && (post->rows != n || post->cols != n)))
+    return fail(CSX_ERR_ARG, "fresnel: pointwise chain grid differs from the field");
   const double2* tw = twiddles(n, s);
   if (!tw) return fail(CSX_ERR_NOMEM, "fft: twiddle table allocation failed");
-  CSX_TRY(launch_cols(E, n, nfield, -1, 1.0, tw, s));
-  CSX_TRY(launch_rows(E, n, nfield, -1, 1, lambdas_dev, ap_rad, z, tw, s));
-  CSX_TRY(launch_cols(E, n, nfield, +1, 1.0 / ((double)n * (double)n), tw, s));
+  CSX_TRY(launch_contig(E, n, nfield, -1, 1.0, tw, s, pre, nullptr));
+  CSX_TRY((launch_panel_t<1, 0>(E, n, nfield, lambdas_dev, ap_rad, z, tw, s)));
+  CSX_TRY(launch_contig(E, n, nfield, +1, 1.0 / ((double)n * (double)n), tw, s, nullptr, post));
   return CSX_OK;
 }
 

@@ -42,8 +42,10 @@ int pointwise_chain(double2* E, int nfield, const PwChain& pc, hipStream_t s);
 // 2-D FFT, csim semantics: dir -1 forward unnormalised, +1 inverse / n^2.
 int fft2_batched(double2* E, int n, int nfield, int dir, hipStream_t s);
 // Angular-spectrum Fresnel step (src/lib/csim_fft.cpp:371-392); lambdas on device.
+// pre / post (optional): pointwise chains on the n x n fields applied as the first pass
+// reads them and as the last pass writes them (the elements around the step, fused).
 int fresnel_batched(double2* E, int n, int nfield, const double* lambdas_dev, double ap_rad,
-                    double z, hipStream_t s);
+                    double z, hipStream_t s, const PwChain* pre = nullptr, const PwChain* post = nullptr);
 
 // Separable linear resampling with precomputed (index, weight) tables, extrap 0.
 struct InterpTable { int lo, hi; double w; };
