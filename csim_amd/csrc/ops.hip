@@ -1,6 +1,7 @@
 // Pointwise, FFT, resampling, hex-mask and reduction kernels (gfx950).
 // Fields are complex128, column-major (rows = y), batches field-major.
 #include "ops.hpp"
+#include "phase.hpp"
 #include "planes.hpp"
 
 #include <algorithm>
@@ -40,7 +41,7 @@ __global__ void k_mirror(double2* __restrict__ E, long npix, long total, const d
     double a = -sign * 2 * 2 * M_PI / lambdas[f];
     double ph = a * h[i - f * npix];
     double sn, cs;
-    sincos(ph, &sn, &cs);
+    sincos_phase(ph, &sn, &cs);
     double2 e = E[i];
     E[i] = make_double2(e.x * cs - e.y * sn, e.x * sn + e.y * cs);
   }
@@ -83,7 +84,7 @@ __global__ void k_psrc(double2* __restrict__ E, int rows, int cols, int nfield,
     double v = 2 * x * tipX / Lx + 2 * y * tiltY / Ly;
     double ph = 2.0 * M_PI * v;
     double sn, cs;
-    sincos(ph, &sn, &cs);
+    sincos_phase(ph, &sn, &cs);
     E[i] = make_double2(flux * cs, flux * sn);
   }
 }
@@ -114,17 +115,19 @@ __device__ __forceinline__ PwField pw_field(const PwChain& pc, int f) {
     F.mir[k] = (k < pc.nops && pc.op[k].kind == PW_MIRROR) ? -pc.op[k].sign * 2 * 2 * M_PI / pc.lambdas[f] : 0.0;
   return F;
 }
-// Value of pixel p (column-major) of field f after the chain: generated from the point
-// source, or the value (er, ei) passed in, times every element factor.
-__device__ __forceinline__ void pw_point(const PwChain& pc, const PwField& F, int f, int p, double& er, double& ei) {
+// Value of pixel p = c rows + r (column-major) of field f after the chain: generated
+// from the point source, or the value (er, ei) passed in, times every element factor.
+// Callers that hold a column fixed pass the same c for all its pixels (the x term is
+// then formed once).
+__device__ __forceinline__ void pw_point_rc(const PwChain& pc, const PwField& F, int f, int p, int r, int c,
+                                            double& er, double& ei) {
 #pragma clang fp contract(off)
   if (F.gen) {
     // efield::init_point_sources (src/data/efield.cpp:179-206)
-    int c = p / pc.rows, r = p - c * pc.rows;
     double x = -(F.Lx - F.ps) / 2.0 + c * F.ps, y = -(F.Ly - F.ps) / 2.0 + r * F.ps;
     double v = 2 * x * F.tx / F.Lx + 2 * y * F.ty / F.Ly;
     double sn, cs;
-    sincos(2.0 * M_PI * v, &sn, &cs);
+    sincos_phase(2.0 * M_PI * v, &sn, &cs);
     er = F.flux * cs; ei = F.flux * sn;
   }
   // unrolled over the largest op count: every index constant (no private arrays in memory)
@@ -140,7 +143,7 @@ __device__ __forceinline__ void pw_point(const PwChain& pc, const PwField& F, in
     } else if (o.kind == PW_MIRROR) {
       double ph = F.mir[k] * gin<double>(o.p)[p];
       double sn, cs;
-      sincos(ph, &sn, &cs);
+      sincos_phase(ph, &sn, &cs);
       double tr = er * cs - ei * sn;
       ei = er * sn + ei * cs;
       er = tr;
@@ -151,6 +154,11 @@ __device__ __forceinline__ void pw_point(const PwChain& pc, const PwField& F, in
       er = tr;
     }
   }
+}
+
+__device__ __forceinline__ void pw_point(const PwChain& pc, const PwField& F, int f, int p, double& er, double& ei) {
+  const int c = p / pc.rows;
+  pw_point_rc(pc, F, f, p, p - c * pc.rows, c, er, ei);
 }
 
 // One pass: point-source generation and/or element factors, one field per
@@ -189,7 +197,7 @@ __global__ void k_pointwise(double2* __restrict__ E, PwChain pc) {
 // Differs from applying the elements in sequence only by FP64 rounding.
 // Phasor of an FP64 phase.  SPLIT (output in two BF16 planes, 16 significant bits): the
 // phase is reduced to [-pi, pi] in FP64 and the sine and cosine taken in FP32, whose
-// error (~1e-7) is far below the planes' rounding (2^-17); otherwise FP64 sincos.
+// error (~1e-7) is far below the planes' rounding (2^-17); otherwise sincos_phase.
 template <bool SPLIT>
 __device__ __forceinline__ void phasor(double ph, double* sn, double* cs) {
   if (SPLIT) {
@@ -200,7 +208,7 @@ __device__ __forceinline__ void phasor(double ph, double* sn, double* cs) {
     *sn = s;
     *cs = c;
   } else {
-    sincos(ph, sn, cs);
+    sincos_phase(ph, sn, cs);
   }
 }
 
@@ -482,16 +490,27 @@ int pointwise_chain(double2* E, int nfield, const PwChain& pc, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------- FFT --
-// Stockham autosort transforms staged in LDS: radix 8 while the length allows, then
-// one radix-4 or radix-2 stage, the butterflies in registers.  Stage of radix R on
-// sub-transforms of length L (L = 1, R, R R', ...):
-//   butterfly b < n/R, k = b mod L, j = b div L:
-//   x_s = in[b + s n/R] w^(s k), w = exp(sign i 2 pi / (L R)), s < R;  y = DFT_R(x);
-//   out[j L R + k + t L] = y_t.
-// Each stage reads its operands into registers, synchronises, writes in place.
-// tw[q] = exp(i 2 pi q / n), q < n (conjugated for sign -1).
-constexpr int kFftThreads = 256;         // threads of a contiguous (one column) transform
-constexpr int kFftPanelThreads = 512;    // threads of a strided (panel of rows) transform
+// Stockham autosort transforms of length N (a power of two) by T = max(1, N / 8)
+// threads each, the data in registers: between stages thread t holds the V = N / T
+// values t + m T (m < V).  Radix 8 while the length allows, then one radix-4 or
+// radix-2 stage.  Stage of radix R on sub-transforms of length L (L = 1, R, R R', ...):
+//   butterfly b = t + q T (q < V / R), k = b mod L, j = b div L:
+//   x_s = v[q + s V / R] w^(s k) = in[b + s N / R] w^(s k), w = exp(sign i 2 pi / (L R));
+//   y = DFT_R(x);  out[j L R + k + u L] = y_u.
+// Between stages the outputs go through LDS: written at their out index, read back in
+// the t + m T layout.  The last stage (L R = N, j = 0) leaves out[b + u N / R] =
+// out[t + (q + u V / R) T]: the register layout itself, so a transform reads its input
+// from and writes its output to global memory with no LDS pass, and the Fresnel
+// step's forward and inverse column transforms meet in registers.
+// LDS indices carry one padding slot per 8 values (fft_lds_ix): the first stages'
+// stride-8 and stride-64 writes then spread over all banks.
+// tw[q] = exp(i 2 pi q / N), q < N (conjugated for sign -1).
+constexpr int kFftBlock = 256;           // threads of a row-pass block (several columns below N = 2048)
+// A column-pass block loads its panel, transforms it and stores it: two blocks per CU
+// (half the LDS and of the 2048 threads each) let one block's loads and stores overlap
+// the other's transforms.
+constexpr int kFftPanelMaxThreads = 512;
+constexpr int kFftPanelLds = 75000;      // bytes of LDS a panel block may use (of 160 KiB per CU)
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -546,74 +565,65 @@ template <int SIGN> struct Dft<8, SIGN> {
   }
 };
 
-// One radix-R stage of a transform of length n at buf by nth threads (tid < nth), Q
-// butterflies per thread at most.  Every thread of the block calls it (it synchronises).
-template <int R, int Q, int SIGN>
-__device__ __forceinline__ void fft_stage(double2* buf, int n, int L, const double2* __restrict__ tw, int tid,
-                                          int nth) {
-  const int nb = n / R, span = n / (L * R);
-  double2 v[Q][R];
-#pragma unroll
-  for (int q = 0; q < Q; q++) {
-    const int b = tid + q * nth;
-    if (b < nb) {
-      const int k = b & (L - 1);
-#pragma unroll
-      for (int s = 0; s < R; s++) v[q][s] = buf[b + s * nb];
-      if (k) {
-#pragma unroll
-        for (int s = 1; s < R; s++) v[q][s] = cmul_tw<SIGN>(v[q][s], tw[(s * k * span) & (n - 1)]);
-      }
-      Dft<R, SIGN>::run(v[q]);
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < Q; q++) {
-    const int b = tid + q * nth;
-    if (b < nb) {
-      const int k = b & (L - 1), j = b / L;
-      const int base = j * L * R + k;
-#pragma unroll
-      for (int t = 0; t < R; t++) buf[base + t * L] = v[q][t];
-    }
-  }
-  __syncthreads();
-}
-template <int R, int SIGN>
-__device__ __forceinline__ void fft_stage_q(double2* buf, int n, int L, const double2* __restrict__ tw, int tid,
-                                            int nth) {
-  // the launch keeps n / R <= (16 / R) nth: at most 16 values per thread in registers
-  const int q = (n / R + nth - 1) / nth;
-  if (q <= 1) fft_stage<R, 1, SIGN>(buf, n, L, tw, tid, nth);
-  else if (q <= 2 || R == 8) fft_stage<R, 2, SIGN>(buf, n, L, tw, tid, nth);
-  else if (q <= 4 || R == 4) fft_stage<R, (R == 8 ? 2 : 4), SIGN>(buf, n, L, tw, tid, nth);
-  else fft_stage<R, (R == 2 ? 8 : (R == 4 ? 4 : 2)), SIGN>(buf, n, L, tw, tid, nth);
-}
-// Whole transform of length n (power of two), in place in LDS.
-template <int SIGN>
-__device__ __forceinline__ void fft_lds(double2* buf, int n, const double2* __restrict__ tw, int tid, int nth) {
-  int L = 1;
-  while (L < n) {
-    const int rem = n / L;
-    if (rem % 8 == 0) { fft_stage_q<8, SIGN>(buf, n, L, tw, tid, nth); L *= 8; }
-    else if (rem % 4 == 0) { fft_stage_q<4, SIGN>(buf, n, L, tw, tid, nth); L *= 4; }
-    else { fft_stage_q<2, SIGN>(buf, n, L, tw, tid, nth); L *= 2; }
-  }
-}
+__host__ __device__ constexpr int fft_threads(int N) { return N >= 8 ? N / 8 : 1; }
+__host__ __device__ constexpr int fft_lds_ix(int i) { return i + (i >> 3); }
+__host__ __device__ constexpr int fft_lds_len(int N) { return fft_lds_ix(N - 1) + 1; }
 
-// sin and cos of a large FP64 argument (|x| up to ~1e9 rad): x is reduced by
-// 2 pi = C1 + C2 (C1 the FP64 nearest 2 pi, C2 the next 53 bits) with fused
-// multiply-adds, r = (x - k C1) - k C2, each step rounded once; |r| <= pi then
-// keeps the full accuracy of sincos.
-__device__ __forceinline__ void sincos_reduced(double x, double* s, double* c) {
-#pragma clang fp contract(off)
-  const double C1 = 6.283185307179586232;           // 2 pi rounded to FP64
-  const double C2 = 2.4492935982947064e-16;         // 2 pi - C1
-  double k = rint(x * 0.15915494309189535);
-  double r = __fma_rn(-k, C1, x);                    // x - k C1, exact product, one rounding
-  r = __fma_rn(-k, C2, r);
-  sincos(r, s, c);
+// One radix-R stage on sub-transforms of length L (see above).  v holds the thread's
+// values in the t + m T layout on entry and on exit.  Every thread of the block calls
+// it: a stage other than the last synchronises around its LDS exchange (lds: this
+// transform's LDS, lds_stride apart for consecutive indices).
+template <int N, int R, int L, int SIGN>
+__device__ __forceinline__ void fft_reg_stage(double2 (&v)[N / fft_threads(N)], int t, const double2* __restrict__ tw,
+                                              double2* lds, int lds_stride) {
+  constexpr int T = fft_threads(N), V = N / T, BQ = V / R, TS = N / (L * R);
+  static_assert(V % R == 0 && L * R <= N, "stage does not fit the register layout");
+  double2 y[BQ][R];
+#pragma unroll
+  for (int q = 0; q < BQ; q++) {
+    const int b = t + q * T, k = b & (L - 1);
+#pragma unroll
+    for (int s = 0; s < R; s++) y[q][s] = v[q + s * BQ];
+    if (L > 1) {
+      // w^0 = 1 exactly: the k = 0 butterflies are multiplied too (exactly), no branch
+#pragma unroll
+      for (int s = 1; s < R; s++) y[q][s] = cmul_tw<SIGN>(y[q][s], tw[s * k * TS]);
+    }
+    Dft<R, SIGN>::run(y[q]);
+  }
+  if (L * R == N) {
+#pragma unroll
+    for (int q = 0; q < BQ; q++)
+#pragma unroll
+      for (int u = 0; u < R; u++) v[q + u * BQ] = y[q][u];
+  } else {
+    __syncthreads();   // every read of the previous exchange is done
+#pragma unroll
+    for (int q = 0; q < BQ; q++) {
+      const int b = t + q * T, k = b & (L - 1), j = b / L;
+#pragma unroll
+      for (int u = 0; u < R; u++) lds[fft_lds_ix(j * L * R + k + u * L) * lds_stride] = y[q][u];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < V; m++) v[m] = lds[fft_lds_ix(t + m * T) * lds_stride];
+  }
+}
+// The stages of a whole transform, from sub-transform length L on.
+template <int N, int L, int SIGN>
+struct FftStages {
+  static constexpr int V = N / fft_threads(N), REM = N / L;
+  static constexpr int R = (REM >= 8 && V % 8 == 0) ? 8 : ((REM >= 4 && V % 4 == 0) ? 4 : 2);
+  static __device__ __forceinline__ void run(double2 (&v)[V], int t, const double2* __restrict__ tw, double2* lds,
+                                             int lds_stride) {
+    fft_reg_stage<N, R, L, SIGN>(v, t, tw, lds, lds_stride);
+    if constexpr (L * R < N) FftStages<N, L * R, SIGN>::run(v, t, tw, lds, lds_stride);
+  }
+};
+template <int N, int SIGN>
+__device__ __forceinline__ void fft_reg(double2 (&v)[N / fft_threads(N)], int t, const double2* __restrict__ tw,
+                                        double2* lds, int lds_stride) {
+  FftStages<N, 1, SIGN>::run(v, t, tw, lds, lds_stride);
 }
 
 __global__ void k_twiddles(double2* tw, int n) {
@@ -624,70 +634,103 @@ __global__ void k_twiddles(double2* tw, int n) {
   }
 }
 
-// Along the contiguous (row) index: one column (n values) of one field per block.  The
+// Along the contiguous (row) index: C = kFftBlock / T columns (n values each, T threads
+// each) per block, column gc = blockIdx.x C + g of the batch (field gc / N).  The
 // field's pointwise chain `pre` (point-source generation, masks, mirrors: the elements
 // before the transform) is applied as the column is read, `post` (the masks and
 // mirrors after it) and `scale` as it is written.
-template <int SIGN>
-__global__ __launch_bounds__(kFftThreads) void k_fft_contig(double2* __restrict__ E, int n, double scale,
-                                                            const double2* __restrict__ tw, PwChain pre, int has_pre,
-                                                            PwChain post, int has_post) {
+template <int N>
+__host__ __device__ constexpr int fft_contig_block() { return fft_threads(N) > kFftBlock ? fft_threads(N) : kFftBlock; }
+template <int N, int SIGN>
+__global__ __launch_bounds__(fft_contig_block<N>()) void k_fft_contig(double2* __restrict__ E, int ncol, double scale,
+                                                                      const double2* __restrict__ tw, PwChain pre,
+                                                                      int has_pre, PwChain post, int has_post) {
+  constexpr int T = fft_threads(N), V = N / T;
   extern __shared__ double2 sbuf[];
-  const int f = blockIdx.y, p0 = blockIdx.x * n;
-  GLOBAL f64x2* Ef = gout<f64x2>(E) + (long)f * n * n;
+  const int g = threadIdx.x / T, t = threadIdx.x - g * T;
+  const int gc = blockIdx.x * (fft_contig_block<N>() / T) + g;
+  const bool live = gc < ncol;               // the last block's spare groups still synchronise
+  const int f = live ? gc / N : 0, p0 = (live ? gc - f * N : 0) * N;
+  GLOBAL f64x2* Ef = gout<f64x2>(E) + (long)f * N * N;
+  double2 v[V];
   if (has_pre) {
     const PwField F = pw_field(pre, f);
-    for (int i = threadIdx.x; i < n; i += kFftThreads) {
+#pragma unroll
+    for (int m = 0; m < V; m++) {
+      const int p = p0 + t + m * T;
       double er = 0.0, ei = 0.0;
-      if (!F.gen) { const f64x2 e = Ef[p0 + i]; er = e.x; ei = e.y; }
-      pw_point(pre, F, f, p0 + i, er, ei);
-      sbuf[i] = make_double2(er, ei);
+      if (!F.gen && live) { const f64x2 e = Ef[p]; er = e.x; ei = e.y; }
+      if (live) pw_point_rc(pre, F, f, p, t + m * T, p0 / N, er, ei);
+      v[m] = make_double2(er, ei);
     }
   } else {
-    for (int i = threadIdx.x; i < n; i += kFftThreads) {
-      const f64x2 e = Ef[p0 + i];
-      sbuf[i] = make_double2(e.x, e.y);
+#pragma unroll
+    for (int m = 0; m < V; m++) {
+      const f64x2 e = live ? Ef[p0 + t + m * T] : f64x2{0.0, 0.0};
+      v[m] = make_double2(e.x, e.y);
     }
   }
-  __syncthreads();
-  fft_lds<SIGN>(sbuf, n, tw, threadIdx.x, kFftThreads);
+  fft_reg<N, SIGN>(v, t, tw, sbuf + g * fft_lds_len(N), 1);
+  if (!live) return;
   if (has_post) {
     const PwField G = pw_field(post, f);
-    for (int i = threadIdx.x; i < n; i += kFftThreads) {
-      double er = sbuf[i].x * scale, ei = sbuf[i].y * scale;
-      pw_point(post, G, f, p0 + i, er, ei);
-      Ef[p0 + i] = f64x2{er, ei};
+#pragma unroll
+    for (int m = 0; m < V; m++) {
+      const int p = p0 + t + m * T;
+      double er = v[m].x * scale, ei = v[m].y * scale;
+      pw_point_rc(post, G, f, p, t + m * T, p0 / N, er, ei);
+      Ef[p] = f64x2{er, ei};
     }
   } else {
-    for (int i = threadIdx.x; i < n; i += kFftThreads) Ef[p0 + i] = f64x2{sbuf[i].x * scale, sbuf[i].y * scale};
+#pragma unroll
+    for (int m = 0; m < V; m++) Ef[p0 + t + m * T] = f64x2{v[m].x * scale, v[m].y * scale};
   }
 }
 
+// Rows per panel: a power of two, at most N, at most kFftPanelMaxThreads / T, the
+// panel's LDS (RP padded rows) within kFftPanelLds.
+template <int N>
+__host__ __device__ constexpr int fft_panel_rows() {
+  int rp = 1;
+  while (2 * rp <= N && 2 * rp * fft_threads(N) <= kFftPanelMaxThreads &&
+         (long)2 * rp * fft_lds_len(N) * 16 <= kFftPanelLds)
+    rp *= 2;
+  return rp;
+}
+
 // Along the strided (column) index: a panel of RP consecutive rows of one field per
-// block (RP n-value transforms, kFftPanelThreads / RP threads each); the panel is read
-// and written as RP-value runs of every column (16 RP contiguous bytes).  Rows sit
-// n + 1 values apart in LDS (the runs' RP values fall on distinct banks).
+// block, RP T threads, thread (rr, t) = threadIdx.x (rr = x mod RP fastest) holding row
+// r0 + rr's values t + m T.  Each global access of a wave is T-stride runs of RP values
+// (16 RP contiguous bytes).  In LDS the rows are interleaved (value i of row rr at
+// fft_lds_ix(i) RP + rr), so lanes of consecutive rows hit consecutive slots.  Panels
+// are numbered XCD-major: the blocks one XCD receives (every 8th in dispatch order)
+// take consecutive panels, so the runs that share a 128-byte line (RP = 2 at
+// N = 2048: four panels) are read through one L2 at about the same time.
 // MODE 0: the transform of sign SIGN.  MODE 1: forward, times the angular-spectrum
-// transfer function, inverse: the Fresnel step's middle pass.
-template <int MODE, int SIGN>
-__global__ __launch_bounds__(kFftPanelThreads) void k_fft_panel(double2* __restrict__ E, int n, int RP,
-                                                                const double* __restrict__ lambdas, double ap_rad,
-                                                                double z, const double2* __restrict__ tw) {
+// transfer function, inverse: the Fresnel step's middle pass (the forward transform's
+// results are the inverse's operands in registers).
+template <int N, int MODE, int SIGN>
+__global__ __launch_bounds__(fft_panel_rows<N>() * fft_threads(N)) void k_fft_panel(
+    double2* __restrict__ E, const double* __restrict__ lambdas, double ap_rad, double z,
+    const double2* __restrict__ tw) {
+  constexpr int T = fft_threads(N), V = N / T, RP = fft_panel_rows<N>(), P = N / RP;
   extern __shared__ double2 sbuf[];
-  const int f = blockIdx.y, r0 = blockIdx.x * RP, ld = n + 1;
-  const int nth = kFftPanelThreads / RP, g = threadIdx.x / nth, tid = threadIdx.x - g * nth;
-  GLOBAL f64x2* F = gout<f64x2>(E) + (long)f * n * n;
-  for (int e = threadIdx.x; e < RP * n; e += kFftPanelThreads) {
-    const int c = e / RP, rr = e - c * RP;
-    const f64x2 v = F[(long)c * n + r0 + rr];
-    sbuf[rr * ld + c] = make_double2(v.x, v.y);
+  const int bx = blockIdx.x;
+  const int panel = (P % 8 == 0) ? (bx & 7) * (P / 8) + (bx >> 3) : bx;
+  const int f = blockIdx.y, r0 = panel * RP;
+  const int rr = threadIdx.x % RP, t = threadIdx.x / RP;
+  GLOBAL f64x2* F = gout<f64x2>(E) + (long)f * N * N + r0 + rr;
+  double2 v[V];
+#pragma unroll
+  for (int m = 0; m < V; m++) {
+    const f64x2 e = F[(long)(t + m * T) * N];
+    v[m] = make_double2(e.x, e.y);
   }
-  __syncthreads();
-  double2* row = sbuf + g * ld;
+  double2* lds = sbuf + rr;
   if (MODE == 0) {
-    fft_lds<SIGN>(row, n, tw, tid, nth);
+    fft_reg<N, SIGN>(v, t, tw, lds, RP);
   } else {
-    fft_lds<-1>(row, n, tw, tid, nth);
+    fft_reg<N, -1>(v, t, tw, lds, RP);
     {
 #pragma clang fp contract(off)
       // csim: exp(i*k*z * sqrt(1. - 1./4.*pow(lambda/apRad, 2.)*H)), Goodman 4-20
@@ -695,35 +738,37 @@ __global__ __launch_bounds__(kFftPanelThreads) void k_fft_panel(double2* __restr
       const double k = 2. * M_PI / lam;
       const double q = lam / ap_rad;
       const double c4 = 1. / 4. * (q * q);
-      const int r = r0 + g;
-      const int pr = r < n / 2 ? r : r - n;      // fft_shift of regspace(-n/2, n/2-1)
-      for (int c = tid; c < n; c += nth) {
-        const int pc = c < n / 2 ? c : c - n;
+      const int r = r0 + rr;
+      const int pr = r < N / 2 ? r : r - N;      // fft_shift of regspace(-n/2, n/2-1)
+#pragma unroll
+      for (int m = 0; m < V; m++) {
+        const int c = t + m * T;
+        const int pc = c < N / 2 ? c : c - N;
         const double H = (double)pc * (double)pc + (double)pr * (double)pr;
         const double ph = k * z * sqrt(1. - c4 * H);
         double sn, cs;
-        sincos_reduced(ph, &sn, &cs);
-        row[c] = cmul(row[c], make_double2(cs, sn));
+        sincos_phase(ph, &sn, &cs);
+        v[m] = cmul(v[m], make_double2(cs, sn));
+        // one value at a time: interleaving the V phasors would not fit the 128 VGPRs a
+        // 1024-thread block leaves
+        __builtin_amdgcn_sched_barrier(0);
       }
     }
-    __syncthreads();
-    fft_lds<+1>(row, n, tw, tid, nth);
+    // the inverse reads the forward's twiddles and LDS slots again: opaque copies of the
+    // pointers and of t keep the compiler from holding the forward's loaded twiddles and
+    // computed addresses in registers across H (they would not fit)
+    const double2* tw_inv = tw;
+    double2* lds_inv = lds;
+    int t_inv = t;
+    asm volatile("" : "+s"(tw_inv), "+v"(lds_inv), "+v"(t_inv));
+    fft_reg<N, +1>(v, t_inv, tw_inv, lds_inv, RP);
   }
-  for (int e = threadIdx.x; e < RP * n; e += kFftPanelThreads) {
-    const int c = e / RP, rr = e - c * RP;
-    const double2 v = sbuf[rr * ld + c];
-    F[(long)c * n + r0 + rr] = f64x2{v.x, v.y};
-  }
-}
-
-// Rows per panel: 128 KiB of LDS at most (with the padding), at least n / 16 threads
-// per transform (16 values per thread per stage), at most n.
-static int panel_rows(int n) {
-  int rp = 8;
-  while (rp > 1 && ((size_t)rp * (n + 1) * sizeof(double2) > (size_t)132 << 10 || kFftPanelThreads / rp < n / 16))
-    rp >>= 1;
-  while (rp > 1 && rp > n) rp >>= 1;
-  return rp;
+  // the stores recompute their addresses (an opaque copy of the base): keeping the loads'
+  // V addresses live through the transforms costs registers the transforms need
+  GLOBAL f64x2* Fw = F;
+  asm volatile("" : "+v"(Fw));
+#pragma unroll
+  for (int m = 0; m < V; m++) Fw[(long)(t + m * T) * N] = f64x2{v[m].x, v[m].y};
 }
 
 static int check_fft_n(int n) {
@@ -749,51 +794,78 @@ static const double2* twiddles(int n, hipStream_t s) {
   return p;
 }
 
-static int launch_contig(double2* E, int n, int nfield, int sign, double scale, const double2* tw, hipStream_t s,
-                         const PwChain* pre = nullptr, const PwChain* post = nullptr) {
-  const size_t lds = sizeof(double2) * n;
+// Dynamic LDS beyond 64 KiB has to be allowed per kernel (once per process).
+template <class K>
+static int allow_lds(K* kernel, size_t bytes) {
+  if (bytes <= ((size_t)64 << 10)) return CSX_OK;
+  CSX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)bytes));
+  return CSX_OK;
+}
+
+template <int N, int SIGN>
+static int launch_contig(double2* E, int nfield, double scale, const double2* tw, hipStream_t s,
+                         const PwChain* pre, const PwChain* post) {
+  constexpr int B = fft_contig_block<N>(), C = B / fft_threads(N);
+  const size_t lds = sizeof(double2) * C * fft_lds_len(N);
+  static const int attr = allow_lds(&k_fft_contig<N, SIGN>, lds);
+  CSX_TRY(attr);
+  const int ncol = N * nfield;
   PwChain none{};
-  const PwChain& a = pre ? *pre : none;
-  const PwChain& b = post ? *post : none;
-  if (sign < 0)
-    hipLaunchKernelGGL(k_fft_contig<-1>, dim3(n, nfield), dim3(kFftThreads), lds, s, E, n, scale, tw, a, pre ? 1 : 0,
-                       b, post ? 1 : 0);
-  else
-    hipLaunchKernelGGL(k_fft_contig<+1>, dim3(n, nfield), dim3(kFftThreads), lds, s, E, n, scale, tw, a, pre ? 1 : 0,
-                       b, post ? 1 : 0);
+  hipLaunchKernelGGL((k_fft_contig<N, SIGN>), dim3((ncol + C - 1) / C), dim3(B), lds, s, E, ncol, scale, tw,
+                     pre ? *pre : none, pre ? 1 : 0, post ? *post : none, post ? 1 : 0);
   CSX_HIP(hipGetLastError());
   return CSX_OK;
 }
-template <int MODE, int SIGN>
-static int launch_panel_t(double2* E, int n, int nfield, const double* lambdas, double ap_rad, double z,
-                          const double2* tw, hipStream_t s) {
-  const int rp = panel_rows(n);
-  const size_t lds = sizeof(double2) * (size_t)rp * (n + 1);
-  static bool attr = false;
-  if (!attr) {   // dynamic LDS beyond 64 KiB
-    CSX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_panel<MODE, SIGN>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)(136 << 10)));
-    attr = true;
-  }
-  hipLaunchKernelGGL((k_fft_panel<MODE, SIGN>), dim3(n / rp, nfield), dim3(kFftPanelThreads), lds, s, E, n, rp,
+template <int N, int MODE, int SIGN>
+static int launch_panel(double2* E, int nfield, const double* lambdas, double ap_rad, double z, const double2* tw,
+                        hipStream_t s) {
+  constexpr int RP = fft_panel_rows<N>();
+  const size_t lds = sizeof(double2) * RP * fft_lds_len(N);
+  static const int attr = allow_lds(&k_fft_panel<N, MODE, SIGN>, lds);
+  CSX_TRY(attr);
+  hipLaunchKernelGGL((k_fft_panel<N, MODE, SIGN>), dim3(N / RP, nfield), dim3(RP * fft_threads(N)), lds, s, E,
                      lambdas, ap_rad, z, tw);
   CSX_HIP(hipGetLastError());
   return CSX_OK;
 }
 
+enum FftOp { kFftForward, kFftInverse, kFftFresnel };
+template <int N>
+static int fft_run(double2* E, int nfield, FftOp op, const double* lambdas, double ap_rad, double z,
+                   const PwChain* pre, const PwChain* post, hipStream_t s) {
+  const double2* tw = twiddles(N, s);
+  if (!tw) return fail(CSX_ERR_NOMEM, "fft: twiddle table allocation failed");
+  const double inv = 1.0 / ((double)N * (double)N);
+  if (op == kFftForward) {
+    CSX_TRY((launch_contig<N, -1>(E, nfield, 1.0, tw, s, nullptr, nullptr)));
+    CSX_TRY((launch_panel<N, 0, -1>(E, nfield, nullptr, 0, 0, tw, s)));
+  } else if (op == kFftInverse) {
+    CSX_TRY((launch_panel<N, 0, +1>(E, nfield, nullptr, 0, 0, tw, s)));
+    CSX_TRY((launch_contig<N, +1>(E, nfield, inv, tw, s, nullptr, nullptr)));
+  } else {
+    CSX_TRY((launch_contig<N, -1>(E, nfield, 1.0, tw, s, pre, nullptr)));
+    CSX_TRY((launch_panel<N, 1, 0>(E, nfield, lambdas, ap_rad, z, tw, s)));
+    CSX_TRY((launch_contig<N, +1>(E, nfield, inv, tw, s, nullptr, post)));
+  }
+  return CSX_OK;
+}
+static int fft_dispatch(int n, double2* E, int nfield, FftOp op, const double* lambdas, double ap_rad, double z,
+                        const PwChain* pre, const PwChain* post, hipStream_t s) {
+  switch (n) {
+#define CSX_FFT_N(N) \
+    case N: return fft_run<N>(E, nfield, op, lambdas, ap_rad, z, pre, post, s);
+    CSX_FFT_N(2) CSX_FFT_N(4) CSX_FFT_N(8) CSX_FFT_N(16) CSX_FFT_N(32) CSX_FFT_N(64) CSX_FFT_N(128)
+    CSX_FFT_N(256) CSX_FFT_N(512) CSX_FFT_N(1024) CSX_FFT_N(2048) CSX_FFT_N(4096)
+#undef CSX_FFT_N
+    default: return fail(CSX_ERR_ARG, "fft: n must be a power of two in [2, 4096]");
+  }
+}
+
 int fft2_batched(double2* E, int n, int nfield, int dir, hipStream_t s) {
   CSX_TRY(check_fft_n(n));
   if (nfield <= 0) return CSX_OK;
-  const double2* tw = twiddles(n, s);
-  if (!tw) return fail(CSX_ERR_NOMEM, "fft: twiddle table allocation failed");
-  if (dir < 0) {
-    CSX_TRY(launch_contig(E, n, nfield, -1, 1.0, tw, s));
-    CSX_TRY((launch_panel_t<0, -1>(E, n, nfield, nullptr, 0, 0, tw, s)));
-  } else {
-    CSX_TRY((launch_panel_t<0, +1>(E, n, nfield, nullptr, 0, 0, tw, s)));
-    CSX_TRY(launch_contig(E, n, nfield, +1, 1.0 / ((double)n * (double)n), tw, s));
-  }
-  return CSX_OK;
+  return fft_dispatch(n, E, nfield, dir < 0 ? kFftForward : kFftInverse, nullptr, 0, 0, nullptr, nullptr, s);
 }
 
 int fresnel_batched(double2* E, int n, int nfield, const double* lambdas_dev, double ap_rad, double z, hipStream_t s,
@@ -802,12 +874,7 @@ int fresnel_batched(double2* E, int n, int nfield, const double* lambdas_dev, do
   if (nfield <= 0) return CSX_OK;
   if ((pre && (pre->rows != n || pre->cols != n)) || (post && (post->rows != n || post->cols != n)))
     return fail(CSX_ERR_ARG, "fresnel: pointwise chain grid differs from the field");
-  const double2* tw = twiddles(n, s);
-  if (!tw) return fail(CSX_ERR_NOMEM, "fft: twiddle table allocation failed");
-  CSX_TRY(launch_contig(E, n, nfield, -1, 1.0, tw, s, pre, nullptr));
-  CSX_TRY((launch_panel_t<1, 0>(E, n, nfield, lambdas_dev, ap_rad, z, tw, s)));
-  CSX_TRY(launch_contig(E, n, nfield, +1, 1.0 / ((double)n * (double)n), tw, s, nullptr, post));
-  return CSX_OK;
+  return fft_dispatch(n, E, nfield, kFftFresnel, lambdas_dev, ap_rad, z, pre, post, s);
 }
 
 // --------------------------------------------------------------- resampling --
@@ -871,7 +938,7 @@ __global__ void k_hex(const int32_t* __restrict__ nsub, const int32_t* __restric
       // csim: exp(phaseSign * 2 * M_PI * i * 2.0 * sag / lambda)
       double ph = phase_sign * 2 * M_PI * 2.0 * sag / lam;
       double sn, cs;
-      sincos(ph, &sn, &cs);
+      sincos_phase(ph, &sn, &cs);
       ar += ns * cs;
       ai += ns * sn;
     }

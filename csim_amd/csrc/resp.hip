@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "phase.hpp"
 #include "gemm.hpp"
 #include "mft.hpp"
 
@@ -50,7 +51,7 @@ __global__ void k_resp_phases(const double* __restrict__ X, int nopt, int ncol, 
     double x = i < nopt ? X[(long)j * nopt + i] : 0.0;
     double ph = a_s[s] * x;
     double sn, cs;
-    sincos(ph, &sn, &cs);
+    sincos_phase(ph, &sn, &cs);
     put(&R[e], cs, sn);
   }
 }

@@ -189,28 +189,54 @@ def test_mft_fp64_tile_variants(ctx, monkeypatch, nin, nout, batch):
     monkeypatch.delenv("CSX_GEMM_VARIANT")
 
 
-@pytest.mark.parametrize("n", [8, 64, 256, 2048])
+@pytest.mark.parametrize("n", [2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096])
 def test_fft2_vs_numpy(ctx, n):
+    # every length the size-specialised kernels cover; batches of 3 (below n = 64 a
+    # row-pass block spans several fields)
     rng = np.random.default_rng(n)
-    E = rand_field(rng, n, n)
+    batch = 3 if n <= 1024 else 1
+    E = np.stack([rand_field(rng, n, n) for _ in range(batch)])
     d = ops.to_device(E)
-    out = ops.to_host(ops.fft2(ctx, d, -1))[0]
-    assert rel_rms(out, np.fft.fft2(E)) < 1e-14
-    back = ops.to_host(ops.fft2(ctx, d, +1))[0]
-    assert rel_rms(back, E) < 1e-14
+    out = ops.to_host(ops.fft2(ctx, d, -1))
+    for b in range(batch):
+        assert rel_rms(out[b], np.fft.fft2(E[b])) < 1e-14, (n, b)
+    back = ops.to_host(ops.fft2(ctx, d, +1))
+    for b in range(batch):
+        assert rel_rms(back[b], E[b]) < 1e-14, (n, b)
 
 
-@pytest.mark.parametrize("n", [128, 2048])
-def test_fresnel_vs_oracle(ctx, n):
+@pytest.mark.parametrize("n,batch", [(16, 3), (128, 2), (512, 2), (2048, 1), (4096, 1)])
+def test_fresnel_vs_oracle(ctx, n, batch):
     rng = np.random.default_rng(5)
-    E = rand_field(rng, n, n)
-    lam, z = 650e-9, 1.0
+    E = np.stack([rand_field(rng, n, n) for _ in range(batch)])
+    lams = [650e-9, 800e-9, 550e-9][:batch]
+    z = 1.0
     ap = n * 10e-6 / 2
-    ref = oracle.fresnel_as(E, lam, ap, z)
-    out = ops.to_host(ops.fresnel_as(ctx, ops.to_device(E), [lam], ap, z))[0]
-    e = rel_rms(out, ref)
-    print(f"fresnel n={n}: {e:.3e}")
-    assert e < 1e-12
+    out = ops.to_host(ops.fresnel_as(ctx, ops.to_device(E), lams, ap, z))
+    for b in range(batch):
+        e = rel_rms(out[b], oracle.fresnel_as(E[b], lams[b], ap, z))
+        print(f"fresnel n={n} field {b}: {e:.3e}")
+        assert e < 1e-12
+
+
+def test_mirror_phasor_accuracy(ctx):
+    # The phasor kernels' sine and cosine (Cody-Waite reduction by pi/2, Taylor series)
+    # against numpy's on phases up to 1.2e7 rad (the angular-spectrum phases of a 1 m
+    # step reach that) and near the quadrant boundaries: unit field, E = exp(i a h).
+    rng = np.random.default_rng(21)
+    n = 256
+    lam, sign = 633e-9, -1.0
+    a = -sign * 2 * 2 * np.pi / lam
+    ph = np.concatenate([rng.uniform(-1.2e7, 1.2e7, n * n // 2),
+                         (np.arange(n * n // 2) - n * n // 4) * (np.pi / 4) + rng.uniform(-1e-9, 1e-9, n * n // 2)])
+    h = (ph / a).reshape(n, n)
+    ph = a * h                                   # the phase the kernel forms, same rounding
+    d = ops.to_device(np.ones((n, n), dtype=np.complex128))
+    ops.apply_mirror(ctx, d, ops.real_to_device(h), [lam], sign=sign)
+    out = ops.to_host(d)[0]
+    err = np.max(np.abs(out - (np.cos(ph) + 1j * np.sin(ph))))
+    print(f"phasor max abs error vs numpy: {err:.3e}")
+    assert err < 3e-16
 
 
 def test_downsample_vs_oracle(ctx):
