@@ -248,10 +248,13 @@ def de_batch_uneven(ctx, precision, steps=5, cpu=False):
     return res
 
 
-def pmc_probe(precision):
-    """Short run for hardware-counter passes (rocprofv3 --pmc): a plain device copy
-    of known size (512 MiB read, 512 MiB written) that calibrates the byte counters,
-    then the workload's steps (graph replay off, so every kernel is its own dispatch)."""
+def pmc_probe(precision, workload="w2"):
+    """Short run for hardware-counter and kernel-trace passes (rocprofv3): a plain
+    device copy of known size (512 MiB read, 512 MiB written) that calibrates the byte
+    counters, then the workload's steps with graph replay off (every kernel its own
+    dispatch).  A one-element fill (`probe_marker`, a torch elementwise kernel) is
+    launched right before the last step, so the last step's dispatches are the ones
+    that follow it in the trace."""
     import torch
     from csim_amd import workloads as W
     from csim_amd.ops import Context
@@ -264,13 +267,19 @@ def pmc_probe(precision):
     torch.cuda.synchronize()
     del x, y
     ctx = Context(0)
-    ch = build_chain(ctx, W.workload2(), precision)
+    w = {"w2": W.workload2, "c0": W.workload0, "c1": W.workload1}[workload]()
+    ch = build_chain(ctx, w, precision)
     ch.set_graph(False)
-    for _ in range(3):
+    for _ in range(2):
         ch.contrast()
     torch.cuda.synchronize()
+    marker = torch.zeros(1, dtype=torch.int32, device="cuda")
+    marker.add_(7)
+    torch.cuda.synchronize()
+    ch.contrast()
+    torch.cuda.synchronize()
     ch.close()
-    print("pmc probe done", flush=True)
+    print(f"probe done: workload {workload}, precision {precision}", flush=True)
     return 0
 
 
@@ -303,14 +312,17 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--precision", default="bf16x6", choices=["fp32", "bf16x3", "bf16x6"],
+    ap.add_argument("--precision", default="bf16x6", choices=["fp64", "fp32", "bf16x3", "bf16x6"],
                     help="headline precision (bf16x6: the one that meets the 1e-11 gate on this workload)")
     ap.add_argument("--also", default="bf16x3", help="comma list of further precisions timed beside it ('' none)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-uneven", action="store_true")
     ap.add_argument("--no-legs", action="store_true", help="skip the C0 and C1 single-field legs")
-    ap.add_argument("--pmc-probe", action="store_true", help="short run for rocprofv3 --pmc counter passes")
+    ap.add_argument("--pmc-probe", action="store_true", help="short run for rocprofv3 counter and trace passes")
+    ap.add_argument("--probe-workload", default="w2", choices=["w2", "c0", "c1"],
+                    help="workload of --pmc-probe: w2 the 27-field bench workload, c0 one 1024^2 field, "
+                         "c1 the 2048^2 PIAA + Fresnel chain")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print("bench.py measures one GPU: every field shards independently, so the whole-job rate at N GPUs is "
@@ -321,7 +333,10 @@ def main(argv=None):
         print("bench.py needs a HIP device", file=sys.stderr)
         return 1
     if a.pmc_probe:
-        return pmc_probe(a.precision)
+        return pmc_probe(a.precision, a.probe_workload)
+    if a.precision == "fp64":
+        print("--precision fp64 is for --pmc-probe only: the bench times FP64 beside the mixed path", file=sys.stderr)
+        return 2
     from csim_amd import _lib
     from csim_amd import workloads as W
     from csim_amd.ops import Context
