@@ -72,6 +72,24 @@ __device__ __forceinline__ bool tile_krange(const GemmProb& P, int m0, int n0, i
   return skip;
 }
 
+// K range of split ks of a problem split ksplit ways: ceil(K / ksplit) rounded up to
+// whole slabs of BK, so every split but the last starts and ends on the slab grid.
+template <int BK>
+__device__ __forceinline__ void split_krange(const GemmProb& P, int ks, int& kbeg, int& kend) {
+  kbeg = 0;
+  kend = P.K;
+  if (P.ksplit > 1) {
+    const int kc = ((P.K + P.ksplit - 1) / P.ksplit + BK - 1) / BK * BK;
+    kbeg = min(P.K, ks * kc);
+    kend = min(P.K, kbeg + kc);
+  }
+}
+// One split's FP64 sums into its workspace slice (the finishing launch adds them).
+__device__ __forceinline__ void split_store(const GemmProb& P, int ks, int m, int n, double re, double im) {
+  if (m < P.M && n < P.N)
+    gout<f64x2>(P.ws)[((long)ks * P.M + m) * P.N + n] = f64x2{re, im};
+}
+
 // Epilogue: alpha, mask (epilogue_value), store (shared by every kernel).
 __device__ __forceinline__ void epilogue_value(const GemmProb& P, int m, int n, double re, double im, double& out_r,
                                                double& out_i) {
@@ -171,8 +189,9 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const GemmProb* __restrict__ pr
 
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   const GemmProb& P = probs[find_prob(probs, nprob, tile)];
-  const int local = tile - P.tile_start;
-  const int tm = local % P.tiles_m, tn = local / P.tiles_m;
+  const int local = tile - P.tile_start, tiles_mn = P.tiles_m * P.tiles_n;
+  const int ks = local / tiles_mn, lmn = local - ks * tiles_mn;   // K split, output tile
+  const int tm = lmn % P.tiles_m, tn = lmn / P.tiles_m;
   const int m0 = tm * kTileM, n0 = tn * kTileN;
   const GLOBAL C2* A = gin<C2>(P.A);
   const GLOBAL C2* B = gin<C2>(P.B);
@@ -189,8 +208,9 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const GemmProb* __restrict__ pr
 #pragma unroll
     for (int j = 0; j < 2; j++) { accR[i][j] = V4{0, 0, 0, 0}; accI[i][j] = V4{0, 0, 0, 0}; }
 
-  int kbeg = 0, kend = K;
-  tile_krange<kTileM, kTileN, BK>(P, m0, n0, kbeg, kend);
+  int kbeg, kend;
+  split_krange<BK>(P, ks, kbeg, kend);
+  if (tile_krange<kTileM, kTileN, BK>(P, m0, n0, kbeg, kend) && P.ksplit > 1) return;
   C2 ra[4], rb[4];
   // unconditional loads from clamped (in-bounds) addresses, zeroed by select
   const int am = m0 + lrow, bn = n0 + lrow;
@@ -242,7 +262,8 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const GemmProb* __restrict__ pr
       for (int v = 0; v < 4; v++) {
         int m = m0 + wm * 32 + i * 16 + Mma<T>::row(lane, v);
         int n = n0 + wn * 32 + j * 16 + (lane & 15);
-        epilogue_store(P, m, n, (double)accR[i][j][v], (double)accI[i][j][v], kend <= kbeg);
+        if (P.ksplit > 1) split_store(P, ks, m, n, (double)accR[i][j][v], (double)accI[i][j][v]);
+        else epilogue_store(P, m, n, (double)accR[i][j][v], (double)accI[i][j][v], kend <= kbeg);
       }
 }
 
@@ -264,8 +285,9 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_f64db(const GemmProb* __r
 
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   const GemmProb& P = probs[find_prob(probs, nprob, tile)];
-  const int local = tile - P.tile_start;
-  const int tm = local % P.tiles_m, tn = local / P.tiles_m;
+  const int local = tile - P.tile_start, tiles_mn = P.tiles_m * P.tiles_n;
+  const int ks = local / tiles_mn, lmn = local - ks * tiles_mn;   // K split, output tile
+  const int tm = lmn % P.tiles_m, tn = lmn / P.tiles_m;
   const int m0 = tm * BM, n0 = tn * BN;
   const GLOBAL f64x2* A = gin<f64x2>(P.A);
   const GLOBAL f64x2* B = gin<f64x2>(P.B);
@@ -316,8 +338,9 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_f64db(const GemmProb* __r
       for (int q = 0; q < 4; q++) Bs[buf][r][kq + q] = (rin && k0 + kq + q < K) ? rb[c][q] : z;
     }
   };
-  int kbeg = 0, kend = K;
-  tile_krange<BM, BN, BK>(P, m0, n0, kbeg, kend);
+  int kbeg, kend;
+  split_krange<BK>(P, ks, kbeg, kend);
+  if (tile_krange<BM, BN, BK>(P, m0, n0, kbeg, kend) && P.ksplit > 1) return;
   const int nslab = (kend - kbeg + BK - 1) / BK;
   gload(kbeg);
   sstore(0, kbeg);
@@ -358,7 +381,8 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_f64db(const GemmProb* __r
       for (int v = 0; v < 4; v++) {
         const int m = m0 + wm * (BM / WM) + i * 16 + Mma<double>::row(lane, v);
         const int n = n0 + wn * (BN / WN) + j * 16 + (lane & 15);
-        epilogue_store(P, m, n, accR[i][j][v], accI[i][j][v], nslab == 0);
+        if (P.ksplit > 1) split_store(P, ks, m, n, accR[i][j][v], accI[i][j][v]);
+        else epilogue_store(P, m, n, accR[i][j][v], accI[i][j][v], nslab == 0);
       }
 }
 
@@ -494,12 +518,8 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
   const GLOBAL unsigned short* B = gin<unsigned short>(P.B);
   const int M = P.M, N = P.N, K = P.K;
   // K range of this block: whole slabs except at K itself
-  int kbeg = 0, kend = K;
-  if (TERMS == 2 && P.ksplit > 1) {
-    const int kc = ((K + P.ksplit - 1) / P.ksplit + BK - 1) / BK * BK;
-    kbeg = min(K, ks * kc);
-    kend = min(K, kbeg + kc);
-  }
+  int kbeg, kend;
+  split_krange<BK>(P, ks, kbeg, kend);
   if (tile_krange<BM, BN, BK>(P, m0, n0, kbeg, kend)) return;   // uniform over the block
 
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -747,6 +767,20 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
       compute(lds);
     }
   }
+  }
+  if (TERMS == 3 && P.ksplit > 1) {
+    // this split's FP64 sums; gemm_splitk_finish adds the splits and applies the epilogue
+#pragma unroll
+    for (int i = 0; i < IM; i++)
+#pragma unroll
+      for (int j = 0; j < JN; j++)
+#pragma unroll
+        for (int v = 0; v < NV; v++)
+          split_store(P, ks, m0 + wm * (BM / WM) + i * MF + Mf::drow(lane, v),
+                      n0 + wn * (BN / WN) + j * MF + Mf::dcol(lane),
+                      (double)accR[i][j][v] + dR[TERMS == 3 ? i : 0][TERMS == 3 ? j : 0][v],
+                      (double)accI[i][j][v] + dI[TERMS == 3 ? i : 0][TERMS == 3 ? j : 0][v]);
+    return;
   }
   if (TERMS == 2 && P.ksplit > 1) {
     // this split's FP32 sums; gemm_splitk_finish adds the splits and applies the epilogue
@@ -1034,7 +1068,9 @@ static const PlanesVariant kVariants3[] = {
 };
 
 std::string gemm_variant_names() {
-  std::string s = "f64:";
+  // the kernel-set revision: split K for every precision (FP64 workspaces for FP64,
+  // FP32 and three-term planes)
+  std::string s = "splitk-all;f64:";
   for (const auto& v : kVariantsF64) { s += v.name; s += ';'; }
   s += "p2:";
   for (const auto& v : kVariants2) { s += v.name; s += ';'; }
@@ -1053,27 +1089,57 @@ int planes_variant_tile(int terms, int v, int* bm, int* bn) {
   *bm = pv.bm; *bn = pv.bn;
   return CSX_OK;
 }
-__global__ void k_splitk_finish(const GemmProb* __restrict__ probs, int nprob, long total) {
-  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+// One thread per output element, n fastest.  Three-term plane outputs take the leading
+// part of every value on the fixed-point grid of its window of kPlaneWindow consecutive
+// columns (planes.hpp): the windows are whole groups of 32 lanes (N and every problem's
+// element offset are multiples of kPlaneWindow, checked at plan time), and every lane of
+// a group runs the loop body.
+__global__ __launch_bounds__(256) void k_splitk_finish(const GemmProb* __restrict__ probs, int nprob, long total) {
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e - threadIdx.x % kPlaneWindow < total; e += stride) {
+    const bool live = e < total;
+    const long el = live ? e : total - 1;
     int lo = 0, hi = nprob - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
-      if (probs[mid].elem_start <= e) lo = mid; else hi = mid - 1;
+      if (probs[mid].elem_start <= el) lo = mid; else hi = mid - 1;
     }
     const GemmProb& P = probs[lo];
-    const long q = e - P.elem_start, mn = (long)P.M * P.N;
+    const long q = el - P.elem_start, mn = (long)P.M * P.N;
     const int m = (int)(q / P.N), n = (int)(q - (long)m * P.N);
     // a block that is zero and never read (K range lo = -1): its tiles wrote no sums
     // and nothing is stored there
-    if (P.kr && P.kr[2 * ((long)(m / kKrBlock) * P.kr_ld + n / kKrBlock)] < 0) continue;
-    const float2* W = reinterpret_cast<const float2*>(P.ws);
+    const bool skip = !live || (P.kr && P.kr[2 * ((long)(m / kKrBlock) * P.kr_ld + n / kKrBlock)] < 0);
     double re = 0, im = 0;
-    for (int k = 0; k < P.ksplit; k++) {         // fixed order: deterministic
-      const float2 w = W[(long)k * mn + q];
-      re += w.x;
-      im += w.y;
+    if (!skip) {
+      if (P.ws_f64) {
+        const f64x2* W = reinterpret_cast<const f64x2*>(P.ws);
+        for (int k = 0; k < P.ksplit; k++) {       // fixed order: deterministic
+          const f64x2 w = W[(long)k * mn + q];
+          re += w.x;
+          im += w.y;
+        }
+      } else {
+        const float2* W = reinterpret_cast<const float2*>(P.ws);
+        for (int k = 0; k < P.ksplit; k++) {
+          const float2 w = W[(long)k * mn + q];
+          re += w.x;
+          im += w.y;
+        }
+      }
     }
-    epilogue_store(P, m, n, re, im);
+    if (P.out_kind == OUT_PLANES && P.terms == 3) {
+      // uniform over the whole grid: a launch's problems share out_kind and terms
+      double vr = 0.0, vi = 0.0;
+      if (!skip) epilogue_value(P, m, n, re, im, vr, vi);
+      const int ew = window_exponent(window_max<kPlaneWindow>((float)fmax(fabs(vr), fabs(vi))));
+      if (!skip) {
+        const long off = P.transC ? (long)n * P.ldc + m : (long)m * P.ldc + n;
+        store_planes3(gout<unsigned short>(P.C), off, P.sC, vr, vi, ew);
+      }
+    } else if (!skip) {
+      epilogue_store(P, m, n, re, im);
+    }
   }
 }
 int gemm_splitk_finish(const GemmProb* probs_dev, int n, long total_elems, hipStream_t s) {

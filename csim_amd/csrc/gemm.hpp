@@ -38,11 +38,13 @@ struct GemmProb {
   // q * sX bf16 elements after its base pointer
   long sA, sB, sC;
   int terms;
-  // split K (two-term BF16 planes only): ksplit > 1 blocks per output tile each take a
-  // K range and write their FP32 sums to ws ([split][m][n] complex float); a finishing
+  // split K: ksplit > 1 blocks per output tile each take a K range (whole slabs) and
+  // write their sums to ws ([split][m][n]: complex float for two-term BF16 planes,
+  // complex double for FP64, FP32 and three-term planes, ws_f64 = 1); a finishing
   // launch (gemm_splitk_finish) adds the splits in order and applies the epilogue
   int ksplit;
-  float* ws;
+  void* ws;
+  int ws_f64;
   long elem_start;     // first output element of this problem among the launch's (finish kernel)
   // optional K range per 64 x 64 block of outputs: int pair [lo, hi) at
   // kr[2 * (mb * kr_ld + nb)] bounds the nonzero products of every output of block

@@ -1,5 +1,6 @@
 // Matrix Fourier transform plans (host side).
 #include "mft.hpp"
+#include "planes.hpp"
 
 #include <algorithm>
 #include <cstdio>
@@ -237,30 +238,47 @@ static std::string signature(const std::vector<GemmProb>& v, csx_precision p) {
   return s;
 }
 
-// K splits worth trying for a launch on a bm x bn tile: two-term planes whose tiles do
-// not fill the chip four times over and whose K is long enough to share (1 = no split).
-static int split_candidate(const std::vector<GemmProb>& v, csx_precision p, int bm, int bn) {
-  if (p != CSX_BF16X3) return 1;
-  if (const char* e = std::getenv("CSX_GEMM_KSPLIT")) return std::max(1, std::atoi(e));
+// K splits worth trying for a launch on a bm x bn tile whose tiles do not fill the chip
+// several times over and whose K is long enough to share: the split that brings the
+// launch to about `target` blocks and half of it (empty: no split).  Per precision:
+// the most splits, the least K per split, and the blocks a launch should reach (1, 2
+// or 4 blocks of a variant fit on each of the 256 CUs).  Three-term plane outputs are
+// finished in whole windows of kPlaneWindow columns, so their N must hold whole
+// windows.
+static std::vector<int> split_candidates(const std::vector<GemmProb>& v, csx_precision p, int bm, int bn) {
+  std::vector<int> out;
+  int kmax_split = 8, kmin_each = 128, target = 2048;
+  if (p == CSX_FP64 || p == CSX_FP32) { kmax_split = 32; kmin_each = 32; target = 1024; }
+  if (p == CSX_BF16X6) { kmax_split = 16; kmin_each = 64; target = 1024; }
   long tiles = 0;
   int kmin = 1 << 30;
   for (const auto& g : v) {
     tiles += (long)((g.M + bm - 1) / bm) * ((g.N + bn - 1) / bn);
     kmin = std::min(kmin, g.K);
+    if (p == CSX_BF16X6 && g.out_kind == OUT_PLANES && (g.N % kPlaneWindow || g.transC)) return out;
   }
-  if (tiles >= 1024 || kmin < 512) return 1;
-  int k = (int)std::min<long>(8, (2048 + tiles - 1) / tiles);
-  while (k > 1 && kmin / k < 128) k--;
-  return std::max(k, 1);
+  if (const char* e = std::getenv("CSX_GEMM_KSPLIT")) {
+    if (std::atoi(e) > 1) out.push_back(std::atoi(e));
+    return out;
+  }
+  if (tiles >= target || kmin < 2 * kmin_each) return out;
+  int k = (int)std::min<long>(kmax_split, (target + tiles - 1) / tiles);
+  while (k > 1 && kmin / k < kmin_each) k--;
+  if (k > 1) out.push_back(k);
+  if (k / 2 > 1) out.push_back(k / 2);
+  return out;
 }
 
-// Sets each problem's K splits and its workspace slice (ws: splits x M x N complex float).
-static void set_split(std::vector<GemmProb>& v, int ksplit, float* ws) {
-  long off = 0;
+// Sets each problem's K splits and its workspace slice (ws: splits x M x N complex, float
+// for two-term planes, double otherwise).
+static size_t split_elem_bytes(csx_precision p) { return p == CSX_BF16X3 ? sizeof(float2) : sizeof(double2); }
+static void set_split(std::vector<GemmProb>& v, csx_precision p, int ksplit, void* ws) {
+  size_t off = 0;
   for (auto& g : v) {
     g.ksplit = ksplit;
-    g.ws = ksplit > 1 ? ws + off : nullptr;
-    off += ksplit > 1 ? 2L * ksplit * g.M * g.N : 0;
+    g.ws = ksplit > 1 ? static_cast<char*>(ws) + off : nullptr;
+    g.ws_f64 = p != CSX_BF16X3;
+    off += ksplit > 1 ? split_elem_bytes(p) * ksplit * g.M * g.N : 0;
   }
 }
 
@@ -301,20 +319,22 @@ static int run_stage(const GemmProb* dev, const std::vector<GemmProb>& v, int ti
 static int choose_variant(std::vector<GemmProb>& v, csx_precision p, DevBuf& dev, DevBuf& ws, hipStream_t s,
                           int* variant, int* ksplit, int* tiles, bool tune = true) {
   const int nv = n_variants(p);
-  std::vector<int> kc(nv, 1);   // K splits worth trying with each variant's tile
+  std::vector<std::vector<int>> kc(nv);   // K splits worth trying with each variant's tile
   int kmax = 1;
-  for (int var = 0; var < nv && p == CSX_BF16X3 && tune; var++) {
-    int bm = 64, bn = 64;
-    CSX_TRY(planes_variant_tile(terms_of(p), var, &bm, &bn));
-    kc[var] = split_candidate(v, p, bm, bn);
-    kmax = std::max(kmax, kc[var]);
+  for (int var = 0; var < nv && tune; var++) {
+    int bm = kTileM, bn = kTileN;
+    if (p == CSX_BF16X3 || p == CSX_BF16X6) CSX_TRY(planes_variant_tile(terms_of(p), var, &bm, &bn));
+    if (p == CSX_FP64) CSX_TRY(f64_variant_tile(var, &bm, &bn));
+    kc[var] = split_candidates(v, p, bm, bn);
+    for (int k : kc[var]) kmax = std::max(kmax, k);
   }
-  if (kmax > 1) CSX_TRY(ws.reserve(sizeof(float) * 2 * (size_t)kmax * total_elems(v)));
+  const size_t wsz = split_elem_bytes(p);
+  if (kmax > 1) CSX_TRY(ws.reserve(wsz * (size_t)kmax * total_elems(v)));
   int forced = forced_variant(p);
   if (nv == 1 || forced >= 0) {
     *variant = nv == 1 ? 0 : forced;
-    *ksplit = std::getenv("CSX_GEMM_KSPLIT") && tune ? kc[*variant] : 1;
-    set_split(v, *ksplit, ws.as<float>());
+    *ksplit = std::getenv("CSX_GEMM_KSPLIT") && tune && !kc[*variant].empty() ? kc[*variant][0] : 1;
+    set_split(v, p, *ksplit, ws.p);
     return upload(v, p, *variant, dev, s, tiles);
   }
   const std::string key = signature(v, p);
@@ -324,7 +344,8 @@ static int choose_variant(std::vector<GemmProb>& v, csx_precision p, DevBuf& dev
     load_tune_cache(ts);
     auto it = ts.choices.find(key);
     if (it != ts.choices.end() && it->second.first < nv &&
-        (!tune || it->second.second <= std::max(1, kc[it->second.first]))) {
+        (!tune || it->second.second == 1 ||
+         std::count(kc[it->second.first].begin(), kc[it->second.first].end(), it->second.second))) {
       *variant = it->second.first;
       *ksplit = tune ? it->second.second : 1;
       if (!ts.source.count(key)) ts.source[key] = 0;
@@ -334,14 +355,14 @@ static int choose_variant(std::vector<GemmProb>& v, csx_precision p, DevBuf& dev
     }
   }
   if (*variant >= 0) {
-    if (*ksplit > 1) CSX_TRY(ws.reserve(sizeof(float) * 2 * (size_t)*ksplit * total_elems(v)));
-    set_split(v, *ksplit, ws.as<float>());
+    if (*ksplit > 1) CSX_TRY(ws.reserve(wsz * (size_t)*ksplit * total_elems(v)));
+    set_split(v, p, *ksplit, ws.p);
     return upload(v, p, *variant, dev, s, tiles);
   }
   std::vector<std::pair<int, int>> cand;
   for (int var = 0; var < nv; var++) cand.push_back({var, 1});
   for (int var = 0; var < nv; var++)
-    if (kc[var] > 1) cand.push_back({var, kc[var]});
+    for (int k : kc[var]) cand.push_back({var, k});
   const long elems = total_elems(v);
   hipEvent_t e0, e1;
   // timing-only events: no system-scope fence (no cache writeback between the timed launches)
@@ -353,7 +374,7 @@ static int choose_variant(std::vector<GemmProb>& v, csx_precision p, DevBuf& dev
     for (size_t c = 0; c < cand.size() && rc == CSX_OK; c++) {
       const int var = cand[c].first, ks = cand[c].second;
       int nt = 0;
-      set_split(v, ks, ws.as<float>());
+      set_split(v, p, ks, ws.p);
       if ((rc = upload(v, p, var, dev, s, &nt)) != CSX_OK) break;
       if ((rc = run_stage(dev.as<GemmProb>(), v, nt, p, s, var, ks, elems)) != CSX_OK) break;
       if (hipEventRecord(e0, s) != hipSuccess) { rc = fail(CSX_ERR_HIP, "autotune: event record"); break; }
@@ -388,7 +409,7 @@ static int choose_variant(std::vector<GemmProb>& v, csx_precision p, DevBuf& dev
   }
   *variant = cand[best].first;
   *ksplit = cand[best].second;
-  set_split(v, *ksplit, ws.as<float>());
+  set_split(v, p, *ksplit, ws.p);
   return upload(v, p, *variant, dev, s, tiles);
 }
 

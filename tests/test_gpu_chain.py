@@ -475,25 +475,28 @@ def test_candidate_batch_babinet_split(ctx, precision, monkeypatch):
     assert f1 < 0.75 * f0
 
 
-def test_split_k_on_every_stage(ctx, monkeypatch):
-    """Split K forced on every contraction of a bf16x3 chain, the FPM -> Lyot stage
-    included (its Lyot-zeroed blocks that the next contraction never reads get no
-    sums and no stores): fields and contrast equal the unsplit ones to FP32 rounding."""
+@pytest.mark.parametrize("precision,tol", [("bf16x3", (5e-6, 1e-5)), ("bf16x6", (1e-9, 1e-10)),
+                                           ("fp64", (1e-13, 1e-13))])
+def test_split_k_on_every_stage(ctx, monkeypatch, precision, tol):
+    """Split K forced on every contraction of a chain, the FPM -> Lyot stage included
+    (its Lyot-zeroed blocks that the next contraction never reads get no sums and no
+    stores): fields and contrast equal the unsplit ones to the rounding of the split
+    sums (FP32 for two-term planes, FP64 otherwise)."""
     w = W.workload2(n_lambda=3, offsets=(0.0, 2.0))
     monkeypatch.setenv("CSX_GEMM_VARIANT", "0")
     res = []
     for split in (None, "2"):
         if split:
             monkeypatch.setenv("CSX_GEMM_KSPLIT", split)
-        ch = build(ctx, w, "bf16x3")
+        ch = build(ctx, w, precision)
         c, _ = ch.contrast()
         res.append((c[0], [ch.field(0, s, li) for s in range(2) for li in range(3)]))
         ch.close()
     monkeypatch.delenv("CSX_GEMM_KSPLIT")
     (c0, f0), (c1, f1) = res
     worst = max(rel_rms(a, b) for a, b in zip(f1, f0))
-    print(f"split K on every stage: fields {worst:.3e}, contrast {c1:.10e} vs {c0:.10e}")
-    assert worst < 5e-6 and abs(c1 - c0) <= 1e-5 * c0
+    print(f"{precision} split K on every stage: fields {worst:.3e}, contrast {c1:.10e} vs {c0:.10e}")
+    assert worst < tol[0] and abs(c1 - c0) <= tol[1] * c0
 
 
 def _with_cube(w):

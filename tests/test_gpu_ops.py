@@ -169,6 +169,52 @@ def test_mft_planes_split_k(ctx, monkeypatch, ksplit, nin, nout, batch):
     monkeypatch.delenv("CSX_GEMM_VARIANT")
 
 
+SPLIT_SIZES = {"fp64": [(1024, 128, 2), (200, 72, 3)], "fp32": [(1024, 128, 2), (200, 72, 3)],
+               # three-term planes: sizes divisible by 32
+               "bf16x6": [(1024, 128, 2), (224, 96, 3)]}
+SPLIT_TOL = {"fp64": (1e-12, 1e-14), "fp32": (2e-6, 2e-6), "bf16x6": (2e-7, 2e-8)}
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x6"])
+@pytest.mark.parametrize("ksplit", [3, 16])
+@pytest.mark.parametrize("case", [0, 1])
+def test_mft_split_k_fp64_workspace(ctx, monkeypatch, prec, ksplit, case):
+    """Split-K contractions whose splits leave FP64 sums (FP64, FP32 and three-term
+    planes: the finishing launch adds them in order and, for three-term plane outputs,
+    puts each window of 32 columns on its fixed-point grid) meet their precision's level
+    against the Bluestein restatement (FP64) or the FP64 path, and agree with the
+    unsplit launch to a hundredth of it (FP64) or a tenth (three-term planes; FP32, whose
+    unsplit sums round in FP32 over the whole K, to its own level), on K split into
+    ragged ranges (1024 / 3, 200 / 16 below one slab per split) for the first tile
+    variant and a 128-wide one."""
+    nin, nout, batch = SPLIT_SIZES[prec][case]
+    tol, tol_split = SPLIT_TOL[prec]
+    rng = np.random.default_rng(nin + 13 * nout + ksplit)
+    E = np.stack([rand_field(rng, nin, nin) for _ in range(batch)])
+    dx, lf = 1e-5, 650e-9
+    dxo = lf / (nin * dx) / 2
+    gi, go = sym(nin, dx), sym(nout, dxo)
+    if prec == "fp64":
+        ref = np.stack([oracle.zoomfft(E[b], (gi[1], gi[1], dx, dx), (nout, go[1], go[1], dxo, dxo), lf)
+                        for b in range(batch)])
+    else:
+        ref = ops.to_host(ops.mft(ctx, ops.to_device(E), gi, gi, go, go, lf, "fp64"))
+    variants = {"fp64": (0, 4), "fp32": (0,), "bf16x6": (0, 1)}[prec]
+    for v in variants:
+        monkeypatch.setenv("CSX_GEMM_VARIANT", str(v))
+        monkeypatch.delenv("CSX_GEMM_KSPLIT", raising=False)
+        base = ops.to_host(ops.mft(ctx, ops.to_device(E), gi, gi, go, go, lf, prec))
+        monkeypatch.setenv("CSX_GEMM_KSPLIT", str(ksplit))
+        out = ops.to_host(ops.mft(ctx, ops.to_device(E), gi, gi, go, go, lf, prec))
+        for b in range(batch):
+            e, d = rel_rms(out[b], ref[b]), rel_rms(out[b], base[b])
+            print(f"{prec} split {ksplit} {nin}->{nout} variant {v} field {b}: vs reference {e:.3e}, "
+                  f"vs unsplit {d:.3e}")
+            assert e < tol and d < tol_split
+    monkeypatch.delenv("CSX_GEMM_KSPLIT")
+    monkeypatch.delenv("CSX_GEMM_VARIANT")
+
+
 @pytest.mark.parametrize("nin,nout,batch", [(1024, 256, 2), (200, 72, 3), (96, 136, 2)])
 def test_mft_fp64_tile_variants(ctx, monkeypatch, nin, nout, batch):
     """Every FP64 tile variant (single- or double-buffered, 64 to 128 wide) meets the
