@@ -94,6 +94,11 @@ SIGNATURES = {
     "csx_intensity": (_i, [_p, _p, _i, _i, _i, _p]),
     "csx_region_mean": (_i, [_p, _p, _i, _i, _p, _i, _d, _p]),
     "csx_hex_resp_objective": (_i, [_p, _p, _i, _i, _i, _p, _d, _d, _p, _i, _i, _i, _p, _i]),
+    "csx_script_parse": (_i, [C.c_char_p, C.c_char_p, _sz]),
+    "csx_run_script": (_i, [_p, C.c_char_p, C.c_char_p, C.c_char_p, _sz]),
+    "csx_response_save": (_i, [C.c_char_p, _p, _i, _i, _i, _i, _i, _d, _p]),
+    "csx_response_load": (_i, [C.c_char_p, _p, _sz, _p, C.POINTER(_d), _p, _i]),
+    "csx_de_optimize": (_i, [C.c_char_p, _i, _p, _p, _p, C.POINTER(_d), C.POINTER(_i)]),
     "csx_philox_fill": (_i, [_u64, _u64, _p, _sz, _i]),
     "csx_philox4x32_10": (None, [_p, _p, _p]),
     "csx_hex_phasor_features": (_i, [_p, _p, _i, _i, _p, _i, _d, _d, _d, _p]),

@@ -181,8 +181,19 @@ int write_fits_matrix(const std::string& name, const double* colmajor, int rows,
   return CSX_OK;
 }
 
+ParseAudit*& parse_audit() {
+  thread_local ParseAudit* a = nullptr;
+  return a;
+}
+
 // FITS matrix -> column-major rows x cols (rows = NAXIS2, cols = NAXIS1), csim_fits.cpp:195-233.
 int fits_matrix(const std::string& path, std::vector<double>& m, int& rows, int& cols, int& slices) {
+  if (ParseAudit* a = parse_audit()) {
+    a->files.push_back(path);
+    m.clear();
+    rows = cols = slices = 0;
+    return CSX_OK;
+  }
   std::vector<double> d;
   std::vector<long> dims;
   CSX_TRY(read_fits(path, d, dims));
@@ -327,6 +338,10 @@ int Element::set_complex_array(Chain&, const std::string& key, const double*, in
 }
 
 static int unknown_key(const std::string& type, const std::string& key) {
+  if (ParseAudit* a = parse_audit()) {
+    a->unknown_keys.push_back(type + ":" + key);
+    return CSX_OK;
+  }
   std::fprintf(stderr, "!!! %s bad set field name: %s\n", type.c_str(), key.c_str());
   return CSX_OK;   // csim prints and continues
 }
@@ -434,6 +449,7 @@ struct RealMap {
     int r, c, s;
     std::vector<double> m;
     CSX_TRY(fits_matrix(trim(path), m, r, c, s));
+    if (parse_audit()) return CSX_OK;   // a device-free check: the name is recorded, nothing read
     return set(m.data(), r, c);
   }
 };
@@ -1624,6 +1640,10 @@ int Chain::make_element(const Block& b, std::unique_ptr<Element>& out) {
   } else if (t == "fpmPupToLyot") {
     e.reset(new FpmPupToLyotElem());
   } else {
+    if (ParseAudit* a = parse_audit()) {
+      a->unknown_elements.push_back(t);
+      return CSX_OK;                       // out stays empty: the element is skipped
+    }
     std::fprintf(stderr, "!!! coronagraph init unknown object name: %s\n", t.c_str());
     return fail(CSX_ERR_UNSUPPORTED, "element type not supported: " + t);
   }
@@ -1699,6 +1719,7 @@ int Chain::parse_top(const std::vector<Block>& blocks) {
           int r = 0, cc = 0, n = 0;
           std::sscanf(a.c_str(), "%d, %d, %d", &r, &cc, &n);
           ef_.rows = r; ef_.cols = cc; nl_size = n;
+          ef_.arraySizePixels = r;            // efield::set_size (:442-452) sets it to the rows
         } else if (k == "constantValue") { ef_.constant = true; ef_.constantValue = std::atof(a.c_str()); }
         else if (k == "pointSource") {
           double fl = 0, tx = 0, ty = 0;
@@ -1712,13 +1733,18 @@ int Chain::parse_top(const std::vector<Block>& blocks) {
           int n = (int)std::round((s1 - s0) / d) + 1;
           ef_.lambdas.clear();
           for (int i = 0; i < n; i++) ef_.lambdas.push_back(s0 + i * d);
+          ef_.arraySizePixels = ef_.rows;     // set_wavelength_range ends in set_size (:365)
         } else if (k == "wavelengthBandwidth") {
           double cw = 0, fr = 0; int n = 0;
           std::sscanf(a.c_str(), "%lf, %lf, %d", &cw, &fr, &n);
           double s0 = cw - cw * fr / 2, s1 = cw + cw * fr / 2, d = (s1 - s0) / ((double)n - 1);
           ef_.lambdas.clear();
           for (int i = 0; i < n; i++) ef_.lambdas.push_back(s0 + i * d);
-        } else if (k == "wavelengthList") ef_.lambdas = parse_list(a);
+          ef_.arraySizePixels = ef_.rows;     // set_wavelength_bandwidth: set_size (:388)
+        } else if (k == "wavelengthList") {
+          ef_.lambdas = parse_list(a);
+          ef_.arraySizePixels = ef_.rows;     // set_wavelength_list: set_size (:416)
+        }
         else if (k == "beamRadiusPhysical") ef_.beamRadius = std::atof(a.c_str());
         else if (k == "pixelScale") ef_.pixelScale = std::atof(a.c_str());
         else if (k == "arraySizePixels") ef_.arraySizePixels = std::atoi(a.c_str());
@@ -1729,13 +1755,17 @@ int Chain::parse_top(const std::vector<Block>& blocks) {
           int r, cc, s, r2, c2, s2;
           CSX_TRY(fits_matrix(trim(a.substr(0, cpos)), re, r, cc, s));
           CSX_TRY(fits_matrix(trim(a.substr(cpos + 1)), im, r2, c2, s2));
+          if (parse_audit()) continue;        // names recorded, nothing read
           ef_.rows = r; ef_.cols = cc;
+          ef_.arraySizePixels = r;            // efield::init(amp, ph) -> set_size (:229)
           for (int sl = 0; sl < s; sl++) {
             std::vector<double2> f((size_t)r * cc);
             for (size_t i = 0; i < f.size(); i++)
               f[i] = make_double2(re[(size_t)sl * r * cc + i], im[(size_t)sl * r * cc + i]);
             ef_.fields[sl] = f;
           }
+        } else if (ParseAudit* au = parse_audit()) {
+          au->unknown_keys.push_back("efield:" + k);
         } else {
           std::fprintf(stderr, "!!!!! efield set: unknown fieldName: %s\n", k.c_str());
         }
@@ -1743,18 +1773,19 @@ int Chain::parse_top(const std::vector<Block>& blocks) {
       if (ef_.lambdas.empty()) return fail(CSX_ERR_PARSE, "efield: no wavelength");
       if (nl_size && (int)ef_.lambdas.size() != nl_size && ef_.lambdas.size() == 1)
         ef_.lambdas.resize(nl_size, ef_.lambdas[0]);
-      if (ef_.beamRadius == 0 || ef_.pixelScale == 0 || ef_.arraySizePixels == 0)
+      const bool audit = parse_audit() != nullptr;   // files not read: sizes unknown
+      if (ef_.beamRadius == 0 || ef_.pixelScale == 0 || (ef_.arraySizePixels == 0 && !audit))
         return fail(CSX_ERR_PARSE, "efield: beamRadiusPhysical, pixelScale and arraySizePixels required");
       ef_.beamSizePixels = std::ceil(2 * ef_.beamRadius / ef_.pixelScale);          // :424
       ef_.arrayRadius = ef_.beamRadius * ef_.arraySizePixels / ef_.beamSizePixels;  // :427
-      if (ef_.rows <= 0 || ef_.cols <= 0) return fail(CSX_ERR_PARSE, "efield: setSize required");
+      if (!audit && (ef_.rows <= 0 || ef_.cols <= 0)) return fail(CSX_ERR_PARSE, "efield: setSize required");
     } else if (t == "coronagraph") {
       std::vector<Block> sub = find_blocks(b);
       for (const Block& sb : sub) {
         if (sb.cmds.empty() || sb.cmds[0].key == "coronagraph") continue;
         std::unique_ptr<Element> e;
         CSX_TRY(make_element(sb, e));
-        elems_.push_back(std::move(e));
+        if (e) elems_.push_back(std::move(e));
       }
     } else if (t == "regionContrast") {   // regionContrast.cpp:36-87
       region_.present = true;
@@ -1773,6 +1804,12 @@ int Chain::parse_top(const std::vector<Block>& blocks) {
 }
 
 int Chain::load_script(const std::string& text) {
+  CSX_TRY(parse_only(text));
+  for (auto& e : elems_) CSX_TRY(e->init(*this));
+  return CSX_OK;
+}
+
+int Chain::parse_only(const std::string& text) {
   Block all = parse_script_text(text);
   std::vector<Block> blocks = find_blocks(all);
   // the telescope must be known before efield keys use the magnification
@@ -1781,8 +1818,7 @@ int Chain::load_script(const std::string& text) {
   for (const Block& b : blocks) if (b.cmds.empty() || b.cmds[0].key != "telescope") ordered.push_back(b);
   CSX_TRY(parse_top(ordered));
   if (tel_.D == 0) return fail(CSX_ERR_PARSE, "script has no telescope block");
-  if (ef_.rows == 0) return fail(CSX_ERR_PARSE, "script has no efield block");
-  for (auto& e : elems_) CSX_TRY(e->init(*this));
+  if (ef_.rows == 0 && !(parse_audit() && ef_.pixelScale > 0)) return fail(CSX_ERR_PARSE, "script has no efield block");
   return CSX_OK;
 }
 
@@ -2105,6 +2141,16 @@ int Chain::ensure_calibration() {
   CSX_HIP(hipMemcpyAsync(&calib_peak_, peak_.p, sizeof(double), hipMemcpyDeviceToHost, stream()));
   CSX_HIP(hipStreamSynchronize(stream()));
   calib_valid_ = true;
+  return CSX_OK;
+}
+
+int Chain::calibration_intensity(std::vector<double>& out, int* rows, int* cols) {
+  CSX_TRY(ensure_calibration());
+  const Geom gc = out_geom_cal_;
+  out.resize((size_t)gc.npix());
+  CSX_HIP(hipMemcpy(out.data(), inten_cal_.p, sizeof(double) * out.size(), hipMemcpyDeviceToHost));
+  *rows = gc.ny;
+  *cols = gc.nx;
   return CSX_OK;
 }
 

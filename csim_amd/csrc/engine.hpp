@@ -26,6 +26,15 @@ std::vector<Block> find_blocks(const Block& b);
 // (NAXIS1 fastest).  csim reads matrices as rows = NAXIS2, cols = NAXIS1.
 int read_fits(const std::string& path, std::vector<double>& data, std::vector<long>& dims);
 
+// Device-free script check (csx_script_parse): while an audit is installed on this
+// thread, FITS matrix loads record the file name and return an empty matrix, unknown
+// keys and element types are recorded (and parsing goes on, as csim prints and goes
+// on), and nothing touches a device.
+struct ParseAudit {
+  std::vector<std::string> files, unknown_keys, unknown_elements;
+};
+ParseAudit*& parse_audit();
+
 // --------------------------------------------------------------- geometry --
 // arrayGeom with uniform grids: x along columns, y along rows.
 struct Geom {
@@ -126,6 +135,12 @@ class Chain {
   Chain(const Chain&) = delete;
   Chain& operator=(const Chain&) = delete;
   int load_script(const std::string& text);
+  // the script's blocks parsed into the telescope, efield, elements and region, with
+  // no element init (no device work, no cached data); load_script = this + init
+  int parse_only(const std::string& text);
+  // sum over sources and wavelengths of the calibration fields' |E|^2 (the
+  // calibration run cached like csx_chain_contrast's), out_rows x out_cols column-major
+  int calibration_intensity(std::vector<double>& out, int* rows, int* cols);
   int set_array(const std::string& elem, const std::string& key, const double* d, int rows, int cols);
   int set_complex_array(const std::string& elem, const std::string& key, const double* d, int rows, int cols,
                         int slice);

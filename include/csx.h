@@ -309,11 +309,58 @@ int csx_region_mean(csx_ctx* ctx, const double* d_intensity, int npix, int n_gro
  * response-matrix objective as a batched complex GEMM on MFMA.  d_M: nslices x
  * (ncol x npix) complex128 with M[s] column-major npix x ncol, ncol = nopt+1;
  * d_X: ncand x nopt; d_obj: ncand.  objective 0 = mean, 1 = max.
- * Precision CSX_FP64 or CSX_FP32 (others: CSX_ERR_UNSUPPORTED). */
+ * Every precision: FP64, FP32, or split-BF16 (bf16x3 / bf16x6: both operands split
+ * into BF16 planes with K = ncol padded by zeros to a multiple of 32). */
 int csx_hex_resp_objective(csx_ctx* ctx, const void* d_M, int npix, int ncol, int nslices,
                            const double* lambdas, double phase_sign, double calib,
                            const double* d_X, int nopt, int ncand, int objective,
                            double* d_obj, csx_precision p);
+
+/* ------------------------------------------------ script driver and tools */
+/* Device-free check of a csim input script (no context, no device, no file read):
+ * a JSON report {"ok", "error", "blocks": [{"type", "keys", "blocks"}...],
+ * "elements": [{"type", "name"}...] (the coronagraph as the engine builds it),
+ * "unknown_keys", "unknown_elements", "files" (every FITS file the script names),
+ * "top_level"}.  The block grammar of src/lib/csim_parser.cpp:60-190; keys as the
+ * element set() functions take them (unknown ones listed, as csim prints and goes
+ * on).  Returns the report length (written up to cap - 1 bytes, NUL-terminated). */
+int csx_script_parse(const char* script_text, char* json, size_t cap);
+
+/* csim's main loop (src/csim.cpp:54-108): the telescope, efield, coronagraph and
+ * regionContrast blocks define the chain; execute, regionContrast
+ * (regionContrast.cpp:145-205: calibration_PSF.fits, normalized_PSF.fits, its
+ * filename), makeContrastCurve (contrastCurve.cpp:68-158), makeHexCFpmResponse
+ * (makeHexCFpmResponse.cpp:91-190: a responseData directory) and
+ * differentialEvolutionOptimizer (differentialEvolutionOptimizer.cpp:216-330,
+ * with an optHexRespObjective sub-block or the additive optChainObjective sub-block
+ * {component, regionContrast} that evaluates every candidate by full propagation)
+ * run in script order; break stops.  PSF, region and curve files go to workdir
+ * (NULL: the current directory); the tools' own directories are taken as given.
+ * log receives the run's messages.  Returns the log length or a negative error. */
+int csx_run_script(csx_ctx* ctx, const char* script_text, const char* workdir, char* log, size_t cap);
+
+/* responseData directory (src/data/responseData.cpp:175-273): responseParams.txt,
+ * responseData_s<s>_p<p>_{re,im}.fits (each slice transposed, NAXIS1 = columns)
+ * and problemParams.txt.  M: interleaved complex, nsrc x npol cubes (source-major),
+ * each slices x (cols x rows) with every slice column-major rows x cols. */
+int csx_response_save(const char* dir, const double* M, int rows, int cols, int slices, int npol, int nsrc,
+                      double calib_max_intensity, const double* lambdas);
+/* sizes[5] = rows, cols, slices, npol, nsrc; M NULL: only the sizes, calibration
+ * peak and wavelengths (lambdas NULL: not returned). */
+int csx_response_load(const char* dir, double* M, size_t capacity_complex, int* sizes,
+                      double* calib_max_intensity, double* lambdas, int lambda_capacity);
+
+/* differentialEvolutionOptimizer::optimize (differentialEvolutionOptimizer.cpp:
+ * 216-330, strategies deStrategy.cpp:23-158) over a caller's objective: the
+ * differentialEvolutionOptimizer block's keys (:66-197; the additive key seed
+ * selects the Philox stream), every generation evaluated as one call
+ * f(user, population (dim x npop column-major), dim, npop, values); a negative
+ * return stops the run.  Writes optBestVal.txt, optValHistory.txt,
+ * bestOptVector.fits (and the population files when saveState) to
+ * outputDirectory. */
+typedef int (*csx_objective_fn)(void* user, const double* population, int dim, int npop, double* values);
+int csx_de_optimize(const char* de_block_text, int dim, csx_objective_fn f, void* user,
+                    double* best_vector, double* best_value, int* iterations);
 
 /* Philox4x32-10 (Salmon et al., SC'11) counter-based generator used for every
  * synthetic input: fills n doubles, uniform [0,1) (normal = 0) or standard
