@@ -188,7 +188,9 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const GemmProb* __restrict__ pr
   __shared__ C2 Bs[kTileN][LDK];
 
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
-  const GemmProb& P = probs[find_prob(probs, nprob, tile)];
+  // by value: the descriptor's fields then live in scalar registers, not reloaded after
+  // every store (a store through one of its pointers could alias a reference to it)
+  const GemmProb P = probs[find_prob(probs, nprob, tile)];
   const int local = tile - P.tile_start, tiles_mn = P.tiles_m * P.tiles_n;
   const int ks = local / tiles_mn, lmn = local - ks * tiles_mn;   // K split, output tile
   const int tm = lmn % P.tiles_m, tn = lmn / P.tiles_m;
@@ -284,7 +286,9 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm_f64db(const GemmProb* __r
   __shared__ f64x2 Bs[2][BN][LDK];
 
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
-  const GemmProb& P = probs[find_prob(probs, nprob, tile)];
+  // by value: the descriptor's fields then live in scalar registers, not reloaded after
+  // every store (a store through one of its pointers could alias a reference to it)
+  const GemmProb P = probs[find_prob(probs, nprob, tile)];
   const int local = tile - P.tile_start, tiles_mn = P.tiles_m * P.tiles_n;
   const int ks = local / tiles_mn, lmn = local - ks * tiles_mn;   // K split, output tile
   const int tm = lmn % P.tiles_m, tn = lmn / P.tiles_m;
@@ -508,7 +512,9 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
   __shared__ __align__(16) unsigned short lds[NBUF * STAGE];
 
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
-  const GemmProb& P = probs[find_prob(probs, nprob, tile)];
+  // by value: the descriptor's fields then live in scalar registers, not reloaded after
+  // every store (a store through one of its pointers could alias a reference to it)
+  const GemmProb P = probs[find_prob(probs, nprob, tile)];
   const int tiles_mn = P.tiles_m * P.tiles_n;
   const int local = tile - P.tile_start;
   const int ks = local / tiles_mn, lmn = local - ks * tiles_mn;   // K split, output tile
@@ -710,7 +716,12 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
         }
     }
    }
-    // the other products are 2^-8 of the leading ones: their FP32 sums are flushed per slab
+  };
+  // The other products are 2^-8 of the leading ones (at most 2^-16 for the terms kept),
+  // so their FP32 sums over 4 slabs (128 values) round at about 2^-30 of the leading
+  // sum, below the 2^-24 of the products left out: they are flushed into FP64 every 4
+  // slabs and at the end.
+  auto flush_rest = [&]() {
     if (TERMS == 3) {
 #pragma unroll
       for (int i = 0; i < (TERMS == 3 ? IM : 1); i++)
@@ -725,6 +736,7 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
         }
     }
   };
+  constexpr int kRestSlabs = 4;
 
   const int nslab = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
   if (nslab > 0) {
@@ -743,6 +755,7 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
       const int nx = s + RING - 1;                 // slot (s - 1) % RING: read by every wave before the last barrier
       if (nx < nslab) dload(kbeg + nx * BK, nx % RING);
       compute(lds + (s % RING) * STAGE);
+      if (s % kRestSlabs == kRestSlabs - 1) flush_rest();
       if (s + 1 < nslab) wait_after(min(nx, nslab - 1) - (s + 1));
       __syncthreads();
     }
@@ -754,6 +767,7 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
       const int cur = s & 1;
       if (s + 1 < nslab) gload(kbeg + (s + 1) * BK);
       compute(lds + cur * STAGE);
+      if (s % kRestSlabs == kRestSlabs - 1) flush_rest();
       if (s + 1 < nslab) sstore(cur ^ 1);
       __syncthreads();
     }
@@ -765,9 +779,11 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
       __syncthreads();
       if (s + 1 < nslab) gload(kbeg + (s + 1) * BK);
       compute(lds);
+      if (s % kRestSlabs == kRestSlabs - 1) flush_rest();
     }
   }
   }
+  flush_rest();
   if (TERMS == 3 && P.ksplit > 1) {
     // this split's FP64 sums; gemm_splitk_finish adds the splits and applies the epilogue
 #pragma unroll
@@ -895,6 +911,105 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
                   (unsigned)h[q][4] | ((unsigned)h[q][5] << 16), (unsigned)h[q][6] | ((unsigned)h[q][7] << 16)};
     }
     return;
+  }
+  constexpr bool kStaged64 = BM * BN * 16 <= 2 * NBUF * STAGE && BN % 64 == 0;
+  if (TERMS == 3 && kStaged64 && P.out_kind == OUT_PLANES && !P.transC && P.terms == 3) {
+    // Three-term planes, whole-grid real mask or none (the common case): the FP64 sums go
+    // through LDS (row ml, column nl ^ 16 (ml & 3)), then each thread finishes 8
+    // consecutive outputs of one row (epilogue_value's arithmetic with 8-byte mask
+    // reads), the 4 threads of a window of 32 columns (4 consecutive lanes: BN / 8 chunks
+    // per row, a multiple of 4) agree on its exponent, and each thread stores one 16-byte
+    // chunk per plane.  Other epilogues take the general path below.
+    const int mk = P.mask_kind;
+    const bool vmask = !P.addend &&
+                       (mk == MASK_NONE || ((mk == MASK_REAL_1PI || mk == MASK_REAL_1PI_F32 ||
+                                             mk == MASK_REAL_1PI_U8 || mk == MASK_REAL) &&
+                                            P.mm0 == 0 && P.mn0 == 0 && P.mM >= M && P.mN >= N));
+    if (vmask) {
+      f64x2* st = reinterpret_cast<f64x2*>(lds);
+      __syncthreads();                             // every wave is done reading operand tiles
+#pragma unroll
+      for (int i = 0; i < IM; i++)
+#pragma unroll
+        for (int j = 0; j < JN; j++)
+#pragma unroll
+          for (int v = 0; v < NV; v++) {
+            const int ml = wm * (BM / WM) + i * MF + Mf::drow(lane, v);
+            const int nl = wn * (BN / WN) + j * MF + Mf::dcol(lane);
+            st[ml * BN + (nl ^ ((ml & 3) << 4))] =
+                f64x2{(double)accR[i][j][v] + dR[TERMS == 3 ? i : 0][TERMS == 3 ? j : 0][v],
+                      (double)accI[i][j][v] + dI[TERMS == 3 ? i : 0][TERMS == 3 ? j : 0][v]};
+          }
+      __syncthreads();
+      const bool empty = nslab == 0;
+      constexpr int CPR = BN / 8;
+      static_assert(!kStaged64 || (CPR % 4 == 0 && NT % 4 == 0 && (BM * CPR) % NT == 0),
+                    "a window is 4 consecutive lanes, every lane runs every iteration");
+      GLOBAL unsigned short* C = gout<unsigned short>(P.C);
+      for (int e = t; e < BM * CPR; e += NT) {     // uniform trip count: BM CPR is a multiple of NT
+        const int ml = e / CPR, cc = e % CPR;
+        const int m = m0 + ml, n = n0 + cc * 8;    // N is a multiple of 32: windows are whole
+        const bool live = m < M && n < N;          // uniform over a window (one row, whole windows)
+        const long off = (long)m * P.ldc + n;
+        double wm8[8];
+        if (live && mk != MASK_NONE && !empty) {
+          const long mo = (long)m * P.ldm + n;
+          if (mk == MASK_REAL_1PI_U8) {
+            const u32x2 w = *reinterpret_cast<const GLOBAL u32x2*>(gin<unsigned char>(P.mask) + mo);
+#pragma unroll
+            for (int k = 0; k < 8; k++) wm8[k] = (double)((w[k >> 2] >> (8 * (k & 3))) & 0xFFu);
+          } else if (mk == MASK_REAL_1PI_F32) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) wm8[k] = (double)gin<float>(P.mask)[mo + k];
+          } else {
+#pragma unroll
+            for (int k = 0; k < 8; k++) wm8[k] = gin<double>(P.mask)[mo + k];
+          }
+        }
+        double vr[8], vi[8];
+        float mx = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+          const f64x2 a = st[ml * BN + ((cc * 8 + k) ^ ((ml & 3) << 4))];
+          double xr = 0.0, xi = 0.0;
+          if (live && !empty) {
+            xr = P.alpha_re * a.x - P.alpha_im * a.y;
+            xi = P.alpha_re * a.y + P.alpha_im * a.x;
+            if (mk == MASK_REAL) {
+              xr *= wm8[k]; xi *= wm8[k];
+            } else if (mk != MASK_NONE) {          // csim mask: (1+i) w
+              const double tr = (xr - xi) * wm8[k];
+              xi = (xr + xi) * wm8[k];
+              xr = tr;
+            }
+          }
+          vr[k] = xr;
+          vi[k] = xi;
+          mx = fmaxf(mx, (float)fmax(fabs(xr), fabs(xi)));
+        }
+        const int ew = window_exponent(window_max<4>(mx));
+        unsigned short h[6][8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+          unsigned short hr[3], hi[3];
+          split3(vr[k], ew, hr);
+          split3(vi[k], ew, hi);
+#pragma unroll
+          for (int tt = 0; tt < 3; tt++) {
+            h[2 * tt][k] = hr[tt];
+            h[2 * tt + 1][k] = hi[tt];
+          }
+        }
+        if (live) {
+#pragma unroll
+          for (int q = 0; q < 6; q++)
+            *reinterpret_cast<GLOBAL u32x4*>(C + q * P.sC + off) =
+                u32x4{(unsigned)h[q][0] | ((unsigned)h[q][1] << 16), (unsigned)h[q][2] | ((unsigned)h[q][3] << 16),
+                      (unsigned)h[q][4] | ((unsigned)h[q][5] << 16), (unsigned)h[q][6] | ((unsigned)h[q][7] << 16)};
+        }
+      }
+      return;
+    }
   }
   if (TERMS == 3 && P.out_kind == OUT_PLANES) {
     // Three-term planes (the next contraction's operand, K along n): every value of a
