@@ -15,7 +15,7 @@ src/tools/regionContrast.cpp:145-205) run two ways on the batched GPU chain:
 The surrogate is trained first on propagations of random thickness vectors
 (t ~ U(0, 1 um)) made by the same chain.  Usage:
 
-    python -m csim_amd.study [--n 512] [--train 90112] [--epochs 10] [--seconds 20]
+    python -m csim_amd.study [--n 512] [--train 200704] [--epochs 10] [--seconds 20]
 
 prints one JSON object (data-generation and training times, held-out rank
 correlation, and the DE traces with the time each reached the plain run's
@@ -90,9 +90,20 @@ def spearman(a, b):
 class ScreenedSurrogate:
     """Ranks mutants by predicted objective, keeps the top `keep`, and learns from the
     exact values of the survivors: their labels go into a device ring buffer and a few
-    Adam steps on random batches of it follow every generation."""
+    Adam steps on random batches of it follow every generation.
 
-    def __init__(self, mlp, lambdas, counts, exact, keep=32, tune_steps=2, lr=3e-4, capacity=32768, seed=5):
+    The exact propagation is the judge: `n_control` of the `keep` exact evaluations of a
+    generation go to mutants drawn at random from those the surrogate did not pick, and
+    the generation's screening gain is the mean log10 contrast of those controls minus
+    that of the picked ones (positive: the surrogate's picks are darker than chance).
+    When its median over the last `fallback_window` generations is not positive, the
+    surrogate is no longer trusted and every later generation is evaluated exactly
+    (plain DE on the same population).  The rank correlation of predicted and exact
+    contrast over the evaluated mutants is recorded too; within a converged population
+    it is close to zero even for a useful surrogate, so it is reported, not judged."""
+
+    def __init__(self, mlp, lambdas, counts, exact, keep=32, tune_steps=2, lr=3e-4, capacity=32768, seed=5,
+                 n_control=4, fallback_window=8):
         import torch
         self.mlp, self.lambdas, self.counts, self.exact = mlp, list(lambdas), counts, exact
         self.keep, self.tune_steps, self.lr = keep, tune_steps, lr
@@ -103,6 +114,11 @@ class ScreenedSurrogate:
         self.gen.manual_seed(seed)
         self.errors = []                # per generation: Spearman(pred, exact) of the survivors
         self.seconds = {"predict": 0.0, "exact": 0.0, "learn": 0.0}   # host wall time per phase
+        self.n_control, self.fallback_window = n_control, fallback_window
+        self.gains = []                 # per generation: screening gain (log10 contrast, controls - picks)
+        self.rng = np.random.Generator(np.random.Philox(seed))
+        self.generation = 0
+        self.fallback_generation = None     # generation from which every mutant is evaluated exactly
 
     def predict(self, thickness):
         import torch
@@ -130,18 +146,36 @@ class ScreenedSurrogate:
             sel = torch.randint(0, self.filled, (nb,), device="cuda", generator=self.gen)
             self.mlp.train_step(self.Xd.index_select(0, sel), self.Yd.index_select(0, sel), self.lr, want_loss=False)
 
+    @property
+    def fallback(self):
+        return self.fallback_generation is not None
+
     def screen_and_eval(self, pop):
         """DE hook: pop (D, P) -> (indices evaluated, their exact values)."""
         import torch
+        self.generation += 1
         t0 = time.perf_counter()
         t = np.ascontiguousarray(pop.T)
+        if self.fallback:                           # plain DE: every mutant exactly
+            vals = self.exact.evaluate(t)
+            self.seconds["exact"] += time.perf_counter() - t0
+            return np.arange(t.shape[0]), vals
         pred = self.predict(t)
-        idx = np.argsort(pred)[:self.keep]
+        order = np.argsort(pred)
+        npick = self.keep - self.n_control
+        ctrl = self.rng.choice(order[npick:], size=min(self.n_control, order.size - npick), replace=False)
+        idx = np.concatenate([order[:npick], ctrl])
         t1 = time.perf_counter()
         vals, prof = self.exact.evaluate(t[idx], with_profiles=True)
         t2 = time.perf_counter()
         if idx.size > 2:
             self.errors.append(spearman(pred[idx], vals))
+        if ctrl.size:
+            lv = np.log10(np.maximum(vals, 1e-300))
+            self.gains.append(float(lv[npick:].mean() - lv[:npick].mean()))
+            recent = self.gains[-self.fallback_window:]
+            if len(recent) == self.fallback_window and np.median(recent) <= 0.0:
+                self.fallback_generation = self.generation + 1
         self._push(features_device(self.mlp.ctx, t[idx], self.lambdas), log_labels(prof).reshape(-1, N_BINS))
         self.tune()
         t3 = time.perf_counter()
@@ -229,6 +263,8 @@ def compare_de(ctx, w, sur, seconds=None, eval_budget=None, plain_pop=64, screen
                        "seconds_to_plain_best": reach[1] if reach else None,
                        "evaluations_to_plain_best": reach[3] if reach else None,
                        "survivor_spearman_median": float(np.median(s1.errors)) if s1.errors else None,
+                       "fallback_generation": s1.fallback_generation,
+                       "screening_gain_median": float(np.median(s1.gains)) if s1.gains else None,
                        "seconds_by_phase": {k: round(v, 3) for k, v in s1.seconds.items()}}
     out["wall_ratio_vs_plain"] = (reach[1] / rp.seconds) if reach else None
     log(f"screened DE: best {rs.best_value:.4e} after {rs.seconds:.1f} s; reached plain best: "
@@ -238,7 +274,7 @@ def compare_de(ctx, w, sur, seconds=None, eval_budget=None, plain_pop=64, screen
     return out
 
 
-def run(n=512, n_train=90112, seconds=20.0, n_lambda=9, plain_pop=64, screen_pop=512, keep=32, epochs=10,
+def run(n=512, n_train=200704, seconds=20.0, n_lambda=9, plain_pop=64, screen_pop=512, keep=32, epochs=10,
         precision="bf16x3", seed=20261207, log=print):
     from .ops import Context
     ctx = Context(0)
@@ -257,7 +293,7 @@ def run(n=512, n_train=90112, seconds=20.0, n_lambda=9, plain_pop=64, screen_pop
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--n", type=int, default=512)
-    ap.add_argument("--train", type=int, default=90112)
+    ap.add_argument("--train", type=int, default=200704)
     ap.add_argument("--seconds", type=float, default=20.0)
     ap.add_argument("--lambdas", type=int, default=9)
     ap.add_argument("--epochs", type=int, default=10)

@@ -6,24 +6,74 @@ without it (all figures are measured, nothing is assumed of the surrogate's qual
     contrast correlates with the exact one, and its top 5 % are darker on average;
   * search: at the same number of exact candidate propagations, DE whose 512-mutant
     generations are screened down to 32 by the surrogate reaches a darker dark hole
-    than plain DE propagating every one of its 64 mutants.
+    than plain DE propagating every one of its 64 mutants;
+  * fallback: the exact propagation judges the surrogate every generation, and a
+    surrogate whose ranking goes wrong is dropped for plain DE.
 """
+import numpy as np
 import pytest
 
 from csim_amd import study
 from csim_amd import workloads as W
+from csim_amd.de import DEConfig, DifferentialEvolution
+from csim_amd.surrogate import MLP, N_BINS, features_device
 
 pytestmark = pytest.mark.gpu
 
 
 def test_surrogate_screening_gain(ctx):
+    """The study's training configuration (90,112 random candidates x 9 wavelengths at
+    512^2, 10 epochs; 22,528 candidates give a held-out Spearman of only 0.30).
+    Thresholds sit below the values it measures (held-out Spearman 0.89, top 5 % at 0.50
+    of the mean) with room for the run-to-run spread of training."""
     w = W.workload3(n=512, n_lambda=9)
-    sur, tr = study.train_surrogate(ctx, w, n_train=22528, epochs=20)
+    sur, tr = study.train_surrogate(ctx, w, n_train=90112, epochs=10)
     print(tr)
     assert tr["label_consistency_max_rel"] < 1e-9          # labels rebuild the exact objective
     assert tr["train_loss"][1] < 0.1 * tr["train_loss"][0]
-    assert tr["holdout_spearman"] > 0.15
-    assert tr["holdout_top5pct_mean_over_mean"] < 0.98
+    assert tr["holdout_spearman"] > 0.75
+    assert tr["holdout_top5pct_mean_over_mean"] < 0.65
     res = study.compare_de(ctx, w, sur, eval_budget=24000)
     print(res)
     assert res["screened"]["best"] < res["plain"]["best"]
+    assert res["screened"]["fallback_generation"] is None  # a good surrogate stays in use
+    assert res["screened"]["screening_gain_median"] > 0
+
+
+class InvertedSurrogate(study.ScreenedSurrogate):
+    """A surrogate whose predictions rank the mutants in the reverse order of their exact
+    contrast (its survivors are the worst mutants): the judge must drop it."""
+
+    def predict(self, thickness):
+        return -self.exact.evaluate(thickness)
+
+
+def test_surrogate_fallback_to_plain_de(ctx):
+    """With a surrogate that ranks backwards, its picks (the worst mutants) are brighter
+    than the random controls in every generation, the fallback engages after
+    fallback_window generations, and from then on every mutant of a generation is
+    evaluated exactly (plain DE)."""
+    w = W.workload3(n=256, n_lambda=3)
+    exact = study.ExactObjective(ctx, w, 16)
+    lambdas = exact.ch.lambdas()
+    nhex = w.meta["n_hex"]
+    n_in = features_device(ctx, np.zeros((1, nhex)), lambdas).shape[1]
+    mlp = MLP(ctx, n_in, 64, 64, N_BINS, seed=3)
+    s = InvertedSurrogate(mlp, lambdas, exact.counts, exact, keep=8, n_control=4, fallback_window=3)
+    evaluated = []
+    inner = s.screen_and_eval
+
+    def screen(pop):
+        idx, vals = inner(pop)
+        evaluated.append(len(idx))
+        return idx, vals
+
+    cfg = DEConfig(population_size=32, global_bounds=(0.0, 1e-6), init_bounds=(0.0, 1e-6), max_iterations=6,
+                   seed=11)
+    res = DifferentialEvolution(exact, nhex, cfg, screen=screen).run()
+    print("screening gains", s.gains, "fallback from generation", s.fallback_generation, "evaluated", evaluated)
+    assert all(g < 0 for g in s.gains)
+    assert s.fallback_generation == 4                       # after 3 judged generations
+    assert evaluated[:3] == [8, 8, 8] and all(n == 32 for n in evaluated[3:])
+    assert np.isfinite(res.best_value)
+    exact.ch.close()
