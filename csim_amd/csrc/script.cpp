@@ -31,6 +31,7 @@
 #include <sys/types.h>
 
 #include "engine.hpp"
+#include "linopt.hpp"
 
 namespace csx {
 
@@ -787,6 +788,218 @@ static int run_curve_block(Chain& ch, const Block& b, const std::string& workdir
   return CSX_OK;
 }
 
+// ------------------------------------------------------- linear optimiser ---
+// linearOptimizer (src/optimization/linearOptimizer.cpp:72-505) on the batched chain:
+// the Jacobian's n + 1 propagations are one candidate batch (x, then x + dx e_i), the
+// normal equations (J^T J + mu^2 I) d = J^T e are formed on the device
+// (linear_normal_equations) and solved on the host by Cholesky for each mu of
+// compute_step's scan (1e7 to 3e8 by 1e7), whose contrasts are one more candidate batch
+// read in csim's order: the last mu that lowered the contrast is kept.
+struct LinearOptimizer {
+  std::string comp, data = "sags", outdir;
+  csx_region region{};
+  bool have_region = false;
+  double jacobian_dx = 1e-8, glo = 0, ghi = 0, stop_contrast = 1e-9, rel_tol = 1e-5, delta_tol = 1e-9;
+  int max_iterations = 20;
+  bool random_init = false, have_init = false;
+  double init_value = 0;
+  double mu0 = 1e7, mu1 = 3e8, mu_step = 1e7;   // compute_step's scan (:375)
+  std::vector<std::string> unknown;
+
+  // linearOptimizer::set (:72-158)
+  void set(const std::string& k, const std::string& arg) {
+    const std::string a = strip(arg);
+    if (k == "linearOptimizer") return;
+    if (k == "dataToOptimize") {
+      const size_t p = a.find(',');
+      comp = strip(a.substr(0, p));
+      if (p != std::string::npos) data = strip(a.substr(p + 1));
+    } else if (k == "initOptVec") {
+      have_init = true;
+      random_init = a == "random";
+      float v = 0;
+      if (!random_init) std::sscanf(a.c_str(), "%f", &v);
+      init_value = v;
+    } else if (k == "contrastRelTolerance") rel_tol = std::atof(a.c_str());
+    else if (k == "jacobianDx") jacobian_dx = std::atof(a.c_str());
+    else if (k == "globalBounds") float_pair(a, glo, ghi);
+    else if (k == "maxIterations") max_iterations = std::atoi(a.c_str());
+    else if (k == "stopContrast") stop_contrast = std::atof(a.c_str());
+    else if (k == "deltaOptVecTolerance") delta_tol = std::atof(a.c_str());
+    else if (k == "outputDirectory") outdir = a;
+    else if (k == "muScan") {                     // additive: start, stop, step of the mu scan
+      std::sscanf(a.c_str(), "%lf, %lf, %lf", &mu0, &mu1, &mu_step);
+    } else if (k == "regularizationMu" || k == "stepSizeScale" || k == "outputInterval" || k == "saveState") {
+      // read by csim but without effect on the step it takes (the mu scan sets mu)
+    } else unknown.push_back(k);
+  }
+
+  // contrast of every row of X (ncand x n) as one candidate batch
+  int contrasts(Chain& ch, const std::vector<double>& X, int ncand, int n, std::vector<double>& c) {
+    CSX_TRY(ch.set_candidates(comp, X.data(), n, ncand));
+    c.assign(ncand, 0.0);
+    return ch.contrast(have_region ? &region : nullptr, c.data(), nullptr);
+  }
+
+  // (A + mu^2 I) d = b by Cholesky (A symmetric positive semi-definite, mu > 0)
+  static bool solve(const std::vector<double>& A, const std::vector<double>& b, double mu, int n,
+                    std::vector<double>& d) {
+    std::vector<double> L(A);
+    for (int i = 0; i < n; i++) L[(size_t)i * n + i] += mu * mu;
+    for (int j = 0; j < n; j++) {
+      double s = L[(size_t)j * n + j];
+      for (int k = 0; k < j; k++) s -= L[(size_t)j * n + k] * L[(size_t)j * n + k];
+      if (!(s > 0)) return false;
+      const double ljj = std::sqrt(s);
+      L[(size_t)j * n + j] = ljj;
+      for (int i = j + 1; i < n; i++) {
+        double t = L[(size_t)i * n + j];
+        for (int k = 0; k < j; k++) t -= L[(size_t)i * n + k] * L[(size_t)j * n + k];
+        L[(size_t)i * n + j] = t / ljj;
+      }
+    }
+    d.assign(n, 0.0);
+    for (int i = 0; i < n; i++) {            // L y = b
+      double t = b[i];
+      for (int k = 0; k < i; k++) t -= L[(size_t)i * n + k] * d[k];
+      d[i] = t / L[(size_t)i * n + i];
+    }
+    for (int i = n - 1; i >= 0; i--) {       // L^T d = y
+      double t = d[i];
+      for (int k = i + 1; k < n; k++) t -= L[(size_t)k * n + i] * d[k];
+      d[i] = t / L[(size_t)i * n + i];
+    }
+    return true;
+  }
+
+  // optimize (:192-300) with the stopping rule of stopping_criteria (:302-313)
+  int run(Chain& ch, std::string& log, double* best_out, std::vector<double>& best_x) {
+    if (comp.empty()) return fail(CSX_ERR_PARSE, "linearOptimizer needs dataToOptimize");
+    if (data != "sags") return fail(CSX_ERR_UNSUPPORTED, "linearOptimizer: only hex-FPM sags (" + data + ")");
+    std::vector<double> x;
+    CSX_TRY(ch.get_opt(comp, data, x));
+    const int n = (int)x.size();
+    if (n == 0) return fail(CSX_ERR_ARG, "linearOptimizer: empty optimisation vector");
+    if (have_init) {
+      DeRandom rng(20261208);
+      for (double& v : x) v = random_init ? ghi * (rng.uniform() - 0.5) : init_value;
+    }
+    if (!outdir.empty()) CSX_TRY(make_dirs(outdir));
+    std::FILE* best_fid = std::fopen(join_path(outdir, "optBestVal.txt").c_str(), "w");
+    std::FILE* hist_fid = std::fopen(join_path(outdir, "optValHistory.txt").c_str(), "w");
+    if (!best_fid || !hist_fid) {
+      if (best_fid) std::fclose(best_fid);
+      if (hist_fid) std::fclose(hist_fid);
+      return fail(CSX_ERR_ARG, "linearOptimizer: cannot write in " + outdir);
+    }
+    int rc = iterate(ch, x, n, log, best_fid, hist_fid, best_out, best_x);
+    std::fclose(best_fid);
+    std::fclose(hist_fid);
+    return rc;
+  }
+
+ private:
+  int iterate(Chain& ch, std::vector<double>& x, int n, std::string& log, std::FILE* best_fid, std::FILE* hist_fid,
+              double* best_out, std::vector<double>& best_x) {
+    auto t0 = std::chrono::steady_clock::now();
+    auto secs = [&]() { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
+    double best = 1e15, last = 0.0, delta_norm = 1e300;
+    DevBuf dF, dA, db;
+    CSX_TRY(dA.reserve(sizeof(double) * (size_t)n * n));
+    CSX_TRY(db.reserve(sizeof(double) * n));
+    std::vector<double> X((size_t)(n + 1) * n), A((size_t)n * n), b(n), c;
+    for (int it = 0;; it++) {
+      // the Jacobian batch: x, then x + dx e_i
+      for (int r = 0; r <= n; r++) {
+        std::copy(x.begin(), x.end(), X.begin() + (size_t)r * n);
+        if (r > 0) X[(size_t)r * n + r - 1] += jacobian_dx;
+      }
+      CSX_TRY(contrasts(ch, X, n + 1, n, c));
+      const double contrast = c[0];
+      int npix = 0, nc = 0, nsl = 0;
+      CSX_TRY(ch.region_fields(have_region ? &region : nullptr, nullptr, &npix, &nc, &nsl));
+      CSX_TRY(dF.reserve(sizeof(double2) * (size_t)nsl * nc * npix));
+      CSX_TRY(ch.region_fields(have_region ? &region : nullptr, dF.as<double2>(), &npix, &nc, &nsl));
+      std::fprintf(hist_fid, "%g %g\n", secs(), contrast);
+      std::fflush(hist_fid);
+      if (contrast < best) {
+        best = contrast;
+        best_x = x;
+        std::fprintf(best_fid, "%g %g\n", secs(), contrast);
+        std::fflush(best_fid);
+        CSX_TRY(save_vector(join_path(outdir, "bestOptVector.fits"), best_x));
+      }
+      const bool stop = std::fabs(last - contrast) < rel_tol || contrast < stop_contrast || delta_norm < delta_tol ||
+                        it > max_iterations;
+      last = contrast;
+      char line[160];
+      std::snprintf(line, sizeof line, "evaluation %d, %.3f s, contrast = %.6e\n", it, secs(), contrast);
+      log += line;
+      if (stop) break;
+      // compute_step: normal equations, then the mu scan as one batch
+      CSX_TRY(linear_normal_equations(dF.p, nsl, n, npix, jacobian_dx, dA.as<double>(), db.as<double>(), ch.stream()));
+      CSX_HIP(hipMemcpyAsync(A.data(), dA.p, sizeof(double) * A.size(), hipMemcpyDeviceToHost, ch.stream()));
+      CSX_HIP(hipMemcpyAsync(b.data(), db.p, sizeof(double) * n, hipMemcpyDeviceToHost, ch.stream()));
+      CSX_HIP(hipStreamSynchronize(ch.stream()));
+      std::vector<double> mus, D;
+      if (!(mu_step > 0) || !(mu0 > 0)) return fail(CSX_ERR_ARG, "linearOptimizer: muScan needs positive values");
+      for (int k = 0; mu0 + k * mu_step <= mu1 + 0.5 * mu_step; k++) mus.push_back(mu0 + k * mu_step);
+      std::vector<double> d;
+      for (double mu : mus) {
+        if (!solve(A, b, mu, n, d)) return fail(CSX_ERR_NUMERIC, "linearOptimizer: normal equations not positive");
+        D.insert(D.end(), d.begin(), d.end());
+      }
+      const int nm = (int)mus.size();
+      std::vector<double> Xm((size_t)nm * n);
+      for (int k = 0; k < nm; k++)
+        for (int i = 0; i < n; i++) Xm[(size_t)k * n + i] = x[i] + D[(size_t)k * n + i];
+      CSX_TRY(contrasts(ch, Xm, nm, n, c));
+      double lastc = 1e6;
+      int kb = 0;
+      for (int k = 0; k < nm; k++) {
+        if (c[k] < lastc) { lastc = c[k]; kb = k; }
+        else break;
+      }
+      double dn = 0;
+      for (int i = 0; i < n; i++) {
+        const double di = D[(size_t)kb * n + i];
+        dn += di * di;
+        x[i] = std::min(ghi, std::max(glo, x[i] + di));
+      }
+      delta_norm = std::sqrt(dn);
+      CSX_TRY(save_vector(join_path(outdir, "deltaOptVec_" + std::to_string(it) + ".fits"),
+                          std::vector<double>(D.begin() + (size_t)kb * n, D.begin() + (size_t)(kb + 1) * n)));
+      std::snprintf(line, sizeof line, "  selected mu = %g, step norm %.3e\n", mus[kb], delta_norm);
+      log += line;
+    }
+    *best_out = best;
+    // the chain returns to the single sag set, holding the best vector
+    CSX_TRY(ch.set_candidates(comp, nullptr, 0, 0));
+    return ch.set_opt(comp, data, best_x);
+  }
+};
+
+static int run_linear_block(Chain& ch, const Block& b, std::string& log) {
+  LinearOptimizer lo;
+  for (const Block& sb : find_blocks(b)) {
+    if (sb.cmds.empty()) continue;
+    if (sb.cmds[0].key == "regionContrast") {
+      region_keys(sb, lo.region);
+      lo.have_region = true;
+    } else {
+      for (const Cmd& c : sb.cmds) lo.set(c.key, c.arg);
+    }
+  }
+  for (const std::string& u : lo.unknown) log += "!!! linearOptimizer bad set field name: " + u + "\n";
+  double best = 0;
+  std::vector<double> x;
+  CSX_TRY(lo.run(ch, log, &best, x));
+  char line[96];
+  std::snprintf(line, sizeof line, "linear optimiser: best contrast %.6e\n", best);
+  log += line;
+  return CSX_OK;
+}
+
 // The script loop of src/csim.cpp:54-108.  The telescope, efield, coronagraph and
 // regionContrast blocks define the chain (parsed once, as csim's globals); the tool
 // blocks run in script order.  Relative output paths are taken as given (csim's
@@ -832,7 +1045,8 @@ static int run_script(csx_ctx* ctx, const std::string& text, const std::string& 
     } else if (t == "nloptOptimizer") {
       return fail(CSX_ERR_UNSUPPORTED, "nloptOptimizer: the nlopt library is not part of this build");
     } else if (t == "linearOptimizer") {
-      return fail(CSX_ERR_UNSUPPORTED, "linearOptimizer runs from the host API (csx_chain_region_fields), not a script");
+      CSX_TRY(need_chain(t));
+      CSX_TRY(run_linear_block(*ch, b, log));
     } else {
       log += "!!! unknown top-level block: " + t + "\n";
     }

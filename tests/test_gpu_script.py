@@ -150,3 +150,42 @@ def test_de_over_full_propagation(ctx, tmp_path):
     print(f"DE over propagations: zero sags {c_zero[0]:.4e}, best {c_best[0]:.4e} (logged {logged:.4e})")
     assert abs(c_best[0] - logged) <= 1e-5 * logged
     assert c_best[0] < c_zero[0]
+
+
+def test_linear_optimizer_script_block_matches_the_python_driver(ctx, tmp_path):
+    """The linearOptimizer block in C++ (Jacobian batch, normal equations on the device,
+    Cholesky solves over the mu scan, the scan's contrasts as one batch) against the
+    Python driver of the same algorithm (csim_amd.linear, torch solves) on the same chain
+    and start: the same contrast after every step and the same best vector."""
+    from csim_amd.linear import LinearConfig, LinearOptimizer
+    w = W.workload2(n=256, n_lambda=2, offsets=(0.0,), rings=3)
+    nhex = w.meta["n_hex"]
+    s0 = np.random.default_rng(41).random(nhex) * 0.2e-6
+    w.meta["sags"] = s0
+    ch = Chain(ctx, w.script)
+    W.apply(ch, w)
+    opt = LinearOptimizer(ch, "fpm", LinearConfig(jacobian_dx=1e-12))
+    import torch
+    _, J, _ = opt.jacobian(s0)
+    scale = float(torch.linalg.matrix_norm(J, ord=2))
+    mu = (0.01 * scale, 0.3 * scale, 0.01 * scale)
+    hi = float(np.float32(1e-6))                      # globalBounds is read through float
+    cfg = LinearConfig(jacobian_dx=1e-12, mu_scan=mu, global_bounds=(0.0, hi), max_iterations=3,
+                       contrast_rel_tolerance=0.0, stop_contrast=0.0)
+    ch.set_opt_data("fpm", "sags", s0)
+    ref = LinearOptimizer(ch, "fpm", cfg).run(s0)
+    d = str(tmp_path)
+    text = script_with_files(w, d)
+    text += ("start\n\tlinearOptimizer\n\tdataToOptimize:fpm, sags\n\tjacobianDx:1e-12\n\tglobalBounds:0, 1e-6\n"
+             f"\tmaxIterations:3\n\tstopContrast:0\n\tcontrastRelTolerance:0\n\tmuScan:{mu[0]!r}, {mu[1]!r}, {mu[2]!r}\n"
+             f"\toutputDirectory:{d}/lin\nend\n")
+    log = script.run_script(ctx, text, workdir=d)
+    print(log)
+    hist = np.loadtxt(os.path.join(d, "lin", "optValHistory.txt"))[:, 1]
+    refc = np.array([h[2] for h in ref.history])
+    print("C++", hist, "python", refc)
+    assert hist.shape == refc.shape
+    assert np.allclose(hist, refc, rtol=1e-5)               # optValHistory.txt is %g
+    best = np.asarray(fits.read_fits(os.path.join(d, "lin", "bestOptVector.fits"))).ravel()
+    assert np.max(np.abs(best - ref.best_vector)) <= 1e-6 * np.max(np.abs(ref.best_vector))
+    assert refc.min() < 0.5 * refc[0]
