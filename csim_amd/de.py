@@ -88,6 +88,8 @@ class DEResult:
     history: list = field(default_factory=list)     # (iteration, wall seconds, best value, evaluations)
     evaluations: int = 0
     seconds: float = 0.0
+    population: np.ndarray | None = None            # final population (D, P) and its values
+    values: np.ndarray | None = None
 
 
 class DifferentialEvolution:
@@ -116,14 +118,20 @@ class DifferentialEvolution:
         return origin[:, None] + spread * c.init_spread * (self.rng.random((self.D, P)) - 0.5) * \
             (self.ihi - self.ilo)[:, None]
 
-    def run(self, time_budget=None, target=None, eval_budget=None) -> DEResult:
+    def run(self, time_budget=None, target=None, eval_budget=None, init_pop=None, init_val=None,
+            stop=None) -> DEResult:
         """Iterates until the csim stopping rules, a wall-time budget (s), a target
-        objective or a budget of exact evaluations is reached."""
+        objective, a budget of exact evaluations or stop() is reached.  init_pop (D, P)
+        (with its values init_val, or evaluated here) replaces the initial population."""
         c = self.c
         t0 = time.perf_counter()
-        origin = c.init_mean if c.init_mean is not None else 0.5 * (self.ilo + self.ihi)
-        pop = self.init_population(origin)
-        val = self._eval(pop)
+        if init_pop is not None:
+            pop = np.array(init_pop, dtype=np.float64)
+            val = np.array(init_val, dtype=np.float64) if init_val is not None else self._eval(pop)
+        else:
+            origin = c.init_mean if c.init_mean is not None else 0.5 * (self.ilo + self.ihi)
+            pop = self.init_population(origin)
+            val = self._eval(pop)
         strategy = STRATEGIES[c.start_strategy]
         best_val, best_vec = np.inf, None
         gbest_val, gbest_vec = np.inf, None
@@ -156,6 +164,8 @@ class DifferentialEvolution:
             if target is not None and gbest_val <= target:
                 break
             if eval_budget is not None and self.evals >= eval_budget:
+                break
+            if stop is not None and stop():
                 break
             if small >= c.max_small_change or do_reinit or spread_mean < c.reinit_spread:
                 best_val = np.inf
@@ -210,4 +220,4 @@ class DifferentialEvolution:
                 prev_best = best_val
             it += 1
             hist.append((it, time.perf_counter() - t0, gbest_val, self.evals))
-        return DEResult(gbest_val, gbest_vec, hist, self.evals, time.perf_counter() - t0)
+        return DEResult(gbest_val, gbest_vec, hist, self.evals, time.perf_counter() - t0, pop, val)

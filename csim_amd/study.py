@@ -96,14 +96,17 @@ class ScreenedSurrogate:
     generation go to mutants drawn at random from those the surrogate did not pick, and
     the generation's screening gain is the mean log10 contrast of those controls minus
     that of the picked ones (positive: the surrogate's picks are darker than chance).
-    When its median over the last `fallback_window` generations is not positive, the
-    surrogate is no longer trusted and every later generation is evaluated exactly
-    (plain DE on the same population).  The rank correlation of predicted and exact
+    When its median over the last `fallback_window` generations falls below
+    `fallback_gain` (picks clearly brighter than chance: the surrogate misleads the
+    search) it is no longer trusted: `fallback` turns true, and the driver continues
+    as plain DE (compare_de); until then a falling-back hook evaluates every mutant
+    exactly.  Within a converged population picks and controls are alike, their gain
+    is noise around zero, and the surrogate stays.  The rank correlation of predicted and exact
     contrast over the evaluated mutants is recorded too; within a converged population
     it is close to zero even for a useful surrogate, so it is reported, not judged."""
 
     def __init__(self, mlp, lambdas, counts, exact, keep=32, tune_steps=2, lr=3e-4, capacity=32768, seed=5,
-                 n_control=4, fallback_window=8):
+                 n_control=4, fallback_window=8, fallback_gain=-0.02, max_generations=None):
         import torch
         self.mlp, self.lambdas, self.counts, self.exact = mlp, list(lambdas), counts, exact
         self.keep, self.tune_steps, self.lr = keep, tune_steps, lr
@@ -114,7 +117,8 @@ class ScreenedSurrogate:
         self.gen.manual_seed(seed)
         self.errors = []                # per generation: Spearman(pred, exact) of the survivors
         self.seconds = {"predict": 0.0, "exact": 0.0, "learn": 0.0}   # host wall time per phase
-        self.n_control, self.fallback_window = n_control, fallback_window
+        self.n_control, self.fallback_window, self.fallback_gain = n_control, fallback_window, fallback_gain
+        self.max_generations = max_generations   # screened generations before plain DE takes over (None: no limit)
         self.gains = []                 # per generation: screening gain (log10 contrast, controls - picks)
         self.rng = np.random.Generator(np.random.Philox(seed))
         self.generation = 0
@@ -174,8 +178,10 @@ class ScreenedSurrogate:
             lv = np.log10(np.maximum(vals, 1e-300))
             self.gains.append(float(lv[npick:].mean() - lv[:npick].mean()))
             recent = self.gains[-self.fallback_window:]
-            if len(recent) == self.fallback_window and np.median(recent) <= 0.0:
+            if len(recent) == self.fallback_window and np.median(recent) < self.fallback_gain:
                 self.fallback_generation = self.generation + 1
+        if self.max_generations is not None and self.generation >= self.max_generations and not self.fallback:
+            self.fallback_generation = self.generation + 1
         self._push(features_device(self.mlp.ctx, t[idx], self.lambdas), log_labels(prof).reshape(-1, N_BINS))
         self.tune()
         t3 = time.perf_counter()
@@ -231,9 +237,11 @@ def train_surrogate(ctx, w, n_train=22528, epochs=20, precision="bf16x3", seed=2
 
 
 def compare_de(ctx, w, sur, seconds=None, eval_budget=None, plain_pop=64, screen_pop=512, precision="bf16x3",
-               seed=20261207, log=print):
+               seed=20261207, log=print, screen_generations=None):
     """Plain DE, then screened DE (stopping at the plain run's best value when a time
     budget is given) starting from the trained model, which is restored afterwards.
+    When the judge drops the surrogate (or after screen_generations screened
+    generations) the screened run continues as plain DE from its best members.
     Budgets: wall seconds or exact candidate evaluations."""
     import torch
     nhex = w.meta["n_hex"]
@@ -252,11 +260,31 @@ def compare_de(ctx, w, sur, seconds=None, eval_budget=None, plain_pop=64, screen
     target = rp.best_value if seconds is not None else None
     sur.mlp.set_params(theta)
     exact_s = ExactObjective(ctx, w, sur.keep, precision)
-    s1 = ScreenedSurrogate(sur.mlp, sur.lambdas, sur.counts, exact_s, keep=sur.keep)
+    s1 = ScreenedSurrogate(sur.mlp, sur.lambdas, sur.counts, exact_s, keep=sur.keep,
+                           max_generations=screen_generations)
     cfg_s = DEConfig(population_size=screen_pop, global_bounds=(0.0, 1e-6), init_bounds=(0.0, 1e-6),
                      max_iterations=10 ** 9, seed=seed + 2)
     de_s = DifferentialEvolution(exact_s, nhex, cfg_s, screen=s1.screen_and_eval)
-    rs = de_s.run(time_budget=seconds, target=target, eval_budget=eval_budget)
+    rs = de_s.run(time_budget=seconds, target=target, eval_budget=eval_budget, stop=lambda: s1.fallback)
+    best_before_fallback = rs.best_value
+    if s1.fallback:
+        # continue as plain DE from the screened population's best members, for what is
+        # left of the budgets; its history continues the screened run's clock and counts
+        order = np.argsort(rs.values)[:plain_pop]
+        exact_f = ExactObjective(ctx, w, plain_pop, precision)
+        de_f = DifferentialEvolution(exact_f, nhex, DEConfig(population_size=plain_pop, global_bounds=(0.0, 1e-6),
+                                                             init_bounds=(0.0, 1e-6), max_iterations=10 ** 9,
+                                                             seed=seed + 3))
+        left_s = None if seconds is None else max(0.0, seconds - rs.seconds)
+        left_e = None if eval_budget is None else max(0, eval_budget - rs.evaluations)
+        rf = de_f.run(time_budget=left_s, target=target, eval_budget=left_e, init_pop=rs.population[:, order],
+                      init_val=rs.values[order])
+        exact_f.ch.close()
+        hist = rs.history + [(g + rs.history[-1][0], t + rs.seconds, min(v, rs.best_value), e + rs.evaluations)
+                             for g, t, v, e in rf.history]
+        rs = type(rs)(min(rs.best_value, rf.best_value),
+                      rf.best_vector if rf.best_value < rs.best_value else rs.best_vector, hist,
+                      rs.evaluations + rf.evaluations, rs.seconds + rf.seconds)
     reach = next((h for h in rs.history if h[2] <= rp.best_value), None)
     out["screened"] = {"population": screen_pop, "keep": s1.keep, "best": rs.best_value, "seconds": rs.seconds,
                        "evaluations": rs.evaluations, "generations": len(rs.history) - 1,
@@ -264,6 +292,7 @@ def compare_de(ctx, w, sur, seconds=None, eval_budget=None, plain_pop=64, screen
                        "evaluations_to_plain_best": reach[3] if reach else None,
                        "survivor_spearman_median": float(np.median(s1.errors)) if s1.errors else None,
                        "fallback_generation": s1.fallback_generation,
+                       "best_before_fallback": best_before_fallback,
                        "screening_gain_median": float(np.median(s1.gains)) if s1.gains else None,
                        "seconds_by_phase": {k: round(v, 3) for k, v in s1.seconds.items()}}
     out["wall_ratio_vs_plain"] = (reach[1] / rp.seconds) if reach else None

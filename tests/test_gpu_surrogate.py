@@ -38,6 +38,13 @@ def test_surrogate_screening_gain(ctx):
     assert res["screened"]["best"] < res["plain"]["best"]
     assert res["screened"]["fallback_generation"] is None  # a good surrogate stays in use
     assert res["screened"]["screening_gain_median"] > 0
+    # the driver's fallback: the screened run stops at the fallback and plain DE continues
+    # from its best members within the same budget of exact evaluations
+    res = study.compare_de(ctx, w, sur, eval_budget=4000, screen_generations=3)
+    print(res)
+    scr = res["screened"]
+    assert scr["fallback_generation"] == 4 and scr["evaluations"] == 4000
+    assert scr["best"] <= scr["best_before_fallback"]
 
 
 class InvertedSurrogate(study.ScreenedSurrogate):
@@ -59,7 +66,8 @@ def test_surrogate_fallback_to_plain_de(ctx):
     nhex = w.meta["n_hex"]
     n_in = features_device(ctx, np.zeros((1, nhex)), lambdas).shape[1]
     mlp = MLP(ctx, n_in, 64, 64, N_BINS, seed=3)
-    s = InvertedSurrogate(mlp, lambdas, exact.counts, exact, keep=8, n_control=4, fallback_window=3)
+    s = InvertedSurrogate(mlp, lambdas, exact.counts, exact, keep=8, n_control=4, fallback_window=3,
+                          fallback_gain=-0.02)
     evaluated = []
     inner = s.screen_and_eval
 
