@@ -156,6 +156,77 @@ __device__ __forceinline__ void pw_point_rc(const PwChain& pc, const PwField& F,
   }
 }
 
+// pw_point_rc on V pixels of one field at once (pixel m = p[m] at row r[m] of column c),
+// element by element: the V map values of an element are loaded before any is used,
+// so their loads are in flight together (one wait per element, where a per-pixel chain
+// waits once per pixel and element).  The element loop is not unrolled (the code of
+// each element kind appears once).  Same arithmetic as pw_point_rc, value for value.
+template <int V>
+__device__ __forceinline__ void pw_points(const PwChain& pc, const PwField& F, int f, const int (&p)[V],
+                                          const int (&r)[V], int c, double (&er)[V], double (&ei)[V]) {
+#pragma clang fp contract(off)
+  if (F.gen) {
+    if (F.tx == 0.0 && F.ty == 0.0) {
+      // on-axis source: the phase 2 pi v is +0 at every pixel, sincos_phase(+0) = (+0, 1)
+#pragma unroll
+      for (int m = 0; m < V; m++) { er[m] = F.flux * 1.0; ei[m] = F.flux * 0.0; }
+    } else {
+#pragma unroll
+      for (int m = 0; m < V; m++) {
+        double x = -(F.Lx - F.ps) / 2.0 + c * F.ps, y = -(F.Ly - F.ps) / 2.0 + r[m] * F.ps;
+        double v = 2 * x * F.tx / F.Lx + 2 * y * F.ty / F.Ly;
+        double sn, cs;
+        sincos_phase(2.0 * M_PI * v, &sn, &cs);
+        er[m] = F.flux * cs; ei[m] = F.flux * sn;
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+  }
+#pragma unroll 1
+  for (int k = F.first; k < pc.nops; k++) {
+    const int kind = pc.op[k].kind;
+    if (kind == PW_MASK_1PI) {
+      const GLOBAL double* mp = gin<double>(pc.op[k].p);
+      double w[V];
+#pragma unroll
+      for (int m = 0; m < V; m++) w[m] = mp[p[m]];
+#pragma unroll
+      for (int m = 0; m < V; m++) {
+        double tr = er[m] * w[m] - ei[m] * w[m];
+        ei[m] = er[m] * w[m] + ei[m] * w[m];
+        er[m] = tr;
+      }
+    } else if (kind == PW_MIRROR) {
+      const GLOBAL double* hp = gin<double>(pc.op[k].p);
+      const double mir = -pc.op[k].sign * 2 * 2 * M_PI / pc.lambdas[f];   // as pw_field
+      double h[V];
+#pragma unroll
+      for (int m = 0; m < V; m++) h[m] = hp[p[m]];
+#pragma unroll
+      for (int m = 0; m < V; m++) {
+        double sn, cs;
+        sincos_phase(mir * h[m], &sn, &cs);
+        double tr = er[m] * cs - ei[m] * sn;
+        ei[m] = er[m] * sn + ei[m] * cs;
+        er[m] = tr;
+        // one phasor at a time: V interleaved phasors would not fit the registers
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    } else if (kind == PW_CFACTOR) {
+      const GLOBAL f64x2* wp = gin<f64x2>(pc.op[k].p) + (long)(f / pc.op[k].per) * ((long)pc.rows * pc.cols);
+      f64x2 w[V];
+#pragma unroll
+      for (int m = 0; m < V; m++) w[m] = wp[p[m]];
+#pragma unroll
+      for (int m = 0; m < V; m++) {
+        double tr = er[m] * w[m].x - ei[m] * w[m].y;
+        ei[m] = er[m] * w[m].y + ei[m] * w[m].x;
+        er[m] = tr;
+      }
+    }
+  }
+}
+
 __device__ __forceinline__ void pw_point(const PwChain& pc, const PwField& F, int f, int p, double& er, double& ei) {
   const int c = p / pc.rows;
   pw_point_rc(pc, F, f, p, p - c * pc.rows, c, er, ei);
@@ -642,7 +713,7 @@ __global__ void k_twiddles(double2* tw, int n) {
 template <int N>
 __host__ __device__ constexpr int fft_contig_block() { return fft_threads(N) > kFftBlock ? fft_threads(N) : kFftBlock; }
 template <int N, int SIGN>
-__global__ __launch_bounds__(fft_contig_block<N>()) void k_fft_contig(double2* __restrict__ E, int ncol, double scale,
+__global__ __launch_bounds__(fft_contig_block<N>()) __attribute__((amdgpu_waves_per_eu(4))) void k_fft_contig(double2* __restrict__ E, int ncol, double scale,
                                                                       const double2* __restrict__ tw, PwChain pre,
                                                                       int has_pre, PwChain post, int has_post) {
   constexpr int T = fft_threads(N), V = N / T;
@@ -655,14 +726,23 @@ __global__ __launch_bounds__(fft_contig_block<N>()) void k_fft_contig(double2* _
   double2 v[V];
   if (has_pre) {
     const PwField F = pw_field(pre, f);
+    int pp[V], rr[V];
+    double er[V], ei[V];
 #pragma unroll
     for (int m = 0; m < V; m++) {
-      const int p = p0 + t + m * T;
-      double er = 0.0, ei = 0.0;
-      if (!F.gen && live) { const f64x2 e = Ef[p]; er = e.x; ei = e.y; }
-      if (live) pw_point_rc(pre, F, f, p, t + m * T, p0 / N, er, ei);
-      v[m] = make_double2(er, ei);
+      rr[m] = t + m * T;
+      pp[m] = p0 + rr[m];
     }
+    if (!F.gen && live) {
+#pragma unroll
+      for (int m = 0; m < V; m++) { const f64x2 e = Ef[pp[m]]; er[m] = e.x; ei[m] = e.y; }
+    } else {
+#pragma unroll
+      for (int m = 0; m < V; m++) { er[m] = 0.0; ei[m] = 0.0; }
+    }
+    if (live) pw_points<V>(pre, F, f, pp, rr, p0 / N, er, ei);
+#pragma unroll
+    for (int m = 0; m < V; m++) v[m] = make_double2(er[m], ei[m]);
   } else {
 #pragma unroll
     for (int m = 0; m < V; m++) {
@@ -674,13 +754,17 @@ __global__ __launch_bounds__(fft_contig_block<N>()) void k_fft_contig(double2* _
   if (!live) return;
   if (has_post) {
     const PwField G = pw_field(post, f);
+    int pp[V], rr[V];
+    double er[V], ei[V];
 #pragma unroll
     for (int m = 0; m < V; m++) {
-      const int p = p0 + t + m * T;
-      double er = v[m].x * scale, ei = v[m].y * scale;
-      pw_point_rc(post, G, f, p, t + m * T, p0 / N, er, ei);
-      Ef[p] = f64x2{er, ei};
+      rr[m] = t + m * T;
+      pp[m] = p0 + rr[m];
+      er[m] = v[m].x * scale; ei[m] = v[m].y * scale;
     }
+    pw_points<V>(post, G, f, pp, rr, p0 / N, er, ei);
+#pragma unroll
+    for (int m = 0; m < V; m++) Ef[pp[m]] = f64x2{er[m], ei[m]};
   } else {
 #pragma unroll
     for (int m = 0; m < V; m++) Ef[p0 + t + m * T] = f64x2{v[m].x * scale, v[m].y * scale};
@@ -755,13 +839,14 @@ __global__ __launch_bounds__(fft_panel_rows<N>() * fft_threads(N)) void k_fft_pa
       }
     }
     // the inverse reads the forward's twiddles and LDS slots again: opaque copies of the
-    // pointers and of t keep the compiler from holding the forward's loaded twiddles and
-    // computed addresses in registers across H (they would not fit)
-    const double2* tw_inv = tw;
-    double2* lds_inv = lds;
+    // twiddle pointer and of t keep the compiler from holding the forward's loaded
+    // twiddles and computed addresses in registers across H (they would not fit).  The
+    // copies keep their address spaces (a generic pointer would turn the inverse's LDS
+    // exchanges and twiddle reads into flat accesses, which wait on both counters).
+    const GLOBAL double2* tw_inv = (const GLOBAL double2*)tw;
     int t_inv = t;
-    asm volatile("" : "+s"(tw_inv), "+v"(lds_inv), "+v"(t_inv));
-    fft_reg<N, +1>(v, t_inv, tw_inv, lds_inv, RP);
+    asm volatile("" : "+s"(tw_inv), "+v"(t_inv));
+    fft_reg<N, +1>(v, t_inv, (const double2*)tw_inv, lds, RP);
   }
   // the stores recompute their addresses (an opaque copy of the base): keeping the loads'
   // V addresses live through the transforms costs registers the transforms need

@@ -219,16 +219,17 @@ def workload_zoomprop(n=256, pad=1.0, zoom=1.0):
                     meta={"seed": seed, "n": n, "fields": 1, "lambdas": [LAMBDA0]})
 
 
-def workload1(n=2048, fresnel_pad=1):
+def workload1(n=2048, fresnel_pad=1, sources=((1.0, 0.0, 0.0),)):
     """2048^2 field, PIAA mirrors around a 1 m Fresnel step, downsampled to 1024^2.
-    fresnel_pad 2: the step on csim's 2n zero pad (fresnelPropagator padFactor)."""
+    fresnel_pad 2: the step on csim's 2n zero pad (fresnelPropagator padFactor).
+    sources: point sources (flux, tipX, tiltY), one on axis by default."""
     seed = BASE_SEED + 1
     pix = PIX
     r_ap = 1000 * n // 2048                  # 1000 px at 2048
     beam_r = r_ap * pix
     h1, c1 = piaa_sag(seed, 2, n, r_ap, radial=0.5e-6)
     h2, c2 = piaa_sag(seed, 3, n, r_ap, radial=-0.5e-6)
-    script = (_telescope(2 * beam_r) + _efield(n, pix, beam_r, [LAMBDA0], [(1.0, 0.0, 0.0)]) +
+    script = (_telescope(2 * beam_r) + _efield(n, pix, beam_r, [LAMBDA0], list(sources)) +
               "start\n\tcoronagraph\n"
               "\tstart\n\t\tmask\n\t\tname:pupil\n\tend\n"
               "\tstart\n\t\tmirror\n\t\tname:aberration\n\tend\n"
@@ -243,7 +244,7 @@ def workload1(n=2048, fresnel_pad=1):
               ("piaa1", "filename"): h1, ("piaa2", "filename"): h2,
               ("lyot", "filename"): circle(n, 900 * n / 2048)}
     return Workload(1, "2048^2 PIAA + 1 m Fresnel + downsample + MFT chain", script, arrays,
-                    meta={"seed": seed, "n": n, "fields": 1, "lambdas": [LAMBDA0], "zernike": (c1, c2)})
+                    meta={"seed": seed, "n": n, "fields": len(sources), "lambdas": [LAMBDA0], "zernike": (c1, c2)})
 
 
 def poly_lambdas(n=9):
