@@ -161,9 +161,12 @@ __device__ __forceinline__ void pw_point_rc(const PwChain& pc, const PwField& F,
 // so their loads are in flight together (one wait per element, where a per-pixel chain
 // waits once per pixel and element).  The element loop is not unrolled (the code of
 // each element kind appears once).  Same arithmetic as pw_point_rc, value for value.
+// The first npre (<= 2) elements after the source may come with their map values
+// already loaded: element F.first + j takes pv[m].x (j = 0) or pv[m].y (j = 1).
 template <int V>
 __device__ __forceinline__ void pw_points(const PwChain& pc, const PwField& F, int f, const int (&p)[V],
-                                          const int (&r)[V], int c, double (&er)[V], double (&ei)[V]) {
+                                          const int (&r)[V], int c, double (&er)[V], double (&ei)[V],
+                                          int npre, const double2 (&pv)[V]) {
 #pragma clang fp contract(off)
   if (F.gen) {
     if (F.tx == 0.0 && F.ty == 0.0) {
@@ -185,11 +188,17 @@ __device__ __forceinline__ void pw_points(const PwChain& pc, const PwField& F, i
 #pragma unroll 1
   for (int k = F.first; k < pc.nops; k++) {
     const int kind = pc.op[k].kind;
+    const int j = k - F.first;
     if (kind == PW_MASK_1PI) {
       const GLOBAL double* mp = gin<double>(pc.op[k].p);
       double w[V];
+      if (j < npre) {
 #pragma unroll
-      for (int m = 0; m < V; m++) w[m] = mp[p[m]];
+        for (int m = 0; m < V; m++) w[m] = j == 0 ? pv[m].x : pv[m].y;
+      } else {
+#pragma unroll
+        for (int m = 0; m < V; m++) w[m] = mp[p[m]];
+      }
 #pragma unroll
       for (int m = 0; m < V; m++) {
         double tr = er[m] * w[m] - ei[m] * w[m];
@@ -200,8 +209,13 @@ __device__ __forceinline__ void pw_points(const PwChain& pc, const PwField& F, i
       const GLOBAL double* hp = gin<double>(pc.op[k].p);
       const double mir = -pc.op[k].sign * 2 * 2 * M_PI / pc.lambdas[f];   // as pw_field
       double h[V];
+      if (j < npre) {
 #pragma unroll
-      for (int m = 0; m < V; m++) h[m] = hp[p[m]];
+        for (int m = 0; m < V; m++) h[m] = j == 0 ? pv[m].x : pv[m].y;
+      } else {
+#pragma unroll
+        for (int m = 0; m < V; m++) h[m] = hp[p[m]];
+      }
 #pragma unroll
       for (int m = 0; m < V; m++) {
         double sn, cs;
@@ -582,6 +596,7 @@ constexpr int kFftBlock = 256;           // threads of a row-pass block (several
 // the other's transforms.
 constexpr int kFftPanelMaxThreads = 512;
 constexpr int kFftPanelLds = 75000;      // bytes of LDS a panel block may use (of 160 KiB per CU)
+constexpr bool kFftPanelBases = true;    // column-pass twiddles from per-thread bases (FftStagesB)
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -640,14 +655,25 @@ __host__ __device__ constexpr int fft_threads(int N) { return N >= 8 ? N / 8 : 1
 __host__ __device__ constexpr int fft_lds_ix(int i) { return i + (i >> 3); }
 __host__ __device__ constexpr int fft_lds_len(int N) { return fft_lds_ix(N - 1) + 1; }
 
+// Radix of the stage on sub-transforms of length L: 8 while the length allows, then 4 or 2.
+__host__ __device__ constexpr int fft_radix(int N, int L) {
+  return (N / L >= 8 && (N / fft_threads(N)) % 8 == 0) ? 8 : ((N / L >= 4 && (N / fft_threads(N)) % 4 == 0) ? 4 : 2);
+}
+__host__ __device__ constexpr int fft_nstages(int N) {
+  int n = 0;
+  for (int L = 1; L < N; L *= fft_radix(N, L)) n++;
+  return n;
+}
+
 // One radix-R stage on sub-transforms of length L (see above).  v holds the thread's
 // values in the t + m T layout on entry and on exit.  Every thread of the block calls
 // it: a stage other than the last synchronises around its LDS exchange (lds: this
-// transform's LDS, lds_stride apart for consecutive indices).
-template <int N, int R, int L, int SIGN>
-__device__ __forceinline__ void fft_reg_stage(double2 (&v)[N / fft_threads(N)], int t, const double2* __restrict__ tw,
+// transform's LDS, lds_stride apart for consecutive indices).  twiddles(q, k, w) puts
+// w^(s k TS) into w[s], 1 <= s < R, for butterfly q (k = its index mod L).
+template <int N, int R, int L, int SIGN, class Tw>
+__device__ __forceinline__ void fft_reg_stage(double2 (&v)[N / fft_threads(N)], int t, const Tw& twiddles,
                                               double2* lds, int lds_stride) {
-  constexpr int T = fft_threads(N), V = N / T, BQ = V / R, TS = N / (L * R);
+  constexpr int T = fft_threads(N), V = N / T, BQ = V / R;
   static_assert(V % R == 0 && L * R <= N, "stage does not fit the register layout");
   double2 y[BQ][R];
 #pragma unroll
@@ -657,8 +683,10 @@ __device__ __forceinline__ void fft_reg_stage(double2 (&v)[N / fft_threads(N)], 
     for (int s = 0; s < R; s++) y[q][s] = v[q + s * BQ];
     if (L > 1) {
       // w^0 = 1 exactly: the k = 0 butterflies are multiplied too (exactly), no branch
+      double2 w[R];
+      twiddles(q, k, w);
 #pragma unroll
-      for (int s = 1; s < R; s++) y[q][s] = cmul_tw<SIGN>(y[q][s], tw[s * k * TS]);
+      for (int s = 1; s < R; s++) y[q][s] = cmul_tw<SIGN>(y[q][s], w[s]);
     }
     Dft<R, SIGN>::run(y[q]);
   }
@@ -680,14 +708,18 @@ __device__ __forceinline__ void fft_reg_stage(double2 (&v)[N / fft_threads(N)], 
     for (int m = 0; m < V; m++) v[m] = lds[fft_lds_ix(t + m * T) * lds_stride];
   }
 }
-// The stages of a whole transform, from sub-transform length L on.
+// The stages of a whole transform, from sub-transform length L on, twiddles read from the
+// table tw[q] = exp(i 2 pi q / N).
 template <int N, int L, int SIGN>
 struct FftStages {
-  static constexpr int V = N / fft_threads(N), REM = N / L;
-  static constexpr int R = (REM >= 8 && V % 8 == 0) ? 8 : ((REM >= 4 && V % 4 == 0) ? 4 : 2);
+  static constexpr int V = N / fft_threads(N), R = fft_radix(N, L), TS = N / (L * R);
   static __device__ __forceinline__ void run(double2 (&v)[V], int t, const double2* __restrict__ tw, double2* lds,
                                              int lds_stride) {
-    fft_reg_stage<N, R, L, SIGN>(v, t, tw, lds, lds_stride);
+    auto twiddles = [&](int, int k, double2 (&w)[R]) {
+#pragma unroll
+      for (int s = 1; s < R; s++) w[s] = tw[s * k * TS];
+    };
+    fft_reg_stage<N, R, L, SIGN>(v, t, twiddles, lds, lds_stride);
     if constexpr (L * R < N) FftStages<N, L * R, SIGN>::run(v, t, tw, lds, lds_stride);
   }
 };
@@ -696,6 +728,57 @@ __device__ __forceinline__ void fft_reg(double2 (&v)[N / fft_threads(N)], int t,
                                         double2* lds, int lds_stride) {
   FftStages<N, 1, SIGN>::run(v, t, tw, lds, lds_stride);
 }
+
+// The same stages with the twiddles formed from one base per butterfly and stage,
+// wb[I][q] = w^(k TS) (stage I, butterfly q, k = (t + q T) mod L).  The bases depend on
+// t only: a block that transforms several columns reads them once, and the stages then
+// make no global memory access.  Powers by products at most three deep (w^2 = w w,
+// w^3 = w^2 w, w^4 = w^2 w^2, w^5 = w^4 w, w^6 = w^4 w^2, w^7 = w^4 w^3): a few ulps
+// from the table's correctly rounded values.
+template <int N>
+struct FftBases {
+  static constexpr int NS = fft_nstages(N), WQ = (N / fft_threads(N)) / 2 > 0 ? (N / fft_threads(N)) / 2 : 1;
+  double2 wb[NS][WQ];
+};
+template <int N, int L, int I, int SIGN>
+struct FftStagesB {
+  static constexpr int T = fft_threads(N), V = N / T, R = fft_radix(N, L), BQ = V / R, TS = N / (L * R);
+  static __device__ __forceinline__ void load(FftBases<N>& B, int t, const GLOBAL f64x2* tw) {
+#pragma unroll
+    for (int q = 0; q < BQ; q++) {
+      const int k = (t + q * T) & (L - 1);
+      if constexpr (L > 1) {
+        const f64x2 w = tw[k * TS];
+        B.wb[I][q] = make_double2(w.x, w.y);
+      } else {
+        B.wb[I][q] = make_double2(1.0, 0.0);
+      }
+    }
+    if constexpr (L * R < N) FftStagesB<N, L * R, I + 1, SIGN>::load(B, t, tw);
+  }
+  static __device__ __forceinline__ void run(double2 (&v)[V], int t, const FftBases<N>& B, double2* lds,
+                                             int lds_stride) {
+    auto twiddles = [&](int q, int, double2 (&w)[R]) {
+      // an opaque copy of the base: the powers are formed here, in every transform,
+      // rather than hoisted out of a caller's loop and held in registers
+      double wx = B.wb[I][q].x, wy = B.wb[I][q].y;
+      asm volatile("" : "+v"(wx), "+v"(wy));
+      w[1] = make_double2(wx, wy);
+      if constexpr (R >= 4) {
+        w[2] = cmul(w[1], w[1]);
+        w[3] = cmul(w[2], w[1]);
+      }
+      if constexpr (R == 8) {
+        w[4] = cmul(w[2], w[2]);
+        w[5] = cmul(w[4], w[1]);
+        w[6] = cmul(w[4], w[2]);
+        w[7] = cmul(w[4], w[3]);
+      }
+    };
+    fft_reg_stage<N, R, L, SIGN>(v, t, twiddles, lds, lds_stride);
+    if constexpr (L * R < N) FftStagesB<N, L * R, I + 1, SIGN>::run(v, t, B, lds, lds_stride);
+  }
+};
 
 __global__ void k_twiddles(double2* tw, int n) {
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
@@ -706,68 +789,104 @@ __global__ void k_twiddles(double2* tw, int n) {
 }
 
 // Along the contiguous (row) index: C = kFftBlock / T columns (n values each, T threads
-// each) per block, column gc = blockIdx.x C + g of the batch (field gc / N).  The
-// field's pointwise chain `pre` (point-source generation, masks, mirrors: the elements
-// before the transform) is applied as the column is read, `post` (the masks and
-// mirrors after it) and `scale` as it is written.
+// each) per block row br, column gc = br C + g of the batch (field gc / N).  The block
+// rows are spread over a persistent grid (blockIdx.x, then gridDim.x apart): while a
+// block transforms one column it has the next one's input in flight, the field values
+// or, where `pre` generates the field, the maps of pre's first two elements when they
+// are masks or mirrors.  The stages take their twiddles from per-thread bases
+// (FftStagesB), so between the input and the output a column's only global accesses
+// are the chains' other maps.  The field's pointwise chain `pre` (point-source
+// generation, masks, mirrors: the elements before the transform) is applied as the
+// column is read, `post` (the masks and mirrors after it) and `scale` as it is written.
 template <int N>
 __host__ __device__ constexpr int fft_contig_block() { return fft_threads(N) > kFftBlock ? fft_threads(N) : kFftBlock; }
-template <int N, int SIGN>
-__global__ __launch_bounds__(fft_contig_block<N>()) __attribute__((amdgpu_waves_per_eu(4))) void k_fft_contig(double2* __restrict__ E, int ncol, double scale,
-                                                                      const double2* __restrict__ tw, PwChain pre,
-                                                                      int has_pre, PwChain post, int has_post) {
-  constexpr int T = fft_threads(N), V = N / T;
+constexpr int kFftRowWaves = 3;          // waves per SIMD a row-pass block is compiled for
+// What a row pass prefetches: the field values (PF 0), or, where `pre` generates the
+// field, the maps of its elements 1 and 2 (PF 1: both masks or mirrors) or nothing (PF 2).
+enum { kPfField = 0, kPfMaps = 1, kPfNone = 2 };
+template <int N, int SIGN, int PF>
+__global__ __launch_bounds__(fft_contig_block<N>()) __attribute__((amdgpu_waves_per_eu(kFftRowWaves)))
+void k_fft_contig(double2* __restrict__ E, int nbr, int ncol, double scale, const double2* __restrict__ tw,
+                  PwChain pre, int has_pre, PwChain post, int has_post) {
+  constexpr int T = fft_threads(N), V = N / T, C = fft_contig_block<N>() / T;
   extern __shared__ double2 sbuf[];
-  const int g = threadIdx.x / T, t = threadIdx.x - g * T;
-  const int gc = blockIdx.x * (fft_contig_block<N>() / T) + g;
-  const bool live = gc < ncol;               // the last block's spare groups still synchronise
-  const int f = live ? gc / N : 0, p0 = (live ? gc - f * N : 0) * N;
-  GLOBAL f64x2* Ef = gout<f64x2>(E) + (long)f * N * N;
-  double2 v[V];
-  if (has_pre) {
-    const PwField F = pw_field(pre, f);
-    int pp[V], rr[V];
-    double er[V], ei[V];
+  const int g = threadIdx.x / T, t0 = threadIdx.x - g * T;
+  double2* lds = sbuf + g * fft_lds_len(N);
+  FftBases<N> B;
+  FftStagesB<N, 1, 0, SIGN>::load(B, t0, gin<f64x2>(tw));
+  constexpr int npre = PF == kPfMaps ? 2 : 0;
+  // the loads of block row br (the spare groups of the last block row read the batch's
+  // last column: their values are never stored)
+  auto fetch = [&](int br, int t, double2 (&dst)[V]) {
+    const int gc = min(br * C + g, ncol - 1);
+    const int f = gc / N, p0 = (gc - f * N) * N + t;
+    if constexpr (PF == kPfField) {
+      const GLOBAL f64x2* Ef = gout<f64x2>(E) + (long)f * N * N + p0;
 #pragma unroll
-    for (int m = 0; m < V; m++) {
-      rr[m] = t + m * T;
-      pp[m] = p0 + rr[m];
-    }
-    if (!F.gen && live) {
+      for (int m = 0; m < V; m++) { const f64x2 e = Ef[m * T]; dst[m] = make_double2(e.x, e.y); }
+    } else if constexpr (PF == kPfMaps) {
+      const GLOBAL double* a = gin<double>(pre.op[1].p) + p0;
+      const GLOBAL double* b = gin<double>(pre.op[2].p) + p0;
 #pragma unroll
-      for (int m = 0; m < V; m++) { const f64x2 e = Ef[pp[m]]; er[m] = e.x; ei[m] = e.y; }
+      for (int m = 0; m < V; m++) dst[m] = make_double2(a[m * T], b[m * T]);
     } else {
 #pragma unroll
-      for (int m = 0; m < V; m++) { er[m] = 0.0; ei[m] = 0.0; }
+      for (int m = 0; m < V; m++) dst[m] = make_double2(0.0, 0.0);
     }
-    if (live) pw_points<V>(pre, F, f, pp, rr, p0 / N, er, ei);
+  };
+  double2 nx[V];
+  if (blockIdx.x < nbr) fetch(blockIdx.x, t0, nx);
+  for (int br = blockIdx.x; br < nbr; br += gridDim.x) {
+    // an opaque copy of t per column: the addresses derived from it are formed again in
+    // every iteration rather than held in registers across the loop (they would not fit)
+    int t = t0;
+    asm volatile("" : "+v"(t));
+    const int gc = br * C + g;
+    const bool live = gc < ncol;               // spare groups of the last block row still synchronise
+    const int f = live ? gc / N : 0, p0 = (live ? gc - f * N : 0) * N;
+    double2 v[V];
+    if (has_pre) {
+      const PwField F = pw_field(pre, f);
+      int pp[V], rr[V];
+      double er[V], ei[V];
 #pragma unroll
-    for (int m = 0; m < V; m++) v[m] = make_double2(er[m], ei[m]);
-  } else {
+      for (int m = 0; m < V; m++) {
+        rr[m] = t + m * T;
+        pp[m] = p0 + rr[m];
+        er[m] = nx[m].x; ei[m] = nx[m].y;      // the field (overwritten where it is generated)
+      }
+      if (live) pw_points<V>(pre, F, f, pp, rr, p0 / N, er, ei, npre, nx);
 #pragma unroll
-    for (int m = 0; m < V; m++) {
-      const f64x2 e = live ? Ef[p0 + t + m * T] : f64x2{0.0, 0.0};
-      v[m] = make_double2(e.x, e.y);
+      for (int m = 0; m < V; m++) v[m] = make_double2(er[m], ei[m]);
+    } else {
+#pragma unroll
+      for (int m = 0; m < V; m++) v[m] = nx[m];
     }
-  }
-  fft_reg<N, SIGN>(v, t, tw, sbuf + g * fft_lds_len(N), 1);
-  if (!live) return;
-  if (has_post) {
-    const PwField G = pw_field(post, f);
-    int pp[V], rr[V];
-    double er[V], ei[V];
+    // the next block row's loads, in flight during the transform.  Every earlier load has
+    // been used by now; waiting for them here keeps the compiler's count of loads in
+    // flight exact where the branch joins (it would otherwise wait for the prefetch too)
+    __builtin_amdgcn_s_waitcnt(0x0F70);                       // vmcnt(0)
+    if (br + (int)gridDim.x < nbr) fetch(br + gridDim.x, t, nx);
+    FftStagesB<N, 1, 0, SIGN>::run(v, t, B, lds, 1);
+    if (!live) continue;
+    GLOBAL f64x2* Ef = gout<f64x2>(E) + (long)f * N * N;
+    if (has_post) {
+      const PwField G = pw_field(post, f);
+      int pp[V], rr[V];
+      double er[V], ei[V];
 #pragma unroll
-    for (int m = 0; m < V; m++) {
-      rr[m] = t + m * T;
-      pp[m] = p0 + rr[m];
-      er[m] = v[m].x * scale; ei[m] = v[m].y * scale;
+      for (int m = 0; m < V; m++) {
+        rr[m] = t + m * T;
+        pp[m] = p0 + rr[m];
+        er[m] = v[m].x * scale; ei[m] = v[m].y * scale;
+      }
+      pw_points<V>(post, G, f, pp, rr, p0 / N, er, ei, 0, v);
+#pragma unroll
+      for (int m = 0; m < V; m++) Ef[pp[m]] = f64x2{er[m], ei[m]};
+    } else {
+#pragma unroll
+      for (int m = 0; m < V; m++) Ef[p0 + t + m * T] = f64x2{v[m].x * scale, v[m].y * scale};
     }
-    pw_points<V>(post, G, f, pp, rr, p0 / N, er, ei);
-#pragma unroll
-    for (int m = 0; m < V; m++) Ef[pp[m]] = f64x2{er[m], ei[m]};
-  } else {
-#pragma unroll
-    for (int m = 0; m < V; m++) Ef[p0 + t + m * T] = f64x2{v[m].x * scale, v[m].y * scale};
   }
 }
 
@@ -804,6 +923,8 @@ __global__ __launch_bounds__(fft_panel_rows<N>() * fft_threads(N)) void k_fft_pa
   const int f = blockIdx.y, r0 = panel * RP;
   const int rr = threadIdx.x % RP, t = threadIdx.x / RP;
   GLOBAL f64x2* F = gout<f64x2>(E) + (long)f * N * N + r0 + rr;
+  FftBases<N> B;
+  if constexpr (kFftPanelBases) FftStagesB<N, 1, 0, -1>::load(B, t, gin<f64x2>(tw));
   double2 v[V];
 #pragma unroll
   for (int m = 0; m < V; m++) {
@@ -812,9 +933,11 @@ __global__ __launch_bounds__(fft_panel_rows<N>() * fft_threads(N)) void k_fft_pa
   }
   double2* lds = sbuf + rr;
   if (MODE == 0) {
-    fft_reg<N, SIGN>(v, t, tw, lds, RP);
+    if constexpr (kFftPanelBases) FftStagesB<N, 1, 0, SIGN>::run(v, t, B, lds, RP);
+    else fft_reg<N, SIGN>(v, t, tw, lds, RP);
   } else {
-    fft_reg<N, -1>(v, t, tw, lds, RP);
+    if constexpr (kFftPanelBases) FftStagesB<N, 1, 0, -1>::run(v, t, B, lds, RP);
+    else fft_reg<N, -1>(v, t, tw, lds, RP);
     {
 #pragma clang fp contract(off)
       // csim: exp(i*k*z * sqrt(1. - 1./4.*pow(lambda/apRad, 2.)*H)), Goodman 4-20
@@ -846,7 +969,8 @@ __global__ __launch_bounds__(fft_panel_rows<N>() * fft_threads(N)) void k_fft_pa
     const GLOBAL double2* tw_inv = (const GLOBAL double2*)tw;
     int t_inv = t;
     asm volatile("" : "+s"(tw_inv), "+v"(t_inv));
-    fft_reg<N, +1>(v, t_inv, (const double2*)tw_inv, lds, RP);
+    if constexpr (kFftPanelBases) FftStagesB<N, 1, 0, +1>::run(v, t_inv, B, lds, RP);
+    else fft_reg<N, +1>(v, t_inv, (const double2*)tw_inv, lds, RP);
   }
   // the stores recompute their addresses (an opaque copy of the base): keeping the loads'
   // V addresses live through the transforms costs registers the transforms need
@@ -879,6 +1003,23 @@ static const double2* twiddles(int n, hipStream_t s) {
   return p;
 }
 
+// Compute units of the current device (queried once per device).
+static int device_cus(int* cus) {
+  static std::mutex mu;
+  static std::map<int, int> known;
+  int dev = 0;
+  CSX_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = known.find(dev);
+  if (it == known.end()) {
+    int n = 0;
+    CSX_HIP(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+    it = known.emplace(dev, std::max(1, n)).first;
+  }
+  *cus = it->second;
+  return CSX_OK;
+}
+
 // Dynamic LDS beyond 64 KiB has to be allowed per kernel (once per process).
 template <class K>
 static int allow_lds(K* kernel, size_t bytes) {
@@ -888,19 +1029,41 @@ static int allow_lds(K* kernel, size_t bytes) {
   return CSX_OK;
 }
 
-template <int N, int SIGN>
-static int launch_contig(double2* E, int nfield, double scale, const double2* tw, hipStream_t s,
-                         const PwChain* pre, const PwChain* post) {
+template <int N, int SIGN, int PF>
+static int launch_contig_pf(double2* E, int nfield, double scale, const double2* tw, hipStream_t s,
+                            const PwChain* pre, const PwChain* post) {
   constexpr int B = fft_contig_block<N>(), C = B / fft_threads(N);
   const size_t lds = sizeof(double2) * C * fft_lds_len(N);
-  static const int attr = allow_lds(&k_fft_contig<N, SIGN>, lds);
+  static const int attr = allow_lds(&k_fft_contig<N, SIGN, PF>, lds);
   CSX_TRY(attr);
-  const int ncol = N * nfield;
+  const int ncol = N * nfield, nbr = (ncol + C - 1) / C;
+  // persistent grid: the blocks that fit on the device at once (kFftRowWaves waves per
+  // SIMD, as the LDS allows), each taking every grid-th block row
+  int cus = 0;
+  CSX_TRY(device_cus(&cus));
+  const int waves = B / 64, per_cu_w = 4 * kFftRowWaves / waves;
+  const int per_cu_l = (int)((160 << 10) / lds);
+  const int grid = std::min(nbr, cus * std::max(1, std::min(per_cu_w, per_cu_l)));
   PwChain none{};
-  hipLaunchKernelGGL((k_fft_contig<N, SIGN>), dim3((ncol + C - 1) / C), dim3(B), lds, s, E, ncol, scale, tw,
+  hipLaunchKernelGGL((k_fft_contig<N, SIGN, PF>), dim3(grid), dim3(B), lds, s, E, nbr, ncol, scale, tw,
                      pre ? *pre : none, pre ? 1 : 0, post ? *post : none, post ? 1 : 0);
   CSX_HIP(hipGetLastError());
   return CSX_OK;
+}
+// A generating chain (its first element the point source) comes only before a forward pass.
+template <int N, int SIGN>
+static int launch_contig(double2* E, int nfield, double scale, const double2* tw, hipStream_t s,
+                         const PwChain* pre, const PwChain* post) {
+  const bool gen = pre && pre->nops > 0 && pre->op[0].kind == PW_SOURCE;
+  if (!gen) return launch_contig_pf<N, SIGN, kPfField>(E, nfield, scale, tw, s, pre, post);
+  if constexpr (SIGN < 0) {
+    auto is_map = [&](int k) {
+      return k < pre->nops && (pre->op[k].kind == PW_MASK_1PI || pre->op[k].kind == PW_MIRROR);
+    };
+    if (is_map(1) && is_map(2)) return launch_contig_pf<N, SIGN, kPfMaps>(E, nfield, scale, tw, s, pre, post);
+    return launch_contig_pf<N, SIGN, kPfNone>(E, nfield, scale, tw, s, pre, post);
+  }
+  return fail(CSX_ERR_ARG, "fft: a generating chain before an inverse transform");
 }
 template <int N, int MODE, int SIGN>
 static int launch_panel(double2* E, int nfield, const double* lambdas, double ap_rad, double z, const double2* tw,
