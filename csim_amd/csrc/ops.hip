@@ -161,13 +161,14 @@ __device__ __forceinline__ void pw_point_rc(const PwChain& pc, const PwField& F,
 // so their loads are in flight together (one wait per element, where a per-pixel chain
 // waits once per pixel and element).  The element loop is not unrolled (the code of
 // each element kind appears once).  Same arithmetic as pw_point_rc, value for value.
-// The first npre (<= 2) elements after the source may come with their map values
-// already loaded: element F.first + j takes pv[m].x (j = 0) or pv[m].y (j = 1).
-template <int V>
+// The first NP (<= 3, <= NX) elements after the source come with their map values
+// already loaded (masks and mirrors): element F.first + j takes pm[j][m].
+template <int V, int NP, int NX>
 __device__ __forceinline__ void pw_points(const PwChain& pc, const PwField& F, int f, const int (&p)[V],
                                           const int (&r)[V], int c, double (&er)[V], double (&ei)[V],
-                                          int npre, const double2 (&pv)[V]) {
+                                          const double (&pm)[NX][V]) {
 #pragma clang fp contract(off)
+  static_assert(NP <= NX && NP <= 3, "preloaded maps");
   if (F.gen) {
     if (F.tx == 0.0 && F.ty == 0.0) {
       // on-axis source: the phase 2 pi v is +0 at every pixel, sincos_phase(+0) = (+0, 1)
@@ -185,47 +186,51 @@ __device__ __forceinline__ void pw_points(const PwChain& pc, const PwField& F, i
       }
     }
   }
+  // E *= (1 + i) w (mask.cpp)
+  auto mask = [&](const double (&w)[V]) {
+#pragma unroll
+    for (int m = 0; m < V; m++) {
+      double tr = er[m] * w[m] - ei[m] * w[m];
+      ei[m] = er[m] * w[m] + ei[m] * w[m];
+      er[m] = tr;
+    }
+  };
+  // E *= exp(i mir h) (mirror.cpp:66-72)
+  auto mirror = [&](int k, const double (&h)[V]) {
+    const double mir = -pc.op[k].sign * 2 * 2 * M_PI / pc.lambdas[f];   // as pw_field
+#pragma unroll
+    for (int m = 0; m < V; m++) {
+      double sn, cs;
+      sincos_phase(mir * h[m], &sn, &cs);
+      double tr = er[m] * cs - ei[m] * sn;
+      ei[m] = er[m] * sn + ei[m] * cs;
+      er[m] = tr;
+      // one phasor at a time: V interleaved phasors would not fit the registers
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  };
+  // the elements whose maps came loaded (masks or mirrors)
+#pragma unroll
+  for (int q = 0; q < NP; q++) {
+    const int k = F.first + q;
+    if (pc.op[k].kind == PW_MASK_1PI) mask(pm[q]);
+    else mirror(k, pm[q]);
+  }
 #pragma unroll 1
-  for (int k = F.first; k < pc.nops; k++) {
+  for (int k = F.first + NP; k < pc.nops; k++) {
     const int kind = pc.op[k].kind;
-    const int j = k - F.first;
     if (kind == PW_MASK_1PI) {
       const GLOBAL double* mp = gin<double>(pc.op[k].p);
       double w[V];
-      if (j < npre) {
 #pragma unroll
-        for (int m = 0; m < V; m++) w[m] = j == 0 ? pv[m].x : pv[m].y;
-      } else {
-#pragma unroll
-        for (int m = 0; m < V; m++) w[m] = mp[p[m]];
-      }
-#pragma unroll
-      for (int m = 0; m < V; m++) {
-        double tr = er[m] * w[m] - ei[m] * w[m];
-        ei[m] = er[m] * w[m] + ei[m] * w[m];
-        er[m] = tr;
-      }
+      for (int m = 0; m < V; m++) w[m] = mp[p[m]];
+      mask(w);
     } else if (kind == PW_MIRROR) {
       const GLOBAL double* hp = gin<double>(pc.op[k].p);
-      const double mir = -pc.op[k].sign * 2 * 2 * M_PI / pc.lambdas[f];   // as pw_field
       double h[V];
-      if (j < npre) {
 #pragma unroll
-        for (int m = 0; m < V; m++) h[m] = j == 0 ? pv[m].x : pv[m].y;
-      } else {
-#pragma unroll
-        for (int m = 0; m < V; m++) h[m] = hp[p[m]];
-      }
-#pragma unroll
-      for (int m = 0; m < V; m++) {
-        double sn, cs;
-        sincos_phase(mir * h[m], &sn, &cs);
-        double tr = er[m] * cs - ei[m] * sn;
-        ei[m] = er[m] * sn + ei[m] * cs;
-        er[m] = tr;
-        // one phasor at a time: V interleaved phasors would not fit the registers
-        __builtin_amdgcn_sched_barrier(0);
-      }
+      for (int m = 0; m < V; m++) h[m] = hp[p[m]];
+      mirror(k, h);
     } else if (kind == PW_CFACTOR) {
       const GLOBAL f64x2* wp = gin<f64x2>(pc.op[k].p) + (long)(f / pc.op[k].per) * ((long)pc.rows * pc.cols);
       f64x2 w[V];
@@ -800,41 +805,47 @@ __global__ void k_twiddles(double2* tw, int n) {
 // column is read, `post` (the masks and mirrors after it) and `scale` as it is written.
 template <int N>
 __host__ __device__ constexpr int fft_contig_block() { return fft_threads(N) > kFftBlock ? fft_threads(N) : kFftBlock; }
-constexpr int kFftRowWaves = 3;          // waves per SIMD a row-pass block is compiled for
-// What a row pass prefetches: the field values (PF 0), or, where `pre` generates the
-// field, the maps of its elements 1 and 2 (PF 1: both masks or mirrors) or nothing (PF 2).
-enum { kPfField = 0, kPfMaps = 1, kPfNone = 2 };
-template <int N, int SIGN, int PF>
+constexpr int kFftRowWaves = 2;          // waves per SIMD a row-pass block is compiled for
+// What a row pass loads ahead.  PF >= 0: the field is generated by `pre` and the next
+// column's maps of pre's elements 1 .. PF (masks or mirrors, PF <= 3) are prefetched;
+// PF = kPfField: the next column's field values are prefetched.  EP: the current
+// column's map of post's first element (a mask or a mirror) is loaded before the
+// transform rather than after it.
+constexpr int kPfField = -1;
+template <int N, int SIGN, int PF, bool EP>
 __global__ __launch_bounds__(fft_contig_block<N>()) __attribute__((amdgpu_waves_per_eu(kFftRowWaves)))
 void k_fft_contig(double2* __restrict__ E, int nbr, int ncol, double scale, const double2* __restrict__ tw,
                   PwChain pre, int has_pre, PwChain post, int has_post) {
   constexpr int T = fft_threads(N), V = N / T, C = fft_contig_block<N>() / T;
+  constexpr int npre = PF > 0 ? PF : 0, NX = PF == kPfField ? 2 : (npre > 0 ? npre : 1);
   extern __shared__ double2 sbuf[];
   const int g = threadIdx.x / T, t0 = threadIdx.x - g * T;
   double2* lds = sbuf + g * fft_lds_len(N);
   FftBases<N> B;
   FftStagesB<N, 1, 0, SIGN>::load(B, t0, gin<f64x2>(tw));
-  constexpr int npre = PF == kPfMaps ? 2 : 0;
   // the loads of block row br (the spare groups of the last block row read the batch's
   // last column: their values are never stored)
-  auto fetch = [&](int br, int t, double2 (&dst)[V]) {
+  auto fetch = [&](int br, int t, double (&dst)[NX][V]) {
     const int gc = min(br * C + g, ncol - 1);
     const int f = gc / N, p0 = (gc - f * N) * N + t;
     if constexpr (PF == kPfField) {
       const GLOBAL f64x2* Ef = gout<f64x2>(E) + (long)f * N * N + p0;
 #pragma unroll
-      for (int m = 0; m < V; m++) { const f64x2 e = Ef[m * T]; dst[m] = make_double2(e.x, e.y); }
-    } else if constexpr (PF == kPfMaps) {
-      const GLOBAL double* a = gin<double>(pre.op[1].p) + p0;
-      const GLOBAL double* b = gin<double>(pre.op[2].p) + p0;
-#pragma unroll
-      for (int m = 0; m < V; m++) dst[m] = make_double2(a[m * T], b[m * T]);
+      for (int m = 0; m < V; m++) { const f64x2 e = Ef[m * T]; dst[0][m] = e.x; dst[1][m] = e.y; }
     } else {
 #pragma unroll
-      for (int m = 0; m < V; m++) dst[m] = make_double2(0.0, 0.0);
+      for (int q = 0; q < npre; q++) {
+        const GLOBAL double* a = gin<double>(pre.op[1 + q].p) + p0;
+#pragma unroll
+        for (int m = 0; m < V; m++) dst[q][m] = a[m * T];
+      }
     }
   };
-  double2 nx[V];
+  double nx[NX][V];
+#pragma unroll
+  for (int q = 0; q < NX; q++)
+#pragma unroll
+    for (int m = 0; m < V; m++) nx[q][m] = 0.0;
   if (blockIdx.x < nbr) fetch(blockIdx.x, t0, nx);
   for (int br = blockIdx.x; br < nbr; br += gridDim.x) {
     // an opaque copy of t per column: the addresses derived from it are formed again in
@@ -853,19 +864,26 @@ void k_fft_contig(double2* __restrict__ E, int nbr, int ncol, double scale, cons
       for (int m = 0; m < V; m++) {
         rr[m] = t + m * T;
         pp[m] = p0 + rr[m];
-        er[m] = nx[m].x; ei[m] = nx[m].y;      // the field (overwritten where it is generated)
+        er[m] = nx[0][m]; ei[m] = nx[NX - 1][m];   // the field (overwritten where it is generated)
       }
-      if (live) pw_points<V>(pre, F, f, pp, rr, p0 / N, er, ei, npre, nx);
+      if (live) pw_points<V, npre>(pre, F, f, pp, rr, p0 / N, er, ei, nx);
 #pragma unroll
       for (int m = 0; m < V; m++) v[m] = make_double2(er[m], ei[m]);
     } else {
 #pragma unroll
-      for (int m = 0; m < V; m++) v[m] = nx[m];
+      for (int m = 0; m < V; m++) v[m] = make_double2(nx[0][m], nx[NX - 1][m]);
     }
-    // the next block row's loads, in flight during the transform.  Every earlier load has
-    // been used by now; waiting for them here keeps the compiler's count of loads in
-    // flight exact where the branch joins (it would otherwise wait for the prefetch too)
+    // the next block row's loads (and this column's post map), in flight during the
+    // transform.  Every earlier load has been used by now; waiting for them here keeps
+    // the compiler's count of loads in flight exact where the branch joins (it would
+    // otherwise wait for these loads too)
     __builtin_amdgcn_s_waitcnt(0x0F70);                       // vmcnt(0)
+    double pe[1][V];
+    if constexpr (EP) {
+      const GLOBAL double* a = gin<double>(post.op[0].p) + p0 + t;
+#pragma unroll
+      for (int m = 0; m < V; m++) pe[0][m] = a[m * T];
+    }
     if (br + (int)gridDim.x < nbr) fetch(br + gridDim.x, t, nx);
     FftStagesB<N, 1, 0, SIGN>::run(v, t, B, lds, 1);
     if (!live) continue;
@@ -880,7 +898,7 @@ void k_fft_contig(double2* __restrict__ E, int nbr, int ncol, double scale, cons
         pp[m] = p0 + rr[m];
         er[m] = v[m].x * scale; ei[m] = v[m].y * scale;
       }
-      pw_points<V>(post, G, f, pp, rr, p0 / N, er, ei, 0, v);
+      pw_points<V, EP ? 1 : 0>(post, G, f, pp, rr, p0 / N, er, ei, pe);
 #pragma unroll
       for (int m = 0; m < V; m++) Ef[pp[m]] = f64x2{er[m], ei[m]};
     } else {
@@ -1029,12 +1047,12 @@ static int allow_lds(K* kernel, size_t bytes) {
   return CSX_OK;
 }
 
-template <int N, int SIGN, int PF>
+template <int N, int SIGN, int PF, bool EP>
 static int launch_contig_pf(double2* E, int nfield, double scale, const double2* tw, hipStream_t s,
                             const PwChain* pre, const PwChain* post) {
   constexpr int B = fft_contig_block<N>(), C = B / fft_threads(N);
   const size_t lds = sizeof(double2) * C * fft_lds_len(N);
-  static const int attr = allow_lds(&k_fft_contig<N, SIGN, PF>, lds);
+  static const int attr = allow_lds(&k_fft_contig<N, SIGN, PF, EP>, lds);
   CSX_TRY(attr);
   const int ncol = N * nfield, nbr = (ncol + C - 1) / C;
   // persistent grid: the blocks that fit on the device at once (kFftRowWaves waves per
@@ -1045,25 +1063,42 @@ static int launch_contig_pf(double2* E, int nfield, double scale, const double2*
   const int per_cu_l = (int)((160 << 10) / lds);
   const int grid = std::min(nbr, cus * std::max(1, std::min(per_cu_w, per_cu_l)));
   PwChain none{};
-  hipLaunchKernelGGL((k_fft_contig<N, SIGN, PF>), dim3(grid), dim3(B), lds, s, E, nbr, ncol, scale, tw,
+  hipLaunchKernelGGL((k_fft_contig<N, SIGN, PF, EP>), dim3(grid), dim3(B), lds, s, E, nbr, ncol, scale, tw,
                      pre ? *pre : none, pre ? 1 : 0, post ? *post : none, post ? 1 : 0);
   CSX_HIP(hipGetLastError());
   return CSX_OK;
 }
-// A generating chain (its first element the point source) comes only before a forward pass.
+// The load-ahead form for a pass's chains.  A generating chain (its first element the
+// point source) comes only before a forward pass, a post chain only after an inverse one
+// (the Fresnel step's passes); the forms beyond the plain ones are built for N >= 512.
+static bool pw_is_map(const PwChain* c, int k) {
+  return c && k < c->nops && (c->op[k].kind == PW_MASK_1PI || c->op[k].kind == PW_MIRROR);
+}
 template <int N, int SIGN>
 static int launch_contig(double2* E, int nfield, double scale, const double2* tw, hipStream_t s,
                          const PwChain* pre, const PwChain* post) {
   const bool gen = pre && pre->nops > 0 && pre->op[0].kind == PW_SOURCE;
-  if (!gen) return launch_contig_pf<N, SIGN, kPfField>(E, nfield, scale, tw, s, pre, post);
-  if constexpr (SIGN < 0) {
-    auto is_map = [&](int k) {
-      return k < pre->nops && (pre->op[k].kind == PW_MASK_1PI || pre->op[k].kind == PW_MIRROR);
-    };
-    if (is_map(1) && is_map(2)) return launch_contig_pf<N, SIGN, kPfMaps>(E, nfield, scale, tw, s, pre, post);
-    return launch_contig_pf<N, SIGN, kPfNone>(E, nfield, scale, tw, s, pre, post);
+  if constexpr (N >= 512 && SIGN < 0) {
+    if (gen) {
+      const int nm = pw_is_map(pre, 1) ? (pw_is_map(pre, 2) ? (pw_is_map(pre, 3) ? 3 : 2) : 1) : 0;
+      if (post) return fail(CSX_ERR_ARG, "fft: a post chain after a forward transform");
+      switch (nm) {
+        case 3: return launch_contig_pf<N, SIGN, 3, false>(E, nfield, scale, tw, s, pre, post);
+        case 2: return launch_contig_pf<N, SIGN, 2, false>(E, nfield, scale, tw, s, pre, post);
+        case 1: return launch_contig_pf<N, SIGN, 1, false>(E, nfield, scale, tw, s, pre, post);
+        default: return launch_contig_pf<N, SIGN, 0, false>(E, nfield, scale, tw, s, pre, post);
+      }
+    }
   }
-  return fail(CSX_ERR_ARG, "fft: a generating chain before an inverse transform");
+  if constexpr (N >= 512 && SIGN > 0) {
+    if (!gen && pw_is_map(post, 0)) return launch_contig_pf<N, SIGN, kPfField, true>(E, nfield, scale, tw, s, pre, post);
+  }
+  if constexpr (SIGN < 0) {
+    if (gen) return launch_contig_pf<N, SIGN, 0, false>(E, nfield, scale, tw, s, pre, post);
+  } else {
+    if (gen) return fail(CSX_ERR_ARG, "fft: a generating chain before an inverse transform");
+  }
+  return launch_contig_pf<N, SIGN, kPfField, false>(E, nfield, scale, tw, s, pre, post);
 }
 template <int N, int MODE, int SIGN>
 static int launch_panel(double2* E, int nfield, const double* lambdas, double ap_rad, double z, const double2* tw,
