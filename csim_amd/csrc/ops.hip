@@ -534,6 +534,117 @@ __global__ __launch_bounds__(256) void k_pointwise_gen_bf2(PwChain pc, int noute
   }
 }
 
+// The generation pass into three-term planes (the bf16x6 working precision), PX = 4
+// pixels per thread: FP64 throughout as k_pointwise_gen (masks, the mirror phases
+// summed into one phasor per pixel and wavelength, FP64 sincos), each 32-pixel window's
+// exponent agreed over 8 lanes, one 8-byte store per plane.  Wavelengths outer, the
+// (at most kBf3Outer) source rows inner with their phasors in registers.
+constexpr int kBf3Outer = 2;
+template <int PX>
+__global__ __launch_bounds__(256) void k_pointwise_gen_bf3(PwChain pc, int nouter) {
+#pragma clang fp contract(off)
+  static_assert(PX == 4, "pixels per thread: a 32-pixel window is 8 lanes");
+  const int nl = pc.nl;
+  const int npix = pc.rows * pc.cols;
+  const int p0 = PX * (blockIdx.x * blockDim.x + threadIdx.x);
+  if (p0 >= npix) return;                      // whole windows: npix is a multiple of 32
+  double cr[PX], ci[PX];
+#pragma unroll
+  for (int x = 0; x < PX; x++) { cr[x] = 1.0; ci[x] = 0.0; }
+  for (int k = 1; k < pc.nops; k++) {
+    if (pc.op[k].kind != PW_MASK_1PI) continue;
+    const GLOBAL f64x2* wp = reinterpret_cast<const GLOBAL f64x2*>(gin<double>(pc.op[k].p) + p0);
+#pragma unroll
+    for (int x2 = 0; x2 < PX / 2; x2++) {      // mask.cpp: E *= (1+i) m
+      const f64x2 w2 = wp[x2];
+#pragma unroll
+      for (int y = 0; y < 2; y++) {
+        const int x = 2 * x2 + y;
+        const double w = w2[y];
+        const double tr = cr[x] * w - ci[x] * w;
+        ci[x] = cr[x] * w + ci[x] * w;
+        cr[x] = tr;
+      }
+    }
+  }
+  // source phasors of every outer row (efield::init_point_sources, src/data/efield.cpp:179-206)
+  const double ps = pc.pixel_scale, Lx = ps * pc.cols, Ly = ps * pc.rows;
+  double sr[kBf3Outer][PX], si[kBf3Outer][PX];
+#pragma unroll
+  for (int o = 0; o < kBf3Outer; o++) {
+    if (o >= nouter) break;
+    const double* prm = pc.src_params + 3 * o * nl;
+    const double flux = prm[0], tx = prm[1], ty = prm[2];
+#pragma unroll
+    for (int x = 0; x < PX; x++) {
+      const int p = p0 + x, c = p / pc.rows, r = p - c * pc.rows;
+      const double xx = -(Lx - ps) / 2.0 + c * ps, yy = -(Ly - ps) / 2.0 + r * ps;
+      const double v = 2 * xx * tx / Lx + 2 * yy * ty / Ly;
+      double sn, cs;
+      sincos_phase(2.0 * M_PI * v, &sn, &cs);
+      sr[o][x] = flux * cs;
+      si[o][x] = flux * sn;
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  const int stride = npix / PX;                // one plane in units of PX bf16 values
+  for (int li = 0; li < nl; li++) {
+    double Fr[PX], Fi[PX], ph[PX];
+#pragma unroll
+    for (int x = 0; x < PX; x++) ph[x] = 0.0;
+    bool any = false;
+    for (int k = 1; k < pc.nops; k++) {
+      if (pc.op[k].kind != PW_MIRROR) continue;
+      const double a = -pc.op[k].sign * 2 * 2 * M_PI / pc.lambdas[li];   // mirror.cpp:66-72
+      const GLOBAL f64x2* hp = reinterpret_cast<const GLOBAL f64x2*>(gin<double>(pc.op[k].p) + p0);
+#pragma unroll
+      for (int x2 = 0; x2 < PX / 2; x2++) {
+        const f64x2 h2 = hp[x2];
+        ph[2 * x2] += a * h2[0];
+        ph[2 * x2 + 1] += a * h2[1];
+      }
+      any = true;
+    }
+#pragma unroll
+    for (int x = 0; x < PX; x++) {
+      if (any) {
+        double sn, cs;
+        sincos_phase(ph[x], &sn, &cs);
+        Fr[x] = cr[x] * cs - ci[x] * sn;
+        Fi[x] = cr[x] * sn + ci[x] * cs;
+      } else {
+        Fr[x] = cr[x];
+        Fi[x] = ci[x];
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int o = 0; o < kBf3Outer; o++) {
+      if (o >= nouter) break;
+      const long f = (long)o * nl + li;
+      double er[PX], ei[PX];
+      float mx = 0.0f;
+#pragma unroll
+      for (int x = 0; x < PX; x++) {
+        er[x] = sr[o][x] * Fr[x] - si[o][x] * Fi[x];
+        ei[x] = sr[o][x] * Fi[x] + si[o][x] * Fr[x];
+        mx = fmaxf(mx, (float)fmax(fabs(er[x]), fabs(ei[x])));
+      }
+      const int e = window_exponent(window_max<kPlaneWindow / PX>(mx));
+      unsigned short h[PX][2][3];
+#pragma unroll
+      for (int x = 0; x < PX; x++) { split3(er[x], e, h[x][0]); split3(ei[x], e, h[x][1]); }
+      GLOBAL u32x2* base = reinterpret_cast<GLOBAL u32x2*>(gout<unsigned short>(pc.out) + f * 6 * (long)npix + p0);
+#pragma unroll
+      for (int t = 0; t < 3; t++)
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+          base[(2 * t + c) * stride] = u32x2{(unsigned)h[0][c][t] | ((unsigned)h[1][c][t] << 16),
+                                             (unsigned)h[2][c][t] | ((unsigned)h[3][c][t] << 16)};
+    }
+  }
+}
+
 int pointwise_chain(double2* E, int nfield, const PwChain& pc, hipStream_t s) {
   if (pc.nops <= 0 || nfield <= 0) return CSX_OK;
   if (pc.out_kind != PW_OUT_C128 && !pc.out) return fail(CSX_ERR_ARG, "pointwise: no output buffer");
@@ -555,6 +666,12 @@ int pointwise_chain(double2* E, int nfield, const PwChain& pc, hipStream_t s) {
     static const GenKernel kGen[2][kMaxPwLam] = {CSX_GEN_ROW(false), CSX_GEN_ROW(true)};
 #undef CSX_GEN_ROW
     static_assert(kMaxPwLam == 16, "one instance per wavelength count");
+    if (pc.out_kind == PW_OUT_PLANES && pc.terms == 3 && npix % kPlaneWindow == 0 && nfield / pc.nl <= kBf3Outer) {
+      hipLaunchKernelGGL(k_pointwise_gen_bf3<4>, dim3((int)((npix / 4 + 255) / 256)), dim3(256), 0, s, pc,
+                         nfield / pc.nl);
+      CSX_HIP(hipGetLastError());
+      return CSX_OK;
+    }
     const int split = pc.out_kind == PW_OUT_PLANES && pc.terms == 2 ? 1 : 0;   // 16-bit planes only
     if (split && npix % 8 == 0 && nfield / pc.nl <= kBf2Outer) {
       // one field per wavelength (sources folded into the next MFT): 4 pixels per thread
