@@ -219,6 +219,34 @@ def workload_zoomprop(n=256, pad=1.0, zoom=1.0):
                     meta={"seed": seed, "n": n, "fields": 1, "lambdas": [LAMBDA0]})
 
 
+def workload_fresnel_mid(n=512):
+    """pupil, aberration -> zoomFftPropagator (dir 1) -> occulting mask -> 0.5 m
+    angular-spectrum step -> mirror -> zoomFftPropagator (dir -1) -> Lyot stop -> image:
+    a Fresnel step that reads its input field (no point-source generation before it),
+    with a mask applied as its first pass reads and a mirror as its last pass writes."""
+    seed = BASE_SEED + 12
+    pix = PIX * 1024 / n
+    r_ap = 500 * n // 1024
+    beam_r = r_ap * pix
+    script = (_telescope(2 * beam_r) + _efield(n, pix, beam_r, [LAMBDA0], [(1.0, 0.0, 0.0)]) +
+              "start\n\tcoronagraph\n\tstart\n\t\tmask\n\t\tname:pupil\n\tend\n"
+              "\tstart\n\t\tmirror\n\t\tname:aberration\n\tend\n"
+              "\tstart\n\t\tzoomFftPropagator\n\t\tname:to fpm\n\t\tdir:1\n\t\tpadFactor:1.0\n"
+              "\t\tzoomFactor:1.0\n\tend\n"
+              "\tstart\n\t\tmask\n\t\tname:occulter\n\t\tposition:0\n\tend\n"
+              "\tstart\n\t\tfresnelPropagator\n\t\tname:gap\n\tend\n"
+              "\tstart\n\t\tmirror\n\t\tname:late\n\t\tposition:0.5\n\tend\n"
+              "\tstart\n\t\tzoomFftPropagator\n\t\tname:to lyot\n\t\tdir:-1\n\t\tpadFactor:1.0\n\tend\n" +
+              _tail(LAMBDA0))
+    arrays = {("pupil", "filename"): pupil_amplitude(n, r_ap, 400 * n / 1024),
+              ("aberration", "filename"): aberration_sag(seed, n),
+              ("occulter", "filename"): 1.0 - circle(n, 0.06 * n),
+              ("late", "filename"): aberration_sag(seed + 1, n),
+              ("lyot", "filename"): circle(n, 450 * n / 1024)}
+    return Workload(12, f"Fresnel step between two zoomFftPropagators, {n}^2", script, arrays,
+                    meta={"seed": seed, "n": n, "fields": 1, "lambdas": [LAMBDA0]})
+
+
 def workload1(n=2048, fresnel_pad=1, sources=((1.0, 0.0, 0.0),)):
     """2048^2 field, PIAA mirrors around a 1 m Fresnel step, downsampled to 1024^2.
     fresnel_pad 2: the step on csim's 2n zero pad (fresnelPropagator padFactor).

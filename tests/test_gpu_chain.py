@@ -581,6 +581,22 @@ def test_fresnel_chain_off_axis_sources_vs_oracle(ctx):
     ch.close()
 
 
+def test_fresnel_reading_its_field_vs_oracle(ctx):
+    """A Fresnel step after a zoomFftPropagator: its first pass reads the field (prefetched
+    a column ahead) and applies the occulting mask before it, its last pass applies the
+    mirror after it (map loaded before the transform); 512^2 against the oracle, FP64."""
+    w = W.workload_fresnel_mid(512)
+    ch = build(ctx, w)
+    oc = OracleChain(w.script, w.arrays)
+    ch.propagate(False)
+    e = rel_rms(ch.field(), oc.field(0, 0))
+    c, _ = ch.contrast()
+    c_ref, _, _ = oc.contrast()
+    print(f"Fresnel step reading its field: field rel rms {e:.3e}, contrast {c[0]:.12e} vs {c_ref:.12e}")
+    assert e < 1e-12 and abs(c[0] - c_ref) <= 1e-12 * abs(c_ref)
+    ch.close()
+
+
 def test_c4_1261_hex_fp64_vs_oracle(ctx):
     """The 20-ring standard layout (1261 hexes) of the surrogate's scale-up through the
     chain at a 256^2 pupil, 2 wavelengths: FP64 fields and contrast against the oracle."""
