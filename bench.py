@@ -200,15 +200,14 @@ def leg_c1_fresnel(leg):
             "three_pass_frac": round(three / t / 8e12, 4)}
 
 
-def de_batch_uneven(ctx, precision, steps=5, cpu=False):
-    """DE generation with uneven pupils: 64 candidates x 9 wavelengths, 32 on 512^2
-    and 32 on 1024^2 pupils (radii halved), one grouped launch per contraction
-    stage; beside it, the two chains run one after the other."""
-    import torch
+def de_group(ctx, precision):
+    """The DE generation with uneven pupils: 32 candidates (random thicknesses, Philox
+    seed 20261203 + 3) on a 512^2 and 32 on a 1024^2 chain of workload 3, 9 wavelengths
+    each, merged into one chain group.  Returns (group, chains, [(workload, sags)])."""
     from csim_amd import workloads as W
     from csim_amd.chain import ChainGroup
     rng = np.random.Generator(np.random.Philox(20261203 + 3))
-    chains, flops, members = [], 0.0, []
+    chains, members = [], []
     for n in (512, 1024):
         w = W.workload3(n=n, n_lambda=9)
         t = rng.random((32, w.meta["n_hex"])) * 1e-6
@@ -217,7 +216,15 @@ def de_batch_uneven(ctx, precision, steps=5, cpu=False):
         ch.contrast()                        # compile, tune, calibrate
         chains.append(ch)
         members.append((w, W.thickness_to_sag(t)))
-    g = ChainGroup(chains)
+    return ChainGroup(chains), chains, members
+
+
+def de_batch_uneven(ctx, precision, steps=5, cpu=False):
+    """DE generation with uneven pupils: 64 candidates x 9 wavelengths, 32 on 512^2
+    and 32 on 1024^2 pupils (radii halved), one grouped launch per contraction
+    stage; beside it, the two chains run one after the other."""
+    import torch
+    g, chains, members = de_group(ctx, precision)
     g.contrast()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -272,9 +279,15 @@ def pmc_probe(precision, workload="w2"):
     torch.cuda.synchronize()
     del x, y
     ctx = Context(0)
-    w = {"w2": W.workload2, "c0": W.workload0, "c1": W.workload1}[workload]()
-    ch = build_chain(ctx, w, precision)
-    ch.set_graph(False)
+    if workload == "c3":
+        ch, chains, _ = de_group(ctx, precision)
+        for c in chains:
+            c.set_graph(False)
+    else:
+        w = {"w2": W.workload2, "c0": W.workload0, "c1": W.workload1}[workload]()
+        ch = build_chain(ctx, w, precision)
+        ch.set_graph(False)
+        chains = [ch]
     for _ in range(2):
         ch.contrast()
     torch.cuda.synchronize()
@@ -283,7 +296,10 @@ def pmc_probe(precision, workload="w2"):
     torch.cuda.synchronize()
     ch.contrast()
     torch.cuda.synchronize()
-    ch.close()
+    if workload == "c3":
+        ch.close()
+    for c in chains:
+        c.close()
     print(f"probe done: workload {workload}, precision {precision}", flush=True)
     return 0
 
@@ -325,9 +341,9 @@ def main(argv=None):
     ap.add_argument("--no-uneven", action="store_true")
     ap.add_argument("--no-legs", action="store_true", help="skip the C0 and C1 single-field legs")
     ap.add_argument("--pmc-probe", action="store_true", help="short run for rocprofv3 counter and trace passes")
-    ap.add_argument("--probe-workload", default="w2", choices=["w2", "c0", "c1"],
+    ap.add_argument("--probe-workload", default="w2", choices=["w2", "c0", "c1", "c3"],
                     help="workload of --pmc-probe: w2 the 27-field bench workload, c0 one 1024^2 field, "
-                         "c1 the 2048^2 PIAA + Fresnel chain")
+                         "c1 the 2048^2 PIAA + Fresnel chain, c3 the DE generation with uneven pupils")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print("bench.py measures one GPU: every field shards independently, so the whole-job rate at N GPUs is "

@@ -916,15 +916,15 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
   if (TERMS == 3 && kStaged64 && P.out_kind == OUT_PLANES && !P.transC && P.terms == 3) {
     // Three-term planes, whole-grid real mask or none (the common case): the FP64 sums go
     // through LDS (row ml, column nl ^ 16 (ml & 3)), then each thread finishes 8
-    // consecutive outputs of one row (epilogue_value's arithmetic with 8-byte mask
-    // reads), the 4 threads of a window of 32 columns (4 consecutive lanes: BN / 8 chunks
-    // per row, a multiple of 4) agree on its exponent, and each thread stores one 16-byte
-    // chunk per plane.  Other epilogues take the general path below.
+    // consecutive outputs of one row (epilogue_value's arithmetic with 8-byte mask reads
+    // and 16-byte reads of the addend's planes), the 4 threads of a window of 32 columns
+    // (4 consecutive lanes: BN / 8 chunks per row, a multiple of 4) agree on its exponent,
+    // and each thread stores one 16-byte chunk per plane.  Other epilogues take the
+    // general path below.
     const int mk = P.mask_kind;
-    const bool vmask = !P.addend &&
-                       (mk == MASK_NONE || ((mk == MASK_REAL_1PI || mk == MASK_REAL_1PI_F32 ||
-                                             mk == MASK_REAL_1PI_U8 || mk == MASK_REAL) &&
-                                            P.mm0 == 0 && P.mn0 == 0 && P.mM >= M && P.mN >= N));
+    const bool vmask = mk == MASK_NONE || ((mk == MASK_REAL_1PI || mk == MASK_REAL_1PI_F32 ||
+                                            mk == MASK_REAL_1PI_U8 || mk == MASK_REAL) &&
+                                           P.mm0 == 0 && P.mn0 == 0 && P.mM >= M && P.mN >= N);
     if (vmask) {
       f64x2* st = reinterpret_cast<f64x2*>(lds);
       __syncthreads();                             // every wave is done reading operand tiles
@@ -966,6 +966,26 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
             for (int k = 0; k < 8; k++) wm8[k] = gin<double>(P.mask)[mo + k];
           }
         }
+        // the addend (stored like the output: three-term planes) of the 8 outputs, each
+        // value summed hi + mid + lo in FP64 as epilogue_value does
+        double ar8[8], ai8[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) { ar8[k] = 0.0; ai8[k] = 0.0; }
+        if (P.addend && live && !empty) {
+          u32x4 aw[6];
+#pragma unroll
+          for (int q = 0; q < 6; q++)
+            aw[q] = *reinterpret_cast<const GLOBAL u32x4*>(gin<unsigned short>(P.addend) + q * P.sC + off);
+#pragma unroll
+          for (int k = 0; k < 8; k++)
+#pragma unroll
+            for (int tt = 0; tt < 3; tt++) {
+              const unsigned short hr = (unsigned short)(aw[2 * tt][k >> 1] >> (16 * (k & 1)));
+              const unsigned short hi = (unsigned short)(aw[2 * tt + 1][k >> 1] >> (16 * (k & 1)));
+              ar8[k] += (double)bf16_val(hr);
+              ai8[k] += (double)bf16_val(hi);
+            }
+        }
         double vr[8], vi[8];
         float mx = 0.0f;
 #pragma unroll
@@ -975,6 +995,7 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
           if (live && !empty) {
             xr = P.alpha_re * a.x - P.alpha_im * a.y;
             xi = P.alpha_re * a.y + P.alpha_im * a.x;
+            if (P.addend) { xr += ar8[k]; xi += ai8[k]; }
             if (mk == MASK_REAL) {
               xr *= wm8[k]; xi *= wm8[k];
             } else if (mk != MASK_NONE) {          // csim mask: (1+i) w
