@@ -5,10 +5,13 @@
 
 namespace csx {
 
+// FP32 -> BF16, round to nearest even (the hardware conversion, v_cvt_pk_bf16_f32).  The
+// empty asm makes x an FP32 value the compiler cannot see through: a narrowing from FP64
+// before it stays its own rounding (two roundings, FP64 -> FP32 -> BF16), not folded into
+// one FP64 -> BF16 rounding.
 __device__ __forceinline__ unsigned short bf16_rne(float x) {
-  unsigned int u = __float_as_uint(x);
-  unsigned int r = u + 0x7FFFu + ((u >> 16) & 1u);
-  return (unsigned short)(r >> 16);
+  asm("" : "+v"(x));
+  return __builtin_bit_cast(unsigned short, (__bf16)x);
 }
 __device__ __forceinline__ float bf16_val(unsigned short h) { return __uint_as_float(((unsigned int)h) << 16); }
 
