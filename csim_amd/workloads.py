@@ -224,7 +224,7 @@ def workload_fresnel_mid(n=512):
     angular-spectrum step -> mirror -> zoomFftPropagator (dir -1) -> Lyot stop -> image:
     a Fresnel step that reads its input field (no point-source generation before it),
     with a mask applied as its first pass reads and a mirror as its last pass writes."""
-    seed = BASE_SEED + 12
+    seed = BASE_SEED + 13
     pix = PIX * 1024 / n
     r_ap = 500 * n // 1024
     beam_r = r_ap * pix
@@ -243,7 +243,44 @@ def workload_fresnel_mid(n=512):
               ("occulter", "filename"): 1.0 - circle(n, 0.06 * n),
               ("late", "filename"): aberration_sag(seed + 1, n),
               ("lyot", "filename"): circle(n, 450 * n / 1024)}
-    return Workload(12, f"Fresnel step between two zoomFftPropagators, {n}^2", script, arrays,
+    return Workload(13, f"Fresnel step between two zoomFftPropagators, {n}^2", script, arrays,
+                    meta={"seed": seed, "n": n, "fields": 1, "lambdas": [LAMBDA0]})
+
+
+def workload_complex_masks(n=512):
+    """complexMask elements (src/elements/complexMask.cpp): a centred complex factor in
+    the pupil plane (before the first MFT: the general pointwise pass), one before and
+    one after a 0.5 m Fresnel step (applied by its first and last passes), and a mirror
+    after the second; the window sizes differ from the field's so the centring counts."""
+    seed = BASE_SEED + 14
+    pix = PIX * 1024 / n
+    r_ap = 500 * n // 1024
+    beam_r = r_ap * pix
+    rng = np.random.Generator(np.random.Philox(seed))
+    script = (_telescope(2 * beam_r) + _efield(n, pix, beam_r, [LAMBDA0], [(1.0, 0.0, 0.0)]) +
+              "start\n\tcoronagraph\n\tstart\n\t\tmask\n\t\tname:pupil\n\tend\n"
+              "\tstart\n\t\tcomplexMask\n\t\tname:apod\n\tend\n"
+              "\tstart\n\t\tzoomFftPropagator\n\t\tname:to fpm\n\t\tdir:1\n\t\tpadFactor:1.0\n"
+              "\t\tzoomFactor:1.0\n\tend\n"
+              "\tstart\n\t\tcomplexMask\n\t\tname:cm1\n\t\tposition:0\n\tend\n"
+              "\tstart\n\t\tfresnelPropagator\n\t\tname:gap\n\tend\n"
+              "\tstart\n\t\tcomplexMask\n\t\tname:cm2\n\t\tposition:0.5\n\tend\n"
+              "\tstart\n\t\tmirror\n\t\tname:late\n\tend\n"
+              "\tstart\n\t\tzoomFftPropagator\n\t\tname:to lyot\n\t\tdir:-1\n\t\tpadFactor:1.0\n\tend\n" +
+              _tail(LAMBDA0))
+
+    def cmask(m):
+        yy, xx = np.mgrid[0:m, 0:m] / m
+        amp = 0.8 + 0.2 * np.cos(2 * np.pi * (xx + 0.5 * yy)) * rng.uniform(0.5, 1.0)
+        ph = rng.uniform(-1, 1) * np.sin(2 * np.pi * xx * 1.5) + rng.uniform(-1, 1) * np.cos(2 * np.pi * yy * 2.5)
+        return amp * np.exp(-1j * ph)
+    arrays = {("pupil", "filename"): pupil_amplitude(n, r_ap, 400 * n / 1024),
+              ("late", "filename"): aberration_sag(seed + 1, n),
+              ("lyot", "filename"): circle(n, 450 * n / 1024)}
+    complex_arrays = {("apod", "mask", 0): cmask(int(0.8 * n) + 1),
+                      ("cm1", "mask", 0): cmask(int(0.75 * n)),
+                      ("cm2", "mask", 0): cmask(int(0.6 * n) + 1)}
+    return Workload(14, f"complexMask elements around a Fresnel step, {n}^2", script, arrays, complex_arrays,
                     meta={"seed": seed, "n": n, "fields": 1, "lambdas": [LAMBDA0]})
 
 

@@ -271,6 +271,14 @@ class OracleChain:
             elif t == "mirror":
                 h = resample_to(m.arrays[(e["name"], "filename")], m.pix, m.cols, g)
                 E = mirror_apply(E, h, lam, float(e.get("sign", 1)))
+            elif t == "complexMask":
+                # complexMask::execute (src/elements/complexMask.cpp:24-55): the field's
+                # centred window (row0 = rows/2 - maskRows/2, likewise columns) times the
+                # maskIndex slice of the complex cube amp exp(-i ph) (:58-67)
+                K = np.asarray(m.arrays[(e["name"], "mask")], dtype=np.complex128)
+                if K.ndim == 3:
+                    K = K[..., int(e.get("maskIndex", 0))]
+                E = center_mask_multiply(E, K)
             elif t == "fresnelPropagator":
                 z = float(e.get("propagationSign", 1)) * abs(els[i + 1]["position"] - els[i - 1]["position"])
                 pad = int(float(e.get("padFactor", 1)))

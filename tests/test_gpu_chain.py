@@ -597,6 +597,28 @@ def test_fresnel_reading_its_field_vs_oracle(ctx):
     ch.close()
 
 
+def test_complex_masks_vs_oracle(ctx):
+    """complexMask elements (complexMask.cpp:24-67): centred complex windows in the pupil
+    plane (the general pointwise pass), before a Fresnel step (its first pass, which
+    reads the field) and after it (its last pass, before a mirror); 512^2 against the
+    oracle, FP64, calibration and occulted."""
+    w = W.workload_complex_masks(512)
+    ch = build(ctx, w)
+    arrays = dict(w.arrays)
+    for (elem, key, sl), a in w.complex_arrays.items():
+        arrays[(elem, key)] = a
+    oc = OracleChain(w.script, arrays)
+    for cal in (True, False):
+        ch.propagate(calibrating=cal)
+        e = rel_rms(ch.field(), oc.field(0, 0, calibrating=cal))
+        print(f"complexMask chain, calibrating={cal}: field rel rms {e:.3e}")
+        assert e < 1e-12
+    c, _ = ch.contrast()
+    c_ref, _, _ = oc.contrast()
+    assert abs(c[0] - c_ref) <= 1e-12 * abs(c_ref)
+    ch.close()
+
+
 def test_c4_1261_hex_fp64_vs_oracle(ctx):
     """The 20-ring standard layout (1261 hexes) of the surrogate's scale-up through the
     chain at a 256^2 pupil, 2 wavelengths: FP64 fields and contrast against the oracle."""

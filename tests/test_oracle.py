@@ -471,3 +471,21 @@ def test_oracle_chain_small_workload_runs_and_normalises():
     assert 0 < c < 1e-3 and peak > 0
     cal = oc.intensity(True) / peak
     assert abs(cal.max() - 1.0) < 1e-15
+
+
+@pytest.mark.parametrize("rows,cols,mr,mc", [(8, 8, 3, 3), (9, 8, 4, 5), (7, 10, 2, 7), (6, 5, 6, 5)])
+def test_complex_mask_centring(rows, cols, mr, mc):
+    """The centred window of complexMask and of the FPM mask cubes (complexMask.cpp:33-43,
+    doubleFraunhoferFPM.cpp:84-97): cube element (r, c) multiplies field element
+    (rows/2 - mr/2 + r, cols/2 - mc/2 + c) with integer halves, nothing else changes.
+    Non-square shapes with odd and even sizes: a transposed or rounded-up offset fails."""
+    rng = np.random.default_rng(rows * 100 + cols)
+    E = rng.standard_normal((rows, cols)) + 1j * rng.standard_normal((rows, cols))
+    K = np.ones((mr, mc), dtype=np.complex128)
+    r, c = mr - 1, mc // 2
+    K[r, c] = 2.0 - 0.5j
+    out = oracle.center_mask_multiply(E, K)
+    expected = E.copy()
+    expected[rows // 2 - mr // 2 + r, cols // 2 - mc // 2 + c] *= 2.0 - 0.5j
+    assert np.array_equal(out, expected)
+
