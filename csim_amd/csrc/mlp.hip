@@ -56,14 +56,34 @@ __global__ __launch_bounds__(256) void k_mlp_gemm(MlpGemm g) {
   for (int i = 0; i < 2; i++)
 #pragma unroll
     for (int j = 0; j < 2; j++) acc[i][j] = f32x4{0, 0, 0, 0};
+  // Each thread moves 8 values of each operand per 32-deep K step: the index that is
+  // contiguous in memory runs fastest over the threads, so each wave's loads are
+  // coalesced whichever way the operand is stored.  The next step's values are loaded
+  // into registers while the current step's MFMAs run.
+  float av[8], bv[8];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const int e = t + i * 256;
+      const int ra = g.transA ? (e & 63) : (e >> 5), ka = g.transA ? (e >> 6) : (e & 31);
+      const int rb = g.transB ? (e >> 5) : (e & 63), kb = g.transB ? (e & 31) : (e >> 6);
+      av[i] = opA(g, m0 + ra, k0 + ka);
+      bv[i] = opB(g, k0 + kb, n0 + rb);
+    }
+  };
+  gload(0);
   for (int k0 = 0; k0 < g.K; k0 += 32) {
     __syncthreads();
-    for (int e = t; e < 64 * 32; e += 256) {
-      int r = e >> 5, k = e & 31;
-      As[r][k] = opA(g, m0 + r, k0 + k);
-      Bs[r][k] = opB(g, k0 + k, n0 + r);
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const int e = t + i * 256;
+      const int ra = g.transA ? (e & 63) : (e >> 5), ka = g.transA ? (e >> 6) : (e & 31);
+      const int rb = g.transB ? (e >> 5) : (e & 63), kb = g.transB ? (e & 31) : (e >> 6);
+      As[ra][ka] = av[i];
+      Bs[rb][kb] = bv[i];
     }
     __syncthreads();
+    if (k0 + 32 < g.K) gload(k0 + 32);
     if (MODE == kBF16) {
       const int fr = lane & 15, fk = 8 * (lane >> 4);
       bf16x8 a[2], b[2];
@@ -136,13 +156,26 @@ __global__ void k_gelu_bwd(float* __restrict__ d, const float* __restrict__ h, l
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x)
     d[i] *= gelu_df(h[i]);
 }
-// db[j] = sum_i d[i][j]
-__global__ void k_colsum(const float* __restrict__ d, int n, int m, float* __restrict__ out) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= m) return;
+// db[j] = sum_i d[i][j]: a block per 64 columns, 16 row groups of 64 threads, group q
+// summing rows q, q + 16, ... in FP64 (loads 16 deep in flight); the groups' sums are
+// then added in a fixed order, so the result does not depend on scheduling
+constexpr int kColsumGroups = 16;
+__global__ __launch_bounds__(64 * kColsumGroups) void k_colsum(const float* __restrict__ d, int n, int m,
+                                                               float* __restrict__ out) {
+  __shared__ double part[kColsumGroups][64];
+  const int c = threadIdx.x & 63, q = threadIdx.x >> 6, j = blockIdx.x * 64 + c;
   double acc = 0;
-  for (int i = 0; i < n; i++) acc += d[(long)i * m + j];
-  out[j] = (float)acc;
+  if (j < m) {
+#pragma unroll 16
+    for (int i = q; i < n; i += kColsumGroups) acc += d[(long)i * m + j];
+  }
+  part[q][c] = acc;
+  __syncthreads();
+  if (q == 0 && j < m) {
+    double s = 0;
+    for (int g = 0; g < kColsumGroups; g++) s += part[g][c];
+    out[j] = (float)s;
+  }
 }
 // Adam (Kingma & Ba 2015) on fp32 master weights
 __global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
@@ -240,7 +273,7 @@ static int mlp_bwd(csx_mlp* m, const float* x, const float* t, int n, double* lo
   MlpGemm g{};
   g = MlpGemm{o[3], m->nh2, 1, dy, m->nout, 0, m->g(4), m->nout, nullptr, nullptr, m->nh2, m->nout, n, EPI_STORE};
   CSX_TRY(launch(g, m->mode, s));
-  hipLaunchKernelGGL(k_colsum, dim3((m->nout + 255) / 256), dim3(256), 0, s, dy, n, m->nout, m->g(5));
+  hipLaunchKernelGGL(k_colsum, dim3((m->nout + 63) / 64), dim3(64 * kColsumGroups), 0, s, dy, n, m->nout, m->g(5));
   g = MlpGemm{dy, m->nout, 0, m->p(4), m->nout, 1, d2, m->nh2, nullptr, nullptr, n, m->nh2, m->nout, EPI_STORE};
   CSX_TRY(launch(g, m->mode, s));
   hipLaunchKernelGGL(k_gelu_bwd, dim3((int)std::min<long>(4096, ((long)n * m->nh2 + 255) / 256)), dim3(256), 0, s, d2,
@@ -248,7 +281,7 @@ static int mlp_bwd(csx_mlp* m, const float* x, const float* t, int n, double* lo
   // layer 2
   g = MlpGemm{o[1], m->nh1, 1, d2, m->nh2, 0, m->g(2), m->nh2, nullptr, nullptr, m->nh1, m->nh2, n, EPI_STORE};
   CSX_TRY(launch(g, m->mode, s));
-  hipLaunchKernelGGL(k_colsum, dim3((m->nh2 + 255) / 256), dim3(256), 0, s, d2, n, m->nh2, m->g(3));
+  hipLaunchKernelGGL(k_colsum, dim3((m->nh2 + 63) / 64), dim3(64 * kColsumGroups), 0, s, d2, n, m->nh2, m->g(3));
   g = MlpGemm{d2, m->nh2, 0, m->p(2), m->nh2, 1, d1, m->nh1, nullptr, nullptr, n, m->nh1, m->nh2, EPI_STORE};
   CSX_TRY(launch(g, m->mode, s));
   hipLaunchKernelGGL(k_gelu_bwd, dim3((int)std::min<long>(4096, ((long)n * m->nh1 + 255) / 256)), dim3(256), 0, s, d1,
@@ -256,7 +289,7 @@ static int mlp_bwd(csx_mlp* m, const float* x, const float* t, int n, double* lo
   // layer 1
   g = MlpGemm{x, m->nin, 1, d1, m->nh1, 0, m->g(0), m->nh1, nullptr, nullptr, m->nin, m->nh1, n, EPI_STORE};
   CSX_TRY(launch(g, m->mode, s));
-  hipLaunchKernelGGL(k_colsum, dim3((m->nh1 + 255) / 256), dim3(256), 0, s, d1, n, m->nh1, m->g(1));
+  hipLaunchKernelGGL(k_colsum, dim3((m->nh1 + 63) / 64), dim3(64 * kColsumGroups), 0, s, d1, n, m->nh1, m->g(1));
   CSX_HIP(hipGetLastError());
   if (!loss) return CSX_OK;   // no read-back: the step stays asynchronous
   std::vector<double> part(blocks);

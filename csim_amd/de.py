@@ -20,33 +20,139 @@ from dataclasses import dataclass, field
 import numpy as np
 
 
-def population_permutations(pop, n, rng):
-    """makePopulationPermutations: slice i = pop[:, si1_i], si1_{i+1} = si1_i rotated by si2[i]+1."""
-    P = pop.shape[1]
+def _rotations(P, n, rng, perm_P):
+    """Index arrays of makePopulationPermutations: slice i takes the members si1_i,
+    si1_{i+1} = si1_i rotated by si2[i] + 1 (si1 from perm_P, si2 from rng)."""
     si2 = rng.permutation(n)
-    si1 = rng.permutation(P)
-    ii1 = np.arange(P)
+    si1 = perm_P(P)
     out = []
     for i in range(n):
-        out.append(pop[:, si1])
+        out.append(si1)
         if i < n - 1:
-            si1 = si1[np.roll(ii1, si2[i] + 1)]
+            k = int(si2[i]) + 1
+            si1 = si1[(np.arange(P) - k) % P] if isinstance(si1, np.ndarray) else si1.roll(k)
     return out
 
 
-def de_rand1(pop, best, F, rng):
-    p = population_permutations(pop, 3, rng)
+def population_permutations(pop, n, rng):
+    """makePopulationPermutations: slice i = pop[:, si1_i], si1_{i+1} = si1_i rotated by si2[i]+1."""
+    return [pop[:, ix] for ix in _rotations(pop.shape[1], n, rng, rng.permutation)]
+
+
+class _NumpyOps:
+    """Array operations of the DE on the host (numpy, Philox stream)."""
+
+    def __init__(self, seed):
+        self.rng = np.random.Generator(np.random.Philox(seed))
+
+    def vec(self, x):
+        return np.asarray(x, dtype=np.float64)
+
+    def rand(self, shape):
+        return self.rng.random(shape)
+
+    def coin(self):
+        return self.rng.random()
+
+    def perms(self, pop, n):
+        return [pop[:, ix] for ix in _rotations(pop.shape[1], n, self.rng, self.rng.permutation)]
+
+    where, minimum, maximum = staticmethod(np.where), staticmethod(np.minimum), staticmethod(np.maximum)
+
+    def spread(self, pop):
+        mean = pop.mean(axis=1)
+        d = np.linalg.norm(pop - mean[:, None], axis=0)
+        return mean, float(d.mean()), float(d.max())
+
+    def argmin(self, val):
+        i = int(np.argmin(val))
+        return i, float(val[i])
+
+    def full(self, n, v):
+        return np.full(n, v, dtype=np.float64)
+
+    def index(self, idx):
+        return np.asarray(idx, dtype=int)
+
+    def host(self, x):
+        return np.asarray(x)
+
+    def copy(self, x):
+        return np.array(x, dtype=np.float64)
+
+
+class _TorchOps:
+    """The same operations on a torch device: a generation of a large population is a
+    few kernels instead of host passes over it.  Bulk random numbers come from a device
+    generator, the few small integers and coin flips of the strategies from a host
+    Philox stream."""
+
+    def __init__(self, seed, device):
+        import torch
+        self.t, self.dev = torch, torch.device(device)
+        self.gen = torch.Generator(device=self.dev)
+        self.gen.manual_seed(seed)
+        self.rng = np.random.Generator(np.random.Philox(seed))
+
+    def vec(self, x):
+        return self.t.as_tensor(np.asarray(x, dtype=np.float64) if not self.t.is_tensor(x) else x,
+                                dtype=self.t.float64, device=self.dev)
+
+    def rand(self, shape):
+        return self.t.rand(shape, generator=self.gen, dtype=self.t.float64, device=self.dev)
+
+    def coin(self):
+        return self.rng.random()
+
+    def perms(self, pop, n):
+        perm_P = lambda P: self.t.randperm(P, generator=self.gen, device=self.dev)
+        return [pop[:, ix] for ix in _rotations(pop.shape[1], n, self.rng, perm_P)]
+
+    def where(self, m, a, b):
+        return self.t.where(m, a, b)
+
+    def minimum(self, a, b):
+        return self.t.minimum(a, b)
+
+    def maximum(self, a, b):
+        return self.t.maximum(a, b)
+
+    def spread(self, pop):
+        mean = pop.mean(dim=1)
+        d = (pop - mean[:, None]).norm(dim=0)
+        st = self.t.stack([d.mean(), d.max()]).cpu()
+        return mean, float(st[0]), float(st[1])
+
+    def argmin(self, val):
+        i = int(self.t.argmin(val))
+        return i, float(val[i])
+
+    def full(self, n, v):
+        return self.t.full((n,), v, dtype=self.t.float64, device=self.dev)
+
+    def index(self, idx):
+        return self.t.as_tensor(np.asarray(idx, dtype=np.int64), device=self.dev)
+
+    def host(self, x):
+        return x.cpu().numpy()
+
+    def copy(self, x):
+        return x.clone()
+
+
+def de_rand1(pop, best, F, ops):
+    p = ops.perms(pop, 3)
     return p[2] + F * (p[0] - p[1]), p[2]
 
 
-def de_local_to_best1(pop, best, F, rng):
-    p = population_permutations(pop, 2, rng)
+def de_local_to_best1(pop, best, F, ops):
+    p = ops.perms(pop, 2)
     return pop + F * (best[:, None] - pop) + F * (p[0] - p[1]), pop
 
 
-def either_or(pop, best, F, rng):
-    p = population_permutations(pop, 3, rng)
-    if rng.random() < 0.5:
+def either_or(pop, best, F, ops):
+    p = ops.perms(pop, 3)
+    if ops.coin() < 0.5:
         return p[2] + F * (p[0] - p[1]), p[2]
     return p[2] + 0.5 * (F + 1.0) * (p[0] + p[1] - 2 * p[2]), p[2]
 
@@ -93,29 +199,32 @@ class DEResult:
 
 
 class DifferentialEvolution:
-    def __init__(self, objective, dim, cfg: DEConfig, screen=None):
+    def __init__(self, objective, dim, cfg: DEConfig, screen=None, device=None):
         """objective(pop (D, P)) -> values (P).  screen(pop (D, K)) -> indices of the
         candidates worth evaluating exactly (surrogate screening), or (indices, exact
-        values) when the screen evaluates its survivors itself; None evaluates all."""
+        values) when the screen evaluates its survivors itself; None evaluates all.
+        device None keeps the population in numpy arrays; a torch device ("cuda") keeps
+        it there as float64 tensors, which objective and screen then receive."""
         self.f = objective
         self.D = dim
         self.c = cfg
         self.screen = screen
-        self.rng = np.random.Generator(np.random.Philox(cfg.seed))
+        self.ops = _NumpyOps(cfg.seed) if device is None else _TorchOps(cfg.seed, device)
+        ops = self.ops
         gl, gu = cfg.global_bounds
         il, iu = cfg.init_bounds
-        self.glo, self.ghi = np.full(dim, gl), np.full(dim, gu)
-        self.ilo, self.ihi = np.full(dim, il), np.full(dim, iu)
+        self.glo, self.ghi = ops.full(dim, gl), ops.full(dim, gu)
+        self.ilo, self.ihi = ops.full(dim, il), ops.full(dim, iu)
         self.evals = 0
 
     def _eval(self, pop):
         self.evals += pop.shape[1]
-        return np.asarray(self.f(pop), dtype=np.float64)
+        return self.ops.vec(self.f(pop))
 
     def init_population(self, origin, spread=1.0):
         c = self.c
         P = c.population_size
-        return origin[:, None] + spread * c.init_spread * (self.rng.random((self.D, P)) - 0.5) * \
+        return origin[:, None] + spread * c.init_spread * (self.ops.rand((self.D, P)) - 0.5) * \
             (self.ihi - self.ilo)[:, None]
 
     def run(self, time_budget=None, target=None, eval_budget=None, init_pop=None, init_val=None,
@@ -123,13 +232,13 @@ class DifferentialEvolution:
         """Iterates until the csim stopping rules, a wall-time budget (s), a target
         objective, a budget of exact evaluations or stop() is reached.  init_pop (D, P)
         (with its values init_val, or evaluated here) replaces the initial population."""
-        c = self.c
+        c, ops = self.c, self.ops
         t0 = time.perf_counter()
         if init_pop is not None:
-            pop = np.array(init_pop, dtype=np.float64)
-            val = np.array(init_val, dtype=np.float64) if init_val is not None else self._eval(pop)
+            pop = ops.copy(ops.vec(init_pop))
+            val = ops.vec(init_val) if init_val is not None else self._eval(pop)
         else:
-            origin = c.init_mean if c.init_mean is not None else 0.5 * (self.ilo + self.ihi)
+            origin = ops.vec(c.init_mean) if c.init_mean is not None else 0.5 * (self.ilo + self.ihi)
             pop = self.init_population(origin)
             val = self._eval(pop)
         strategy = STRATEGIES[c.start_strategy]
@@ -139,19 +248,14 @@ class DifferentialEvolution:
 
         def find_best():
             nonlocal best_val, best_vec, gbest_val, gbest_vec
-            i = int(np.argmin(val))
-            if val[i] < best_val:
-                best_val, best_vec = float(val[i]), pop[:, i].copy()
+            i, v = ops.argmin(val)
+            if v < best_val:
+                best_val, best_vec = v, ops.copy(pop[:, i])
                 if best_val < gbest_val:
-                    gbest_val, gbest_vec = best_val, best_vec.copy()
-
-        def stats():
-            mean = pop.mean(axis=1)
-            d = np.linalg.norm(pop - mean[:, None], axis=0)
-            return mean, d.mean(), d.max()
+                    gbest_val, gbest_vec = best_val, best_vec
 
         find_best()
-        mean, spread_mean, spread_max = stats()
+        mean, spread_mean, spread_max = ops.spread(pop)
         hist.append((0, time.perf_counter() - t0, gbest_val, self.evals))
         small, prev_best, retries = 0, 0.0, 0
         sel_base = sel_end = False
@@ -175,38 +279,37 @@ class DifferentialEvolution:
                 find_best()
                 small, do_reinit = 0, False
                 retries += 1
-            mutated, original = strategy(pop, best_vec, c.f_weight, self.rng)
+            mutated, original = strategy(pop, best_vec, c.f_weight, ops)
             if c.crossover_prob < 1.0:
-                cross = self.rng.random(pop.shape) < c.crossover_prob
-                cand = np.where(cross, mutated, pop)
+                cand = ops.where(ops.rand(tuple(pop.shape)) < c.crossover_prob, mutated, pop)
             else:
                 cand = mutated
             # bounce-back within [mean + init bounds] clipped to the global bounds
-            up = np.minimum(mean + self.ihi, self.ghi)
-            lo = np.maximum(mean + self.ilo, self.glo)
-            r = self.rng.random(cand.shape)
+            up = ops.minimum(mean + self.ihi, self.ghi)
+            lo = ops.maximum(mean + self.ilo, self.glo)
+            r = ops.rand(tuple(cand.shape))
             below = cand < lo[:, None]
             above = ~below & (cand > up[:, None])
-            cand = np.where(below, lo[:, None] + r * (original - lo[:, None]), cand)
-            cand = np.where(above, up[:, None] + r * (original - up[:, None]), cand)
+            cand = ops.where(below, lo[:, None] + r * (original - lo[:, None]), cand)
+            cand = ops.where(above, up[:, None] + r * (original - up[:, None]), cand)
             # selection (optionally on a surrogate-screened subset)
             if self.screen is not None:
                 res = self.screen(cand)
-                cv = np.full(cand.shape[1], np.inf)
+                cv = ops.full(cand.shape[1], np.inf)
                 if isinstance(res, tuple):          # the screen evaluated its survivors itself
                     idx = np.asarray(res[0], dtype=int)
                     self.evals += idx.size
-                    cv[idx] = res[1]
+                    cv[ops.index(idx)] = ops.vec(res[1])
                 else:
                     idx = np.asarray(res, dtype=int)
                     if idx.size:
-                        cv[idx] = self._eval(cand[:, idx])
+                        cv[ops.index(idx)] = self._eval(cand[:, ops.index(idx)])
             else:
                 cv = self._eval(cand)
             better = cv < val
-            pop[:, better] = cand[:, better]
-            val[better] = cv[better]
-            mean, spread_mean, spread_max = stats()
+            pop = ops.where(better[None, :], cand, pop)
+            val = ops.where(better, cv, val)
+            mean, spread_mean, spread_max = ops.spread(pop)
             find_best()
             if it % c.output_interval == 0:
                 if spread_mean < c.base_strategy_spread_switch and not sel_base:
@@ -220,4 +323,5 @@ class DifferentialEvolution:
                 prev_best = best_val
             it += 1
             hist.append((it, time.perf_counter() - t0, gbest_val, self.evals))
-        return DEResult(gbest_val, gbest_vec, hist, self.evals, time.perf_counter() - t0, pop, val)
+        return DEResult(gbest_val, ops.host(gbest_vec), hist, self.evals, time.perf_counter() - t0, ops.host(pop),
+                        ops.host(val))

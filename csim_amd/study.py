@@ -73,7 +73,9 @@ class ExactObjective:
             self.calls += k
         return (vals, prof) if with_profiles else vals
 
-    def __call__(self, pop):            # DE interface: pop (D, P)
+    def __call__(self, pop):            # DE interface: pop (D, P), numpy or a torch tensor
+        if not isinstance(pop, np.ndarray):
+            pop = pop.cpu().numpy()
         return self.evaluate(np.ascontiguousarray(pop.T))
 
 
@@ -155,13 +157,17 @@ class ScreenedSurrogate:
         return self.fallback_generation is not None
 
     def screen_and_eval(self, pop):
-        """DE hook: pop (D, P) -> (indices evaluated, their exact values)."""
+        """DE hook: pop (D, P), numpy or a device tensor -> (indices evaluated, their
+        exact values).  A device population stays there for the prediction and the
+        features; only the evaluated candidates go to the host."""
         import torch
         self.generation += 1
         t0 = time.perf_counter()
-        t = np.ascontiguousarray(pop.T)
+        dev = torch.is_tensor(pop)
+        t = pop.T.contiguous() if dev else np.ascontiguousarray(pop.T)
+        host = (lambda x: x.cpu().numpy()) if dev else (lambda x: x)
         if self.fallback:                           # plain DE: every mutant exactly
-            vals = self.exact.evaluate(t)
+            vals = self.exact.evaluate(host(t))
             self.seconds["exact"] += time.perf_counter() - t0
             return np.arange(t.shape[0]), vals
         pred = self.predict(t)
@@ -169,8 +175,9 @@ class ScreenedSurrogate:
         npick = self.keep - self.n_control
         ctrl = self.rng.choice(order[npick:], size=min(self.n_control, order.size - npick), replace=False)
         idx = np.concatenate([order[:npick], ctrl])
+        sel = t[torch.as_tensor(idx, device=t.device)] if dev else t[idx]
         t1 = time.perf_counter()
-        vals, prof = self.exact.evaluate(t[idx], with_profiles=True)
+        vals, prof = self.exact.evaluate(host(sel), with_profiles=True)
         t2 = time.perf_counter()
         if idx.size > 2:
             self.errors.append(spearman(pred[idx], vals))
@@ -182,7 +189,7 @@ class ScreenedSurrogate:
                 self.fallback_generation = self.generation + 1
         if self.max_generations is not None and self.generation >= self.max_generations and not self.fallback:
             self.fallback_generation = self.generation + 1
-        self._push(features_device(self.mlp.ctx, t[idx], self.lambdas), log_labels(prof).reshape(-1, N_BINS))
+        self._push(features_device(self.mlp.ctx, sel, self.lambdas), log_labels(prof).reshape(-1, N_BINS))
         self.tune()
         t3 = time.perf_counter()
         self.seconds["predict"] += t1 - t0
@@ -264,7 +271,8 @@ def compare_de(ctx, w, sur, seconds=None, eval_budget=None, plain_pop=64, screen
                            max_generations=screen_generations)
     cfg_s = DEConfig(population_size=screen_pop, global_bounds=(0.0, 1e-6), init_bounds=(0.0, 1e-6),
                      max_iterations=10 ** 9, seed=seed + 2)
-    de_s = DifferentialEvolution(exact_s, nhex, cfg_s, screen=s1.screen_and_eval)
+    # the screened population (512 mutants a generation) lives on the GPU
+    de_s = DifferentialEvolution(exact_s, nhex, cfg_s, screen=s1.screen_and_eval, device="cuda")
     rs = de_s.run(time_budget=seconds, target=target, eval_budget=eval_budget, stop=lambda: s1.fallback)
     best_before_fallback = rs.best_value
     if s1.fallback:

@@ -56,3 +56,44 @@ def test_de_on_reference_response_fixture():
     start = f(np.zeros((499, 1)))[0]
     assert res.best_value < start / 10
     assert res.best_value > 1.3e-10 * 0.5        # cannot beat the reference's long-run optimum by much
+
+
+def test_device_population_same_rules():
+    """The population held as torch tensors (here on the CPU device) follows the same
+    rules: every candidate inside the bounds, convergence on the quadratic, and a
+    screen that evaluates a subset sees tensors and changes only those members."""
+    import torch
+    D = 6
+    target = np.linspace(0.1, 0.9, D)
+    seen = []
+
+    def f(pop):
+        assert torch.is_tensor(pop) and pop.dtype == torch.float64
+        seen.append((float(pop.min()), float(pop.max())))
+        return ((pop - torch.as_tensor(target)[:, None]) ** 2).sum(dim=0)
+
+    cfg = DEConfig(population_size=40, max_iterations=300, global_bounds=(0.0, 1.0), init_bounds=(-0.5, 0.5),
+                   init_mean=np.full(D, 0.5), output_interval=10)
+    res = DifferentialEvolution(f, D, cfg, device="cpu").run()
+    assert min(lo for lo, _ in seen) >= -1e-12 and max(hi for _, hi in seen) <= 1 + 1e-12
+    assert res.best_value < 1e-6
+    assert np.allclose(res.best_vector, target, atol=2e-3)
+    assert isinstance(res.population, np.ndarray) and res.population.shape == (D, 40)
+
+    picked = []
+
+    def screen(cand):
+        assert torch.is_tensor(cand)
+        idx = np.arange(0, cand.shape[1], 4)
+        picked.append(idx)
+        return idx, f(cand[:, torch.as_tensor(idx)])
+
+    cfg2 = DEConfig(population_size=40, max_iterations=0, global_bounds=(0.0, 1.0), init_bounds=(-0.5, 0.5),
+                    init_mean=np.full(D, 0.5))                # one generation
+    de = DifferentialEvolution(f, D, cfg2, screen=screen, device="cpu")
+    init = de.init_population(de.ops.vec(np.full(D, 0.5)))
+    res2 = de.run(init_pop=init, init_val=f(init))
+    assert len(picked) == 1 and res2.evaluations == 10
+    untouched = np.setdiff1d(np.arange(40), picked[0])
+    assert np.array_equal(res2.population[:, untouched], init.numpy()[:, untouched])
+    assert not np.array_equal(res2.population[:, picked[0]], init.numpy()[:, picked[0]])
