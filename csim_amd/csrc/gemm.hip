@@ -478,6 +478,16 @@ template <> struct MfmaBf16<32> {
   static __device__ __forceinline__ int dcol(int lane) { return lane & 31; }
 };
 
+// Position of output column nl of tile row ml in the LDS staging of plane epilogues
+// (rows of BN >= 16 elements, BN a multiple of 16; a permutation of each row's aligned
+// 16-element groups).  The finishing loop has thread c of a row read columns 8 c + k,
+// k = 0..7: the chunk index XORed into the low bits spreads those 8 lanes over 8
+// distinct positions of a 16-element group for every k, and the row parity in bit 3
+// keeps the next row's 8 lanes off them (the 16 lanes a 128-byte LDS cycle serves for
+// 8-byte elements, or the 8 lanes of one row for 16-byte ones).  A store instruction of
+// the MFMA layout writes 16 consecutive columns of one row: they stay distinct.
+__device__ __forceinline__ int stage_col(int ml, int nl) { return nl ^ ((nl >> 3) & 7) ^ ((ml & 1) << 3); }
+
 // Zero chunk that direct-to-LDS loads of k beyond K read instead of the matrix.
 static __device__ __align__(16) unsigned int kZeroChunk[4];
 
@@ -815,11 +825,10 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
   }
   // BF16-plane outputs are staged through LDS so that every lane stores 16
   // contiguous bytes of one plane (the MFMA layout gives a lane 4 rows of one
-  // column).  Chunk cc (8 columns) of tile row ml sits at cc ^ (2 ((ml >> 2) & 3)):
-  // the 4 rows one store instruction covers land on distinct banks.
+  // column); column nl of tile row ml sits at stage_col(ml, nl).
   constexpr bool kStaged = 2 * TERMS * BM * BN <= NBUF * STAGE;
   if (TERMS == 2 && BN % 64 == 0 && kStaged && P.out_kind == OUT_PLANES && !P.transC && P.terms == TERMS) {
-    // The FP32 sums go through LDS (row ml, column nl ^ 16 (ml & 3)); then each thread
+    // The FP32 sums go through LDS (row ml, position stage_col(ml, nl)); then each thread
     // finishes 8 consecutive outputs of one row with 16-byte reads of the addend planes
     // and of a whole-grid real mask, the epilogue arithmetic of epilogue_value, the BF16
     // split, and one 16-byte store per plane.
@@ -833,7 +842,7 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
         for (int v = 0; v < NV; v++) {
           const int ml = wm * (BM / WM) + i * MF + Mf::drow(lane, v);
           const int nl = wn * (BN / WN) + j * MF + Mf::dcol(lane);
-          st[ml * BN + (nl ^ ((ml & 3) << 4))] = make_float2(accR[i][j][v], accI[i][j][v]);
+          st[ml * BN + stage_col(ml, nl)] = make_float2(accR[i][j][v], accI[i][j][v]);
         }
     __syncthreads();
     const bool empty = nslab == 0;
@@ -877,7 +886,7 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
       unsigned short h[4][8];
 #pragma unroll
       for (int k = 0; k < 8; k++) {
-        const float2 a = st[ml * BN + ((cc * 8 + k) ^ ((ml & 3) << 4))];
+        const float2 a = st[ml * BN + stage_col(ml, cc * 8 + k)];
         double vr = 0.0, vi = 0.0;
         if (!empty) {
           if (vmask) {
@@ -915,7 +924,7 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
   constexpr bool kStaged64 = BM * BN * 16 <= 2 * NBUF * STAGE && BN % 64 == 0;
   if (TERMS == 3 && kStaged64 && P.out_kind == OUT_PLANES && !P.transC && P.terms == 3) {
     // Three-term planes, whole-grid real mask or none (the common case): the FP64 sums go
-    // through LDS (row ml, column nl ^ 16 (ml & 3)), then each thread finishes 8
+    // through LDS (row ml, position stage_col(ml, nl)), then each thread finishes 8
     // consecutive outputs of one row (epilogue_value's arithmetic with 8-byte mask reads
     // and 16-byte reads of the addend's planes), the 4 threads of a window of 32 columns
     // (4 consecutive lanes: BN / 8 chunks per row, a multiple of 4) agree on its exponent,
@@ -936,7 +945,7 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
           for (int v = 0; v < NV; v++) {
             const int ml = wm * (BM / WM) + i * MF + Mf::drow(lane, v);
             const int nl = wn * (BN / WN) + j * MF + Mf::dcol(lane);
-            st[ml * BN + (nl ^ ((ml & 3) << 4))] =
+            st[ml * BN + stage_col(ml, nl)] =
                 f64x2{(double)accR[i][j][v] + dR[TERMS == 3 ? i : 0][TERMS == 3 ? j : 0][v],
                       (double)accI[i][j][v] + dI[TERMS == 3 ? i : 0][TERMS == 3 ? j : 0][v]};
           }
@@ -990,7 +999,7 @@ void k_gemm_planes(const GemmProb* __restrict__ probs, int nprob) {
         float mx = 0.0f;
 #pragma unroll
         for (int k = 0; k < 8; k++) {
-          const f64x2 a = st[ml * BN + ((cc * 8 + k) ^ ((ml & 3) << 4))];
+          const f64x2 a = st[ml * BN + stage_col(ml, cc * 8 + k)];
           double xr = 0.0, xi = 0.0;
           if (live && !empty) {
             xr = P.alpha_re * a.x - P.alpha_im * a.y;
