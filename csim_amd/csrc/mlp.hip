@@ -28,14 +28,18 @@ struct MlpGemm {
   float* C2;                             // GELU output (EPI_BIAS_GELU)
   const float* bias;                     // N
   int M, N, K, epi;
+  // split K (EPI_STORE only): blockIdx.z sums K range [z kchunk, (z + 1) kchunk) into
+  // ws[z][M][N]; k_splitk_sum adds the ranges in order into C
+  int kchunk;
+  float* ws;
 };
 
-__device__ __forceinline__ float opA(const MlpGemm& g, int m, int k) {
-  if (m >= g.M || k >= g.K) return 0.f;
+__device__ __forceinline__ float opA(const MlpGemm& g, int m, int k, int kend) {
+  if (m >= g.M || k >= kend) return 0.f;
   return g.transA ? g.A[(long)k * g.lda + m] : g.A[(long)m * g.lda + k];
 }
-__device__ __forceinline__ float opB(const MlpGemm& g, int k, int n) {
-  if (n >= g.N || k >= g.K) return 0.f;
+__device__ __forceinline__ float opB(const MlpGemm& g, int k, int n, int kend) {
+  if (n >= g.N || k >= kend) return 0.f;
   return g.transB ? g.B[(long)n * g.ldb + k] : g.B[(long)k * g.ldb + n];
 }
 __device__ __forceinline__ float gelu_f(float u) { return 0.5f * u * (1.0f + erff(u * 0.70710678118654752f)); }
@@ -60,6 +64,7 @@ __global__ __launch_bounds__(256) void k_mlp_gemm(MlpGemm g) {
   // contiguous in memory runs fastest over the threads, so each wave's loads are
   // coalesced whichever way the operand is stored.  The next step's values are loaded
   // into registers while the current step's MFMAs run.
+  const int kbeg = g.ws ? blockIdx.z * g.kchunk : 0, kend = g.ws ? min(g.K, kbeg + g.kchunk) : g.K;
   float av[8], bv[8];
   auto gload = [&](int k0) {
 #pragma unroll
@@ -67,12 +72,12 @@ __global__ __launch_bounds__(256) void k_mlp_gemm(MlpGemm g) {
       const int e = t + i * 256;
       const int ra = g.transA ? (e & 63) : (e >> 5), ka = g.transA ? (e >> 6) : (e & 31);
       const int rb = g.transB ? (e >> 5) : (e & 63), kb = g.transB ? (e & 31) : (e >> 6);
-      av[i] = opA(g, m0 + ra, k0 + ka);
-      bv[i] = opB(g, k0 + kb, n0 + rb);
+      av[i] = opA(g, m0 + ra, k0 + ka, kend);
+      bv[i] = opB(g, k0 + kb, n0 + rb, kend);
     }
   };
-  gload(0);
-  for (int k0 = 0; k0 < g.K; k0 += 32) {
+  gload(kbeg);
+  for (int k0 = kbeg; k0 < kend; k0 += 32) {
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < 8; i++) {
@@ -83,7 +88,7 @@ __global__ __launch_bounds__(256) void k_mlp_gemm(MlpGemm g) {
       Bs[rb][kb] = bv[i];
     }
     __syncthreads();
-    if (k0 + 32 < g.K) gload(k0 + 32);
+    if (k0 + 32 < kend) gload(k0 + 32);
     if (MODE == kBF16) {
       const int fr = lane & 15, fk = 8 * (lane >> 4);
       bf16x8 a[2], b[2];
@@ -121,6 +126,10 @@ __global__ __launch_bounds__(256) void k_mlp_gemm(MlpGemm g) {
         int m = m0 + wm * 32 + i * 16 + 4 * (lane >> 4) + v;
         int n = n0 + wn * 32 + j * 16 + (lane & 15);
         if (m >= g.M || n >= g.N) continue;
+        if (g.ws) {
+          g.ws[((long)blockIdx.z * g.M + m) * g.N + n] = acc[i][j][v];
+          continue;
+        }
         float x = acc[i][j][v];
         if (g.epi != EPI_STORE) x += g.bias[n];
         g.C[(long)m * g.ldc + n] = x;
@@ -128,11 +137,28 @@ __global__ __launch_bounds__(256) void k_mlp_gemm(MlpGemm g) {
       }
 }
 
+// C[m][n] = sum over the K ranges z of ws[z][m][n], in order
+__global__ void k_splitk_sum(const float* __restrict__ ws, int nsplit, int M, int N, float* __restrict__ C, int ldc) {
+  const long total = (long)M * N;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    float x = 0.f;
+    for (int z = 0; z < nsplit; z++) x += ws[z * total + e];
+    C[(e / N) * ldc + e % N] = x;
+  }
+}
+
 static int launch(const MlpGemm& g, int mode, hipStream_t s) {
-  dim3 grid((g.N + 63) / 64, (g.M + 63) / 64);
+  const int nsplit = g.ws ? (g.K + g.kchunk - 1) / g.kchunk : 1;
+  dim3 grid((g.N + 63) / 64, (g.M + 63) / 64, nsplit);
   if (mode == kBF16) hipLaunchKernelGGL(k_mlp_gemm<kBF16>, grid, dim3(256), 0, s, g);
   else hipLaunchKernelGGL(k_mlp_gemm<kF32>, grid, dim3(256), 0, s, g);
   CSX_HIP(hipGetLastError());
+  if (g.ws) {
+    const long total = (long)g.M * g.N;
+    hipLaunchKernelGGL(k_splitk_sum, dim3((int)std::min<long>(1024, (total + 255) / 256)), dim3(256), 0, s, g.ws,
+                       nsplit, g.M, g.N, g.C, g.ldc);
+    CSX_HIP(hipGetLastError());
+  }
   return CSX_OK;
 }
 
@@ -222,6 +248,7 @@ struct csx_mlp {
   DevBuf act;                   // activations for the largest batch seen
   int act_batch = 0;
   DevBuf partial;
+  DevBuf ws;                    // split-K partial sums of the weight gradients
   long step = 0;
   float* p(int i) { return P.as<float>() + off[i]; }
   float* g(int i) { return G.as<float>() + off[i]; }
@@ -255,6 +282,20 @@ static int mlp_fwd(csx_mlp* m, const float* x, int n, float** outs) {
   return CSX_OK;
 }
 
+// Weight gradients contract over the batch (K = n) into few output tiles: a long K
+// range is split so that the launch has enough blocks (at least 128-deep ranges).
+static int launch_wgrad(csx_mlp* m, MlpGemm g, hipStream_t s) {
+  const int tiles = ((g.M + 63) / 64) * ((g.N + 63) / 64);
+  int nsplit = std::min(8, std::max(1, g.K / 128));
+  while (nsplit > 1 && tiles * (nsplit / 2) >= 256) nsplit /= 2;
+  if (nsplit > 1) {
+    g.kchunk = ((g.K + nsplit - 1) / nsplit + 31) / 32 * 32;
+    CSX_TRY(m->ws.reserve(sizeof(float) * (size_t)((g.K + g.kchunk - 1) / g.kchunk) * g.M * g.N));
+    g.ws = m->ws.as<float>();
+  }
+  return launch(g, m->mode, s);
+}
+
 // loss + gradients into m->G
 static int mlp_bwd(csx_mlp* m, const float* x, const float* t, int n, double* loss) {
   float* o[5];
@@ -272,7 +313,7 @@ static int mlp_bwd(csx_mlp* m, const float* x, const float* t, int n, double* lo
   // layer 3: dW3 = a2^T dy, db3 = colsum dy, d2 = dy W3^T * gelu'(h2)
   MlpGemm g{};
   g = MlpGemm{o[3], m->nh2, 1, dy, m->nout, 0, m->g(4), m->nout, nullptr, nullptr, m->nh2, m->nout, n, EPI_STORE};
-  CSX_TRY(launch(g, m->mode, s));
+  CSX_TRY(launch_wgrad(m, g, s));
   hipLaunchKernelGGL(k_colsum, dim3((m->nout + 63) / 64), dim3(64 * kColsumGroups), 0, s, dy, n, m->nout, m->g(5));
   g = MlpGemm{dy, m->nout, 0, m->p(4), m->nout, 1, d2, m->nh2, nullptr, nullptr, n, m->nh2, m->nout, EPI_STORE};
   CSX_TRY(launch(g, m->mode, s));
@@ -280,7 +321,7 @@ static int mlp_bwd(csx_mlp* m, const float* x, const float* t, int n, double* lo
                      o[2], (long)n * m->nh2);
   // layer 2
   g = MlpGemm{o[1], m->nh1, 1, d2, m->nh2, 0, m->g(2), m->nh2, nullptr, nullptr, m->nh1, m->nh2, n, EPI_STORE};
-  CSX_TRY(launch(g, m->mode, s));
+  CSX_TRY(launch_wgrad(m, g, s));
   hipLaunchKernelGGL(k_colsum, dim3((m->nh2 + 63) / 64), dim3(64 * kColsumGroups), 0, s, d2, n, m->nh2, m->g(3));
   g = MlpGemm{d2, m->nh2, 0, m->p(2), m->nh2, 1, d1, m->nh1, nullptr, nullptr, n, m->nh1, m->nh2, EPI_STORE};
   CSX_TRY(launch(g, m->mode, s));
@@ -288,7 +329,7 @@ static int mlp_bwd(csx_mlp* m, const float* x, const float* t, int n, double* lo
                      o[0], (long)n * m->nh1);
   // layer 1
   g = MlpGemm{x, m->nin, 1, d1, m->nh1, 0, m->g(0), m->nh1, nullptr, nullptr, m->nin, m->nh1, n, EPI_STORE};
-  CSX_TRY(launch(g, m->mode, s));
+  CSX_TRY(launch_wgrad(m, g, s));
   hipLaunchKernelGGL(k_colsum, dim3((m->nh1 + 63) / 64), dim3(64 * kColsumGroups), 0, s, d1, n, m->nh1, m->g(1));
   CSX_HIP(hipGetLastError());
   if (!loss) return CSX_OK;   // no read-back: the step stays asynchronous

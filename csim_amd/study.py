@@ -335,9 +335,10 @@ def main():
     ap.add_argument("--lambdas", type=int, default=9)
     ap.add_argument("--epochs", type=int, default=10)
     ap.add_argument("--precision", default="bf16x3")
+    ap.add_argument("--seed", type=int, default=20261207, help="training data, model and DE streams")
     a = ap.parse_args()
     res = run(n=a.n, n_train=a.train, seconds=a.seconds, n_lambda=a.lambdas, epochs=a.epochs, precision=a.precision,
-              log=lambda s: print(s, flush=True))
+              seed=a.seed, log=lambda s: print(s, flush=True))
     print(json.dumps(res))
 
 
