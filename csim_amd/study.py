@@ -94,7 +94,7 @@ class ScreenedSurrogate:
     exact values of the survivors: their labels go into a device ring buffer and a few
     Adam steps on random batches of it follow every generation.
 
-    The exact propagation is the judge: `n_control` of the `keep` exact evaluations of a
+    The exact propagation checks the surrogate: `n_control` of the `keep` exact evaluations of a
     generation go to mutants drawn at random from those the surrogate did not pick, and
     the generation's screening gain is the mean log10 contrast of those controls minus
     that of the picked ones (positive: the surrogate's picks are darker than chance).
@@ -105,7 +105,7 @@ class ScreenedSurrogate:
     exactly.  Within a converged population picks and controls are alike, their gain
     is noise around zero, and the surrogate stays.  The rank correlation of predicted and exact
     contrast over the evaluated mutants is recorded too; within a converged population
-    it is close to zero even for a useful surrogate, so it is reported, not judged."""
+    it is close to zero even for a useful surrogate, so it is reported, not acted on."""
 
     def __init__(self, mlp, lambdas, counts, exact, keep=32, tune_steps=2, lr=3e-4, capacity=32768, seed=5,
                  n_control=4, fallback_window=8, fallback_gain=-0.02, max_generations=None):
@@ -247,7 +247,7 @@ def compare_de(ctx, w, sur, seconds=None, eval_budget=None, plain_pop=64, screen
                seed=20261207, log=print, screen_generations=None):
     """Plain DE, then screened DE (stopping at the plain run's best value when a time
     budget is given) starting from the trained model, which is restored afterwards.
-    When the judge drops the surrogate (or after screen_generations screened
+    When the exact check drops the surrogate (or after screen_generations screened
     generations) the screened run continues as plain DE from its best members.
     Budgets: wall seconds or exact candidate evaluations."""
     import torch
