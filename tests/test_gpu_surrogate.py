@@ -7,7 +7,7 @@ without it (all figures are measured, nothing is assumed of the surrogate's qual
   * search: at the same number of exact candidate propagations, DE whose 512-mutant
     generations are screened down to 32 by the surrogate reaches a darker dark hole
     than plain DE propagating every one of its 64 mutants;
-  * fallback: the exact propagation judges the surrogate every generation, and a
+  * fallback: the exact propagation checks the surrogate every generation, and a
     surrogate whose ranking goes wrong is dropped for plain DE.
 """
 import numpy as np
@@ -49,7 +49,7 @@ def test_surrogate_screening_gain(ctx):
 
 class InvertedSurrogate(study.ScreenedSurrogate):
     """A surrogate whose predictions rank the mutants in the reverse order of their exact
-    contrast (its survivors are the worst mutants): the judge must drop it."""
+    contrast (its survivors are the worst mutants): the exact check must drop it."""
 
     def predict(self, thickness):
         return -self.exact.evaluate(thickness)
@@ -81,7 +81,7 @@ def test_surrogate_fallback_to_plain_de(ctx):
     res = DifferentialEvolution(exact, nhex, cfg, screen=screen).run()
     print("screening gains", s.gains, "fallback from generation", s.fallback_generation, "evaluated", evaluated)
     assert all(g < 0 for g in s.gains)
-    assert s.fallback_generation == 4                       # after 3 judged generations
+    assert s.fallback_generation == 4                       # after 3 checked generations
     assert evaluated[:3] == [8, 8, 8] and all(n == 32 for n in evaluated[3:])
     assert np.isfinite(res.best_value)
     exact.ch.close()
