@@ -31,8 +31,9 @@ def _params(p, sizes):
     return flat, out
 
 
+@pytest.mark.parametrize("nb", [96, 1000])       # 1000: the weight gradients split K (7 ranges, a short last one)
 @pytest.mark.parametrize("fp32", [True, False])
-def test_mlp_gradients_vs_fp64_oracle(ctx, fp32, monkeypatch):
+def test_mlp_gradients_vs_fp64_oracle(ctx, fp32, nb, monkeypatch):
     import torch
     from csim_amd._lib import check, lib
     monkeypatch.setenv("CSX_MLP_FP32", "1" if fp32 else "0")
@@ -40,7 +41,6 @@ def test_mlp_gradients_vs_fp64_oracle(ctx, fp32, monkeypatch):
     p = _mlp(ctx, sizes, seed=5)
     flat, P = _params(p, sizes)
     rng = np.random.default_rng(2)
-    nb = 96
     x = rng.random((nb, 218)).astype(np.float32)
     t = (rng.standard_normal((nb, 32)) * 0.5 - 6).astype(np.float32)
     dx = torch.from_numpy(x).cuda()
@@ -55,7 +55,7 @@ def test_mlp_gradients_vs_fp64_oracle(ctx, fp32, monkeypatch):
     lref, gref = oracle.mlp_loss_grad(x.astype(np.float64), t.astype(np.float64), P)
     gflat = np.concatenate([v.ravel() for v in gref])
     rel = np.linalg.norm(g - gflat) / np.linalg.norm(gflat)
-    print(f"fp32={fp32}: loss {loss.value:.6e} vs {lref:.6e}; gradient rel err {rel:.3e}")
+    print(f"fp32={fp32} batch {nb}: loss {loss.value:.6e} vs {lref:.6e}; gradient rel err {rel:.3e}")
     tol = 1e-5 if fp32 else 3e-2
     assert abs(loss.value - lref) <= tol * abs(lref)
     assert rel < tol
